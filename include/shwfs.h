@@ -1,0 +1,107 @@
+/*
+ * shwfs.h -- C ABI of the MI355X Shack-Hartmann wavefront-sensor library (libshwfs.so).
+ *
+ * Plain C types only: every array argument is a device pointer on the plan's device,
+ * every stream is a hipStream_t passed as void*.  All functions return 0 on success or a
+ * negative error code; shwfs_last_error() returns the message of the last failure on the
+ * calling thread.  No function allocates device memory except the *_create calls.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the
+ * SPECULA checkout).  The reference path is Python; the ctypes binding a maintainer
+ * adds on the reference side is shown in INTEGRATION.md.
+ */
+#ifndef SHWFS_H
+#define SHWFS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SHWFS_OK              0
+#define SHWFS_ERR_ARG        -1   /* invalid argument / unsupported geometry            */
+#define SHWFS_ERR_HIP        -2   /* HIP runtime error                                  */
+#define SHWFS_ERR_UNSUPPORTED -3  /* geometry outside the compiled kernel instantiations */
+
+typedef struct shwfs_plan shwfs_plan;
+typedef struct shslopec_plan shslopec_plan;
+
+/* Geometry of one SH sensor, derived on the host by the restatement of
+ * SH.set_in_ef / calc_trigger_geometry (processing_objects/sh.py:112-323). */
+typedef struct shwfs_geom {
+    int32_t n_lenses;      /* subapertures per side: Lenslet.dimx == dimy (sh.py:67)          */
+    int32_t ef_size;       /* input ElectricField side [px]                                   */
+    int32_t np_sub;        /* phase px per subaperture side: SH._ovs_np_sub (sh.py:218)       */
+    int32_t fft_size;      /* reference zero-padded FFT size N: SH._fft_size (sh.py:219)      */
+    int32_t spec_size;     /* M: power-spectrum grid, power of two >= 2*np_sub-1             */
+    int32_t subap_npx;     /* CCD px per subaperture side: SH._subap_npx                     */
+    int32_t lgs;           /* 1: per-subaperture ConvolutionKernel correlation (sh.py:382-394) */
+    int32_t parseval;      /* 1: sum of the binned image = norm_const * sum(A^2) exactly      */
+    int32_t max_frames;    /* largest n_frames accepted by shwfs_forward                      */
+    int32_t reserved;
+    double  k_rad_per_nm;  /* 2*pi/lambda (electric_field.py:65)                              */
+    double  tilt_rad;      /* pi/N: half-pixel tilt coefficient (sh.py:469-477)               */
+    double  tilt_a0;       /* first tilt abscissa: -ceil(np_sub/2)                            */
+    double  norm_const;    /* parseval mode: sum over all CCD px of the unscaled image / sum(A^2) */
+    double  pixel_area;    /* ElectricField.pixel_pitch**2 (electric_field.py:87-89)           */
+} shwfs_geom;
+
+/* Library identification. */
+const char* shwfs_version(void);
+const char* shwfs_last_error(void);
+
+/* Create a sensor plan on `device`.
+ *   row_w, col_w : binning operators (host, FP64).
+ *       NGS: real, subap_npx x spec_size, row-major:  B = row_w . S . col_w^T, where S is the
+ *            spec_size^2 power spectrum of one subaperture.  Encodes fftshift (sh.py:396),
+ *            FoV mask (sh.py:399), FoV cut (sh.py:405) and the CCD rebin (lib/toccd.py:33-63).
+ *       LGS: complex interleaved (re,im), subap_npx x (2*np_sub-1):  B = Re(row_w . D . col_w^T)
+ *            with D the kernel-weighted autocorrelation on the lag grid.
+ * Replaces the geometry/state that SH.setup() builds (sh.py:439-467). */
+int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double* col_w,
+                      int device, shwfs_plan** out);
+int shwfs_plan_destroy(shwfs_plan* plan);
+
+/* LGS only: per-subaperture kernel spectra sampled on the lag grid, device complex64
+ * interleaved, layout [n_lenses^2][2*np_sub-1][2*np_sub-1], subaperture s = row*n + col,
+ * lag index (ly + np_sub-1, lx + np_sub-1).  Copied into the plan (device-to-device).
+ * Replaces ConvolutionKernel.kernels as read at sh.py:383-388. */
+int shwfs_set_kernels(shwfs_plan* plan, const float* kspec_lags, void* stream);
+
+/* SH forward for n_frames ElectricFields: one CCD intensity per frame, normalised to
+ * S0 * masked_area (sh.py:422-428).  A, phase_nm: ef_size x ef_size float32 per frame
+ * (a_frame_stride may be 0 to share one amplitude); ccd: (n*subap_npx)^2 float32 per frame.
+ * Replaces SH.trigger_code + SH.post_trigger (processing_objects/sh.py:348-428). */
+int shwfs_forward(shwfs_plan* plan, int n_frames,
+                  const float* A, int64_t a_frame_stride,
+                  const float* phase_nm, int64_t phase_frame_stride,
+                  double S0, float* ccd, int64_t ccd_frame_stride, void* stream);
+
+/* ShSlopec plan.  Either `block_origin` (n_sub linear pixel offsets of square np_sub^2
+ * blocks, row stride frame_w) or `idx` (n_sub x np_sub^2 linear pixel indices, int32) must
+ * be given (host pointers).  xw, yw, mw: np_sub^2 weights (host, FP64): the x/y CoG weights
+ * and mask_weighted of ShSlopec.computeXYweights (processing_objects/sh_slopec.py:86-132).
+ * Replaces the state built by ShSlopec.__init__/set_xy_weights (sh_slopec.py:26-84). */
+int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
+                         const int32_t* block_origin, const int32_t* idx,
+                         const double* xw, const double* yw, const double* mw,
+                         int max_frames, int device, shslopec_plan** out);
+int shslopec_plan_destroy(shslopec_plan* plan);
+
+/* Thresholded centre of gravity for n_frames pixel frames.
+ *   thr_mode: 0 = no threshold, 1 = subtract thr then clamp at 0, 2 = pedestal (zero below thr)
+ *   slopes  : float32, 2*n_sub per frame, [sx..., sy...] (data_objects/slopes.py:27-32)
+ *   flux    : FP64, n_sub per frame (pre-threshold subaperture sums, sh_slopec.py:365)
+ *   totals  : FP64, 2 per frame: sum and mean of flux (sh_slopec.py:432-434)
+ * Replaces ShSlopec.calc_slopes_nofor (processing_objects/sh_slopec.py:308-438). */
+int shslopec_forward(shslopec_plan* plan, int n_frames,
+                     const float* pixels, int64_t pixel_frame_stride,
+                     float thr, int thr_mode,
+                     float* slopes, int64_t slopes_frame_stride,
+                     double* flux, double* totals, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SHWFS_H */

@@ -1,0 +1,465 @@
+"""FP64 NumPy restatement of the reference SH / ShSlopec / toccd / LGS-kernel algorithms.
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` -- never by the product package ``specula_amd``.
+
+Every function cites the reference file:line it follows (paths relative to the
+reference checkout).  The restatement is deliberately literal: same loop structure
+(per-subaperture-row FFT batches), same FFT library (numpy pocketfft), same rebin
+index arithmetic, so that it reproduces the reference's numbers.  Deviations:
+
+* ``cutpixels == 0``: the reference slice ``psf[:, c//2:-c//2]`` is empty and raises
+  (sh.py:405,410).  Here the slice end is ``N - ceil(c/2)``, which equals the
+  reference's end for every ``c >= 1`` and is ``N`` for ``c == 0``.
+* Only the ``fov_ovs == 1`` path without rotation/shift is restated (no Interp2D,
+  no edge extrapolation); ``sh_forward`` raises otherwise.
+
+Pinning (what checks each function; see tests/test_oracle_pins.py):
+  make_xy / make_mask ....... closed forms + reference docstring examples (make_xy.py:93-119)
+  sh_geometry ............... hand-derived geometries (survey cfg0/cfg1/cfg2, test_sh.py
+                              geometry, params_scao_sh_test.yml geometry: N=32 cut 24)
+  sh_forward (NGS) .......... brute-force O(N^4) DFT on tiny N, Parseval flux known answer
+                              (test_sh.py:18-42), 1-px tilt shift (test_sh.py:44-95)
+  sh_forward (LGS) .......... direct spatial circular correlation on tiny N, flux preservation
+  toccd / rebin2d ........... block sums (integer ratio), brute-force replicate-then-mean
+                              (LCM ratio), total conservation
+  shslopec .................. hand-computed tiny cases (threshold, clamp, 1e-3*mean cut),
+                              1-px tilt slope 2/(npx-1) (test_sh_slopec.py:109-114)
+  detect_subaps ............. idx layout / ordering of scao_subaps_n8_th0.5_ref.fits
+  lgs_map_sh ................ IDL golden lgs_map_sh_ref.fits (test_convolution_kernel.py:175-276)
+"""
+import math
+
+import numpy as np
+
+ASEC2RAD = np.pi / (3600 * 180)          # specula/__init__.py:21
+RAD2ASEC = 1.0 / ASEC2RAD                # specula/__init__.py:22
+
+
+# --------------------------------------------------------------------------- grids / masks
+
+def make_xy(sampling, ratio, dtype=None):
+    """specula/lib/make_xy.py:12-177 (cartesian, not quarter, not fft-ordered)."""
+    if sampling <= 1:
+        raise ValueError('make_xy: sampling must be larger than 1')
+    x0 = (sampling - 1) / 2.0                                   # make_xy.py:157
+    x = (np.arange(sampling, dtype=dtype) - x0) / (sampling / 2.0) * ratio   # :161
+    x = np.tile(x, (sampling, 1))                               # :172
+    y = np.transpose(x).copy()                                  # :173
+    return x, y
+
+
+def make_mask(np_size, obsratio=0.0, diaratio=1.0, xc=0.0, yc=0.0, square=False, inverse=False):
+    """specula/lib/make_mask.py:7-147 (no spider, no centeronpixel)."""
+    if diaratio is None:
+        diaratio = 1.0
+    if obsratio is None:
+        obsratio = 0.0
+    x, y = make_xy(np_size, 1.0)
+    ir = obsratio
+    if square:                                                  # make_mask.py:60-64
+        sel = ((np.abs(x - xc) <= diaratio) & (np.abs(y - yc) <= diaratio) &
+               ((np.abs(x - xc) >= diaratio * ir) | (np.abs(y - yc) >= diaratio * ir)))
+    else:                                                       # make_mask.py:65-69
+        r2 = (x - xc) ** 2 + (y - yc) ** 2
+        sel = (r2 < diaratio ** 2) & (r2 >= (diaratio * ir) ** 2)
+    mask = np.zeros((np_size, np_size), dtype=np.uint8)
+    mask[sel] = 1
+    if inverse:
+        mask = 1 - mask
+    return mask
+
+
+def lenslet(n_lenses):
+    """specula/data_objects/lenslet.py:6-39 -> lenses[i][j] = [x(j), y(i), 2/n]."""
+    if n_lenses > 1:
+        x, y = make_xy(n_lenses, 1.0)
+    else:
+        x = np.zeros((1, 1))
+        y = np.zeros((1, 1))
+    size = 2.0 / n_lenses
+    return [[[x[i, j], y[i, j], size] for j in range(n_lenses)] for i in range(n_lenses)]
+
+
+# --------------------------------------------------------------------------- geometry
+
+def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_pxscale,
+                subap_on_diameter, subap_npx, FoVres30mas=False, squaremask=True,
+                fov_ovs_coeff=0.0, do_not_double_fov_ovs=False, set_fov_res_to_turbpxsc=False,
+                fov_resolution_arcsec=None):
+    """Restates SH.__init__ (sh.py:43-110), set_in_ef (sh.py:112-237) and the geometry part of
+    calc_trigger_geometry (sh.py:239-282).  Angles in arcsec as in the YAML keys.
+    ``fov_resolution_arcsec`` mirrors assigning ``sh._fov_resolution_arcsec`` before setup()."""
+    lens = lenslet(subap_on_diameter)
+    n_lenses = subap_on_diameter
+    dimx = len(lens)
+    _subap_wanted_fov = subap_wanted_fov / RAD2ASEC                  # sh.py:68
+    _sensor_pxscale = sensor_pxscale / RAD2ASEC                      # sh.py:69
+    _fov_resolution_arcsec = 0.03 if FoVres30mas else 0              # sh.py:73
+    if fov_resolution_arcsec is not None:
+        _fov_resolution_arcsec = fov_resolution_arcsec
+    _fov_ovs_coeff = fov_ovs_coeff
+    l0 = lens[0][0]
+
+    _np_sub = max([1, round((ef_size * l0[2]) / 2.0)])               # sh.py:118
+    if _np_sub * n_lenses > ef_size:                                 # sh.py:119-120
+        _np_sub -= 1
+    np_sub = (ef_size * l0[2]) / 2.0                                 # sh.py:122
+    sensor_pxscale_arcsec = _sensor_pxscale * RAD2ASEC
+    dSubApInM = np_sub * pixel_pitch
+    turbulence_pxscale = wavelengthInNm * 1e-9 / dSubApInM * RAD2ASEC
+    subap_wanted_fov_arcsec = _subap_wanted_fov * RAD2ASEC
+    subap_real_fov_arcsec = _sensor_pxscale * subap_npx * RAD2ASEC
+
+    if _fov_resolution_arcsec == 0:                                  # sh.py:130-179
+        if set_fov_res_to_turbpxsc:
+            if turbulence_pxscale >= sensor_pxscale_arcsec:
+                raise ValueError('set_fov_res_to_turbpxsc property should be set to one only if '
+                                 'turb. pix. sc. is < sensor pix. sc.')
+            _fov_resolution_arcsec = turbulence_pxscale
+        elif turbulence_pxscale < sensor_pxscale_arcsec and sensor_pxscale_arcsec / 2.0 > 0.5:
+            _fov_resolution_arcsec = turbulence_pxscale * 0.5
+        else:
+            i = 0
+            resTry = turbulence_pxscale / (i + 2)
+            while resTry >= sensor_pxscale_arcsec:
+                i += 1
+                resTry = turbulence_pxscale / (i + 2)
+            iMin = i
+            nTry = 10
+            resTry = np.zeros(nTry)
+            scaleTry = np.zeros(nTry)
+            fftScaleTry = np.zeros(nTry)
+            subapRealTry = np.zeros(nTry)
+            mcmxTry = np.zeros(nTry)
+            for i in range(nTry):
+                resTry[i] = turbulence_pxscale / (iMin + i + 2)
+                scaleTry[i] = round(turbulence_pxscale / resTry[i])
+                fftScaleTry[i] = (wavelengthInNm / 1e9 * dimx / (ef_size * pixel_pitch * scaleTry[i])
+                                  * RAD2ASEC)
+                subapRealTry[i] = round(subap_wanted_fov_arcsec / fftScaleTry[i] / 2.0) * 2
+                mcmxTry[i] = np.lcm(int(subap_npx), int(subapRealTry[i]))
+            FoV = subapRealTry * fftScaleTry
+            FoVerror = np.abs(FoV - subap_wanted_fov_arcsec) / subap_wanted_fov_arcsec
+            idxGood = np.where(FoVerror < 0.02)[0]
+            if len(idxGood) == 0:
+                idxGood = np.arange(nTry)
+            ratioMcm = mcmxTry[idxGood] / scaleTry[idxGood]
+            idxMin = np.argmin(ratioMcm)
+            if (idxGood[idxMin] != 0 and mcmxTry[idxGood[0]] / mcmxTry[idxGood[idxMin]]
+                    > scaleTry[idxGood[idxMin]] / scaleTry[idxGood[0]]):
+                _fov_resolution_arcsec = resTry[idxGood[idxMin]]
+            else:
+                _fov_resolution_arcsec = resTry[idxGood[0]]
+
+    scale_ovs = round(turbulence_pxscale / _fov_resolution_arcsec)  # sh.py:185
+    dTelPaddedInM = ef_size * pixel_pitch * scale_ovs
+    dSubApPaddedInM = dTelPaddedInM / dimx
+    fft_pxscale_arcsec = wavelengthInNm * 1e-9 / dSubApPaddedInM * RAD2ASEC
+    subap_real_fov_pix = round(subap_real_fov_arcsec / fft_pxscale_arcsec / 2.0) * 2.0   # :192
+    subap_real_fov_arcsec = subap_real_fov_pix * fft_pxscale_arcsec
+    turbulence_fov_pix = int(scale_ovs * np_sub)                     # :196
+
+    fov_ovs = 1
+    if turbulence_fov_pix > 2 * subap_real_fov_pix:                  # :199-215
+        fov_ovs = 1
+        if _fov_ovs_coeff != 0.0:
+            fov_ovs = _fov_ovs_coeff
+    else:
+        ratio = float(subap_real_fov_pix) / float(turbulence_fov_pix)
+        np_factor = 1 if abs(np_sub - int(np_sub)) < 1e-3 else int(np_sub)
+        if do_not_double_fov_ovs and _fov_ovs_coeff == 0.0:
+            _fov_ovs_coeff = 1.0
+            fov_ovs = np.ceil(np_factor * ratio / 2.0) * 2.0 / float(np_factor)
+        else:
+            if _fov_ovs_coeff == 0.0:
+                _fov_ovs_coeff = 2.0
+            if ratio < 2:
+                fov_ovs = np.ceil(np_factor * _fov_ovs_coeff) / float(np_factor)
+            else:
+                fov_ovs = np.ceil(np_factor * ratio * _fov_ovs_coeff) / float(np_factor)
+
+    _sensor_pxscale = subap_real_fov_arcsec / subap_npx / RAD2ASEC   # :217
+    ovs_np_sub = int(ef_size * fov_ovs * l0[2] * 0.5)                # :218
+    fft_size = ovs_np_sub * scale_ovs                                # :219
+
+    chk = (ef_size * round(fov_ovs) * l0[2]) / 2.0                   # :234
+    if abs(chk - round(chk)) > 1e-4:
+        raise ValueError(f'ERROR: interpolated input phase size {ef_size} * {round(fov_ovs)} '
+                         f'is not divisible by  {n_lenses} subapertures.')
+
+    # calc_trigger_geometry (sh.py:239-282)
+    wf1_pitch = pixel_pitch / fov_ovs if fov_ovs != 1 else pixel_pitch   # :254-259
+    fp4_pixel_pitch = wavelengthInNm / 1e9 / (wf1_pitch * fft_size)      # :268
+    fov_complete = fft_size * fp4_pixel_pitch
+    sensor_subap_fov = _sensor_pxscale * subap_npx
+    fov_cut = fov_complete - sensor_subap_fov
+    cutpixels = int(np.round(fov_cut / fp4_pixel_pitch) / 2 * 2)         # :274
+    cutsize = fft_size - cutpixels
+    fp_mask = make_mask(fft_size, diaratio=_subap_wanted_fov / fov_complete, square=squaremask)  # :282
+
+    return dict(ef_size=ef_size, pixel_pitch=pixel_pitch, wavelengthInNm=wavelengthInNm,
+                n_lenses=n_lenses, np_sub=_np_sub, ovs_np_sub=ovs_np_sub, fov_ovs=fov_ovs,
+                scale_ovs=scale_ovs, fft_size=fft_size, cutpixels=cutpixels, cutsize=cutsize,
+                subap_npx=subap_npx, ccd_side=subap_npx * n_lenses, fp_mask=fp_mask,
+                fp4_pixel_pitch=fp4_pixel_pitch, fov_complete=fov_complete,
+                fov_resolution_arcsec=_fov_resolution_arcsec, sensor_pxscale=_sensor_pxscale,
+                turbulence_pxscale=turbulence_pxscale)
+
+
+def tilt_f(p, c):
+    """SH.get_tlt_f, sh.py:469-477: half-pixel tilt exp(-2 pi i (x+y) / (2 (p+c)))."""
+    a = np.arange(-p // 2, p // 2)
+    xx, yy = np.meshgrid(a, a)
+    return np.exp(-2 * np.pi * 1j * (xx + yy) / (2 * (p + c)))
+
+
+# --------------------------------------------------------------------------- rebin / toccd
+
+def rebin2d(a, shape, sample=False):
+    """specula/lib/rebin.py:12-35."""
+    if a.shape == tuple(shape):
+        return a
+    if sample:
+        slices = [slice(0, old, float(old) / new) for old, new in zip(a.shape, shape)]
+        idx = np.mgrid[slices].astype(np.int32)
+        return a[tuple(idx)]
+    M, N = a.shape
+    m, n = shape
+    if m <= M and n <= M:
+        if (M // m != M / m) or (N // n != N / n):
+            raise ValueError('Resampling by non-integer factors is not supported')
+        return a.reshape((m, M // m, n, N // n)).mean(3).mean(1)
+    raise ValueError('Upsampling with sample=False is not supported')
+
+
+def toccd(a, newshape, set_total=None):
+    """specula/lib/toccd.py:33-63 (CPU path)."""
+    newshape = tuple(newshape)
+    if a.shape == newshape:
+        return a
+    if set_total is None:
+        set_total = a.sum()
+    mcmx = math.lcm(a.shape[0], newshape[0])
+    mcmy = math.lcm(a.shape[1], newshape[1])
+    temp = rebin2d(a, (mcmx, a.shape[1]), sample=True)
+    temp = rebin2d(temp, (newshape[0], a.shape[1]))
+    temp = rebin2d(temp, (newshape[0], mcmy), sample=True)
+    rebinned = rebin2d(temp, newshape)
+    return rebinned / rebinned.sum() * set_total
+
+
+# --------------------------------------------------------------------------- SH forward
+
+def sh_forward(A, phase_nm, geom, S0=1.0, kernels=None, return_psfimage=False):
+    """SH.trigger_code + post_trigger (sh.py:348-428) in FP64, fov_ovs == 1 path.
+
+    ``kernels``: (n_lenses**2, N, N) complex array as produced by ``process_kernels``
+    (ConvolutionKernel.kernels), or None for the NGS path.
+    Returns the CCD intensity (ccd_side x ccd_side) float64."""
+    if geom['fov_ovs'] != 1:
+        raise NotImplementedError('oracle restates the fov_ovs == 1 path only')
+    A = np.asarray(A, dtype=np.float64)
+    phase_nm = np.asarray(phase_nm, dtype=np.float64)
+    n = geom['n_lenses']
+    p = geom['ovs_np_sub']
+    N = geom['fft_size']
+    c = geom['cutpixels']
+    cs = geom['cutsize']
+    lam = geom['wavelengthInNm']
+    tltf = tilt_f(p, N - p)                                          # sh.py:280
+    fp_mask = geom['fp_mask'].astype(np.float64)
+    wf3 = np.zeros((n, N, N), dtype=np.complex128)                   # sh.py:265
+    psfimage = np.zeros((cs * n, cs * n))                            # sh.py:276
+    lo, hi = c // 2, N - (c - c // 2)                                # slice fix (see header)
+    for i in range(n):                                               # sh.py:364
+        sl = slice(i * p, (i + 1) * p)
+        phi = phase_nm[sl, :] * ((2 * np.pi) / lam)                  # electric_field.py:65
+        ef_row = np.exp(1j * phi) * A[sl, :]                         # electric_field.py:73-74
+        cube = ef_row.reshape(p, n, p).swapaxes(0, 1)                # sh.py:373
+        wf3[:, :p, :p] = cube * tltf[np.newaxis, :, :]               # sh.py:376
+        fp4 = np.fft.fft2(wf3, axes=(1, 2))                          # sh.py:378
+        psf_shifted = np.real(fp4 * np.conj(fp4))                    # sh.py:25-27,379
+        if kernels is not None:                                      # sh.py:382-394
+            kern = kernels[i * n:(i + 1) * n, :, :]
+            psf_fft = np.fft.fft2(psf_shifted) * kern
+            psf = np.fft.ifft2(psf_fft, norm='forward').real         # in-place scipy ifft2, norm='forward'
+        else:
+            psf = np.fft.fftshift(psf_shifted, axes=(1, 2))          # sh.py:396
+        psf = psf * fp_mask[np.newaxis, :, :]                        # sh.py:399
+        cut = psf[:, lo:hi, lo:hi]                                   # sh.py:405 (fixed)
+        psfimage[i * cs:(i + 1) * cs, :] = cut.swapaxes(0, 1).reshape(-1, n * cs)   # sh.py:410-413
+    ccd_side = geom['ccd_side']
+    out = toccd(psfimage, (ccd_side, ccd_side))                      # sh.py:420
+    phot = S0 * (geom['pixel_pitch'] ** 2) * np.sum(A)               # sh.py:426; electric_field.py:87-89
+    out = out * (phot / out.sum())                                   # sh.py:427
+    if return_psfimage:
+        return out, psfimage
+    return out
+
+
+# --------------------------------------------------------------------------- ShSlopec
+
+def compute_xy_weights(np_sub, exp_weight=1.0):
+    """ShSlopec.computeXYweights, sh_slopec.py:86-132 (quadcell off, weightedPixRad == 0)."""
+    x, y = np.meshgrid(np.linspace(-1, 1, np_sub), np.linspace(-1, 1, np_sub))   # :98
+    xc, yc = x, y
+    x = np.where(x > 0, np.power(x, exp_weight), -np.power(np.abs(x), exp_weight))   # :108
+    y = np.where(y > 0, np.power(y, exp_weight), -np.power(np.abs(y), exp_weight))
+    xc = np.where(x > 0, xc, -np.abs(xc))
+    yc = np.where(y > 0, yc, -np.abs(yc))
+    mask_weighted = np.ones((np_sub, np_sub))
+    return dict(x=x, y=y, xc=xc, yc=yc, mask_weighted=mask_weighted)
+
+
+def shslopec(pixels, idxs, thr_value=-1.0, exp_weight=1.0, thr_pedestal=False):
+    """ShSlopec.calc_slopes_nofor, sh_slopec.py:308-438, in FP64.
+    Returns dict(slopes=[sx..., sy...], flux (n_sub,), total, mean)."""
+    pixels = np.asarray(pixels, dtype=np.float64)
+    idxs = np.asarray(idxs).astype(np.int64)
+    n_sub, npix = idxs.shape
+    np_sub = int(math.sqrt(npix))
+    w = compute_xy_weights(np_sub, exp_weight)
+    rows = idxs // pixels.shape[1]                                    # utils.py:37-49
+    cols = idxs - rows * pixels.shape[1]
+    p = pixels[rows, cols].T.copy()                                   # :330-331 -> (np^2, n_sub)
+    flux = p.sum(axis=0)                                              # :365
+    if thr_pedestal or thr_value > 0:                                 # :377-380
+        thr = thr_value
+    else:
+        thr = 0
+    if thr_pedestal:                                                  # :382-386
+        p = np.where(p < thr, 0, p)
+    else:
+        p = p - thr
+        p = np.where(p < 0, 0, p)
+    subap_tot = np.sum(p * w['mask_weighted'].reshape(npix, 1), axis=0)   # :392
+    mean_tot = np.mean(subap_tot)
+    with np.errstate(divide='ignore'):
+        factor = 1.0 / subap_tot                                      # :394
+    factor = np.where(factor > 1.0 / (mean_tot * 1e-3), 0, factor)    # :400
+    sx = np.sum(p * w['x'].reshape(npix, 1) * factor[np.newaxis, :], axis=0)   # :403
+    sy = np.sum(p * w['y'].reshape(npix, 1) * factor[np.newaxis, :], axis=0)   # :404
+    return dict(slopes=np.concatenate([sx, sy]), flux=flux, total=flux.sum(), mean=flux.mean())
+
+
+def detect_subaps(image, n_lenses, energy_th):
+    """ShSubapCalibrator._detect_subaps, sh_subap_calibrator.py:46-92.
+    Returns (idxs (n_sub, np_sub**2) int64, display_map (n_sub,) int64)."""
+    npx = image.shape[0]
+    lens = lenslet(n_lenses)
+    spot = np.zeros((n_lenses, n_lenses))
+    xs = np.zeros((n_lenses, n_lenses))
+    ys = np.zeros((n_lenses, n_lenses))
+    np_sub = 0
+    masks = {}
+    for i in range(n_lenses):
+        for j in range(n_lenses):
+            l = lens[i][j]
+            xs[i, j] = npx / 2.0 * (1 + l[0])
+            ys[i, j] = npx / 2.0 * (1 + l[1])
+            np_sub = round(npx / 2.0 * l[2])
+            r0, r1 = int(np.round(xs[i, j] - np_sub / 2)), int(np.round(xs[i, j] + np_sub / 2))
+            c0, c1 = int(np.round(ys[i, j] - np_sub / 2)), int(np.round(ys[i, j] + np_sub / 2))
+            masks[i, j] = (r0, r1, c0, c1)
+            spot[i, j] = image[r0:r1, c0:c1].sum()
+    idxs, dmap = [], []
+    for i in range(n_lenses):
+        for j in range(n_lenses):
+            if spot[i, j] > energy_th * spot.max():                   # :72
+                r0, r1, c0, c1 = masks[i, j]
+                rr, cc = np.meshgrid(np.arange(r0, r1), np.arange(c0, c1), indexing='ij')
+                idxs.append(np.ravel_multi_index((rr.ravel(), cc.ravel()), image.shape))
+                dmap.append(j * n_lenses + i)                         # :77
+    if not idxs:
+        raise ValueError('Error: no subapertures selected')
+    return np.array(idxs, dtype=np.int64), np.array(dmap, dtype=np.int64)
+
+
+# --------------------------------------------------------------------------- LGS kernels
+
+def lgs_map_sh(nsh, diam, rl, zb, dz, profz, fwhmb, ps, ssp, overs=2, theta=(0.0, 0.0),
+               rprof_type=0, dtype=np.float32):
+    """convolution_kernel.py:14-108 (doCube=True, no pupil mask).  Computed in ``dtype``
+    (float32 by default, as the reference)."""
+    theta = np.asarray(theta, dtype=np.float64)
+    ossp = ssp * overs
+    xsh, ysh = np.meshgrid(np.linspace(-diam / 2, diam / 2, nsh, dtype=dtype),
+                           np.linspace(-diam / 2, diam / 2, nsh, dtype=dtype))
+    xfov, yfov = np.meshgrid(np.linspace(-ssp * ps / 2, ssp * ps / 2, ossp, dtype=dtype),
+                             np.linspace(-ssp * ps / 2, ssp * ps / 2, ossp, dtype=dtype))
+    sigma = (fwhmb * ASEC2RAD * zb) / (2 * np.sqrt(2 * np.log(2)))
+    one_over_sigma2 = 1.0 / sigma ** 2
+    exp_sigma = -0.5 * one_over_sigma2
+    rb = np.array([theta[0] * ASEC2RAD * zb, theta[1] * ASEC2RAD * zb, 0], dtype=dtype)
+    kv = np.array([0, 0, 1], dtype=dtype)
+    BL = zb * kv + rb - np.array(rl, dtype=dtype)
+    el = BL / BL[2]
+    rs_x, rs_y = xsh, ysh
+    rf_x = np.tile(xfov * ASEC2RAD * zb, (nsh, nsh)).reshape(ossp * nsh, ossp * nsh)
+    rf_y = np.tile(yfov * ASEC2RAD * zb, (nsh, nsh)).reshape(ossp * nsh, ossp * nsh)
+    es_x = (rf_x - np.repeat(np.repeat(rs_x, ossp, axis=0), ossp, axis=1)) / zb
+    es_y = (rf_y - np.repeat(np.repeat(rs_y, ossp, axis=0), ossp, axis=1)) / zb
+    es_z = 1.0
+    fmap = np.zeros((nsh * ossp, nsh * ossp), dtype=dtype)
+    if rprof_type == 0:
+        gnorm = 1.0 / (sigma * np.pi * np.sqrt(2.0))
+    elif rprof_type == 1:
+        gnorm = 1.0 / (np.pi / 4 * (fwhmb * ASEC2RAD * zb) ** 2)
+    else:
+        raise ValueError('Unsupported radial profile type')
+    for iz in range(len(dz)):
+        if profz[iz] > 0:
+            d2 = ((rf_x + dz[iz] * es_x - (rb[0] + dz[iz] * el[0])) ** 2 +
+                  (rf_y + dz[iz] * es_y - (rb[1] + dz[iz] * el[1])) ** 2 +
+                  (0.0 + dz[iz] * es_z - (rb[2] + dz[iz] * el[2])) ** 2)
+            if rprof_type == 0:
+                fmap += (gnorm * profz[iz]) * np.exp(d2 * exp_sigma, dtype=dtype)
+            else:
+                fmap += (gnorm * profz[iz]) * ((d2 * one_over_sigma2) <= 1.0)
+    ccd = rebin2d(fmap, (ssp * nsh, ssp * nsh), sample=True)
+    ccd = ccd.reshape(nsh, ssp, nsh, ssp).transpose((2, 0, 1, 3)).reshape(nsh * nsh, ssp, ssp)
+    return ccd
+
+
+def process_kernels(real_kernels, dimx, dimy, return_fft=True):
+    """ConvolutionKernel.process_kernels, convolution_kernel.py:258-274 (FP64 output)."""
+    if np.any(~np.isfinite(real_kernels)):
+        raise ValueError('Kernel contains non-finite values!')
+    out = np.zeros(real_kernels.shape, dtype=np.complex128 if return_fft else np.float64)
+    for i in range(dimx):
+        for j in range(dimy):
+            k = np.array(real_kernels[i * dimx + j, :, :])
+            total = np.sum(k)
+            if total > 0:
+                k = k / total
+            out[j * dimx + i] = np.fft.ifft2(k) if return_fft else k
+    return out
+
+
+def convolution_kernels_for_sh(geom, launcher_pos, spot_size, zlayer, zprofile, zfocus=90e3,
+                               theta=(0.0, 0.0), seeing=0.0, airmass=1.0, positive_shift_tt=True):
+    """ConvolutionKernel as built by SH.calc_trigger_geometry (sh.py:307-320) and
+    calculate_lgs_map (convolution_kernel.py:179-225) -> ``kernels`` (n^2, N, N) complex."""
+    n = geom['n_lenses']
+    pxscale = geom['fp4_pixel_pitch'] * RAD2ASEC
+    zf = zfocus if zfocus != -1 else np.sum(np.array(zlayer) * np.array(zprofile)) / np.sum(zprofile)
+    lay = np.array(zlayer) * airmass
+    zf = zf * airmass
+    spot = np.sqrt(seeing ** 2 + spot_size ** 2)
+    if spot <= 0:
+        raise ValueError('Spot size must be greater than zero')
+    lgs_tt = np.array([0.5, 0.5] if positive_shift_tt else [-0.5, -0.5]) * pxscale + np.asarray(theta)
+    real = lgs_map_sh(n, geom['pixel_pitch'] * geom['ef_size'], launcher_pos, zf, lay - zf,
+                      zprofile, spot, pxscale, geom['fft_size'], overs=1, theta=lgs_tt)
+    return process_kernels(real, n, n, return_fft=True)
+
+
+def gaussian_kernels_for_sh(geom, spot_size, positive_shift_tt=True):
+    """GaussianConvolutionKernel.calculate_lgs_map, gaussian_convolution_kernel.py:38-57."""
+    n = max(geom['n_lenses'], 2)
+    pxscale = geom['fp4_pixel_pitch'] * RAD2ASEC
+    lgs_tt = [x * pxscale for x in ([0.5, 0.5] if positive_shift_tt else [-0.5, -0.5])]
+    real = lgs_map_sh(n, geom['pixel_pitch'] * geom['ef_size'], 0, 90e3, [0], profz=[1.0],
+                      fwhmb=spot_size, ps=pxscale, ssp=geom['fft_size'], overs=1, theta=lgs_tt)
+    return process_kernels(real, n, n, return_fft=True)

@@ -1,0 +1,93 @@
+"""ctypes binding of libshwfs.so (declarations in include/shwfs.h).
+
+torch is imported before the library is loaded: torch ships its own HIP runtime whose
+SONAME (libamdhip64.so.7) then satisfies libshwfs.so's dependency, so torch streams and
+allocations and the library's launches share one runtime.  There is no CPU fallback: if the
+library is missing every entry point raises.
+"""
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'lib', 'libshwfs.so')
+
+# every symbol declared in include/shwfs.h
+EXPORTS = (
+    'shwfs_version', 'shwfs_last_error',
+    'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_set_kernels', 'shwfs_forward',
+    'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
+)
+
+
+class ShwfsGeom(ctypes.Structure):
+    """Mirror of ``shwfs_geom`` in include/shwfs.h."""
+    _fields_ = [
+        ('n_lenses', ctypes.c_int32), ('ef_size', ctypes.c_int32), ('np_sub', ctypes.c_int32),
+        ('fft_size', ctypes.c_int32), ('spec_size', ctypes.c_int32), ('subap_npx', ctypes.c_int32),
+        ('lgs', ctypes.c_int32), ('parseval', ctypes.c_int32), ('max_frames', ctypes.c_int32),
+        ('reserved', ctypes.c_int32),
+        ('k_rad_per_nm', ctypes.c_double), ('tilt_rad', ctypes.c_double), ('tilt_a0', ctypes.c_double),
+        ('norm_const', ctypes.c_double), ('pixel_area', ctypes.c_double),
+    ]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+_F = ctypes.c_float
+
+
+def _declare(lib):
+    sig = {
+        'shwfs_version': (ctypes.c_char_p, []),
+        'shwfs_last_error': (ctypes.c_char_p, []),
+        'shwfs_plan_create': (_I, [ctypes.POINTER(ShwfsGeom), _P, _P, _I, ctypes.POINTER(_P)]),
+        'shwfs_plan_destroy': (_I, [_P]),
+        'shwfs_set_kernels': (_I, [_P, _P, _P]),
+        'shwfs_forward': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P]),
+        'shslopec_plan_create': (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _I, ctypes.POINTER(_P)]),
+        'shslopec_plan_destroy': (_I, [_P]),
+        'shslopec_forward': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def load_library(path=LIB_PATH):
+    """Load libshwfs.so (after torch, see module docstring).  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  -- must precede the dlopen (shared HIP runtime)
+    if not os.path.exists(path):
+        raise RuntimeError(f'{path} not found: build it with `python -m specula_amd.build`')
+    lib = ctypes.CDLL(path)
+    _declare(lib)
+    _lib = lib
+    return lib
+
+
+def lib():
+    return load_library()
+
+
+def check(rc, what=''):
+    if rc != 0:
+        msg = lib().shwfs_last_error().decode(errors='replace')
+        raise RuntimeError(f'{what} failed ({rc}): {msg}')
+
+
+def stream_handle(stream=None):
+    """hipStream_t of a torch stream (default: the current stream) as a ctypes pointer."""
+    import torch
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    """Device pointer of a torch tensor."""
+    return ctypes.c_void_p(t.data_ptr())

@@ -1,0 +1,683 @@
+// shwfs.hip -- MI355X (gfx950) Shack-Hartmann wavefront-sensor hot path and its C ABI.
+//
+// Forward model of one subaperture (SH.trigger_code, processing_objects/sh.py:364-420):
+//   E      = A * exp(i*(2 pi/lambda)*phi) * tilt                        (np x np, zero padded to N)
+//   I      = |FFT_N(E)|^2 ;  NGS: psf = fftshift(I);  LGS: psf = Re IFFT_N(FFT_N(I) * K)
+//   B      = rebin(cut(mask * psf))                                       (npx x npx CCD pixels)
+// Because E has only np x np non-zero samples, |FFT_N(E)|^2 is a trigonometric polynomial whose
+// autocorrelation support is (2np-1)^2 lags.  Its power spectrum S on any grid M >= 2np-1
+// determines it exactly, and every step after |.|^2 (shift, mask, cut, rebin, and for LGS the
+// kernel correlation) is linear.  So the kernels compute
+//   NGS:  B = P . S . Q^T            (P, Q: npx x M real binning operators built on the host)
+//   LGS:  B = Re(P . (N^2 C(-m) K(m)) . Q^T)   (C: autocorrelation on the lag grid)
+// where S = |FFT_M(E)|^2 comes from register-resident pruned FFTs of size M = 16 or 32 instead
+// of the N = 64/128 transforms of the reference.  Normalisation to S0*masked_area
+// (sh.py:422-428) is exact: either analytic by Parseval (no cut, all-ones mask, so the image sum
+// is norm_const * sum(A^2)) or by an FP64 reduction of the image followed by a rescale.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -shared -fPIC (see specula_amd/build.py).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/shwfs.h"
+#include "fft_reg.h"
+
+using namespace shwfs;
+
+// ============================================================================ error plumbing
+namespace {
+thread_local std::string g_last_error;
+
+int set_error(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+}  // namespace
+
+#define SHWFS_HIP_CHECK(expr)                                                              \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(SHWFS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define SHWFS_LAUNCH_CHECK()                                                               \
+    do {                                                                                   \
+        hipError_t e_ = hipGetLastError();                                                 \
+        if (e_ != hipSuccess)                                                              \
+            return set_error(SHWFS_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// ============================================================================ device helpers
+namespace {
+
+constexpr int kWave = 64;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Block-wide FP64 sum; result valid in every thread.  `red` needs blockDim/64 doubles.
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += red[w];   // fixed order: deterministic
+    return s;
+}
+
+// Phase of one pupil sample: (2 pi / lambda) * phi - (pi / N) * (ax + ay), reduced to
+// [-pi, pi] in FP64 before the FP32 sincos (the unreduced phase reaches tens of radians).
+__device__ __forceinline__ float2 field_sample(float a, float ph, double k, double tilt, double ax_ay) {
+    double psi = (double)ph * k - tilt * ax_ay;
+    psi -= kTwoPi * rint(psi * (1.0 / kTwoPi));
+    float s, c;
+    sincosf((float)psi, &s, &c);
+    return make_float2(a * c, a * s);
+}
+
+// ---------------------------------------------------------------- amplitude statistics
+// Per frame: sum(A) (masked area, electric_field.py:87-89) and sum(A^2), FP64, deterministic.
+// Last-block-done pattern: every block writes a partial, the last one to arrive reduces the
+// partials in fixed order and resets the arrival counter (so graph replays need no memset).
+__global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, int64_t n_px,
+                                 double* __restrict__ part, unsigned* __restrict__ counter,
+                                 double* __restrict__ out) {
+    __shared__ double red[16];
+    __shared__ bool last;
+    const int f = blockIdx.y;
+    const float* a = A + f * a_stride;
+    double s1 = 0.0, s2 = 0.0;
+    const int64_t n4 = n_px / 4;
+    const float4* a4 = reinterpret_cast<const float4*>(a);
+    const bool aligned = (reinterpret_cast<uintptr_t>(a) % 16) == 0;
+    if (aligned) {
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n4;
+             q += (int64_t)gridDim.x * blockDim.x) {
+            float4 v = a4[q];
+            s1 += (double)v.x + (double)v.y + (double)v.z + (double)v.w;
+            s2 += (double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z + (double)v.w * v.w;
+        }
+        for (int64_t q = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_px;
+             q += (int64_t)gridDim.x * blockDim.x) {
+            double v = a[q];
+            s1 += v; s2 += v * v;
+        }
+    } else {
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_px;
+             q += (int64_t)gridDim.x * blockDim.x) {
+            double v = a[q];
+            s1 += v; s2 += v * v;
+        }
+    }
+    s1 = block_sum(s1, red);
+    s2 = block_sum(s2, red);
+    double* p = part + ((int64_t)f * gridDim.x + blockIdx.x) * 2;
+    if (threadIdx.x == 0) {
+        p[0] = s1;
+        p[1] = s2;
+        __threadfence();
+        unsigned prev = atomicAdd(&counter[f], 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double t1 = 0.0, t2 = 0.0;
+        const double* pf = part + (int64_t)f * gridDim.x * 2;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+            t1 += __builtin_nontemporal_load(&pf[2 * b]);
+            t2 += __builtin_nontemporal_load(&pf[2 * b + 1]);
+        }
+        // per-thread strided partials, then fixed-order block reduction
+        t1 = block_sum(t1, red);
+        t2 = block_sum(t2, red);
+        if (threadIdx.x == 0) {
+            out[2 * f] = t1;
+            out[2 * f + 1] = t2;
+            counter[f] = 0u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- NGS forward
+struct NgsArgs {
+    const float* A;
+    int64_t a_stride;
+    const float* phase;
+    int64_t ph_stride;
+    float* ccd;
+    int64_t ccd_stride;
+    const float* roww;   // npx_pad x M   (P)
+    const float* colw;   // npx_pad x M   (Q)
+    const double* amp;   // [frame][2]: sum(A), sum(A^2)     (frame index f * amp_fstep)
+    int amp_fstep;
+    double* bpart;       // [frame][n_tiles]: sum of unscaled B   (non-Parseval mode)
+    int n_lenses, ef_size, npx, npx_pad, n_sub, n_tiles, parseval;
+    double k, tilt, a0, S0, pixel_area, norm_const;
+};
+
+// Work split of one block = TS subapertures, TS*NP threads:
+//   row pass    : thread (s, y)  -> pruned FFT_M of pupil row y of subaperture s
+//   column pass : thread (s, kx) -> pruned FFT_M of column kx, |.|^2, U = P.S  (in registers)
+//   bin pass    : thread (s, Y)  -> CCD row Y of subaperture s: B[Y,:] = U[Y,:].Q^T
+// LDS holds G1 (complex, TS x M x NP) and later U (real, TS x npx_pad x M) in the same buffer.
+template <int NP, int M, int TS>
+__global__ void __launch_bounds__(TS * NP)
+sh_ngs_forward_kernel(NgsArgs g) {
+    extern __shared__ float4 lds_raw[];
+    float2* g1 = reinterpret_cast<float2*>(lds_raw);
+    float* us = reinterpret_cast<float*>(lds_raw);
+    __shared__ double red[8];
+
+    constexpr int G1S = M * NP + 1;            // complex stride per subaperture (odd: spreads banks)
+    const int US = g.npx_pad * M + 4;          // float stride per subaperture in U
+    constexpr int NT = TS * NP;
+    constexpr int CPT = (TS * M + NT - 1) / NT;   // columns per thread
+
+    const int t = threadIdx.x;
+    const int f = blockIdx.y;
+    const int tile = blockIdx.x;
+    const int n = g.n_lenses;
+
+    // ---- row pass
+    {
+        const int s = t % TS, y = t / TS;
+        const int sub = tile * TS + s;
+        float2 e[NP];
+        if (sub < g.n_sub) {
+            const int i = sub / n, j = sub - (sub / n) * n;
+            const int64_t off = (int64_t)(i * NP + y) * g.ef_size + (int64_t)j * NP;
+            const float* ar = g.A + f * g.a_stride + off;
+            const float* pr = g.phase + f * g.ph_stride + off;
+            const double ay = g.a0 + y;
+#pragma unroll
+            for (int x = 0; x < NP; ++x)
+                e[x] = field_sample(ar[x], pr[x], g.k, g.tilt, g.a0 + x + ay);
+        } else {
+#pragma unroll
+            for (int x = 0; x < NP; ++x) e[x] = make_float2(0.f, 0.f);
+        }
+        float2 r[M];
+        fft_pruned<M, NP, false>(e, r);
+#pragma unroll
+        for (int kx = 0; kx < M; ++kx) g1[s * G1S + kx * NP + y] = r[kx];
+    }
+    __syncthreads();
+
+    // ---- column pass: S = |FFT_M over y|^2, kept in registers across the LDS reuse
+    float S[CPT][M];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = t + q * NT;
+        if (c < TS * M) {
+            const int s = c % TS, kx = c / TS;
+            float2 col[NP];
+#pragma unroll
+            for (int y = 0; y < NP; ++y) col[y] = g1[s * G1S + kx * NP + y];
+            float2 G[M];
+            fft_pruned<M, NP, false>(col, G);
+#pragma unroll
+            for (int ky = 0; ky < M; ++ky) S[q][ky] = G[ky].x * G[ky].x + G[ky].y * G[ky].y;
+        }
+    }
+    __syncthreads();
+
+    // U[s][Y][kx] = sum_ky P[Y][ky] S[ky][kx]   (P row index uniform across the block)
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int c = t + q * NT;
+        if (c < TS * M) {
+            const int s = c % TS, kx = c / TS;
+            for (int Y0 = 0; Y0 < g.npx_pad; Y0 += 4) {
+                float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+                const float* p0 = g.roww + Y0 * M;
+#pragma unroll
+                for (int ky = 0; ky < M; ++ky) {
+                    acc0 = fmaf(p0[ky], S[q][ky], acc0);
+                    acc1 = fmaf(p0[M + ky], S[q][ky], acc1);
+                    acc2 = fmaf(p0[2 * M + ky], S[q][ky], acc2);
+                    acc3 = fmaf(p0[3 * M + ky], S[q][ky], acc3);
+                }
+                float* u = us + s * US + Y0 * M + kx;
+                u[0] = acc0;
+                u[M] = acc1;
+                u[2 * M] = acc2;
+                u[3 * M] = acc3;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- bin pass: B[Y][X] = sum_kx U[Y][kx] Q[X][kx]; store CCD row segments
+    double scale = 1.0;
+    if (g.parseval) {
+        const double* am = g.amp + (int64_t)f * g.amp_fstep;
+        scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
+    }
+    const int W = n * g.npx;
+    double bsum = 0.0;
+    for (int c = t; c < TS * g.npx; c += NT) {
+        const int s = c % TS, Y = c / TS;
+        const int sub = tile * TS + s;
+        if (sub >= g.n_sub) continue;
+        const int i = sub / n, j = sub - (sub / n) * n;
+        float u[M];
+#pragma unroll
+        for (int kx = 0; kx < M; ++kx) u[kx] = us[s * US + Y * M + kx];
+        float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx;
+        for (int X0 = 0; X0 < g.npx; X0 += 4) {
+            float b[4] = {0.f, 0.f, 0.f, 0.f};
+            const float* q0 = g.colw + X0 * M;
+#pragma unroll
+            for (int kx = 0; kx < M; ++kx) {
+                b[0] = fmaf(q0[kx], u[kx], b[0]);
+                b[1] = fmaf(q0[M + kx], u[kx], b[1]);
+                b[2] = fmaf(q0[2 * M + kx], u[kx], b[2]);
+                b[3] = fmaf(q0[3 * M + kx], u[kx], b[3]);
+            }
+            const int nx = min(4, g.npx - X0);
+            for (int q = 0; q < nx; ++q) {
+                bsum += (double)b[q];
+                dst[X0 + q] = (float)((double)b[q] * scale);
+            }
+        }
+    }
+    if (!g.parseval) {
+        bsum = block_sum(bsum, red);
+        if (t == 0) g.bpart[(int64_t)f * g.n_tiles + tile] = bsum;
+    }
+}
+
+// Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
+__global__ void sh_rescale_kernel(float* __restrict__ ccd, int64_t ccd_stride, int64_t n_px,
+                                  const double* __restrict__ bpart, int n_tiles,
+                                  const double* __restrict__ amp, int amp_fstep,
+                                  double S0, double pixel_area) {
+    __shared__ double red[16];
+    const int f = blockIdx.y;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < n_tiles; b += blockDim.x) t += bpart[(int64_t)f * n_tiles + b];
+    t = block_sum(t, red);
+    const double scale = S0 * pixel_area * amp[(int64_t)f * amp_fstep] / t;
+    float* c = ccd + f * ccd_stride;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_px;
+         q += (int64_t)gridDim.x * blockDim.x)
+        c[q] = (float)((double)c[q] * scale);
+}
+
+// ---------------------------------------------------------------- ShSlopec CoG
+// One wavefront per subaperture: FP64 moments (flux, thresholded total, x and y numerators).
+__global__ void cog_moments_kernel(const float* __restrict__ pix, int64_t pix_stride,
+                                   const int32_t* __restrict__ origin, const int32_t* __restrict__ idx,
+                                   int n_sub, int np_sub, int frame_w,
+                                   const double* __restrict__ xw, const double* __restrict__ yw,
+                                   const double* __restrict__ mw, float thr, int thr_mode,
+                                   double* __restrict__ mom) {
+    const int lane = threadIdx.x % kWave;
+    const int sub = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int f = blockIdx.y;
+    if (sub >= n_sub) return;
+    const float* p = pix + f * pix_stride;
+    const int npp = np_sub * np_sub;
+    double flux = 0.0, tot = 0.0, sx = 0.0, sy = 0.0;
+    for (int q = lane; q < npp; q += kWave) {
+        int64_t addr;
+        if (origin) {
+            const int r = q / np_sub, c = q - (q / np_sub) * np_sub;
+            addr = (int64_t)origin[sub] + (int64_t)r * frame_w + c;
+        } else {
+            addr = idx[(int64_t)sub * npp + q];
+        }
+        const float v = p[addr];
+        flux += (double)v;
+        float w;
+        if (thr_mode == 1) {              // pedestal: zero below threshold (sh_slopec.py:382-383)
+            w = v < thr ? 0.f : v;
+        } else {                          // subtract then clamp at zero (sh_slopec.py:385-386)
+            w = v - thr;
+            w = w < 0.f ? 0.f : w;
+        }
+        const double wd = (double)w;
+        tot += wd * mw[q];
+        sx += wd * xw[q];
+        sy += wd * yw[q];
+    }
+    flux = wave_sum(flux);
+    tot = wave_sum(tot);
+    sx = wave_sum(sx);
+    sy = wave_sum(sy);
+    if (lane == 0) {
+        double* m = mom + ((int64_t)f * n_sub + sub) * 4;
+        m[0] = flux; m[1] = tot; m[2] = sx; m[3] = sy;
+    }
+}
+
+// One block per frame: mean of the thresholded totals, dim-subaperture cut-off
+// (factor = 0 where 1/tot > 1/(1e-3 mean), sh_slopec.py:393-400), slopes and flux outputs.
+__global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
+                                    float* __restrict__ slopes, int64_t slopes_stride,
+                                    double* __restrict__ flux, double* __restrict__ totals) {
+    __shared__ double red[16];
+    const int f = blockIdx.x;
+    const double* m = mom + (int64_t)f * n_sub * 4;
+    double st = 0.0, sf = 0.0;
+    for (int s = threadIdx.x; s < n_sub; s += blockDim.x) {
+        sf += m[4 * s];
+        st += m[4 * s + 1];
+    }
+    sf = block_sum(sf, red);
+    st = block_sum(st, red);
+    const double mean_tot = st / n_sub;
+    const double limit = 1.0 / (mean_tot * 1e-3);
+    float* sl = slopes + f * slopes_stride;
+    for (int s = threadIdx.x; s < n_sub; s += blockDim.x) {
+        double factor = 1.0 / m[4 * s + 1];
+        if (factor > limit) factor = 0.0;
+        sl[s] = (float)(m[4 * s + 2] * factor);
+        sl[n_sub + s] = (float)(m[4 * s + 3] * factor);
+        flux[(int64_t)f * n_sub + s] = m[4 * s];
+    }
+    if (threadIdx.x == 0) {
+        totals[2 * f] = sf;
+        totals[2 * f + 1] = sf / n_sub;
+    }
+}
+
+}  // namespace
+
+// ============================================================================ plans
+struct shwfs_plan {
+    shwfs_geom g{};
+    int device = 0;
+    int n_sub = 0;
+    int npx_pad = 0;
+    int ts = 0;
+    int n_tiles = 0;
+    int amp_blocks = 0;
+    float* d_roww = nullptr;
+    float* d_colw = nullptr;
+    float* d_kspec = nullptr;
+    double* d_amp = nullptr;       // [max_frames][2]
+    double* d_amp_part = nullptr;  // [max_frames][amp_blocks][2]
+    unsigned* d_counter = nullptr; // [max_frames]
+    double* d_bpart = nullptr;     // [max_frames][n_tiles]
+};
+
+struct shslopec_plan {
+    int device = 0;
+    int n_sub = 0, np_sub = 0, frame_h = 0, frame_w = 0, max_frames = 0;
+    int32_t* d_origin = nullptr;
+    int32_t* d_idx = nullptr;
+    double* d_w = nullptr;     // xw | yw | mw
+    double* d_mom = nullptr;   // [max_frames][n_sub][4]
+};
+
+namespace {
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+template <int NP, int M>
+int launch_ngs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
+    constexpr int TS = 256 / NP;
+    const int g1_bytes = TS * (M * NP + 1) * (int)sizeof(float2);
+    const int u_bytes = TS * (p->npx_pad * M + 4) * (int)sizeof(float);
+    const int lds = (g1_bytes > u_bytes ? g1_bytes : u_bytes);
+    if (lds > 150 * 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "LDS budget exceeded for this geometry");
+    dim3 grid(p->n_tiles, n_frames);
+    hipLaunchKernelGGL((sh_ngs_forward_kernel<NP, M, TS>), grid, dim3(TS * NP), lds, st, a);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+int ngs_tile_size(int np) { return 256 / np; }
+
+using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
+
+NgsLauncher pick_ngs(int np, int M) {
+    switch (np) {
+        case 4: return M == 8 ? launch_ngs<4, 8> : nullptr;
+        case 5: return M == 16 ? launch_ngs<5, 16> : nullptr;
+        case 6: return M == 16 ? launch_ngs<6, 16> : nullptr;
+        case 8: return M == 16 ? launch_ngs<8, 16> : nullptr;
+        case 10: return M == 32 ? launch_ngs<10, 32> : nullptr;
+        case 12: return M == 32 ? launch_ngs<12, 32> : nullptr;
+        case 16: return M == 32 ? launch_ngs<16, 32> : nullptr;
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* shwfs_version(void) { return "shwfs-gfx950 0.1"; }
+
+const char* shwfs_last_error(void) { return g_last_error.c_str(); }
+
+int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double* col_w,
+                      int device, shwfs_plan** out) {
+    if (!geom || !row_w || !col_w || !out) return set_error(SHWFS_ERR_ARG, "null argument");
+    const shwfs_geom& g = *geom;
+    if (g.n_lenses <= 0 || g.np_sub <= 0 || g.subap_npx <= 0 || g.max_frames <= 0)
+        return set_error(SHWFS_ERR_ARG, "non-positive geometry field");
+    if (g.ef_size != g.n_lenses * g.np_sub)
+        return set_error(SHWFS_ERR_ARG, "ef_size must equal n_lenses * np_sub");
+    if (g.spec_size < 2 * g.np_sub - 1)
+        return set_error(SHWFS_ERR_ARG, "spec_size must be >= 2*np_sub-1");
+    if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS kernel path not available in this build");
+    if (!pick_ngs(g.np_sub, g.spec_size))
+        return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
+    DeviceGuard dg(device);
+    auto* p = new shwfs_plan();
+    p->g = g;
+    p->device = device;
+    p->n_sub = g.n_lenses * g.n_lenses;
+    p->npx_pad = (g.subap_npx + 3) / 4 * 4;
+    p->ts = ngs_tile_size(g.np_sub);
+    p->n_tiles = (p->n_sub + p->ts - 1) / p->ts;
+    p->amp_blocks = 64;
+    const int M = g.spec_size;
+    std::vector<float> rw((size_t)p->npx_pad * M, 0.f), cw((size_t)p->npx_pad * M, 0.f);
+    for (int Y = 0; Y < g.subap_npx; ++Y)
+        for (int k = 0; k < M; ++k) {
+            rw[(size_t)Y * M + k] = (float)row_w[(size_t)Y * M + k];
+            cw[(size_t)Y * M + k] = (float)col_w[(size_t)Y * M + k];
+        }
+    auto cleanup = [&](int code) { shwfs_plan_destroy(p); return code; };
+    hipError_t e = hipSuccess;
+#define PLAN_ALLOC(ptr, bytes)                                                                  \
+    e = hipMalloc((void**)&(ptr), (bytes));                                                     \
+    if (e != hipSuccess) return cleanup(set_error(SHWFS_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)));
+    PLAN_ALLOC(p->d_roww, rw.size() * sizeof(float));
+    PLAN_ALLOC(p->d_colw, cw.size() * sizeof(float));
+    PLAN_ALLOC(p->d_amp, (size_t)g.max_frames * 2 * sizeof(double));
+    PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
+    PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
+    PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_tiles * sizeof(double));
+#undef PLAN_ALLOC
+    if ((e = hipMemcpy(p->d_roww, rw.data(), rw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(p->d_colw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemset(p->d_counter, 0, (size_t)g.max_frames * sizeof(unsigned))) != hipSuccess)
+        return cleanup(set_error(SHWFS_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e)));
+    *out = p;
+    return SHWFS_OK;
+}
+
+int shwfs_plan_destroy(shwfs_plan* p) {
+    if (!p) return SHWFS_OK;
+    DeviceGuard dg(p->device);
+    (void)hipFree(p->d_roww);
+    (void)hipFree(p->d_colw);
+    (void)hipFree(p->d_kspec);
+    (void)hipFree(p->d_amp);
+    (void)hipFree(p->d_amp_part);
+    (void)hipFree(p->d_counter);
+    (void)hipFree(p->d_bpart);
+    delete p;
+    return SHWFS_OK;
+}
+
+int shwfs_set_kernels(shwfs_plan* p, const float* kspec, void* stream) {
+    (void)p; (void)kspec; (void)stream;
+    return set_error(SHWFS_ERR_UNSUPPORTED, "LGS kernel path not available in this build");
+}
+
+int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
+                  const float* phase, int64_t ph_stride, double S0, float* ccd,
+                  int64_t ccd_stride, void* stream) {
+    if (!p || !A || !phase || !ccd) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_frames <= 0 || n_frames > p->g.max_frames)
+        return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
+    const shwfs_geom& g = p->g;
+    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
+    const int64_t ccd_side = (int64_t)g.n_lenses * g.subap_npx;
+    if (a_stride != 0 && a_stride < ef_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
+    if (n_frames > 1 && ph_stride < ef_px) return set_error(SHWFS_ERR_ARG, "phase frame stride too small");
+    if (n_frames > 1 && ccd_stride < ccd_side * ccd_side)
+        return set_error(SHWFS_ERR_ARG, "ccd frame stride too small");
+    DeviceGuard dg(p->device);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+
+    // amplitude statistics: one pass per distinct amplitude frame
+    const int amp_frames = (a_stride == 0) ? 1 : n_frames;
+    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
+                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp);
+    SHWFS_LAUNCH_CHECK();
+
+    NgsArgs a{};
+    a.A = A; a.a_stride = a_stride;
+    a.phase = phase; a.ph_stride = ph_stride;
+    a.ccd = ccd; a.ccd_stride = ccd_stride;
+    a.roww = p->d_roww; a.colw = p->d_colw;
+    a.amp = p->d_amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
+    a.bpart = p->d_bpart;
+    a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
+    a.n_sub = p->n_sub; a.n_tiles = p->n_tiles; a.parseval = g.parseval;
+    a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0;
+    a.S0 = S0; a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
+    NgsLauncher L = pick_ngs(g.np_sub, g.spec_size);
+    int rc = L(p, a, n_frames, st);
+    if (rc) return rc;
+    if (!g.parseval) {
+        hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, n_frames), dim3(256), 0, st, ccd, ccd_stride,
+                           ccd_side * ccd_side, p->d_bpart, p->n_tiles, p->d_amp,
+                           (a_stride == 0) ? 0 : 2, S0, g.pixel_area);
+        SHWFS_LAUNCH_CHECK();
+    }
+    return SHWFS_OK;
+}
+
+int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
+                         const int32_t* block_origin, const int32_t* idx,
+                         const double* xw, const double* yw, const double* mw,
+                         int max_frames, int device, shslopec_plan** out) {
+    if (!out || !xw || !yw || !mw) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_sub <= 0 || np_sub <= 0 || max_frames <= 0 || frame_h <= 0 || frame_w <= 0)
+        return set_error(SHWFS_ERR_ARG, "non-positive size");
+    if (!block_origin && !idx) return set_error(SHWFS_ERR_ARG, "need block_origin or idx");
+    const int64_t npp = (int64_t)np_sub * np_sub;
+    const int64_t frame_px = (int64_t)frame_h * frame_w;
+    // host-side bounds check of every pixel address the kernel will read
+    if (block_origin) {
+        for (int s = 0; s < n_sub; ++s) {
+            const int64_t o = block_origin[s];
+            const int64_t r0 = o / frame_w, c0 = o % frame_w;
+            if (o < 0 || r0 + np_sub > frame_h || c0 + np_sub > frame_w)
+                return set_error(SHWFS_ERR_ARG, "subaperture block outside the pixel frame");
+        }
+    } else {
+        for (int64_t q = 0; q < (int64_t)n_sub * npp; ++q)
+            if (idx[q] < 0 || idx[q] >= frame_px) return set_error(SHWFS_ERR_ARG, "pixel index out of range");
+    }
+    DeviceGuard dg(device);
+    auto* p = new shslopec_plan();
+    p->device = device; p->n_sub = n_sub; p->np_sub = np_sub;
+    p->frame_h = frame_h; p->frame_w = frame_w; p->max_frames = max_frames;
+    auto cleanup = [&](int code) { shslopec_plan_destroy(p); return code; };
+    hipError_t e;
+    std::vector<double> w(3 * npp);
+    memcpy(w.data(), xw, npp * sizeof(double));
+    memcpy(w.data() + npp, yw, npp * sizeof(double));
+    memcpy(w.data() + 2 * npp, mw, npp * sizeof(double));
+    if ((e = hipMalloc((void**)&p->d_w, w.size() * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void**)&p->d_mom, (size_t)max_frames * n_sub * 4 * sizeof(double))) != hipSuccess ||
+        (e = hipMemcpy(p->d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
+    if (block_origin) {
+        if ((e = hipMalloc((void**)&p->d_origin, n_sub * sizeof(int32_t))) != hipSuccess ||
+            (e = hipMemcpy(p->d_origin, block_origin, n_sub * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
+    } else {
+        if ((e = hipMalloc((void**)&p->d_idx, n_sub * npp * sizeof(int32_t))) != hipSuccess ||
+            (e = hipMemcpy(p->d_idx, idx, n_sub * npp * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
+            return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
+    }
+    *out = p;
+    return SHWFS_OK;
+}
+
+int shslopec_plan_destroy(shslopec_plan* p) {
+    if (!p) return SHWFS_OK;
+    DeviceGuard dg(p->device);
+    (void)hipFree(p->d_origin);
+    (void)hipFree(p->d_idx);
+    (void)hipFree(p->d_w);
+    (void)hipFree(p->d_mom);
+    delete p;
+    return SHWFS_OK;
+}
+
+int shslopec_forward(shslopec_plan* p, int n_frames, const float* pixels, int64_t pix_stride,
+                     float thr, int thr_mode, float* slopes, int64_t slopes_stride,
+                     double* flux, double* totals, void* stream) {
+    if (!p || !pixels || !slopes || !flux || !totals) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_frames <= 0 || n_frames > p->max_frames)
+        return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
+    if (thr_mode != 0 && thr_mode != 1) return set_error(SHWFS_ERR_ARG, "thr_mode must be 0 or 1");
+    if (n_frames > 1 && pix_stride < (int64_t)p->frame_h * p->frame_w)
+        return set_error(SHWFS_ERR_ARG, "pixel frame stride too small");
+    if (n_frames > 1 && slopes_stride < 2 * (int64_t)p->n_sub)
+        return set_error(SHWFS_ERR_ARG, "slopes frame stride too small");
+    DeviceGuard dg(p->device);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int npp = p->np_sub * p->np_sub;
+    const double* xw = p->d_w;
+    const double* yw = p->d_w + npp;
+    const double* mw = p->d_w + 2 * npp;
+    constexpr int kSubPerBlock = 4;
+    dim3 grid((p->n_sub + kSubPerBlock - 1) / kSubPerBlock, n_frames);
+    hipLaunchKernelGGL(cog_moments_kernel, grid, dim3(kSubPerBlock * kWave), 0, st, pixels, pix_stride,
+                       p->d_origin, p->d_idx, p->n_sub, p->np_sub, p->frame_w, xw, yw, mw, thr,
+                       thr_mode, p->d_mom);
+    SHWFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cog_finalize_kernel, dim3(n_frames), dim3(1024), 0, st, p->d_mom, p->n_sub,
+                       slopes, slopes_stride, flux, totals);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+}  // extern "C"

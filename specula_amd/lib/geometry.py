@@ -1,0 +1,147 @@
+"""SH geometry derivation on the host (FP64 scalars).
+
+Restates SH.set_in_ef (specula/processing_objects/sh.py:112-237) and the geometry part of
+SH.calc_trigger_geometry (sh.py:239-282): subaperture sampling, the FoV-resolution search,
+FFT oversampling, FFT size, focal-plane cut and FoV mask.  Same keys give the same FFT size,
+cut and mask as the reference.
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from specula_amd import RAD2ASEC
+from specula_amd.lib.grids import make_mask, make_xy
+
+
+@dataclass
+class SHGeometry:
+    ef_size: int
+    pixel_pitch: float
+    wavelengthInNm: float
+    n_lenses: int
+    np_sub: int              # SH._np_sub
+    ovs_np_sub: int          # SH._ovs_np_sub: phase px per subaperture fed to the FFT
+    fov_ovs: float
+    scale_ovs: int
+    fft_size: int            # N
+    cutpixels: int
+    cutsize: int
+    subap_npx: int
+    fp4_pixel_pitch: float   # [rad]
+    fov_complete: float      # [rad]
+    fov_resolution_arcsec: float
+    sensor_pxscale: float    # [rad] (rewritten by set_in_ef, sh.py:217)
+    turbulence_pxscale: float
+    fp_mask: np.ndarray = field(repr=False)
+
+    @property
+    def ccd_side(self):
+        return self.subap_npx * self.n_lenses
+
+    @property
+    def needs_interpolation(self):
+        return self.fov_ovs != 1
+
+
+def _lens_size(n_lenses):
+    # Lenslet(n).get(0, 0)[2] == 2/n for every lens (lenslet.py:21-27)
+    return 2.0 / n_lenses
+
+
+def _search_fov_resolution(turb_px, sensor_px_arcsec, wanted_fov_arcsec, subap_npx, n_lenses,
+                           ef_size, pixel_pitch, wavelengthInNm):
+    """Automatic FoV-resolution choice, sh.py:142-179."""
+    i = 0
+    while turb_px / (i + 2) >= sensor_px_arcsec:
+        i += 1
+    i_min = i
+    n_try = 10
+    res = np.array([turb_px / (i_min + k + 2) for k in range(n_try)])
+    scale = np.array([float(round(turb_px / r)) for r in res])
+    fft_scale = wavelengthInNm / 1e9 * n_lenses / (ef_size * pixel_pitch * scale) * RAD2ASEC
+    subap_real = np.array([round(wanted_fov_arcsec / s / 2.0) * 2.0 for s in fft_scale])
+    mcm = np.array([float(np.lcm(int(subap_npx), int(sr))) for sr in subap_real])
+    fov_err = np.abs(subap_real * fft_scale - wanted_fov_arcsec) / wanted_fov_arcsec
+    good = np.where(fov_err < 0.02)[0]
+    if len(good) == 0:
+        good = np.arange(n_try)
+    best = int(np.argmin(mcm[good] / scale[good]))
+    if good[best] != 0 and mcm[good[0]] / mcm[good[best]] > scale[good[best]] / scale[good[0]]:
+        return res[good[best]]
+    return res[good[0]]
+
+
+def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_pxscale,
+                subap_on_diameter, subap_npx, FoVres30mas=False, squaremask=True, fov_ovs_coeff=0.0,
+                do_not_double_fov_ovs=False, set_fov_res_to_turbpxsc=False,
+                fov_resolution_arcsec=0.0):
+    """Geometry for an ElectricField of side ``ef_size`` (angles in arcsec, as the YAML keys).
+    ``fov_resolution_arcsec`` != 0 plays the role of a pre-set ``SH._fov_resolution_arcsec``."""
+    n = int(subap_on_diameter)
+    lsize = _lens_size(n)
+    wanted_fov_rad = subap_wanted_fov / RAD2ASEC
+    sensor_px_rad = sensor_pxscale / RAD2ASEC
+    fov_res = fov_resolution_arcsec if fov_resolution_arcsec else (0.03 if FoVres30mas else 0.0)
+
+    np_sub_int = max(1, round(ef_size * lsize / 2.0))
+    if np_sub_int * n > ef_size:
+        np_sub_int -= 1
+    np_sub = ef_size * lsize / 2.0
+    sensor_px_arcsec = sensor_px_rad * RAD2ASEC
+    turb_px = wavelengthInNm * 1e-9 / (np_sub * pixel_pitch) * RAD2ASEC
+    wanted_fov_arcsec = wanted_fov_rad * RAD2ASEC
+    real_fov_arcsec = sensor_px_rad * subap_npx * RAD2ASEC
+
+    if fov_res == 0:
+        if set_fov_res_to_turbpxsc:
+            if turb_px >= sensor_px_arcsec:
+                raise ValueError('set_fov_res_to_turbpxsc property should be set to one only if '
+                                 'turb. pix. sc. is < sensor pix. sc.')
+            fov_res = turb_px
+        elif turb_px < sensor_px_arcsec and sensor_px_arcsec / 2.0 > 0.5:
+            fov_res = turb_px * 0.5
+        else:
+            fov_res = _search_fov_resolution(turb_px, sensor_px_arcsec, wanted_fov_arcsec, subap_npx,
+                                             n, ef_size, pixel_pitch, wavelengthInNm)
+
+    scale_ovs = round(turb_px / fov_res)
+    fft_px_arcsec = wavelengthInNm * 1e-9 / (ef_size * pixel_pitch * scale_ovs / n) * RAD2ASEC
+    real_fov_pix = round(real_fov_arcsec / fft_px_arcsec / 2.0) * 2.0
+    real_fov_arcsec = real_fov_pix * fft_px_arcsec
+    turb_fov_pix = int(scale_ovs * np_sub)
+
+    coeff = fov_ovs_coeff
+    if turb_fov_pix > 2 * real_fov_pix:
+        fov_ovs = coeff if coeff != 0.0 else 1
+    else:
+        ratio = float(real_fov_pix) / float(turb_fov_pix)
+        np_factor = 1 if abs(np_sub - int(np_sub)) < 1e-3 else int(np_sub)
+        if do_not_double_fov_ovs and coeff == 0.0:
+            fov_ovs = math.ceil(np_factor * ratio / 2.0) * 2.0 / float(np_factor)
+        else:
+            if coeff == 0.0:
+                coeff = 2.0
+            base = coeff if ratio < 2 else ratio * coeff
+            fov_ovs = math.ceil(np_factor * base) / float(np_factor)
+
+    sensor_px_rad = real_fov_arcsec / subap_npx / RAD2ASEC
+    ovs_np_sub = int(ef_size * fov_ovs * lsize * 0.5)
+    fft_size = ovs_np_sub * scale_ovs
+
+    chk = (ef_size * round(fov_ovs) * lsize) / 2.0
+    if abs(chk - round(chk)) > 1e-4:
+        raise ValueError(f'ERROR: interpolated input phase size {ef_size} * {round(fov_ovs)} is not '
+                         f'divisible by  {n} subapertures.')
+
+    wf1_pitch = pixel_pitch / fov_ovs if fov_ovs != 1 else pixel_pitch
+    fp4 = wavelengthInNm / 1e9 / (wf1_pitch * fft_size)
+    fov_complete = fft_size * fp4
+    cut = int(np.round((fov_complete - sensor_px_rad * subap_npx) / fp4) / 2 * 2)
+    mask = make_mask(fft_size, diaratio=wanted_fov_rad / fov_complete, square=squaremask)
+    return SHGeometry(ef_size=ef_size, pixel_pitch=pixel_pitch, wavelengthInNm=wavelengthInNm,
+                      n_lenses=n, np_sub=np_sub_int, ovs_np_sub=ovs_np_sub, fov_ovs=fov_ovs,
+                      scale_ovs=scale_ovs, fft_size=fft_size, cutpixels=cut, cutsize=fft_size - cut,
+                      subap_npx=subap_npx, fp4_pixel_pitch=fp4, fov_complete=fov_complete,
+                      fov_resolution_arcsec=fov_res, sensor_pxscale=sensor_px_rad,
+                      turbulence_pxscale=turb_px, fp_mask=mask)

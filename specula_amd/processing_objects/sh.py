@@ -1,0 +1,230 @@
+"""Drop-in SH (Shack-Hartmann WFS) processing object running on gfx950 HIP kernels.
+
+Same constructor keywords (= YAML keys), ports and lifecycle as
+specula/processing_objects/sh.py:30-110; the per-frame body (sh.py:348-428: field, FFT, |.|^2,
+kernel correlation, mask, cut, CCD rebin, flux normalisation) is one ``shwfs_forward`` call.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from specula_amd import RAD2ASEC
+from specula_amd import _native
+from specula_amd.base import BaseProcessingObj, BaseValue, InputValue
+from specula_amd.data_objects.electric_field import ElectricField
+from specula_amd.data_objects.intensity import Intensity
+from specula_amd.data_objects.lenslet import Lenslet
+from specula_amd.data_objects.laser_launch_telescope import LaserLaunchTelescope
+from specula_amd.lib.binning import build_operators
+from specula_amd.lib.geometry import sh_geometry
+
+
+class SHPlan:
+    """Owns a ``shwfs_plan`` (device copies of the binning operators and FP64 scratch)."""
+
+    def __init__(self, geom, lgs=False, max_frames=1, device=0):
+        row, col, M, norm = build_operators(geom, lgs=lgs)
+        if lgs:
+            raise NotImplementedError('LGS plans are created by SH with kernel spectra')
+        self.geom = geom
+        self.M = M
+        self.parseval = norm is not None
+        self.max_frames = max_frames
+        self.device = device
+        self.row_op = row
+        self.col_op = col
+        g = _native.ShwfsGeom()
+        g.n_lenses = geom.n_lenses
+        g.ef_size = geom.ef_size
+        g.np_sub = geom.ovs_np_sub
+        g.fft_size = geom.fft_size
+        g.spec_size = M
+        g.subap_npx = geom.subap_npx
+        g.lgs = 0
+        g.parseval = 1 if self.parseval else 0
+        g.max_frames = max_frames
+        g.k_rad_per_nm = 2 * np.pi / geom.wavelengthInNm
+        g.tilt_rad = np.pi / geom.fft_size
+        g.tilt_a0 = float((-geom.ovs_np_sub) // 2)
+        g.norm_const = norm if norm is not None else 0.0
+        g.pixel_area = geom.pixel_pitch ** 2
+        self._geom_c = g
+        rw = np.ascontiguousarray(row, dtype=np.float64)
+        cw = np.ascontiguousarray(col, dtype=np.float64)
+        h = ctypes.c_void_p()
+        lib = _native.lib()
+        _native.check(lib.shwfs_plan_create(ctypes.byref(g), rw.ctypes.data_as(ctypes.c_void_p),
+                                            cw.ctypes.data_as(ctypes.c_void_p), device, ctypes.byref(h)),
+                      'shwfs_plan_create')
+        self._h = h
+
+    def forward(self, A, phase, out, S0, n_frames=1, a_frame_stride=None, stream=None):
+        """A, phase: (n_frames,) ef x ef float32 device tensors (A may be 2-D = shared);
+        out: (n_frames,) ccd x ccd float32."""
+        for t in (A, phase, out):
+            if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError('SH buffers must be contiguous float32 device tensors')
+        ef = self.geom.ef_size
+        ccd = self.geom.ccd_side
+        if phase.numel() != n_frames * ef * ef or out.numel() != n_frames * ccd * ccd:
+            raise ValueError('SH buffer sizes do not match the plan geometry')
+        if a_frame_stride is None:
+            a_frame_stride = 0 if A.numel() == ef * ef else ef * ef
+        if A.numel() < (ef * ef if a_frame_stride == 0 else n_frames * ef * ef):
+            raise ValueError('amplitude buffer too small')
+        _native.check(_native.lib().shwfs_forward(
+            self._h, n_frames, _native.ptr(A), a_frame_stride, _native.ptr(phase), ef * ef,
+            float(S0), _native.ptr(out), ccd * ccd, _native.stream_handle(stream)), 'shwfs_forward')
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            try:
+                _native.lib().shwfs_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+
+class SH(BaseProcessingObj):
+    """specula/processing_objects/sh.py:30-477, HIP forward."""
+
+    def __init__(self,
+                 wavelengthInNm: float,
+                 subap_wanted_fov: float,
+                 sensor_pxscale: float,
+                 subap_on_diameter: int,
+                 subap_npx: int,
+                 FoVres30mas: bool = False,
+                 squaremask: bool = True,
+                 fov_ovs_coeff: float = 0,
+                 xShiftPhInPixel: float = 0,
+                 yShiftPhInPixel: float = 0,
+                 aXShiftPhInPixel: float = 0,
+                 aYShiftPhInPixel: float = 0,
+                 rotAnglePhInDeg: float = 0,
+                 aRotAnglePhInDeg: float = 0,
+                 do_not_double_fov_ovs: bool = False,
+                 set_fov_res_to_turbpxsc: bool = False,
+                 laser_launch_tel: LaserLaunchTelescope = None,
+                 target_device_idx: int = None,
+                 precision: int = None):
+        super().__init__(target_device_idx=target_device_idx, precision=precision)
+        if self.target_device_idx < 0:
+            raise ValueError('SH runs on the GPU only (target_device_idx >= 0)')
+        if self.precision != 1:
+            raise ValueError('the HIP SH path computes in FP32 (with FP64 reductions): use precision=1')
+        self._wavelengthInNm = wavelengthInNm
+        self._lenslet = Lenslet(subap_on_diameter)
+        self._subap_wanted_fov_arcsec = subap_wanted_fov
+        self._subap_wanted_fov = subap_wanted_fov / RAD2ASEC
+        self._sensor_pxscale_arcsec = sensor_pxscale
+        self._sensor_pxscale = sensor_pxscale / RAD2ASEC
+        self._subap_npx = subap_npx
+        self._fov_ovs_coeff = fov_ovs_coeff
+        self._squaremask = squaremask
+        self._FoVres30mas = FoVres30mas
+        self._fov_resolution_arcsec = 0.03 if FoVres30mas else 0
+        self._rotAnglePhInDeg = rotAnglePhInDeg
+        self._aRotAnglePhInDeg = aRotAnglePhInDeg
+        self._xShiftPhInPixel = xShiftPhInPixel
+        self._yShiftPhInPixel = yShiftPhInPixel
+        self._aXShiftPhInPixel = aXShiftPhInPixel
+        self._aYShiftPhInPixel = aYShiftPhInPixel
+        self._set_fov_res_to_turbpxsc = set_fov_res_to_turbpxsc
+        self._do_not_double_fov_ovs = do_not_double_fov_ovs
+        self._laser_launch_tel = laser_launch_tel
+        self._np_sub = 0
+        self._fft_size = 0
+        self._fov_ovs = 1
+        self._floatShifts = False
+        self._plan = None
+        self._graph_key = None
+        self.use_graph = False
+
+        self._ccd_side = self._subap_npx * self._lenslet.n_lenses
+        self._out_i = Intensity(self._ccd_side, self._ccd_side, precision=self.precision,
+                                target_device_idx=self.target_device_idx)
+        if self._laser_launch_tel is not None:
+            self.inputs['sodium_altitude'] = InputValue(type=BaseValue, optional=True)
+            self.inputs['sodium_intensity'] = InputValue(type=BaseValue, optional=True)
+        self.inputs['in_ef'] = InputValue(type=ElectricField)
+        self.outputs['out_i'] = self._out_i
+
+    # ------------------------------------------------------------------ geometry
+    def set_in_ef(self, in_ef):
+        """sh.py:112-237 (+ calc_trigger_geometry's cut/mask, sh.py:262-282)."""
+        if in_ef.size[0] != in_ef.size[1]:
+            raise ValueError('SH needs a square ElectricField')
+        g = sh_geometry(in_ef.size[0], in_ef.pixel_pitch, self._wavelengthInNm,
+                        self._subap_wanted_fov_arcsec, self._sensor_pxscale_arcsec,
+                        self._lenslet.n_lenses, self._subap_npx, FoVres30mas=self._FoVres30mas,
+                        squaremask=self._squaremask, fov_ovs_coeff=self._fov_ovs_coeff,
+                        do_not_double_fov_ovs=self._do_not_double_fov_ovs,
+                        set_fov_res_to_turbpxsc=self._set_fov_res_to_turbpxsc,
+                        fov_resolution_arcsec=self._fov_resolution_arcsec)
+        self.geometry = g
+        self._fov_resolution_arcsec = g.fov_resolution_arcsec
+        self._np_sub = g.np_sub
+        self._ovs_np_sub = g.ovs_np_sub
+        self._fov_ovs = g.fov_ovs
+        self._fft_size = g.fft_size
+        self._cutpixels = g.cutpixels
+        self._cutsize = g.cutsize
+        self._fp_mask = g.fp_mask
+        self._sensor_pxscale = g.sensor_pxscale
+
+    def _check_supported(self):
+        rot = self._rotAnglePhInDeg + self._aRotAnglePhInDeg
+        shift = np.round(np.array([self._xShiftPhInPixel + self._aXShiftPhInPixel,
+                                   self._yShiftPhInPixel + self._aYShiftPhInPixel]) * self._fov_ovs)
+        if self._fov_ovs != 1 or rot != 0 or np.sum(np.abs(shift)) != 0:
+            raise NotImplementedError(
+                f'SH geometry needs field interpolation (fov_ovs={self._fov_ovs}, rotation={rot}, '
+                f'shift={shift.tolist()}); set fov_ovs_coeff=1.0 so that fov_ovs == 1')
+        if self._laser_launch_tel is not None:
+            raise NotImplementedError('LGS (laser_launch_tel) SH path is not available in this build')
+
+    def setup(self):
+        super().setup()
+        in_ef = self.inputs['in_ef'].get(target_device_idx=self.target_device_idx)
+        self.set_in_ef(in_ef)
+        self._check_supported()
+        self._plan = SHPlan(self.geometry, lgs=False, max_frames=1, device=self.target_device_idx)
+        self.in_ef = in_ef
+        self.build_stream(allow_parallel=False, capture=False)
+
+    # ------------------------------------------------------------------ per frame
+    def _inputs_fp32(self):
+        ef = self.local_inputs.get('in_ef') or self.in_ef
+        A, ph = ef.A, ef.phaseInNm
+        if A.dtype != torch.float32 or not A.is_contiguous() or A.device != self.device:
+            A = A.to(self.device, torch.float32).contiguous()
+        if ph.dtype != torch.float32 or not ph.is_contiguous() or ph.device != self.device:
+            ph = ph.to(self.device, torch.float32).contiguous()
+        return ef, A, ph
+
+    def trigger_code(self):
+        ef, A, ph = self._inputs_fp32()
+        self._plan.forward(A, ph, self._out_i.i, ef.S0, n_frames=1, stream=self.stream)
+
+    def trigger(self):
+        if self.ready and self.use_graph:
+            ef, A, ph = self._inputs_fp32()
+            key = (A.data_ptr(), ph.data_ptr(), float(ef.S0), self._out_i.i.data_ptr())
+            if key != self._graph_key:
+                self._keep = (A, ph)
+                self.capture_stream()
+                self._graph_key = key
+        super().trigger()
+
+    def post_trigger(self):
+        super().post_trigger()
+        self._out_i.generation_time = self.current_time
+
+    def get_tlt_f(self, p, c):
+        """Half-pixel tilt (sh.py:469-477), for inspection; the kernels evaluate it inline."""
+        a = np.arange(-p // 2, p // 2)
+        xx, yy = np.meshgrid(a, a)
+        return np.exp(-2 * np.pi * 1j * (xx + yy) / (2 * (p + c)))
