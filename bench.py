@@ -1,0 +1,170 @@
+"""Benchmark: SH frames/s on the ELT-class NGS sensor (80x80 subapertures, N=64), one MI355X.
+
+One step = the hot path over one batch of 100 sequential frames of a seeded von Karman screen
+moving at 10 m/s (1 ms per frame): SH forward (field, spectra, binning, normalisation) plus the
+ShSlopec centre of gravity, with all inputs resident in HBM before the timed region.
+
+Prints one JSON line (rank 0).  The `roofline` object is for the dominant kernel (the fused SH
+forward): algorithmic bytes per launch / its average duration measured with HIP events recorded
+on its own stream.  `cpu_baseline` times the FP64 NumPy oracle (a restatement of the reference
+CPU path) on a bounded sample of the same frames, single-threaded.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--frames', type=int, default=100)
+    ap.add_argument('--cpu-seconds', type=float, default=15.0,
+                    help='budget for the CPU baseline sample (0 disables it)')
+    return ap.parse_args()
+
+
+def cpu_baseline(wl, A, frames, idxs, budget_s):
+    """Oracle SH + ShSlopec per frame on one host core (bounded sample)."""
+    os.environ.setdefault('OMP_NUM_THREADS', '1')
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import sh_oracle as O
+    go = O.sh_geometry(ef_size=wl.ef_size, pixel_pitch=wl.pixel_pitch,
+                       fov_resolution_arcsec=wl.fov_resolution_arcsec or None, **wl.sh_kwargs)
+    A64 = A.astype(np.float64)
+    times = []
+    t_start = time.perf_counter()
+    k = 0
+    while True:
+        t0 = time.perf_counter()
+        img = O.sh_forward(A64, frames[k % len(frames)].astype(np.float64), go)
+        O.shslopec(img, idxs)
+        times.append(time.perf_counter() - t0)
+        k += 1
+        if time.perf_counter() - t_start > budget_s or k >= 50:
+            break
+    warm = times[1:] if len(times) > 1 else times
+    per_frame = float(np.median(warm))
+    return {'value': 1.0 / per_frame, 'unit': 'frames/s', 'cores': 1, 'kind': 'port',
+            'sample': f'{len(times)} frames of elt_ngs_80x80 (FP64 NumPy oracle SH + ShSlopec, '
+                      f'median of {len(warm)} after 1 warm-up)',
+            'cpu': _cpu_model(), 'cpu_count': os.cpu_count(),
+            'affinity': len(os.sched_getaffinity(0))}
+
+
+def _cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f'bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported',
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from specula_amd import _native, workloads
+    from specula_amd.lib.subaps import detect_subaps
+    from specula_amd.processing_objects.sh import SHPlan
+    from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weights
+
+    dev = torch.device('cuda:0')
+    wl = workloads.elt_ngs_80x80()
+    g = wl.geometry()
+    F = args.frames
+    A = wl.pupil().astype(np.float32)
+    frames = wl.frames(F).astype(np.float32)
+
+    plan = SHPlan(g, max_frames=F, device=0)
+    At = torch.tensor(A, device=dev)
+    ph = torch.tensor(frames, device=dev)
+    ccd = torch.empty((F, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
+
+    # valid subapertures: flux > 0.5 max on a flat wavefront (ShSubapCalibrator convention)
+    flat = torch.zeros((1, g.ef_size, g.ef_size), dtype=torch.float32, device=dev)
+    flat_img = torch.empty((1, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
+    plan.forward(At, flat, flat_img, 1.0, n_frames=1)
+    idxs, _ = detect_subaps(flat_img[0].cpu().numpy(), g.n_lenses, 0.5)
+    n_sub = idxs.shape[0]
+    sl_plan = SlopecPlan(idxs, (g.ccd_side, g.ccd_side), compute_xy_weights(g.subap_npx), max_frames=F, device=0)
+    slopes = torch.empty((F, 2 * n_sub), dtype=torch.float32, device=dev)
+    flux = torch.empty((F, n_sub), dtype=torch.float64, device=dev)
+    totals = torch.empty((F, 2), dtype=torch.float64, device=dev)
+
+    stream = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+
+    def step(k=None):
+        if k is not None:
+            _native.check(_native.lib().shwfs_plan_set_timing_events(
+                plan._h, ev[k][0].cuda_event, ev[k][1].cuda_event), 'set_timing_events')
+        plan.forward(At, ph, ccd, 1.0, n_frames=F, stream=stream)
+        if k is not None:
+            _native.check(_native.lib().shwfs_plan_set_timing_events(plan._h, None, None), 'set_timing_events')
+        sl_plan.forward(ccd, slopes, flux, totals, n_frames=F, stream=stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(k)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms_per_step = elapsed * 1e3 / args.steps
+    fps = F * args.steps / elapsed
+    px = g.ef_size * g.ef_size
+    bytes_per_frame = 4 * (2 * px + g.ccd_side * g.ccd_side)      # A + phase read, CCD written
+    achieved = bytes_per_frame * F / (kern_ms * 1e-3) / 1e9
+    N = g.fft_size
+    fft_flop = 5 * N * N * np.log2(N * N) * g.n_lenses ** 2       # 5 N^2 log2(N^2) per subaperture
+    result = {
+        'metric': 'SH frames/s (subap-FFTs/s, % HBM roofline); adjoint vs push-pull IM time',
+        'value': round(fps, 1),
+        'unit': 'frames/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 4),
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'f32 (f64 flux/CoG accumulation)',
+        'data': 'synthetic: seeded von Karman screen (r0=0.15 m, L0=25 m) moving at 10 m/s',
+        'config': {'workload': 'elt_ngs_80x80', 'pupil_px': g.ef_size, 'subaps': f'{g.n_lenses}x{g.n_lenses}',
+                   'np_sub': g.ovs_np_sub, 'fft_size': N, 'ccd_px_per_subap': g.subap_npx,
+                   'frames_per_step': F, 'valid_subaps': int(n_sub), 'parallelism': 'single GPU'},
+        'subap_ffts_per_s': round(fps * g.n_lenses ** 2, 1),
+        'sh_kernel_ms_per_launch': round(kern_ms, 5),
+        'sh_kernel_frames_per_s': round(F / (kern_ms * 1e-3), 1),
+        'fft_gflops_nominal': round(fft_flop * F / (kern_ms * 1e-3) / 1e9, 1),
+        'fft_fp32_peak_frac': round(fft_flop * F / (kern_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+        'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
+                     'kernel': 'sh_ngs_forward_kernel<8,16,32>',
+                     'bytes_per_launch': bytes_per_frame * F},
+    }
+    if args.cpu_seconds > 0:
+        result['cpu_baseline'] = cpu_baseline(wl, A, frames, idxs, args.cpu_seconds)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -78,6 +78,12 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
                   const float* phase_nm, int64_t phase_frame_stride,
                   double S0, float* ccd, int64_t ccd_frame_stride, void* stream);
 
+/* Optional timing hook: when both events are non-null (hipEvent_t), shwfs_forward records
+ * `start` immediately before and `stop` immediately after the main SH kernel launch, on the
+ * call's stream, so that the caller can time that kernel alone.  Pass nulls to disable.
+ * No reference counterpart (measurement support). */
+int shwfs_plan_set_timing_events(shwfs_plan* plan, void* start, void* stop);
+
 /* ShSlopec plan.  Either `block_origin` (n_sub linear pixel offsets of square np_sub^2
  * blocks, row stride frame_w) or `idx` (n_sub x np_sub^2 linear pixel indices, int32) must
  * be given (host pointers).  xw, yw, mw: np_sub^2 weights (host, FP64): the x/y CoG weights
@@ -90,7 +96,8 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
 int shslopec_plan_destroy(shslopec_plan* plan);
 
 /* Thresholded centre of gravity for n_frames pixel frames.
- *   thr_mode: 0 = no threshold, 1 = subtract thr then clamp at 0, 2 = pedestal (zero below thr)
+ *   thr_mode: 0 = subtract thr then clamp at 0 (thr = 0 when no threshold is set),
+ *             1 = pedestal: zero the pixels below thr (sh_slopec.py:374-386)
  *   slopes  : float32, 2*n_sub per frame, [sx..., sy...] (data_objects/slopes.py:27-32)
  *   flux    : FP64, n_sub per frame (pre-threshold subaperture sums, sh_slopec.py:365)
  *   totals  : FP64, 2 per frame: sum and mean of flux (sh_slopec.py:432-434)

@@ -196,6 +196,8 @@ def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_p
     fov_cut = fov_complete - sensor_subap_fov
     cutpixels = int(np.round(fov_cut / fp4_pixel_pitch) / 2 * 2)         # :274
     cutsize = fft_size - cutpixels
+    if cutpixels < 0:
+        raise ValueError(f'sensor FoV exceeds the FFT FoV (cutpixels={cutpixels})')
     fp_mask = make_mask(fft_size, diaratio=_subap_wanted_fov / fov_complete, square=squaremask)  # :282
 
     return dict(ef_size=ef_size, pixel_pitch=pixel_pitch, wavelengthInNm=wavelengthInNm,

@@ -140,8 +140,8 @@ __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, 
         double t1 = 0.0, t2 = 0.0;
         const double* pf = part + (int64_t)f * gridDim.x * 2;
         for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-            t1 += __builtin_nontemporal_load(&pf[2 * b]);
-            t2 += __builtin_nontemporal_load(&pf[2 * b + 1]);
+            t1 += pf[2 * b];
+            t2 += pf[2 * b + 1];
         }
         // per-thread strided partials, then fixed-order block reduction
         t1 = block_sum(t1, red);
@@ -416,6 +416,8 @@ struct shwfs_plan {
     double* d_amp_part = nullptr;  // [max_frames][amp_blocks][2]
     unsigned* d_counter = nullptr; // [max_frames]
     double* d_bpart = nullptr;     // [max_frames][n_tiles]
+    hipEvent_t ev_start = nullptr; // optional timing hook around the main kernel
+    hipEvent_t ev_stop = nullptr;
 };
 
 struct shslopec_plan {
@@ -541,6 +543,14 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     return SHWFS_OK;
 }
 
+int shwfs_plan_set_timing_events(shwfs_plan* p, void* start, void* stop) {
+    if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
+    if ((start == nullptr) != (stop == nullptr)) return set_error(SHWFS_ERR_ARG, "give both events or neither");
+    p->ev_start = reinterpret_cast<hipEvent_t>(start);
+    p->ev_stop = reinterpret_cast<hipEvent_t>(stop);
+    return SHWFS_OK;
+}
+
 int shwfs_set_kernels(shwfs_plan* p, const float* kspec, void* stream) {
     (void)p; (void)kspec; (void)stream;
     return set_error(SHWFS_ERR_UNSUPPORTED, "LGS kernel path not available in this build");
@@ -580,8 +590,10 @@ int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0;
     a.S0 = S0; a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
     NgsLauncher L = pick_ngs(g.np_sub, g.spec_size);
+    if (p->ev_start) SHWFS_HIP_CHECK(hipEventRecord(p->ev_start, st));
     int rc = L(p, a, n_frames, st);
     if (rc) return rc;
+    if (p->ev_stop) SHWFS_HIP_CHECK(hipEventRecord(p->ev_stop, st));
     if (!g.parseval) {
         hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, n_frames), dim3(256), 0, st, ccd, ccd_stride,
                            ccd_side * ccd_side, p->d_bpart, p->n_tiles, p->d_amp,

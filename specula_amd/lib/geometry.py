@@ -138,6 +138,8 @@ def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_p
     fp4 = wavelengthInNm / 1e9 / (wf1_pitch * fft_size)
     fov_complete = fft_size * fp4
     cut = int(np.round((fov_complete - sensor_px_rad * subap_npx) / fp4) / 2 * 2)
+    if cut < 0:
+        raise ValueError(f'sensor FoV exceeds the FFT FoV (cutpixels={cut})')
     mask = make_mask(fft_size, diaratio=wanted_fov_rad / fov_complete, square=squaremask)
     return SHGeometry(ef_size=ef_size, pixel_pitch=pixel_pitch, wavelengthInNm=wavelengthInNm,
                       n_lenses=n, np_sub=np_sub_int, ovs_np_sub=ovs_np_sub, fov_ovs=fov_ovs,

@@ -1,0 +1,81 @@
+"""Synthetic SH workloads used by the benchmark, the smoke check and the tests.
+
+Each workload is an SH sensor geometry (YAML keys of processing_objects/sh.py:43-63) plus a
+seeded input generator.  Sizes:
+
+  scao_ngs_20x20   D=8 m, 160 px pupil, 14% obstruction, lambda 750 nm, 20x20 subapertures of
+                   8x8 px, N=32, 8x8 CCD px per subaperture (4:1), r0=0.15 m, L0=25 m
+  elt_ngs_80x80    D=39 m, 640 px pupil, 28% obstruction, lambda 750 nm, 80x80 subapertures of
+                   8x8 px, N=64, 16x16 CCD px (4:1), 100 frames of a screen moving at 10 m/s, 1 ms
+  lgs_40x40        D=39 m, 640 px pupil, 28% obstruction, lambda 589 nm, 40x40 subapertures of
+                   16x16 px, N=128, 16x16 CCD px (8:1)
+
+The 80x80 and 40x40 FFT sizes are reached by pinning the internal FoV resolution to
+(lambda/d_sub)/8, the value SH._fov_resolution_arcsec takes when it is set before setup()
+(sh.py:130: the automatic search is skipped when it is non-zero).
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from specula_amd import RAD2ASEC
+from specula_amd.lib.phasescreen import annular_pupil, shifted_frames, von_karman_opd
+
+
+@dataclass
+class Workload:
+    name: str
+    ef_size: int
+    pixel_pitch: float
+    obstruction: float
+    sh_kwargs: dict
+    fov_resolution_arcsec: float
+    seed: int
+
+    def geometry(self):
+        from specula_amd.lib.geometry import sh_geometry
+        return sh_geometry(self.ef_size, self.pixel_pitch, fov_resolution_arcsec=self.fov_resolution_arcsec,
+                           **self.sh_kwargs)
+
+    def pupil(self):
+        return annular_pupil(self.ef_size, self.obstruction)
+
+    def screen(self, width=None):
+        w = width or max(256, 1 << int(np.ceil(np.log2(self.ef_size + 64))))
+        return von_karman_opd(w, self.pixel_pitch, r0=0.15, L0=25.0, seed=self.seed)
+
+    def frames(self, n_frames, wind_speed=10.0, dt=1e-3):
+        """(n_frames, ef, ef) OPD [nm]: the screen translated by wind_speed*dt per frame."""
+        scr = self.screen()
+        return shifted_frames(scr, self.ef_size, n_frames, wind_speed * dt / self.pixel_pitch)
+
+
+def _lambda_over_d(lam_nm, d_sub):
+    return lam_nm * 1e-9 / d_sub * RAD2ASEC
+
+
+def scao_ngs_20x20():
+    pitch = 8.0 / 160
+    ld = _lambda_over_d(750, 8 * pitch)
+    kw = dict(wavelengthInNm=750, subap_wanted_fov=1.05 * 8 * ld, sensor_pxscale=ld,
+              subap_on_diameter=20, subap_npx=8, fov_ovs_coeff=1.0)
+    return Workload('scao_ngs_20x20', 160, pitch, 0.14, kw, 0.0, seed=1)
+
+
+def elt_ngs_80x80():
+    pitch = 39.0 / 640
+    ld = _lambda_over_d(750, 8 * pitch)
+    kw = dict(wavelengthInNm=750, subap_wanted_fov=8 * ld, sensor_pxscale=ld / 2,
+              subap_on_diameter=80, subap_npx=16, fov_ovs_coeff=1.0)
+    return Workload('elt_ngs_80x80', 640, pitch, 0.28, kw, ld / 8, seed=2)
+
+
+def lgs_40x40():
+    pitch = 39.0 / 640
+    ld = _lambda_over_d(589, 16 * pitch)
+    kw = dict(wavelengthInNm=589, subap_wanted_fov=16 * ld, sensor_pxscale=ld,
+              subap_on_diameter=40, subap_npx=16, fov_ovs_coeff=1.0)
+    return Workload('lgs_40x40', 640, pitch, 0.28, kw, ld / 8, seed=3)
+
+
+WORKLOADS = {w().name: w for w in (scao_ngs_20x20, elt_ngs_80x80, lgs_40x40)}

@@ -1,0 +1,58 @@
+"""The C-ABI library: it loads, exports every symbol include/shwfs.h declares, and rejects bad
+arguments with a message before touching the GPU.  CPU only (no kernels are launched)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+from specula_amd import _native
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), 'include', 'shwfs.h')
+
+
+def declared():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return set(re.findall(r'\b((?:shwfs|shslopec)_[a-z_]+)\s*\(', text))
+
+
+def test_library_exports_header_symbols():
+    lib = _native.load_library()
+    names = declared()
+    assert names == set(_native.EXPORTS)
+    for n in names:
+        assert hasattr(lib, n), n
+    assert lib.shwfs_version().startswith(b'shwfs-gfx950')
+
+
+def test_geom_struct_layout():
+    # int32 x 10 then 5 doubles, no padding surprises
+    assert ctypes.sizeof(_native.ShwfsGeom) == 10 * 4 + 5 * 8
+
+
+def test_argument_errors_are_reported():
+    lib = _native.load_library()
+    h = ctypes.c_void_p()
+    rc = lib.shwfs_plan_create(None, None, None, 0, ctypes.byref(h))
+    assert rc == -1 and b'null' in lib.shwfs_last_error()
+    g = _native.ShwfsGeom(n_lenses=4, ef_size=30, np_sub=8, spec_size=16, subap_npx=8, max_frames=1)
+    w = np.zeros((8, 16))
+    rc = lib.shwfs_plan_create(ctypes.byref(g), w.ctypes.data_as(ctypes.c_void_p),
+                               w.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(h))
+    assert rc == -1 and b'ef_size' in lib.shwfs_last_error()
+    g.ef_size = 32
+    g.spec_size = 8
+    rc = lib.shwfs_plan_create(ctypes.byref(g), w.ctypes.data_as(ctypes.c_void_p),
+                               w.ctypes.data_as(ctypes.c_void_p), 0, ctypes.byref(h))
+    assert rc == -1 and b'spec_size' in lib.shwfs_last_error()
+    idx = np.array([[0, 1, 2, 99]], dtype=np.int32)
+    wt = np.zeros(4)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+    rc = lib.shslopec_plan_create(1, 2, 4, 4, None, vp(idx), vp(wt), vp(wt), vp(wt), 1, 0, ctypes.byref(h))
+    assert rc == -1 and b'out of range' in lib.shwfs_last_error()
+    org = np.array([3], dtype=np.int32)
+    rc = lib.shslopec_plan_create(1, 2, 4, 4, vp(org), None, vp(wt), vp(wt), vp(wt), 1, 0, ctypes.byref(h))
+    assert rc == -1 and b'outside' in lib.shwfs_last_error()
+    rc = lib.shwfs_plan_set_timing_events(None, None, None)
+    assert rc == -1

@@ -1,0 +1,160 @@
+"""GPU parity of the HIP SH forward and ShSlopec CoG against the FP64 oracle (calls the C ABI)."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import check_intensity, check_slopes, oracle_geometry
+from oracle import sh_oracle as O
+from specula_amd import workloads
+from specula_amd.lib.geometry import sh_geometry
+from specula_amd.lib.phasescreen import annular_pupil, von_karman_opd
+from specula_amd.lib.subaps import detect_subaps
+from specula_amd.processing_objects.sh import SHPlan
+from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weights
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+
+
+def gpu_sh(geom, A, frames, S0=1.0, a_shared=True):
+    plan = SHPlan(geom, max_frames=len(frames), device=0)
+    ccd = geom.ccd_side
+    At = torch.tensor(np.asarray(A, np.float32), device=DEV)
+    if not a_shared:
+        At = At.expand(len(frames), *At.shape).contiguous()
+    ph = torch.tensor(np.asarray(frames, np.float32), device=DEV)
+    out = torch.empty((len(frames), ccd, ccd), dtype=torch.float32, device=DEV)
+    plan.forward(At, ph, out, S0, n_frames=len(frames))
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), plan
+
+
+def run_case(kw, ef, pitch, obs, seed, fov_res=0.0, n_frames=1, S0=1.0):
+    print(f'case ef={ef} kw={kw}', flush=True)
+    g = sh_geometry(ef, pitch, fov_resolution_arcsec=fov_res, **kw)
+    go = O.sh_geometry(ef_size=ef, pixel_pitch=pitch, fov_resolution_arcsec=fov_res or None, **kw)
+    A = annular_pupil(ef, obs)
+    scr = von_karman_opd(1 << int(np.ceil(np.log2(ef + 8))), pitch, seed=seed)
+    frames = np.stack([scr[:ef, k:k + ef] for k in range(n_frames)]).astype(np.float32)
+    out, plan = gpu_sh(g, A, frames, S0=S0)
+    for k in range(n_frames):
+        ref = O.sh_forward(A, frames[k].astype(np.float64), go, S0=S0)
+        st = check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
+    return st, plan
+
+
+def test_scao_ngs_20x20_parity():
+    wl = workloads.scao_ngs_20x20()
+    st, plan = run_case(wl.sh_kwargs, wl.ef_size, wl.pixel_pitch, wl.obstruction, wl.seed, n_frames=3, S0=7.5)
+    assert plan.parseval and plan.M == 16
+
+
+@pytest.mark.parametrize('kw', [
+    dict(wavelengthInNm=500, subap_wanted_fov=1.5, sensor_pxscale=0.25, subap_on_diameter=20, subap_npx=6,
+         fov_ovs_coeff=1.0),    # N=42, cut 12, clipping FoV mask, 5:1 rebin
+    dict(wavelengthInNm=500, subap_wanted_fov=1.5, sensor_pxscale=0.25, subap_on_diameter=20, subap_npx=4,
+         fov_ovs_coeff=1.0),    # N=30, cut 16, 14 -> 4 LCM rebin
+    dict(wavelengthInNm=500, subap_wanted_fov=1.2, sensor_pxscale=0.2, subap_on_diameter=20, subap_npx=6,
+         fov_ovs_coeff=1.0),    # N=24, cut 10, 14 -> 6 LCM rebin
+])
+def test_cut_mask_lcm_parity(kw):
+    st, plan = run_case(kw, 120, 0.05, 0.1, seed=5)
+    assert not plan.parseval
+
+
+def test_elt_ngs_80x80_parity():
+    wl = workloads.elt_ngs_80x80()
+    g = wl.geometry()
+    assert (g.fft_size, g.cutpixels, g.subap_npx, g.ovs_np_sub) == (64, 0, 16, 8)
+    frames = wl.frames(2).astype(np.float32)
+    A = wl.pupil()
+    out, plan = gpu_sh(g, A, frames)
+    go = oracle_geometry(wl)
+    for k in range(2):
+        ref = O.sh_forward(A, frames[k].astype(np.float64), go)
+        check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
+
+
+def test_batch_equals_single_and_stride():
+    wl = workloads.scao_ngs_20x20()
+    g = wl.geometry()
+    A = wl.pupil()
+    frames = wl.frames(4).astype(np.float32)
+    batch, _ = gpu_sh(g, A, frames)
+    per, _ = gpu_sh(g, A, frames, a_shared=False)
+    assert np.array_equal(batch, per)
+    for k in range(4):
+        single, _ = gpu_sh(g, A, frames[k:k + 1])
+        assert np.array_equal(single[0], batch[k])
+
+
+def test_flux_known_answer():
+    """test_sh.py:18-42: sum of the intensity == S0 * masked area, here at fov_ovs == 1."""
+    kw = dict(wavelengthInNm=500, subap_wanted_fov=3, sensor_pxscale=0.5, subap_on_diameter=20,
+              subap_npx=6, fov_ovs_coeff=1.0, fov_resolution_arcsec=0.5 * 0.34377467707849396 / 1.0)
+    g = sh_geometry(120, 0.05, **kw)
+    A = np.ones((120, 120))
+    out, _ = gpu_sh(g, A, np.zeros((1, 120, 120), np.float32), S0=100)
+    np.testing.assert_allclose(out[0].astype(np.float64).sum(), 100 * 120 * 120 * 0.05 ** 2, rtol=1e-6)
+
+
+def test_one_pixel_tilt_shifts_one_pixel():
+    """test_sh.py:44-95 at fov_ovs == 1: a tilt of one CCD pixel moves the spots one pixel in x."""
+    ef, pitch, pxs, npx = 120, 0.05, 0.5, 6
+    kw = dict(wavelengthInNm=500, subap_wanted_fov=npx * pxs, sensor_pxscale=pxs, subap_on_diameter=20,
+              subap_npx=npx, fov_ovs_coeff=1.0)
+    g = sh_geometry(ef, pitch, **kw)
+    tilt_value = np.radians(ef * pitch * 1 / (60 * 60) * pxs)
+    tilt = np.linspace(-tilt_value / 2, tilt_value / 2, ef)
+    ph = np.broadcast_to(tilt, (ef, ef)) * 1e9
+    out, _ = gpu_sh(g, np.ones((ef, ef)), np.stack([np.zeros((ef, ef)), ph]).astype(np.float32))
+    flat, tilted = out[0].astype(np.float64), out[1].astype(np.float64)
+    shifted = np.roll(flat, (0, 1))
+    shifted[:, ::npx] = 0
+    tilted[:, ::npx] = 0
+    np.testing.assert_array_almost_equal(tilted, shifted, decimal=3)
+
+
+def _slopec_case(wl, pedestal=False, thr=0.0, idx_mode=False, exp_weight=1.0):
+    g = wl.geometry()
+    A = wl.pupil()
+    frames = wl.frames(2).astype(np.float32)
+    flat, _ = gpu_sh(g, A, np.zeros((1, wl.ef_size, wl.ef_size), np.float32))
+    idxs, dmap = detect_subaps(flat[0], g.n_lenses, 0.5)
+    out, _ = gpu_sh(g, A, frames)
+    H, W = out.shape[1:]
+    if idx_mode:   # a permuted pixel order forces the general index-list addressing
+        perm = np.random.default_rng(0).permutation(idxs.shape[1])
+        idxs_used = idxs[:, perm]
+        weights = compute_xy_weights(g.subap_npx, exp_weight)
+        weights = {k: v.ravel()[perm].reshape(v.shape) for k, v in weights.items()}
+    else:
+        idxs_used = idxs
+        weights = compute_xy_weights(g.subap_npx, exp_weight)
+    plan = SlopecPlan(idxs_used, (H, W), weights, max_frames=2, device=0)
+    assert plan.block_mode == (not idx_mode)
+    n_sub = idxs.shape[0]
+    pix = torch.tensor(out, device=DEV)
+    slopes = torch.empty((2, 2 * n_sub), dtype=torch.float32, device=DEV)
+    flux = torch.empty((2, n_sub), dtype=torch.float64, device=DEV)
+    totals = torch.empty((2, 2), dtype=torch.float64, device=DEV)
+    plan.forward(pix, slopes, flux, totals, thr=thr, pedestal=pedestal, n_frames=2)
+    torch.cuda.synchronize()
+    for k in range(2):
+        ref = O.shslopec(out[k].astype(np.float64), idxs, thr_value=thr if thr > 0 else -1,
+                         exp_weight=exp_weight, thr_pedestal=pedestal)
+        check_slopes(slopes[k].cpu().numpy(), ref['slopes'], g.subap_npx)
+        np.testing.assert_allclose(flux[k].cpu().numpy(), ref['flux'], rtol=1e-12, atol=1e-12 * ref['flux'].max())
+        np.testing.assert_allclose(totals[k].cpu().numpy(), [ref['total'], ref['mean']], rtol=1e-12)
+    return n_sub
+
+
+def test_slopec_parity_elt():
+    n_sub = _slopec_case(workloads.elt_ngs_80x80())
+    assert 4000 < n_sub < 5200
+
+
+@pytest.mark.parametrize('opts', [dict(idx_mode=True), dict(thr=2e-7), dict(thr=2e-7, pedestal=True),
+                                  dict(exp_weight=1.5)])
+def test_slopec_modes(opts):
+    _slopec_case(workloads.scao_ngs_20x20(), **opts)
