@@ -88,19 +88,24 @@ def test_batch_equals_single_and_stride():
         assert np.array_equal(single[0], batch[k])
 
 
-def test_flux_known_answer():
+@pytest.mark.parametrize('kw', [
+    dict(wavelengthInNm=500, subap_wanted_fov=1.5, sensor_pxscale=0.25, subap_on_diameter=20, subap_npx=6,
+         fov_ovs_coeff=1.0),                                             # summed normalisation (cut 12)
+    dict(wavelengthInNm=500, subap_wanted_fov=2.5, sensor_pxscale=2.0625 / 8, subap_on_diameter=20,
+         subap_npx=8, fov_ovs_coeff=1.0),                                # Parseval normalisation
+])
+def test_flux_known_answer(kw):
     """test_sh.py:18-42: sum of the intensity == S0 * masked area, here at fov_ovs == 1."""
-    kw = dict(wavelengthInNm=500, subap_wanted_fov=3, sensor_pxscale=0.5, subap_on_diameter=20,
-              subap_npx=6, fov_ovs_coeff=1.0, fov_resolution_arcsec=0.5 * 0.34377467707849396 / 1.0)
     g = sh_geometry(120, 0.05, **kw)
     A = np.ones((120, 120))
-    out, _ = gpu_sh(g, A, np.zeros((1, 120, 120), np.float32), S0=100)
-    np.testing.assert_allclose(out[0].astype(np.float64).sum(), 100 * 120 * 120 * 0.05 ** 2, rtol=1e-6)
+    A[:7, :30] = 0
+    out, plan = gpu_sh(g, A, np.zeros((1, 120, 120), np.float32), S0=100)
+    np.testing.assert_allclose(out[0].astype(np.float64).sum(), 100 * A.sum() * 0.05 ** 2, rtol=1e-6)
 
 
 def test_one_pixel_tilt_shifts_one_pixel():
     """test_sh.py:44-95 at fov_ovs == 1: a tilt of one CCD pixel moves the spots one pixel in x."""
-    ef, pitch, pxs, npx = 120, 0.05, 0.5, 6
+    ef, pitch, pxs, npx = 120, 0.05, 0.25, 6
     kw = dict(wavelengthInNm=500, subap_wanted_fov=npx * pxs, sensor_pxscale=pxs, subap_on_diameter=20,
               subap_npx=npx, fov_ovs_coeff=1.0)
     g = sh_geometry(ef, pitch, **kw)
