@@ -39,7 +39,7 @@ typedef struct shwfs_geom {
     int32_t lgs;           /* 1: per-subaperture ConvolutionKernel correlation (sh.py:382-394) */
     int32_t parseval;      /* 1: sum of the binned image = norm_const * sum(A^2) exactly      */
     int32_t max_frames;    /* largest n_frames accepted by shwfs_forward                      */
-    int32_t reserved;
+    int32_t variant;       /* kernel choice: 0 auto, 1 register GEMV binning, 2 MFMA binning  */
     double  k_rad_per_nm;  /* 2*pi/lambda (electric_field.py:65)                              */
     double  tilt_rad;      /* pi/N: half-pixel tilt coefficient (sh.py:469-477)               */
     double  tilt_a0;       /* first tilt abscissa: -ceil(np_sub/2)                            */

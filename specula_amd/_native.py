@@ -25,7 +25,7 @@ class ShwfsGeom(ctypes.Structure):
         ('n_lenses', ctypes.c_int32), ('ef_size', ctypes.c_int32), ('np_sub', ctypes.c_int32),
         ('fft_size', ctypes.c_int32), ('spec_size', ctypes.c_int32), ('subap_npx', ctypes.c_int32),
         ('lgs', ctypes.c_int32), ('parseval', ctypes.c_int32), ('max_frames', ctypes.c_int32),
-        ('reserved', ctypes.c_int32),
+        ('variant', ctypes.c_int32),
         ('k_rad_per_nm', ctypes.c_double), ('tilt_rad', ctypes.c_double), ('tilt_a0', ctypes.c_double),
         ('norm_const', ctypes.c_double), ('pixel_area', ctypes.c_double),
     ]

@@ -303,6 +303,122 @@ sh_ngs_forward_kernel(NgsArgs g) {
     }
 }
 
+// ---------------------------------------------------------------- NGS forward, MFMA binning (M = 16)
+// One wavefront owns SW = 64/NP consecutive subapertures (a block holds 4 independent waves):
+//   A: lane (q, x)  -> E along column x of subaperture q, pruned FFT_16 over y    -> G1[q][ky][x]
+//   B: lane (q, ky) -> pruned FFT_16 over x of row ky, |.|^2                      -> S[q][ky][kx]
+//   C: per subaperture, B = P S Q^T as 2 x 4 v_mfma_f32_16x16x4f32:
+//        T = S Q^T  with the K index permuted as k = 4*(lane/16) + step, so that each lane reads
+//                   one float4 of S and one float4 of Q;
+//        B = P T    where T, left in the accumulator layout (lane l, reg r = T[4(l/16)+r][l%16]),
+//                   is directly the B operand of step r under the same K permutation.
+//      The result lands as lane l, reg r = B[Y = 4(l/16)+r][X = l%16].
+// G1 (complex, 8 per row) and S (real, 16 per row) occupy the same 64 bytes per (q, ky) row, and
+// each row is read and rewritten by the same lane, so they share one LDS buffer.
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+__global__ void __launch_bounds__(256)
+sh_ngs_mfma_kernel(NgsArgs g) {
+    constexpr int M = 16;
+    constexpr int SW = 64 / NP;
+    constexpr int XS = 8;
+    static_assert(NP <= 8, "M = 16 path needs np_sub <= 8");
+    __shared__ float4 lds_all[4][SW * M * XS / 2];
+    __shared__ double red[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float2* g1 = reinterpret_cast<float2*>(lds_all[w]);
+    float* sS = reinterpret_cast<float*>(lds_all[w]);
+    const int f = blockIdx.y;
+    const int sub0 = (blockIdx.x * 4 + w) * SW;
+    const int n = g.n_lenses;
+
+    {   // phase A: column transforms
+        const int q = lane / NP, x = lane - (lane / NP) * NP;
+        const int sub = sub0 + q;
+        float2 e[NP];
+        if (q < SW && sub < g.n_sub) {
+            const int i = sub / n, j = sub - (sub / n) * n;
+            const int64_t off = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x;
+            const float* ar = g.A + f * g.a_stride + off;
+            const float* pr = g.phase + f * g.ph_stride + off;
+            const double ax = g.a0 + x;
+#pragma unroll
+            for (int y = 0; y < NP; ++y)
+                e[y] = field_sample(ar[(int64_t)y * g.ef_size], pr[(int64_t)y * g.ef_size], g.k, g.tilt,
+                                    ax + g.a0 + y);
+        } else {
+#pragma unroll
+            for (int y = 0; y < NP; ++y) e[y] = make_float2(0.f, 0.f);
+        }
+        float2 G[M];
+        fft_pruned<M, NP, false>(e, G);
+        if (q < SW) {
+#pragma unroll
+            for (int ky = 0; ky < M; ++ky) g1[(q * M + ky) * XS + x] = G[ky];
+        }
+    }
+    __syncthreads();
+
+    for (int r = lane; r < SW * M; r += 64) {   // phase B: row transforms and power spectrum
+        float2 row[NP];
+#pragma unroll
+        for (int x = 0; x < NP; ++x) row[x] = g1[r * XS + x];
+        float2 G[M];
+        fft_pruned<M, NP, false>(row, G);
+        float4* srow = reinterpret_cast<float4*>(sS + r * M);
+#pragma unroll
+        for (int c = 0; c < M / 4; ++c)
+            srow[c] = make_float4(G[4 * c].x * G[4 * c].x + G[4 * c].y * G[4 * c].y,
+                                  G[4 * c + 1].x * G[4 * c + 1].x + G[4 * c + 1].y * G[4 * c + 1].y,
+                                  G[4 * c + 2].x * G[4 * c + 2].x + G[4 * c + 2].y * G[4 * c + 2].y,
+                                  G[4 * c + 3].x * G[4 * c + 3].x + G[4 * c + 3].y * G[4 * c + 3].y);
+    }
+    __syncthreads();
+
+    // phase C: binning on the matrix cores
+    const int i16 = lane & 15, g4 = lane >> 4;
+    const floatx4 qv = *reinterpret_cast<const floatx4*>(g.colw + i16 * M + 4 * g4);
+    const floatx4 pv = *reinterpret_cast<const floatx4*>(g.roww + i16 * M + 4 * g4);
+    double scale = 1.0;
+    if (g.parseval) {
+        const double* am = g.amp + (int64_t)f * g.amp_fstep;
+        scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
+    }
+    const int W = n * g.npx;
+    double bsum = 0.0;
+    for (int q = 0; q < SW; ++q) {
+        const int sub = sub0 + q;
+        if (sub >= g.n_sub) break;                       // wave-uniform
+        const floatx4 sv = *reinterpret_cast<const floatx4*>(sS + (q * M + i16) * M + 4 * g4);
+        floatx4 t = {0.f, 0.f, 0.f, 0.f};
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[0], qv[0], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[1], qv[1], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[2], qv[2], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[3], qv[3], t, 0, 0, 0);
+        floatx4 b = {0.f, 0.f, 0.f, 0.f};
+        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[0], t[0], b, 0, 0, 0);
+        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[1], t[1], b, 0, 0, 0);
+        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[2], t[2], b, 0, 0, 0);
+        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[3], t[3], b, 0, 0, 0);
+        const int i = sub / n, j = sub - (sub / n) * n;
+        float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + 4 * g4) * W + (int64_t)j * g.npx + i16;
+        if (i16 < g.npx) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (4 * g4 + r < g.npx) {
+                    bsum += (double)b[r];
+                    dst[(int64_t)r * W] = (float)((double)b[r] * scale);
+                }
+            }
+        }
+    }
+    if (!g.parseval) {
+        bsum = block_sum(bsum, red);
+        if (threadIdx.x == 0) g.bpart[(int64_t)f * g.n_tiles + blockIdx.x] = bsum;
+    }
+}
+
 // Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
 __global__ void sh_rescale_kernel(float* __restrict__ ccd, int64_t ccd_stride, int64_t n_px,
                                   const double* __restrict__ bpart, int n_tiles,
@@ -409,6 +525,8 @@ struct shwfs_plan {
     int ts = 0;
     int n_tiles = 0;
     int amp_blocks = 0;
+    int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
+    bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
     float* d_roww = nullptr;
     float* d_colw = nullptr;
     float* d_kspec = nullptr;
@@ -457,7 +575,27 @@ int launch_ngs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
 
 int ngs_tile_size(int np) { return 256 / np; }
 
+template <int NP>
+int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
+    dim3 grid(p->n_tiles, n_frames);
+    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP>), grid, dim3(256), 0, st, a);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
 using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
+
+// MFMA binning path: M = 16 (np_sub 5..8) and at most 16 CCD px per subaperture.
+NgsLauncher pick_ngs_mfma(int np, int M, int npx) {
+    if (M != 16 || npx > 16) return nullptr;
+    switch (np) {
+        case 5: return launch_ngs_mfma<5>;
+        case 6: return launch_ngs_mfma<6>;
+        case 7: return launch_ngs_mfma<7>;
+        case 8: return launch_ngs_mfma<8>;
+        default: return nullptr;
+    }
+}
 
 NgsLauncher pick_ngs(int np, int M) {
     switch (np) {
@@ -491,19 +629,26 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     if (g.spec_size < 2 * g.np_sub - 1)
         return set_error(SHWFS_ERR_ARG, "spec_size must be >= 2*np_sub-1");
     if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS kernel path not available in this build");
-    if (!pick_ngs(g.np_sub, g.spec_size))
+    NgsLauncher mf = pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx);
+    NgsLauncher v1 = pick_ngs(g.np_sub, g.spec_size);
+    bool use_mfma = (g.variant == 2) || (g.variant == 0 && mf != nullptr);
+    if (use_mfma && !mf) return set_error(SHWFS_ERR_UNSUPPORTED, "MFMA variant needs spec_size 16, np_sub 5..8, subap_npx <= 16");
+    if (!use_mfma && !v1)
         return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
+    if (g.variant < 0 || g.variant > 2) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1 or 2");
     DeviceGuard dg(device);
     auto* p = new shwfs_plan();
     p->g = g;
     p->device = device;
     p->n_sub = g.n_lenses * g.n_lenses;
     p->npx_pad = (g.subap_npx + 3) / 4 * 4;
-    p->ts = ngs_tile_size(g.np_sub);
+    p->mfma = use_mfma;
+    p->rows_alloc = use_mfma ? 16 : p->npx_pad;
+    p->ts = use_mfma ? 4 * (64 / g.np_sub) : ngs_tile_size(g.np_sub);
     p->n_tiles = (p->n_sub + p->ts - 1) / p->ts;
     p->amp_blocks = 64;
     const int M = g.spec_size;
-    std::vector<float> rw((size_t)p->npx_pad * M, 0.f), cw((size_t)p->npx_pad * M, 0.f);
+    std::vector<float> rw((size_t)p->rows_alloc * M, 0.f), cw((size_t)p->rows_alloc * M, 0.f);
     for (int Y = 0; Y < g.subap_npx; ++Y)
         for (int k = 0; k < M; ++k) {
             rw[(size_t)Y * M + k] = (float)row_w[(size_t)Y * M + k];
@@ -589,7 +734,7 @@ int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
     a.n_sub = p->n_sub; a.n_tiles = p->n_tiles; a.parseval = g.parseval;
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0;
     a.S0 = S0; a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
-    NgsLauncher L = pick_ngs(g.np_sub, g.spec_size);
+    NgsLauncher L = p->mfma ? pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx) : pick_ngs(g.np_sub, g.spec_size);
     if (p->ev_start) SHWFS_HIP_CHECK(hipEventRecord(p->ev_start, st));
     int rc = L(p, a, n_frames, st);
     if (rc) return rc;

@@ -23,7 +23,9 @@ from specula_amd.lib.geometry import sh_geometry
 class SHPlan:
     """Owns a ``shwfs_plan`` (device copies of the binning operators and FP64 scratch)."""
 
-    def __init__(self, geom, lgs=False, max_frames=1, device=0):
+    def __init__(self, geom, lgs=False, max_frames=1, device=0, variant=0):
+        """variant: 0 picks the kernel automatically, 1 forces register-GEMV binning, 2 forces
+        MFMA binning (spec_size 16, np_sub 5..8, subap_npx <= 16)."""
         row, col, M, norm = build_operators(geom, lgs=lgs)
         if lgs:
             raise NotImplementedError('LGS plans are created by SH with kernel spectra')
@@ -44,6 +46,7 @@ class SHPlan:
         g.lgs = 0
         g.parseval = 1 if self.parseval else 0
         g.max_frames = max_frames
+        g.variant = variant
         g.k_rad_per_nm = 2 * np.pi / geom.wavelengthInNm
         g.tilt_rad = np.pi / geom.fft_size
         g.tilt_a0 = float((-geom.ovs_np_sub) // 2)

@@ -16,8 +16,8 @@ pytestmark = pytest.mark.gpu
 DEV = 'cuda:0'
 
 
-def gpu_sh(geom, A, frames, S0=1.0, a_shared=True):
-    plan = SHPlan(geom, max_frames=len(frames), device=0)
+def gpu_sh(geom, A, frames, S0=1.0, a_shared=True, variant=0):
+    plan = SHPlan(geom, max_frames=len(frames), device=0, variant=variant)
     ccd = geom.ccd_side
     At = torch.tensor(np.asarray(A, np.float32), device=DEV)
     if not a_shared:
@@ -29,23 +29,25 @@ def gpu_sh(geom, A, frames, S0=1.0, a_shared=True):
     return out.cpu().numpy(), plan
 
 
-def run_case(kw, ef, pitch, obs, seed, fov_res=0.0, n_frames=1, S0=1.0):
-    print(f'case ef={ef} kw={kw}', flush=True)
+def run_case(kw, ef, pitch, obs, seed, fov_res=0.0, n_frames=1, S0=1.0, variant=0):
+    print(f'case ef={ef} kw={kw} variant={variant}', flush=True)
     g = sh_geometry(ef, pitch, fov_resolution_arcsec=fov_res, **kw)
     go = O.sh_geometry(ef_size=ef, pixel_pitch=pitch, fov_resolution_arcsec=fov_res or None, **kw)
     A = annular_pupil(ef, obs)
     scr = von_karman_opd(1 << int(np.ceil(np.log2(ef + 8))), pitch, seed=seed)
     frames = np.stack([scr[:ef, k:k + ef] for k in range(n_frames)]).astype(np.float32)
-    out, plan = gpu_sh(g, A, frames, S0=S0)
+    out, plan = gpu_sh(g, A, frames, S0=S0, variant=variant)
     for k in range(n_frames):
         ref = O.sh_forward(A, frames[k].astype(np.float64), go, S0=S0)
         st = check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
     return st, plan
 
 
-def test_scao_ngs_20x20_parity():
+@pytest.mark.parametrize('variant', [1, 2])
+def test_scao_ngs_20x20_parity(variant):
     wl = workloads.scao_ngs_20x20()
-    st, plan = run_case(wl.sh_kwargs, wl.ef_size, wl.pixel_pitch, wl.obstruction, wl.seed, n_frames=3, S0=7.5)
+    st, plan = run_case(wl.sh_kwargs, wl.ef_size, wl.pixel_pitch, wl.obstruction, wl.seed, n_frames=3, S0=7.5,
+                        variant=variant)
     assert plan.parseval and plan.M == 16
 
 
@@ -57,18 +59,20 @@ def test_scao_ngs_20x20_parity():
     dict(wavelengthInNm=500, subap_wanted_fov=1.2, sensor_pxscale=0.2, subap_on_diameter=20, subap_npx=6,
          fov_ovs_coeff=1.0),    # N=24, cut 10, 14 -> 6 LCM rebin
 ])
-def test_cut_mask_lcm_parity(kw):
-    st, plan = run_case(kw, 120, 0.05, 0.1, seed=5)
+@pytest.mark.parametrize('variant', [1, 2])
+def test_cut_mask_lcm_parity(kw, variant):
+    st, plan = run_case(kw, 120, 0.05, 0.1, seed=5, variant=variant)
     assert not plan.parseval
 
 
-def test_elt_ngs_80x80_parity():
+@pytest.mark.parametrize('variant', [1, 2])
+def test_elt_ngs_80x80_parity(variant):
     wl = workloads.elt_ngs_80x80()
     g = wl.geometry()
     assert (g.fft_size, g.cutpixels, g.subap_npx, g.ovs_np_sub) == (64, 0, 16, 8)
     frames = wl.frames(2).astype(np.float32)
     A = wl.pupil()
-    out, plan = gpu_sh(g, A, frames)
+    out, plan = gpu_sh(g, A, frames, variant=variant)
     go = oracle_geometry(wl)
     for k in range(2):
         ref = O.sh_forward(A, frames[k].astype(np.float64), go)
@@ -83,6 +87,8 @@ def test_batch_equals_single_and_stride():
     batch, _ = gpu_sh(g, A, frames)
     per, _ = gpu_sh(g, A, frames, a_shared=False)
     assert np.array_equal(batch, per)
+    v1, _ = gpu_sh(g, A, frames, variant=1)
+    np.testing.assert_allclose(v1, batch, rtol=1e-4, atol=1e-5 * batch.max())
     for k in range(4):
         single, _ = gpu_sh(g, A, frames[k:k + 1])
         assert np.array_equal(single[0], batch[k])
