@@ -19,6 +19,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -401,16 +402,42 @@ sh_ngs_mfma_kernel(NgsArgs g) {
         b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[1], t[1], b, 0, 0, 0);
         b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[2], t[2], b, 0, 0, 0);
         b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[3], t[3], b, 0, 0, 0);
-        const int i = sub / n, j = sub - (sub / n) * n;
-        float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + 4 * g4) * W + (int64_t)j * g.npx + i16;
-        if (i16 < g.npx) {
+        // stage the 16 x 16 tile over this subaperture's (already consumed) spectrum in LDS;
+        // rows/columns >= npx are exactly zero (zero-padded operators)
+        float* bt = sS + q * (M * M);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                if (4 * g4 + r < g.npx) {
-                    bsum += (double)b[r];
-                    dst[(int64_t)r * W] = (float)((double)b[r] * scale);
-                }
-            }
+        for (int r = 0; r < 4; ++r) {
+            bt[(4 * g4 + r) * M + i16] = b[r];
+            if (!g.parseval && 4 * g4 + r < g.npx && i16 < g.npx) bsum += (double)b[r];
+        }
+    }
+    __syncthreads();
+    // coalesced stores: consecutive lanes walk X, then subapertures, then CCD rows
+    const int nq = min(SW, g.n_sub - sub0);
+    float* ccdf = g.ccd + f * g.ccd_stride;
+    if (g.npx == 16 && nq == SW) {
+        constexpr int PER4 = SW * 16 * 4;                 // float4 groups per wave
+        for (int e = lane; e < PER4; e += 64) {
+            const int x4 = e & 3, t2 = e >> 2;
+            const int q = t2 % SW, Y = t2 / SW;
+            const int sub = sub0 + q;
+            const int i = sub / n, j = sub - (sub / n) * n;
+            const float4 v = *reinterpret_cast<const float4*>(sS + q * (M * M) + Y * M + 4 * x4);
+            const float sc = (float)scale;
+            *reinterpret_cast<float4*>(ccdf + (int64_t)(i * 16 + Y) * W + j * 16 + 4 * x4) =
+                g.parseval ? make_float4((float)(v.x * scale), (float)(v.y * scale), (float)(v.z * scale),
+                                         (float)(v.w * scale))
+                           : make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc);
+        }
+    } else if (nq > 0) {
+        const int per = nq * g.npx * g.npx;
+        for (int e = lane; e < per; e += 64) {
+            const int X = e % g.npx, t2 = e / g.npx;
+            const int q = t2 % nq, Y = t2 / nq;
+            const int sub = sub0 + q;
+            const int i = sub / n, j = sub - (sub / n) * n;
+            ccdf[(int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X] =
+                (float)((double)sS[q * (M * M) + Y * M + X] * scale);
         }
     }
     if (!g.parseval) {
@@ -483,6 +510,65 @@ __global__ void cog_moments_kernel(const float* __restrict__ pix, int64_t pix_st
     }
 }
 
+// Fast path for square subaperture blocks with separable weights (xw[r][c] = xv[c],
+// yw[r][c] = yv[r], mask_weighted = 1, the ShSlopec default): one lane per pixel row, NP lanes per
+// subaperture, subapertures taken in row-major order of their blocks so that the lanes of a wave
+// read neighbouring 64-byte row segments.  FP64 accumulation throughout; x weights are
+// wave-uniform (scalar loads), y weights multiply the row total.
+template <int NP, bool VEC4>
+__global__ void __launch_bounds__(256)
+cog_rows_kernel(const float* __restrict__ pix, int64_t pix_stride, const int32_t* __restrict__ origin,
+                const int32_t* __restrict__ order, int n_sub, int frame_w,
+                const double* __restrict__ xv, const double* __restrict__ yv, float thr, int thr_mode,
+                double* __restrict__ mom) {
+    constexpr int SPW = 64 / NP;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = lane / NP, r = lane - (lane / NP) * NP;
+    const int k = (blockIdx.x * 4 + w) * SPW + q;
+    const int f = blockIdx.y;
+    const bool active = k < n_sub;
+    double flux = 0.0, tot = 0.0, sx = 0.0;
+    if (active) {
+        const float* row = pix + f * pix_stride + origin[k] + (int64_t)r * frame_w;
+        float v[NP];
+        if constexpr (VEC4) {
+#pragma unroll
+            for (int c = 0; c < NP / 4; ++c) {
+                const float4 t = reinterpret_cast<const float4*>(row)[c];
+                v[4 * c] = t.x; v[4 * c + 1] = t.y; v[4 * c + 2] = t.z; v[4 * c + 3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < NP; ++c) v[c] = row[c];
+        }
+#pragma unroll
+        for (int c = 0; c < NP; ++c) {
+            flux += (double)v[c];
+            float wv;
+            if (thr_mode == 1) {
+                wv = v[c] < thr ? 0.f : v[c];
+            } else {
+                wv = v[c] - thr;
+                wv = wv < 0.f ? 0.f : wv;
+            }
+            tot += (double)wv;
+            sx += (double)wv * xv[c];
+        }
+    }
+    double sy = active ? yv[r] * tot : 0.0;
+#pragma unroll
+    for (int off = NP / 2; off > 0; off >>= 1) {
+        flux += __shfl_xor(flux, off, kWave);
+        tot += __shfl_xor(tot, off, kWave);
+        sx += __shfl_xor(sx, off, kWave);
+        sy += __shfl_xor(sy, off, kWave);
+    }
+    if (active && r == 0) {
+        double* m = mom + ((int64_t)f * n_sub + order[k]) * 4;
+        m[0] = flux; m[1] = tot; m[2] = sx; m[3] = sy;
+    }
+}
+
 // One block per frame: mean of the thresholded totals, dim-subaperture cut-off
 // (factor = 0 where 1/tot > 1/(1e-3 mean), sh_slopec.py:393-400), slopes and flux outputs.
 __global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
@@ -541,6 +627,11 @@ struct shwfs_plan {
 struct shslopec_plan {
     int device = 0;
     int n_sub = 0, np_sub = 0, frame_h = 0, frame_w = 0, max_frames = 0;
+    bool rows_path = false;        // cog_rows_kernel (separable weights, square blocks)
+    bool vec4 = false;
+    int32_t* d_sorted = nullptr;   // row-major sorted block origins
+    int32_t* d_order = nullptr;    // subaperture index of each sorted slot
+    double* d_sep = nullptr;       // xv | yv
     int32_t* d_origin = nullptr;
     int32_t* d_idx = nullptr;
     double* d_w = nullptr;     // xw | yw | mw
@@ -788,6 +879,36 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
         if ((e = hipMalloc((void**)&p->d_origin, n_sub * sizeof(int32_t))) != hipSuccess ||
             (e = hipMemcpy(p->d_origin, block_origin, n_sub * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
+        // separable default weights -> row kernel with row-major processing order
+        bool sep = (np_sub == 4 || np_sub == 8 || np_sub == 16);
+        for (int64_t r = 0; sep && r < np_sub; ++r)
+            for (int64_t c = 0; c < np_sub; ++c) {
+                const int64_t q = r * np_sub + c;
+                if (mw[q] != 1.0 || xw[q] != xw[c] || yw[q] != yw[r * np_sub]) { sep = false; break; }
+            }
+        if (sep) {
+            std::vector<int32_t> order(n_sub), sorted(n_sub);
+            for (int s = 0; s < n_sub; ++s) order[s] = s;
+            std::stable_sort(order.begin(), order.end(),
+                             [&](int32_t a, int32_t b) { return block_origin[a] < block_origin[b]; });
+            bool aligned = (frame_w % 4 == 0) && (np_sub % 4 == 0);
+            for (int s = 0; s < n_sub; ++s) {
+                sorted[s] = block_origin[order[s]];
+                if (sorted[s] % 4 != 0) aligned = false;
+            }
+            std::vector<double> sepw(2 * np_sub);
+            for (int c = 0; c < np_sub; ++c) sepw[c] = xw[c];
+            for (int r = 0; r < np_sub; ++r) sepw[np_sub + r] = yw[(int64_t)r * np_sub];
+            if ((e = hipMalloc((void**)&p->d_sorted, n_sub * sizeof(int32_t))) != hipSuccess ||
+                (e = hipMalloc((void**)&p->d_order, n_sub * sizeof(int32_t))) != hipSuccess ||
+                (e = hipMalloc((void**)&p->d_sep, sepw.size() * sizeof(double))) != hipSuccess ||
+                (e = hipMemcpy(p->d_sorted, sorted.data(), n_sub * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(p->d_order, order.data(), n_sub * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess ||
+                (e = hipMemcpy(p->d_sep, sepw.data(), sepw.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+                return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
+            p->rows_path = true;
+            p->vec4 = aligned;
+        }
     } else {
         if ((e = hipMalloc((void**)&p->d_idx, n_sub * npp * sizeof(int32_t))) != hipSuccess ||
             (e = hipMemcpy(p->d_idx, idx, n_sub * npp * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
@@ -804,6 +925,9 @@ int shslopec_plan_destroy(shslopec_plan* p) {
     (void)hipFree(p->d_idx);
     (void)hipFree(p->d_w);
     (void)hipFree(p->d_mom);
+    (void)hipFree(p->d_sorted);
+    (void)hipFree(p->d_order);
+    (void)hipFree(p->d_sep);
     delete p;
     return SHWFS_OK;
 }
@@ -825,11 +949,26 @@ int shslopec_forward(shslopec_plan* p, int n_frames, const float* pixels, int64_
     const double* xw = p->d_w;
     const double* yw = p->d_w + npp;
     const double* mw = p->d_w + 2 * npp;
-    constexpr int kSubPerBlock = 4;
-    dim3 grid((p->n_sub + kSubPerBlock - 1) / kSubPerBlock, n_frames);
-    hipLaunchKernelGGL(cog_moments_kernel, grid, dim3(kSubPerBlock * kWave), 0, st, pixels, pix_stride,
-                       p->d_origin, p->d_idx, p->n_sub, p->np_sub, p->frame_w, xw, yw, mw, thr,
-                       thr_mode, p->d_mom);
+    if (p->rows_path) {
+        const int spb = 4 * (64 / p->np_sub);
+        dim3 grid((p->n_sub + spb - 1) / spb, n_frames);
+        const double* xv = p->d_sep;
+        const double* yv = p->d_sep + p->np_sub;
+#define COG_ROWS(NPV, V4) hipLaunchKernelGGL((cog_rows_kernel<NPV, V4>), grid, dim3(256), 0, st, pixels, pix_stride, \
+                                             p->d_sorted, p->d_order, p->n_sub, p->frame_w, xv, yv, thr, thr_mode, p->d_mom)
+        switch (p->np_sub) {
+            case 4: if (p->vec4) COG_ROWS(4, true); else COG_ROWS(4, false); break;
+            case 8: if (p->vec4) COG_ROWS(8, true); else COG_ROWS(8, false); break;
+            default: if (p->vec4) COG_ROWS(16, true); else COG_ROWS(16, false); break;
+        }
+#undef COG_ROWS
+    } else {
+        constexpr int kSubPerBlock = 4;
+        dim3 grid((p->n_sub + kSubPerBlock - 1) / kSubPerBlock, n_frames);
+        hipLaunchKernelGGL(cog_moments_kernel, grid, dim3(kSubPerBlock * kWave), 0, st, pixels, pix_stride,
+                           p->d_origin, p->d_idx, p->n_sub, p->np_sub, p->frame_w, xw, yw, mw, thr,
+                           thr_mode, p->d_mom);
+    }
     SHWFS_LAUNCH_CHECK();
     hipLaunchKernelGGL(cog_finalize_kernel, dim3(n_frames), dim3(1024), 0, st, p->d_mom, p->n_sub,
                        slopes, slopes_stride, flux, totals);
