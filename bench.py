@@ -181,7 +181,7 @@ def main():
         'fft_fp32_peak_frac': round(fft_flop * F / (kern_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-                     'kernel': 'sh_ngs_forward_kernel<8,16,32>',
+                     'kernel': 'sh_ngs_mfma_kernel<8>',
                      'bytes_per_launch': bytes_per_frame * F},
     }
     if args.cpu_seconds > 0:

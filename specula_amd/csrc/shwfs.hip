@@ -91,6 +91,15 @@ __device__ __forceinline__ float2 field_sample(float a, float ph, double k, doub
     return make_float2(a * c, a * s);
 }
 
+// Same sample with the reduction done in revolutions: r = psi/(2 pi) - rint(psi/(2 pi)) in FP64,
+// then the hardware v_sin_f32 / v_cos_f32 (which take revolutions) on |r| <= 1/2.
+__device__ __forceinline__ float2 field_sample_rev(float a, float ph, double k_rev, double tilt_rev, double ax_ay) {
+    double rv = (double)ph * k_rev - tilt_rev * ax_ay;
+    rv -= rint(rv);
+    const float rf = (float)rv;
+    return make_float2(a * __builtin_amdgcn_cosf(rf), a * __builtin_amdgcn_sinf(rf));
+}
+
 // ---------------------------------------------------------------- amplitude statistics
 // Per frame: sum(A) (masked area, electric_field.py:87-89) and sum(A^2), FP64, deterministic.
 // Last-block-done pattern: every block writes a partial, the last one to arrive reduces the
@@ -323,9 +332,10 @@ __global__ void __launch_bounds__(256)
 sh_ngs_mfma_kernel(NgsArgs g) {
     constexpr int M = 16;
     constexpr int SW = 64 / NP;
-    constexpr int XS = 8;
+    constexpr int RS = 20;             // words per (q, ky) row: 16 used + 4 pad (conflict-free b128 rows)
+    constexpr int QS = M * RS + 16;    // words per subaperture: +16 rotates banks between subapertures
     static_assert(NP <= 8, "M = 16 path needs np_sub <= 8");
-    __shared__ float4 lds_all[4][SW * M * XS / 2];
+    __shared__ float4 lds_all[4][SW * QS / 4];
     __shared__ double red[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     float2* g1 = reinterpret_cast<float2*>(lds_all[w]);
@@ -344,10 +354,11 @@ sh_ngs_mfma_kernel(NgsArgs g) {
             const float* ar = g.A + f * g.a_stride + off;
             const float* pr = g.phase + f * g.ph_stride + off;
             const double ax = g.a0 + x;
+            const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
 #pragma unroll
             for (int y = 0; y < NP; ++y)
-                e[y] = field_sample(ar[(int64_t)y * g.ef_size], pr[(int64_t)y * g.ef_size], g.k, g.tilt,
-                                    ax + g.a0 + y);
+                e[y] = field_sample_rev(ar[(int64_t)y * g.ef_size], pr[(int64_t)y * g.ef_size], k_rev, tilt_rev,
+                                        ax + g.a0 + y);
         } else {
 #pragma unroll
             for (int y = 0; y < NP; ++y) e[y] = make_float2(0.f, 0.f);
@@ -356,18 +367,19 @@ sh_ngs_mfma_kernel(NgsArgs g) {
         fft_pruned<M, NP, false>(e, G);
         if (q < SW) {
 #pragma unroll
-            for (int ky = 0; ky < M; ++ky) g1[(q * M + ky) * XS + x] = G[ky];
+            for (int ky = 0; ky < M; ++ky) g1[(q * QS + ky * RS) / 2 + x] = G[ky];
         }
     }
     __syncthreads();
 
     for (int r = lane; r < SW * M; r += 64) {   // phase B: row transforms and power spectrum
+        const int rowoff = (r / M) * QS + (r % M) * RS;
         float2 row[NP];
 #pragma unroll
-        for (int x = 0; x < NP; ++x) row[x] = g1[r * XS + x];
+        for (int x = 0; x < NP; ++x) row[x] = g1[rowoff / 2 + x];
         float2 G[M];
         fft_pruned<M, NP, false>(row, G);
-        float4* srow = reinterpret_cast<float4*>(sS + r * M);
+        float4* srow = reinterpret_cast<float4*>(sS + rowoff);
 #pragma unroll
         for (int c = 0; c < M / 4; ++c)
             srow[c] = make_float4(G[4 * c].x * G[4 * c].x + G[4 * c].y * G[4 * c].y,
@@ -391,7 +403,7 @@ sh_ngs_mfma_kernel(NgsArgs g) {
     for (int q = 0; q < SW; ++q) {
         const int sub = sub0 + q;
         if (sub >= g.n_sub) break;                       // wave-uniform
-        const floatx4 sv = *reinterpret_cast<const floatx4*>(sS + (q * M + i16) * M + 4 * g4);
+        const floatx4 sv = *reinterpret_cast<const floatx4*>(sS + q * QS + i16 * RS + 4 * g4);
         floatx4 t = {0.f, 0.f, 0.f, 0.f};
         t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[0], qv[0], t, 0, 0, 0);
         t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[1], qv[1], t, 0, 0, 0);
@@ -404,10 +416,10 @@ sh_ngs_mfma_kernel(NgsArgs g) {
         b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[3], t[3], b, 0, 0, 0);
         // stage the 16 x 16 tile over this subaperture's (already consumed) spectrum in LDS;
         // rows/columns >= npx are exactly zero (zero-padded operators)
-        float* bt = sS + q * (M * M);
+        float* bt = sS + q * QS;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            bt[(4 * g4 + r) * M + i16] = b[r];
+            bt[(4 * g4 + r) * RS + i16] = b[r];
             if (!g.parseval && 4 * g4 + r < g.npx && i16 < g.npx) bsum += (double)b[r];
         }
     }
@@ -415,19 +427,18 @@ sh_ngs_mfma_kernel(NgsArgs g) {
     // coalesced stores: consecutive lanes walk X, then subapertures, then CCD rows
     const int nq = min(SW, g.n_sub - sub0);
     float* ccdf = g.ccd + f * g.ccd_stride;
-    if (g.npx == 16 && nq == SW) {
-        constexpr int PER4 = SW * 16 * 4;                 // float4 groups per wave
-        for (int e = lane; e < PER4; e += 64) {
-            const int x4 = e & 3, t2 = e >> 2;
-            const int q = t2 % SW, Y = t2 / SW;
-            const int sub = sub0 + q;
-            const int i = sub / n, j = sub - (sub / n) * n;
-            const float4 v = *reinterpret_cast<const float4*>(sS + q * (M * M) + Y * M + 4 * x4);
-            const float sc = (float)scale;
-            *reinterpret_cast<float4*>(ccdf + (int64_t)(i * 16 + Y) * W + j * 16 + 4 * x4) =
-                g.parseval ? make_float4((float)(v.x * scale), (float)(v.y * scale), (float)(v.z * scale),
-                                         (float)(v.w * scale))
-                           : make_float4(v.x * sc, v.y * sc, v.z * sc, v.w * sc);
+    if (64 % (4 * SW) == 0 && g.npx == 16 && nq == SW) {
+        // lane -> fixed (subaperture q, float4 column x4); CCD rows Y = lane/32 + 2t
+        const int x4 = lane & 3, q = (lane >> 2) % SW;
+        const int sub = sub0 + q;
+        const int i = sub / n, j = sub - (sub / n) * n;
+        float* base = ccdf + (int64_t)(i * 16) * W + j * 16 + 4 * x4;
+        const float* src = sS + q * QS + 4 * x4;
+        constexpr int YSTEP = 64 / (4 * SW);
+        for (int Y = (lane >> 2) / SW; Y < 16; Y += YSTEP) {
+            const float4 v = *reinterpret_cast<const float4*>(src + Y * RS);
+            *reinterpret_cast<float4*>(base + (int64_t)Y * W) =
+                make_float4((float)(v.x * scale), (float)(v.y * scale), (float)(v.z * scale), (float)(v.w * scale));
         }
     } else if (nq > 0) {
         const int per = nq * g.npx * g.npx;
@@ -437,7 +448,7 @@ sh_ngs_mfma_kernel(NgsArgs g) {
             const int sub = sub0 + q;
             const int i = sub / n, j = sub - (sub / n) * n;
             ccdf[(int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X] =
-                (float)((double)sS[q * (M * M) + Y * M + X] * scale);
+                (float)((double)sS[q * QS + Y * RS + X] * scale);
         }
     }
     if (!g.parseval) {
