@@ -177,6 +177,7 @@ struct NgsArgs {
     const double* amp;   // [frame][2]: sum(A), sum(A^2)     (frame index f * amp_fstep)
     int amp_fstep;
     double* bpart;       // [frame][n_tiles]: sum of unscaled B   (non-Parseval mode)
+    const float2* kspec; // LGS: [n_sub][L][L] kernel spectra on the lag grid (L = 2 np - 1)
     int n_lenses, ef_size, npx, npx_pad, n_sub, n_tiles, parseval;
     double k, tilt, a0, S0, pixel_area, norm_const;
 };
@@ -457,6 +458,169 @@ sh_ngs_mfma_kernel(NgsArgs g) {
     }
 }
 
+// ---------------------------------------------------------------- LGS forward (sh.py:381-394)
+// psf = Re IFFT_N(FFT_N(|FFT_N(E)|^2) * K).  With D[m] = (N^2/M^2) FFT_M(S)[m mod M] K[m] on the
+// lag grid |m| <= np-1 (S = |FFT_M(E)|^2, M >= 2np-1), the binned image is B = Re(P D Q^T) where
+// P, Q (npx x L, complex) already carry N^2/M^2, the rebin, the cut and the mask.
+// One block = TS subapertures, 256 threads:
+//   1: task (s, x)   column FFT over y of the field                         -> buf[s][ky][x]
+//   2: task (s, ky)  pruned FFT over x, S = |.|^2, forward FFT over kx of S  -> buf[s][lx][ky] (L lags)
+//   3: task (s, lx)  forward FFT over ky, D = F * K_s, V[Y] = sum_my P[Y][my] D[my]
+//   4: task (s, Y, half)  B[Y][X] = Re sum_lx V[Y][lx] Q[X][lx]
+template <int NP, int M, int TS>
+__global__ void __launch_bounds__(256)
+sh_lgs_forward_kernel(NgsArgs g) {
+    constexpr int L = 2 * NP - 1;
+    static_assert(L <= M, "lag grid must fit the spectrum grid");
+    extern __shared__ float4 lds_raw[];
+    float2* buf = reinterpret_cast<float2*>(lds_raw);
+    __shared__ double red[4];
+    const int t = threadIdx.x;
+    const int f = blockIdx.y;
+    const int n = g.n_lenses;
+    const int sub0 = blockIdx.x * TS;
+    const float2* Pc = reinterpret_cast<const float2*>(g.roww);   // npx_pad x L complex
+    const float2* Qc = reinterpret_cast<const float2*>(g.colw);
+
+    // phase 1
+    for (int task = t; task < TS * NP; task += 256) {
+        const int s = task / NP, x = task - (task / NP) * NP;
+        const int sub = sub0 + s;
+        float2 e[NP];
+        if (sub < g.n_sub) {
+            const int i = sub / n, j = sub - (sub / n) * n;
+            const int64_t off = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x;
+            const float* ar = g.A + f * g.a_stride + off;
+            const float* pr = g.phase + f * g.ph_stride + off;
+            const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
+            const double ax = g.a0 + x;
+#pragma unroll
+            for (int y = 0; y < NP; ++y)
+                e[y] = field_sample_rev(ar[(int64_t)y * g.ef_size], pr[(int64_t)y * g.ef_size], k_rev, tilt_rev,
+                                        ax + g.a0 + y);
+        } else {
+#pragma unroll
+            for (int y = 0; y < NP; ++y) e[y] = make_float2(0.f, 0.f);
+        }
+        float2 G[M];
+        fft_pruned<M, NP, false>(e, G);
+#pragma unroll
+        for (int ky = 0; ky < M; ++ky) buf[(s * M + ky) * NP + x] = G[ky];
+    }
+    __syncthreads();
+
+    // phase 2 (read rows into registers, then overwrite the buffer with lag rows)
+    constexpr int T2 = (TS * M + 255) / 256;
+    float2 R[T2][L];
+#pragma unroll
+    for (int h = 0; h < T2; ++h) {
+        const int task = t + 256 * h;
+        if (task < TS * M) {
+            const int s = task / M, ky = task - (task / M) * M;
+            float2 row[NP];
+#pragma unroll
+            for (int x = 0; x < NP; ++x) row[x] = buf[(s * M + ky) * NP + x];
+            float2 G[M];
+            fft_pruned<M, NP, false>(row, G);
+#pragma unroll
+            for (int kx = 0; kx < M; ++kx) G[kx] = make_float2(G[kx].x * G[kx].x + G[kx].y * G[kx].y, 0.f);
+            fft_radix2<M, false>(G);
+#pragma unroll
+            for (int li = 0; li < L; ++li) R[h][li] = G[(li - (NP - 1) + M) % M];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < T2; ++h) {
+        const int task = t + 256 * h;
+        if (task < TS * M) {
+            const int s = task / M, ky = task - (task / M) * M;
+#pragma unroll
+            for (int li = 0; li < L; ++li) buf[(s * L + li) * M + ky] = R[h][li];
+        }
+    }
+    __syncthreads();
+
+    // phase 3: column transforms over ky, kernel multiply, left binning operator
+    constexpr int T3 = (TS * L + 255) / 256;
+    float2 V[T3][16];
+#pragma unroll
+    for (int h = 0; h < T3; ++h) {
+        const int task = t + 256 * h;
+#pragma unroll
+        for (int Y = 0; Y < 16; ++Y) V[h][Y] = make_float2(0.f, 0.f);
+        if (task < TS * L) {
+            const int s = task / L, li = task - (task / L) * L;
+            const int sub = sub0 + s;
+            float2 F[M];
+#pragma unroll
+            for (int ky = 0; ky < M; ++ky) F[ky] = buf[(s * L + li) * M + ky];
+            fft_radix2<M, false>(F);
+            if (sub < g.n_sub) {
+                const float2* kr = g.kspec + (int64_t)sub * L * L + li;
+#pragma unroll
+                for (int mi = 0; mi < L; ++mi) {
+                    const float2 d = cmul(F[(mi - (NP - 1) + M) % M], kr[mi * L]);
+#pragma unroll
+                    for (int Y = 0; Y < 16; ++Y) {          // rows >= npx of P are zero
+                        const float2 pc = Pc[Y * L + mi];
+                        V[h][Y].x = fmaf(pc.x, d.x, fmaf(-pc.y, d.y, V[h][Y].x));
+                        V[h][Y].y = fmaf(pc.x, d.y, fmaf(pc.y, d.x, V[h][Y].y));
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < T3; ++h) {
+        const int task = t + 256 * h;
+        if (task < TS * L) {
+            const int s = task / L, li = task - (task / L) * L;
+#pragma unroll
+            for (int Y = 0; Y < 16; ++Y) buf[(s * 16 + Y) * L + li] = V[h][Y];
+        }
+    }
+    __syncthreads();
+
+    // phase 4: right binning operator, real part, scale, store
+    double scale = 1.0;
+    if (g.parseval) {
+        const double* am = g.amp + (int64_t)f * g.amp_fstep;
+        scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
+    }
+    const int W = n * g.npx;
+    const int halfw = (g.npx + 1) / 2;
+    double bsum = 0.0;
+    for (int task = t; task < TS * g.npx * 2; task += 256) {
+        const int s = task / (2 * g.npx), rem = task - s * 2 * g.npx;
+        const int Y = rem >> 1, hx = rem & 1;
+        const int sub = sub0 + s;
+        if (sub >= g.n_sub) continue;
+        const int i = sub / n, j = sub - (sub / n) * n;
+        const float2* vrow = buf + (s * 16 + Y) * L;
+        float2 v[L];
+#pragma unroll
+        for (int li = 0; li < L; ++li) v[li] = vrow[li];
+        float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx;
+        const int x0 = hx * halfw, x1 = min(g.npx, x0 + halfw);
+        for (int X = x0; X < x1; ++X) {
+            float acc = 0.f;
+#pragma unroll
+            for (int li = 0; li < L; ++li) {
+                const float2 qc = Qc[X * L + li];
+                acc = fmaf(v[li].x, qc.x, fmaf(-v[li].y, qc.y, acc));
+            }
+            bsum += (double)acc;
+            dst[X] = (float)((double)acc * scale);
+        }
+    }
+    if (!g.parseval) {
+        bsum = block_sum(bsum, red);
+        if (t == 0) g.bpart[(int64_t)f * g.n_tiles + blockIdx.x] = bsum;
+    }
+}
+
 // Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
 __global__ void sh_rescale_kernel(float* __restrict__ ccd, int64_t ccd_stride, int64_t n_px,
                                   const double* __restrict__ bpart, int n_tiles,
@@ -624,6 +788,8 @@ struct shwfs_plan {
     int amp_blocks = 0;
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
+    bool kernels_set = false;      // LGS: kernel spectra uploaded
+    int L = 0;                     // LGS lag-grid size 2 np - 1
     float* d_roww = nullptr;
     float* d_colw = nullptr;
     float* d_kspec = nullptr;
@@ -687,6 +853,26 @@ int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t s
 
 using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
 
+template <int NP, int M, int TS>
+int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
+    constexpr int L = 2 * NP - 1;
+    constexpr int e1 = M * NP, e2 = L * M, e3 = 16 * L;
+    constexpr int lds = TS * (e1 > e2 ? (e1 > e3 ? e1 : e3) : (e2 > e3 ? e2 : e3)) * (int)sizeof(float2);
+    static_assert(lds <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
+    dim3 grid(p->n_tiles, n_frames);
+    hipLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3(256), lds, st, a);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+NgsLauncher pick_lgs(int np, int M, int* ts) {
+    if (np == 16 && M == 32) { *ts = 8; return launch_lgs<16, 32, 8>; }
+    if (np == 12 && M == 32) { *ts = 8; return launch_lgs<12, 32, 8>; }
+    if (np == 8 && M == 16) { *ts = 16; return launch_lgs<8, 16, 16>; }
+    if (np == 6 && M == 16) { *ts = 16; return launch_lgs<6, 16, 16>; }
+    return nullptr;
+}
+
 // MFMA binning path: M = 16 (np_sub 5..8) and at most 16 CCD px per subaperture.
 NgsLauncher pick_ngs_mfma(int np, int M, int npx) {
     if (M != 16 || npx > 16) return nullptr;
@@ -730,14 +916,22 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         return set_error(SHWFS_ERR_ARG, "ef_size must equal n_lenses * np_sub");
     if (g.spec_size < 2 * g.np_sub - 1)
         return set_error(SHWFS_ERR_ARG, "spec_size must be >= 2*np_sub-1");
-    if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS kernel path not available in this build");
-    NgsLauncher mf = pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx);
-    NgsLauncher v1 = pick_ngs(g.np_sub, g.spec_size);
-    bool use_mfma = (g.variant == 2) || (g.variant == 0 && mf != nullptr);
-    if (use_mfma && !mf) return set_error(SHWFS_ERR_UNSUPPORTED, "MFMA variant needs spec_size 16, np_sub 5..8, subap_npx <= 16");
-    if (!use_mfma && !v1)
-        return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
     if (g.variant < 0 || g.variant > 2) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1 or 2");
+    NgsLauncher mf = nullptr, v1 = nullptr, lg = nullptr;
+    int lgs_ts = 0;
+    bool use_mfma = false;
+    if (g.lgs) {
+        lg = pick_lgs(g.np_sub, g.spec_size, &lgs_ts);
+        if (!lg) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS path instantiated for (np_sub, spec_size) in {(16,32),(12,32),(8,16),(6,16)}");
+        if (g.subap_npx > 16) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS path needs subap_npx <= 16");
+    } else {
+        mf = pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx);
+        v1 = pick_ngs(g.np_sub, g.spec_size);
+        use_mfma = (g.variant == 2) || (g.variant == 0 && mf != nullptr);
+        if (use_mfma && !mf) return set_error(SHWFS_ERR_UNSUPPORTED, "MFMA variant needs spec_size 16, np_sub 5..8, subap_npx <= 16");
+        if (!use_mfma && !v1)
+            return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
+    }
     DeviceGuard dg(device);
     auto* p = new shwfs_plan();
     p->g = g;
@@ -745,17 +939,31 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     p->n_sub = g.n_lenses * g.n_lenses;
     p->npx_pad = (g.subap_npx + 3) / 4 * 4;
     p->mfma = use_mfma;
-    p->rows_alloc = use_mfma ? 16 : p->npx_pad;
-    p->ts = use_mfma ? 4 * (64 / g.np_sub) : ngs_tile_size(g.np_sub);
-    p->n_tiles = (p->n_sub + p->ts - 1) / p->ts;
+    p->L = 2 * g.np_sub - 1;
     p->amp_blocks = 64;
     const int M = g.spec_size;
-    std::vector<float> rw((size_t)p->rows_alloc * M, 0.f), cw((size_t)p->rows_alloc * M, 0.f);
-    for (int Y = 0; Y < g.subap_npx; ++Y)
-        for (int k = 0; k < M; ++k) {
-            rw[(size_t)Y * M + k] = (float)row_w[(size_t)Y * M + k];
-            cw[(size_t)Y * M + k] = (float)col_w[(size_t)Y * M + k];
+    std::vector<float> rw, cw;
+    if (g.lgs) {
+        p->rows_alloc = 16;
+        p->ts = lgs_ts;
+        rw.assign((size_t)16 * p->L * 2, 0.f);
+        cw.assign((size_t)16 * p->L * 2, 0.f);
+        for (size_t q = 0; q < (size_t)g.subap_npx * p->L * 2; ++q) {
+            rw[q] = (float)row_w[q];
+            cw[q] = (float)col_w[q];
         }
+    } else {
+        p->rows_alloc = use_mfma ? 16 : p->npx_pad;
+        p->ts = use_mfma ? 4 * (64 / g.np_sub) : ngs_tile_size(g.np_sub);
+        rw.assign((size_t)p->rows_alloc * M, 0.f);
+        cw.assign((size_t)p->rows_alloc * M, 0.f);
+        for (int Y = 0; Y < g.subap_npx; ++Y)
+            for (int k = 0; k < M; ++k) {
+                rw[(size_t)Y * M + k] = (float)row_w[(size_t)Y * M + k];
+                cw[(size_t)Y * M + k] = (float)col_w[(size_t)Y * M + k];
+            }
+    }
+    p->n_tiles = (p->n_sub + p->ts - 1) / p->ts;
     auto cleanup = [&](int code) { shwfs_plan_destroy(p); return code; };
     hipError_t e = hipSuccess;
 #define PLAN_ALLOC(ptr, bytes)                                                                  \
@@ -767,6 +975,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
     PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
     PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_tiles * sizeof(double));
+    if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
 #undef PLAN_ALLOC
     if ((e = hipMemcpy(p->d_roww, rw.data(), rw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(p->d_colw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
@@ -799,8 +1008,13 @@ int shwfs_plan_set_timing_events(shwfs_plan* p, void* start, void* stop) {
 }
 
 int shwfs_set_kernels(shwfs_plan* p, const float* kspec, void* stream) {
-    (void)p; (void)kspec; (void)stream;
-    return set_error(SHWFS_ERR_UNSUPPORTED, "LGS kernel path not available in this build");
+    if (!p || !kspec) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (!p->g.lgs) return set_error(SHWFS_ERR_ARG, "plan was not created for the LGS path");
+    DeviceGuard dg(p->device);
+    SHWFS_HIP_CHECK(hipMemcpyAsync(p->d_kspec, kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float),
+                                   hipMemcpyDeviceToDevice, reinterpret_cast<hipStream_t>(stream)));
+    p->kernels_set = true;
+    return SHWFS_OK;
 }
 
 int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
@@ -836,7 +1050,11 @@ int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
     a.n_sub = p->n_sub; a.n_tiles = p->n_tiles; a.parseval = g.parseval;
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0;
     a.S0 = S0; a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
-    NgsLauncher L = p->mfma ? pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx) : pick_ngs(g.np_sub, g.spec_size);
+    a.kspec = reinterpret_cast<const float2*>(p->d_kspec);
+    if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan: call shwfs_set_kernels first");
+    int lts = 0;
+    NgsLauncher L = g.lgs ? pick_lgs(g.np_sub, g.spec_size, &lts)
+                          : (p->mfma ? pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx) : pick_ngs(g.np_sub, g.spec_size));
     if (p->ev_start) SHWFS_HIP_CHECK(hipEventRecord(p->ev_start, st));
     int rc = L(p, a, n_frames, st);
     if (rc) return rc;

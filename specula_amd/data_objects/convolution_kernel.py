@@ -1,0 +1,228 @@
+"""LGS elongation kernels (host, NumPy): specula/data_objects/convolution_kernel.py:14-380 and
+gaussian_convolution_kernel.py:5-107.
+
+``ConvolutionKernel.kernels`` holds, per subaperture, ifft2 of the flux-normalised geometric
+sodium-beacon image (process_kernels, convolution_kernel.py:258-274), stored with the reference's
+index transposition ``kernels[j*dimx+i] <- real_kernels[i*dimx+j]``.  The HIP SH path only needs
+these spectra on the (2 np - 1)^2 lag grid: ``lag_spectra(np_sub)`` gathers them.
+
+Kernels are recomputed when the SHA1 of the parameters changes (convolution_kernel.py:161-174,
+307-330); unlike the reference, no FITS cache file is written to the working directory.
+"""
+import hashlib
+import json
+
+import numpy as np
+
+from specula_amd import ASEC2RAD
+from specula_amd.base import BaseDataObj
+
+
+def _sample_rebin(a, shape):
+    """rebin2d(sample=True) (rebin.py:18-21): nearest sample at float(old)/new steps."""
+    slices = [slice(0, old, float(old) / new) for old, new in zip(a.shape, shape)]
+    idx = np.mgrid[slices].astype(np.int32)
+    return a[tuple(idx)]
+
+
+def lgs_map_sh(nsh, diam, rl, zb, dz, profz, fwhmb, ps, ssp, overs=2, theta=(0.0, 0.0),
+               rprof_type=0, dtype=np.float32):
+    """Geometric sodium-LGS image seen by every subaperture (convolution_kernel.py:14-108):
+    a Gaussian (or top-hat) beam cross-section, sampled along the line of sight at the layer
+    offsets ``dz`` with weights ``profz``, projected from the launcher at ``rl``.  Returns
+    (nsh*nsh, ssp, ssp), index [q*nsh + p] for subaperture row p, column q."""
+    theta = np.asarray(theta, dtype=np.float64)
+    ossp = ssp * overs
+    lin = np.linspace(-diam / 2, diam / 2, nsh, dtype=dtype)
+    fov = np.linspace(-ssp * ps / 2, ssp * ps / 2, ossp, dtype=dtype)
+    xsh, ysh = np.meshgrid(lin, lin)
+    xfov, yfov = np.meshgrid(fov, fov)
+    sigma = (fwhmb * ASEC2RAD * zb) / (2 * np.sqrt(2 * np.log(2)))
+    inv_s2 = 1.0 / sigma ** 2
+    rb = np.array([theta[0] * ASEC2RAD * zb, theta[1] * ASEC2RAD * zb, 0], dtype=dtype)
+    beam = zb * np.array([0, 0, 1], dtype=dtype) + rb - np.array(rl, dtype=dtype)
+    el = beam / beam[2]
+    fx = np.tile(xfov * ASEC2RAD * zb, (nsh, nsh)).reshape(ossp * nsh, ossp * nsh)
+    fy = np.tile(yfov * ASEC2RAD * zb, (nsh, nsh)).reshape(ossp * nsh, ossp * nsh)
+    ex = (fx - np.repeat(np.repeat(xsh, ossp, axis=0), ossp, axis=1)) / zb
+    ey = (fy - np.repeat(np.repeat(ysh, ossp, axis=0), ossp, axis=1)) / zb
+    if rprof_type == 0:
+        gnorm = 1.0 / (sigma * np.pi * np.sqrt(2.0))
+    elif rprof_type == 1:
+        gnorm = 1.0 / (np.pi / 4 * (fwhmb * ASEC2RAD * zb) ** 2)
+    else:
+        raise ValueError('Unsupported radial profile type')
+    fmap = np.zeros((nsh * ossp, nsh * ossp), dtype=dtype)
+    for z, w in zip(dz, profz):
+        if w <= 0:
+            continue
+        d2 = ((fx + z * ex - (rb[0] + z * el[0])) ** 2 +
+              (fy + z * ey - (rb[1] + z * el[1])) ** 2 +
+              (0.0 + z * 1.0 - (rb[2] + z * el[2])) ** 2)
+        if rprof_type == 0:
+            fmap += (gnorm * w) * np.exp(d2 * (-0.5 * inv_s2), dtype=dtype)
+        else:
+            fmap += (gnorm * w) * ((d2 * inv_s2) <= 1.0)
+    ccd = _sample_rebin(fmap, (ssp * nsh, ssp * nsh))
+    return ccd.reshape(nsh, ssp, nsh, ssp).transpose((2, 0, 1, 3)).reshape(nsh * nsh, ssp, ssp)
+
+
+class ConvolutionKernel(BaseDataObj):
+    """specula/data_objects/convolution_kernel.py:110-380 (host arrays)."""
+
+    def __init__(self, dimx: int, dimy: int, pxscale: float, pupil_size_m: float, dimension: int,
+                 launcher_pos: list = [0.0, 0.0, 0.0], seeing: float = 0.0, launcher_size: float = 0.0,
+                 zfocus: float = 90e3, theta: list = [0.0, 0.0], airmass: float = 1.0,
+                 oversampling: int = 1, return_fft: bool = True, positive_shift_tt: bool = True,
+                 target_device_idx: int = None, precision: int = None):
+        super().__init__(target_device_idx=target_device_idx, precision=precision)
+        if len(launcher_pos) != 3:
+            raise ValueError('Launcher position must be a three-elements vector [m]')
+        self.dimx, self.dimy = dimx, dimy
+        self.pxscale = pxscale
+        self.pupil_size_m = pupil_size_m
+        self.dimension = dimension
+        self.seeing = seeing
+        self.zlayer = None
+        self.zprofile = None
+        self.zfocus = zfocus
+        self.theta = np.array(theta, dtype=np.float64)
+        self.return_fft = return_fft
+        self.launcher_size = launcher_size
+        self.airmass = airmass
+        self.oversampling = oversampling
+        self.launcher_pos = np.array(launcher_pos, dtype=np.float64)
+        self.positive_shift_tt = positive_shift_tt
+        self.spot_size = np.sqrt(seeing ** 2 + launcher_size ** 2)
+        self.real_kernels = None
+        self.kernels = None
+        self._kernel_fn = None
+        self.version = 0          # bumped each time the kernels are recomputed
+
+    def calculate_focus(self):
+        return float(np.sum(np.array(self.zlayer) * np.array(self.zprofile)) / np.sum(self.zprofile))
+
+    def _lgs_tt(self):
+        return np.array([0.5, 0.5] if self.positive_shift_tt else [-0.5, -0.5]) * self.pxscale + self.theta
+
+    def build(self):
+        """Parameter hash (convolution_kernel.py:161-174)."""
+        if len(self.zlayer) != len(self.zprofile):
+            raise ValueError('Number of elements of zlayer and zprofile must be the same')
+        zfocus = self.zfocus if self.zfocus != -1 else self.calculate_focus()
+        lay = np.array(self.zlayer) * self.airmass
+        zfocus *= self.airmass
+        self.spot_size = np.sqrt(self.seeing ** 2 + self.launcher_size ** 2)
+        items = [self.dimx, self.pupil_size_m, self.launcher_pos, zfocus, lay, self.zprofile,
+                 self.spot_size, self.pxscale, self.dimension, self.oversampling, self._lgs_tt(),
+                 'float32' if self.precision == 1 else 'float64']
+        return 'ConvolutionKernel' + self.generate_hash(items)
+
+    @staticmethod
+    def generate_hash(items):
+        plain = []
+        for it in items:
+            if isinstance(it, np.ndarray):
+                plain.append(it.tolist())
+            elif isinstance(it, tuple):
+                plain.append([float(x) for x in it])
+            elif hasattr(it, 'item') and hasattr(it, 'dtype'):
+                plain.append(it.item())
+            else:
+                plain.append(it)
+        return hashlib.sha1(json.dumps(plain).encode('utf-8')).hexdigest()
+
+    def calculate_lgs_map(self):
+        """convolution_kernel.py:179-225."""
+        if len(self.zlayer) != len(self.zprofile):
+            raise ValueError('Number of elements of zlayer and zprofile must be the same')
+        self.spot_size = np.sqrt(self.seeing ** 2 + self.launcher_size ** 2)
+        if self.spot_size <= 0:
+            raise ValueError('Spot size must be greater than zero')
+        zfocus = self.zfocus if self.zfocus != -1 else self.calculate_focus()
+        lay = np.array(self.zlayer) * self.airmass
+        zfocus *= self.airmass
+        self.real_kernels = lgs_map_sh(self.dimx, self.pupil_size_m, self.launcher_pos, zfocus, lay - zfocus,
+                                       self.zprofile, self.spot_size, self.pxscale, self.dimension,
+                                       overs=self.oversampling, theta=self._lgs_tt())
+        self.process_kernels(return_fft=self.return_fft)
+
+    def process_kernels(self, return_fft=False):
+        """Normalise each subaperture kernel and store ifft2 of it, transposed index
+        (convolution_kernel.py:258-274)."""
+        rk = np.asarray(self.real_kernels)
+        if np.any(~np.isfinite(rk)):
+            raise ValueError('Kernel contains non-finite values!')
+        n = self.dimx
+        tot = rk.sum(axis=(1, 2), keepdims=True)
+        norm = np.where(tot > 0, rk / np.where(tot > 0, tot, 1), rk)
+        src = np.arange(n * self.dimy).reshape(n, self.dimy)          # i*dimx + j
+        dst = src.T.reshape(-1)                                         # j*dimx + i
+        out = np.empty((n * self.dimy,) + rk.shape[1:], dtype=np.complex128 if return_fft else np.float64)
+        vals = np.fft.ifft2(norm, axes=(1, 2)) if return_fft else norm
+        out[dst] = vals[src.reshape(-1)]
+        self.kernels = out
+        self.version += 1
+
+    def prepare_for_sh(self, sodium_altitude=None, sodium_intensity=None, current_time=None):
+        """Recompute only when the parameter hash changes (convolution_kernel.py:307-330)."""
+        if sodium_altitude is not None:
+            self.zlayer = sodium_altitude
+        if sodium_intensity is not None:
+            self.zprofile = sodium_intensity
+        fn = self.build()
+        if fn != self._kernel_fn:
+            self._kernel_fn = fn
+            self.calculate_lgs_map()
+        if current_time is not None:
+            self.generation_time = current_time
+
+    def lag_spectra(self, np_sub):
+        """(n_sub, L, L) complex64 with L = 2 np - 1: kernels[s][m_y mod N][m_x mod N] for
+        |m| <= np - 1, subaperture s = row*n + col."""
+        N = self.dimension
+        lags = np.arange(-(np_sub - 1), np_sub) % N
+        return np.ascontiguousarray(self.kernels[:, lags][:, :, lags].astype(np.complex64))
+
+    def kernels_normalised(self, tol=1e-6):
+        """True when every kernel has unit sum (DC of ifft2 == 1/N^2): the LGS image then
+        preserves flux and the Parseval normalisation applies."""
+        dc = self.kernels[:, 0, 0].real * self.dimension ** 2
+        return bool(np.all(np.abs(dc - 1.0) < tol))
+
+
+class GaussianConvolutionKernel(ConvolutionKernel):
+    """specula/data_objects/gaussian_convolution_kernel.py:5-57 (on-axis launcher, one layer)."""
+
+    def __init__(self, dimx: int, dimy: int, pxscale: float, dimension: int, spot_size: float,
+                 pupil_size_m: float = 0.0, oversampling: int = 1, return_fft: bool = True,
+                 positive_shift_tt: bool = True, airmass: float = 1.0, target_device_idx: int = None,
+                 precision: int = None):
+        super().__init__(dimx=dimx, dimy=dimy, pxscale=pxscale, pupil_size_m=pupil_size_m,
+                         dimension=dimension, airmass=airmass, oversampling=oversampling,
+                         return_fft=return_fft, positive_shift_tt=positive_shift_tt,
+                         target_device_idx=target_device_idx, precision=precision)
+        self.spot_size = spot_size
+
+    def build(self):
+        self.dimx = max(self.dimx, 2)
+        tt = [x * self.pxscale for x in ([0.5, 0.5] if self.positive_shift_tt else [-0.5, -0.5])]
+        items = [self.dimx, self.pupil_size_m, 90e3, self.spot_size,
+                 'float32' if self.precision == 1 else 'float64', self.pxscale, self.dimension, 3, tt,
+                 [0, 0, 0], [90e3], [1.0]]
+        return 'ConvolutionKernel' + self.generate_hash(items)
+
+    def calculate_lgs_map(self):
+        tt = [x * self.pxscale for x in ([0.5, 0.5] if self.positive_shift_tt else [-0.5, -0.5])]
+        self.real_kernels = lgs_map_sh(self.dimx, self.pupil_size_m, 0, 90e3, [0], profz=[1.0],
+                                       fwhmb=self.spot_size, ps=self.pxscale, ssp=self.dimension,
+                                       overs=1, theta=tt)
+        self.process_kernels(return_fft=self.return_fft)
+
+    def prepare_for_sh(self, sodium_altitude=None, sodium_intensity=None, current_time=None):
+        fn = self.build()
+        if fn != self._kernel_fn:
+            self._kernel_fn = fn
+            self.calculate_lgs_map()
+        if current_time is not None:
+            self.generation_time = current_time

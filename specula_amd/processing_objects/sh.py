@@ -12,6 +12,7 @@ import torch
 from specula_amd import RAD2ASEC
 from specula_amd import _native
 from specula_amd.base import BaseProcessingObj, BaseValue, InputValue
+from specula_amd.data_objects.convolution_kernel import ConvolutionKernel, GaussianConvolutionKernel
 from specula_amd.data_objects.electric_field import ElectricField
 from specula_amd.data_objects.intensity import Intensity
 from specula_amd.data_objects.lenslet import Lenslet
@@ -23,14 +24,21 @@ from specula_amd.lib.geometry import sh_geometry
 class SHPlan:
     """Owns a ``shwfs_plan`` (device copies of the binning operators and FP64 scratch)."""
 
-    def __init__(self, geom, lgs=False, max_frames=1, device=0, variant=0):
+    def __init__(self, geom, lgs=False, max_frames=1, device=0, variant=0, kernels_normalised=True):
         """variant: 0 picks the kernel automatically, 1 forces register-GEMV binning, 2 forces
-        MFMA binning (spec_size 16, np_sub 5..8, subap_npx <= 16)."""
+        MFMA binning (spec_size 16, np_sub 5..8, subap_npx <= 16).
+        lgs: per-subaperture kernel correlation; spectra come from ``set_kernels``.  With
+        ``kernels_normalised`` (unit-sum kernels) the Parseval normalisation applies."""
         row, col, M, norm = build_operators(geom, lgs=lgs)
         if lgs:
-            raise NotImplementedError('LGS plans are created by SH with kernel spectra')
+            N = geom.fft_size
+            row = row * (N * N / float(M * M))          # D = (N^2/M^2) FFT_M(S) K
+            if not kernels_normalised:
+                norm = None
         self.geom = geom
+        self.lgs = lgs
         self.M = M
+        self.L = 2 * geom.ovs_np_sub - 1
         self.parseval = norm is not None
         self.max_frames = max_frames
         self.device = device
@@ -43,7 +51,7 @@ class SHPlan:
         g.fft_size = geom.fft_size
         g.spec_size = M
         g.subap_npx = geom.subap_npx
-        g.lgs = 0
+        g.lgs = 1 if lgs else 0
         g.parseval = 1 if self.parseval else 0
         g.max_frames = max_frames
         g.variant = variant
@@ -53,14 +61,27 @@ class SHPlan:
         g.norm_const = norm if norm is not None else 0.0
         g.pixel_area = geom.pixel_pitch ** 2
         self._geom_c = g
-        rw = np.ascontiguousarray(row, dtype=np.float64)
-        cw = np.ascontiguousarray(col, dtype=np.float64)
+        if lgs:     # complex operators as interleaved (re, im)
+            rw = np.ascontiguousarray(np.stack([row.real, row.imag], axis=-1), dtype=np.float64)
+            cw = np.ascontiguousarray(np.stack([col.real, col.imag], axis=-1), dtype=np.float64)
+        else:
+            rw = np.ascontiguousarray(row, dtype=np.float64)
+            cw = np.ascontiguousarray(col, dtype=np.float64)
         h = ctypes.c_void_p()
         lib = _native.lib()
         _native.check(lib.shwfs_plan_create(ctypes.byref(g), rw.ctypes.data_as(ctypes.c_void_p),
                                             cw.ctypes.data_as(ctypes.c_void_p), device, ctypes.byref(h)),
                       'shwfs_plan_create')
         self._h = h
+
+    def set_kernels(self, kspec, stream=None):
+        """kspec: (n_sub, L, L) complex64 device tensor of kernel spectra on the lag grid."""
+        n_sub = self.geom.n_lenses ** 2
+        if (kspec.dtype != torch.complex64 or not kspec.is_cuda or not kspec.is_contiguous()
+                or tuple(kspec.shape) != (n_sub, self.L, self.L)):
+            raise ValueError(f'kernel spectra must be a contiguous complex64 ({n_sub}, {self.L}, {self.L}) tensor')
+        _native.check(_native.lib().shwfs_set_kernels(self._h, _native.ptr(kspec), _native.stream_handle(stream)),
+                      'shwfs_set_kernels')
 
     def forward(self, A, phase, out, S0, n_frames=1, a_frame_stride=None, stream=None):
         """A, phase: (n_frames,) ef x ef float32 device tensors (A may be 2-D = shared);
@@ -186,17 +207,64 @@ class SH(BaseProcessingObj):
             raise NotImplementedError(
                 f'SH geometry needs field interpolation (fov_ovs={self._fov_ovs}, rotation={rot}, '
                 f'shift={shift.tolist()}); set fov_ovs_coeff=1.0 so that fov_ovs == 1')
-        if self._laser_launch_tel is not None:
-            raise NotImplementedError('LGS (laser_launch_tel) SH path is not available in this build')
 
     def setup(self):
         super().setup()
         in_ef = self.inputs['in_ef'].get(target_device_idx=self.target_device_idx)
         self.set_in_ef(in_ef)
         self._check_supported()
-        self._plan = SHPlan(self.geometry, lgs=False, max_frames=1, device=self.target_device_idx)
         self.in_ef = in_ef
+        self._kernelobj = None
+        self._kernel_version = None
+        if self._laser_launch_tel is None:
+            self._plan = SHPlan(self.geometry, lgs=False, max_frames=1, device=self.target_device_idx)
+        else:
+            self._make_kernel_object(in_ef)
         self.build_stream(allow_parallel=False, capture=False)
+
+    def _make_kernel_object(self, in_ef):
+        """Kernel object as SH.calc_trigger_geometry builds it (sh.py:290-323)."""
+        g = self.geometry
+        llt = self._laser_launch_tel
+        common = dict(dimx=self._lenslet.dimx, dimy=self._lenslet.dimy,
+                      pxscale=g.fp4_pixel_pitch * RAD2ASEC, pupil_size_m=in_ef.pixel_pitch * in_ef.size[0],
+                      dimension=g.fft_size, oversampling=1, return_fft=True, positive_shift_tt=True,
+                      target_device_idx=-1, precision=self.precision)
+        if len(llt.tel_pos) == 0:
+            self._kernelobj = GaussianConvolutionKernel(spot_size=llt.spot_size, **common)
+        else:
+            theta = llt.beacon_tt if len(llt.beacon_tt) != 0 else []
+            self._kernelobj = ConvolutionKernel(launcher_pos=llt.tel_pos, seeing=0.0, launcher_size=llt.spot_size,
+                                                zfocus=llt.beacon_focus, theta=theta if len(theta) else [0.0, 0.0],
+                                                **common)
+
+    def prepare_trigger(self, t):
+        super().prepare_trigger(t)
+        if self._kernelobj is not None:
+            self.prepare_kernels()
+
+    def prepare_kernels(self):
+        """sh.py:330-346, then upload the lag-grid spectra when the kernels changed."""
+        if len(self._laser_launch_tel.tel_pos) != 0:
+            alt = self.local_inputs.get('sodium_altitude')
+            inten = self.local_inputs.get('sodium_intensity')
+            if alt is None or inten is None:
+                raise ValueError('sodium_altitude and sodium_intensity must be provided')
+            alt, inten = alt.value, inten.value
+            alt = alt.tolist() if hasattr(alt, 'tolist') else alt
+            inten = inten.tolist() if hasattr(inten, 'tolist') else inten
+        else:
+            alt = inten = None
+        self._kernelobj.prepare_for_sh(sodium_altitude=alt, sodium_intensity=inten, current_time=self.current_time)
+        if self._kernelobj.version != self._kernel_version:
+            normalised = self._kernelobj.kernels_normalised()
+            if self._plan is None or self._plan.parseval != normalised:
+                self._plan = SHPlan(self.geometry, lgs=True, max_frames=1, device=self.target_device_idx,
+                                    kernels_normalised=normalised)
+                self._graph_key = None
+            self._kspec = torch.tensor(self._kernelobj.lag_spectra(self.geometry.ovs_np_sub), device=self.device)
+            self._plan.set_kernels(self._kspec, stream=self.stream)
+            self._kernel_version = self._kernelobj.version
 
     # ------------------------------------------------------------------ per frame
     def _inputs_fp32(self):

@@ -79,3 +79,26 @@ def lgs_40x40():
 
 
 WORKLOADS = {w().name: w for w in (scao_ngs_20x20, elt_ngs_80x80, lgs_40x40)}
+
+
+LGS_LAYERS_M = np.linspace(80e3, 100e3, 11)
+
+
+def sodium_profile(layers=LGS_LAYERS_M, centre=90e3, fwhm=10e3):
+    """Normalised Gaussian sodium profile on the given layer altitudes."""
+    sigma = fwhm / (2 * np.sqrt(2 * np.log(2)))
+    prof = np.exp(-0.5 * ((layers - centre) / sigma) ** 2)
+    return prof / prof.sum()
+
+
+def lgs_kernel(geom, launcher_pos=(20.0, 0.0, 0.0), spot_size=1.0, zfocus=90e3):
+    """ConvolutionKernel of the lgs_40x40 workload: 11-layer Gaussian Na profile (90 km, FWHM
+    10 km) seen from a launcher 20 m off-axis, 1.0" spot, no seeing term."""
+    from specula_amd.data_objects.convolution_kernel import ConvolutionKernel
+    k = ConvolutionKernel(dimx=geom.n_lenses, dimy=geom.n_lenses, pxscale=geom.fp4_pixel_pitch * RAD2ASEC,
+                          pupil_size_m=geom.pixel_pitch * geom.ef_size, dimension=geom.fft_size,
+                          launcher_pos=list(launcher_pos), seeing=0.0, launcher_size=spot_size, zfocus=zfocus,
+                          theta=[0.0, 0.0], oversampling=1, return_fft=True, positive_shift_tt=True,
+                          target_device_idx=-1, precision=1)
+    k.prepare_for_sh(sodium_altitude=LGS_LAYERS_M.tolist(), sodium_intensity=sodium_profile().tolist())
+    return k
