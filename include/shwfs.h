@@ -108,6 +108,34 @@ int shslopec_forward(shslopec_plan* plan, int n_frames,
                      float* slopes, int64_t slopes_frame_stride,
                      double* flux, double* totals, void* stream);
 
+/* ---- adjoint (vector-Jacobian products) ------------------------------------------------
+ * No reference counterpart: the reference builds interaction matrices by push-pull
+ * (processing_objects/im_rec_calibrator.py:42-82, lib/modal_pushpull_signal.py:35-65).
+ *
+ * shslopec_vjp: dL/dpixels from dL/dslopes for the CoG of shslopec_forward (moments are
+ * recomputed; the dim-subaperture factor is held fixed).  grad_pixels is fully overwritten
+ * (zero outside the subapertures), frame stride = frame_h*frame_w. */
+int shslopec_vjp(shslopec_plan* plan, int n_frames, const float* pixels, int64_t pixel_frame_stride,
+                 float thr, int thr_mode, const float* grad_slopes, int64_t grad_slopes_stride,
+                 float* grad_pixels, void* stream);
+
+/* shwfs_vjp: dL/dphase_nm from dL/d(CCD intensity) for shwfs_forward (NGS path, Parseval
+ * normalisation).  phase_frame_stride may be 0 to evaluate several cotangents at one phase. */
+int shwfs_vjp(shwfs_plan* plan, int n_frames, const float* A, int64_t a_frame_stride,
+              const float* phase_nm, int64_t phase_frame_stride, double S0,
+              const float* grad_ccd, int64_t grad_ccd_stride, float* grad_phase, int64_t grad_phase_stride,
+              void* stream);
+
+/* Interaction matrix from the block-diagonal slope Jacobian (fov_ovs == 1):
+ *   im[m][slot]           = scale * sum_{p in subaperture} jx[p] * modes[m][p]
+ *   im[m][n_valid + slot] = scale * sum_{p in subaperture} jy[p] * modes[m][p]
+ * jx, jy: ef x ef maps (d slope / d phase_nm); modes: n_modes maps of ef x ef (stride
+ * mode_stride); slot: n_lenses^2 int32, slope index of each subaperture or -1; scale = 2
+ * reproduces the push-pull convention IM = (s(+a) - s(-a)) / a. */
+int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy,
+                           int n_modes, const float* modes, int64_t mode_stride, const int32_t* slot,
+                           int n_valid, float scale, float* im, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -16,6 +16,7 @@ EXPORTS = (
     'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events',
     'shwfs_set_kernels', 'shwfs_forward',
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
+    'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian',
 )
 
 
@@ -52,6 +53,9 @@ def _declare(lib):
         'shslopec_plan_create': (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _I, ctypes.POINTER(_P)]),
         'shslopec_plan_destroy': (_I, [_P]),
         'shslopec_forward': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
+        'shslopec_vjp': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P]),
+        'shwfs_vjp': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P, _I64, _P]),
+        'shwfs_im_from_jacobian': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

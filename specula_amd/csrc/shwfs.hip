@@ -748,7 +748,8 @@ cog_rows_kernel(const float* __restrict__ pix, int64_t pix_stride, const int32_t
 // (factor = 0 where 1/tot > 1/(1e-3 mean), sh_slopec.py:393-400), slopes and flux outputs.
 __global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
                                     float* __restrict__ slopes, int64_t slopes_stride,
-                                    double* __restrict__ flux, double* __restrict__ totals) {
+                                    double* __restrict__ flux, double* __restrict__ totals,
+                                    double* __restrict__ factor_out) {
     __shared__ double red[16];
     const int f = blockIdx.x;
     const double* m = mom + (int64_t)f * n_sub * 4;
@@ -765,13 +766,235 @@ __global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
     for (int s = threadIdx.x; s < n_sub; s += blockDim.x) {
         double factor = 1.0 / m[4 * s + 1];
         if (factor > limit) factor = 0.0;
-        sl[s] = (float)(m[4 * s + 2] * factor);
-        sl[n_sub + s] = (float)(m[4 * s + 3] * factor);
-        flux[(int64_t)f * n_sub + s] = m[4 * s];
+        factor_out[(int64_t)f * n_sub + s] = factor;
+        if (slopes) {
+            sl[s] = (float)(m[4 * s + 2] * factor);
+            sl[n_sub + s] = (float)(m[4 * s + 3] * factor);
+        }
+        if (flux) flux[(int64_t)f * n_sub + s] = m[4 * s];
     }
-    if (threadIdx.x == 0) {
+    if (totals && threadIdx.x == 0) {
         totals[2 * f] = sf;
         totals[2 * f + 1] = sf / n_sub;
+    }
+}
+
+
+// ============================================================================ adjoint (VJP)
+__global__ void fill_zero_kernel(float* __restrict__ p, int64_t n) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        p[q] = 0.f;
+}
+
+// CoG backward (sh_slopec.py:374-404 differentiated with the dim-subaperture factor held fixed):
+//   dL/dp = factor * (gx (xw - sx mw) + gy (yw - sy mw)) where the thresholded pixel is alive.
+// One wavefront per subaperture; pixels outside every subaperture keep the zero fill.
+__global__ void cog_vjp_kernel(const float* __restrict__ pix, int64_t pix_stride,
+                               const int32_t* __restrict__ origin, const int32_t* __restrict__ idx,
+                               int n_sub, int np_sub, int frame_w,
+                               const double* __restrict__ xw, const double* __restrict__ yw,
+                               const double* __restrict__ mw, float thr, int thr_mode,
+                               const double* __restrict__ mom, const double* __restrict__ factor,
+                               const float* __restrict__ gsl, int64_t gsl_stride,
+                               float* __restrict__ gpix, int64_t gpix_stride) {
+    const int lane = threadIdx.x % kWave;
+    const int sub = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int f = blockIdx.y;
+    if (sub >= n_sub) return;
+    const double fac = factor[(int64_t)f * n_sub + sub];
+    const double* m = mom + ((int64_t)f * n_sub + sub) * 4;
+    const double sx = m[2] * fac, sy = m[3] * fac;
+    const double gx = gsl[f * gsl_stride + sub], gy = gsl[f * gsl_stride + n_sub + sub];
+    const int npp = np_sub * np_sub;
+    for (int q = lane; q < npp; q += kWave) {
+        int64_t addr;
+        if (origin) {
+            const int r = q / np_sub, c = q - (q / np_sub) * np_sub;
+            addr = (int64_t)origin[sub] + (int64_t)r * frame_w + c;
+        } else {
+            addr = idx[(int64_t)sub * npp + q];
+        }
+        const float v = pix[f * pix_stride + addr];
+        const bool alive = (thr_mode == 1) ? !(v < thr) : (v - thr > 0.f);
+        const double gp = alive ? fac * (gx * (xw[q] - sx * mw[q]) + gy * (yw[q] - sy * mw[q])) : 0.0;
+        gpix[f * gpix_stride + addr] = (float)gp;
+    }
+}
+
+struct VjpArgs {
+    const float* A;
+    int64_t a_stride;
+    const float* phase;
+    int64_t ph_stride;
+    const float* gccd;     // dL/d(CCD intensity), per frame
+    int64_t gccd_stride;
+    float* gphase;         // dL/d(phase [nm]), per frame
+    int64_t gph_stride;
+    const float* roww;     // P (rows_alloc x M)
+    const float* colw;     // Q
+    const double* amp;
+    int amp_fstep;
+    int n_lenses, ef_size, npx, n_sub;
+    double k, tilt, a0, S0, pixel_area, norm_const;
+};
+
+// SH backward for the NGS path with Parseval normalisation (I = c B, c independent of the
+// phase): with G_B = c dL/dI on the subaperture tile,
+//   G_S = P^T G_B Q,   X = G_S conj(F),   H = FFT_M(X) on the np x np support,
+//   dL/dphi = -2 (2 pi / lambda) Im(H E).
+template <int NP, int M, int TS>
+__global__ void __launch_bounds__(TS * NP)
+sh_ngs_vjp_kernel(VjpArgs g) {
+    extern __shared__ float4 lds_raw[];
+    constexpr int G1S = M * NP + 1;
+    constexpr int NT = TS * NP;
+    constexpr int CPT = (TS * M + NT - 1) / NT;
+    float2* buf = reinterpret_cast<float2*>(lds_raw);
+    float* gb = reinterpret_cast<float*>(buf + TS * G1S);
+    const int npx2 = g.npx * g.npx;
+    const int t = threadIdx.x, f = blockIdx.y, tile = blockIdx.x, n = g.n_lenses;
+    const double* am = g.amp + (int64_t)f * g.amp_fstep;
+    const float c = (float)(g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]));
+
+    for (int e = t; e < TS * npx2; e += NT) {         // G_B tiles
+        const int s = e / npx2, rem = e - s * npx2;
+        const int Y = rem / g.npx, X = rem - Y * g.npx;
+        const int sub = tile * TS + s;
+        float v = 0.f;
+        if (sub < g.n_sub) {
+            const int i = sub / n, j = sub - (sub / n) * n;
+            v = g.gccd[f * g.gccd_stride + (int64_t)(i * g.npx + Y) * (n * g.npx) + j * g.npx + X] * c;
+        }
+        gb[s * npx2 + rem] = v;
+    }
+    // row pass: field rows (kept for the last step) and their pruned transforms
+    const int s_r = t % TS, y_r = t / TS;
+    const int sub_r = tile * TS + s_r;
+    float2 e[NP];
+    int64_t row_off = 0;
+    if (sub_r < g.n_sub) {
+        const int i = sub_r / n, j = sub_r - (sub_r / n) * n;
+        row_off = (int64_t)(i * NP + y_r) * g.ef_size + (int64_t)j * NP;
+        const float* ar = g.A + f * g.a_stride + row_off;
+        const float* pr = g.phase + f * g.ph_stride + row_off;
+        const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
+#pragma unroll
+        for (int x = 0; x < NP; ++x)
+            e[x] = field_sample_rev(ar[x], pr[x], k_rev, tilt_rev, g.a0 + x + g.a0 + y_r);
+    } else {
+#pragma unroll
+        for (int x = 0; x < NP; ++x) e[x] = make_float2(0.f, 0.f);
+    }
+    {
+        float2 r[M];
+        fft_pruned<M, NP, false>(e, r);
+#pragma unroll
+        for (int kx = 0; kx < M; ++kx) buf[s_r * G1S + kx * NP + y_r] = r[kx];
+    }
+    __syncthreads();
+    // column pass: F, G_S, X, transform over ky (first NP outputs kept)
+    float2 Z[CPT][NP];
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int cidx = t + q * NT;
+#pragma unroll
+        for (int y = 0; y < NP; ++y) Z[q][y] = make_float2(0.f, 0.f);
+        if (cidx < TS * M) {
+            const int s = cidx % TS, kx = cidx / TS;
+            float2 col[NP];
+#pragma unroll
+            for (int y = 0; y < NP; ++y) col[y] = buf[s * G1S + kx * NP + y];
+            float2 F[M];
+            fft_pruned<M, NP, false>(col, F);
+            float gs[M];
+#pragma unroll
+            for (int ky = 0; ky < M; ++ky) gs[ky] = 0.f;
+            const float* gbs = gb + s * npx2;
+            for (int Y = 0; Y < g.npx; ++Y) {
+                float u = 0.f;
+                for (int X = 0; X < g.npx; ++X) u = fmaf(gbs[Y * g.npx + X], g.colw[X * M + kx], u);
+#pragma unroll
+                for (int ky = 0; ky < M; ++ky) gs[ky] = fmaf(g.roww[Y * M + ky], u, gs[ky]);
+            }
+            float2 X[M];
+#pragma unroll
+            for (int ky = 0; ky < M; ++ky) X[ky] = make_float2(gs[ky] * F[ky].x, -gs[ky] * F[ky].y);
+            fft_radix2<M, false>(X);
+#pragma unroll
+            for (int y = 0; y < NP; ++y) Z[q][y] = X[y];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+        const int cidx = t + q * NT;
+        if (cidx < TS * M) {
+            const int s = cidx % TS, kx = cidx / TS;
+#pragma unroll
+            for (int y = 0; y < NP; ++y) buf[s * G1S + y * M + kx] = Z[q][y];
+        }
+    }
+    __syncthreads();
+    // row pass back: transform over kx, phase gradient
+    if (sub_r < g.n_sub) {
+        float2 H[M];
+#pragma unroll
+        for (int kx = 0; kx < M; ++kx) H[kx] = buf[s_r * G1S + y_r * M + kx];
+        fft_radix2<M, false>(H);
+        float* gr = g.gphase + f * g.gph_stride + row_off;
+        const float two_k = (float)(2.0 * g.k);
+#pragma unroll
+        for (int x = 0; x < NP; ++x) {
+            const float im = H[x].x * e[x].y + H[x].y * e[x].x;     // Im(H E)
+            gr[x] = -two_k * im;
+        }
+    }
+}
+
+// Interaction matrix from the block-diagonal Jacobian: with fov_ovs == 1 each phase pixel feeds
+// one subaperture, so two VJPs (cotangent 1 on every x slope, then on every y slope) give
+// Jx, Jy (pupil maps, slope per nm) and IM[m][slot] = scale * sum_{p in subap} J[p] mode_m[p]
+// (scale 2 reproduces the push-pull convention (s(+a) - s(-a))/a of im_rec_calibrator.py:58-63).
+// Block = one strip of NP pupil rows x a chunk of modes; one thread per pupil column.
+template <int NP>
+__global__ void im_from_jacobian_kernel(const float* __restrict__ jx, const float* __restrict__ jy, int ef,
+                                        int n_lenses, const float* __restrict__ modes, int64_t mode_stride,
+                                        int n_modes, int modes_per_block, const int32_t* __restrict__ slot,
+                                        int n_valid, float scale, float* __restrict__ im) {
+    const int i = blockIdx.x;
+    const int col = threadIdx.x;
+    const bool active = col < ef;
+    float jxr[NP], jyr[NP];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+        const int64_t o = (int64_t)(i * NP + r) * ef + col;
+        jxr[r] = active ? jx[o] : 0.f;
+        jyr[r] = active ? jy[o] : 0.f;
+    }
+    const int j = col / NP, xin = col - j * NP;
+    const int sl = (active && xin == 0) ? slot[i * n_lenses + j] : -1;
+    const int m0 = blockIdx.y * modes_per_block;
+    const int m1 = min(n_modes, m0 + modes_per_block);
+    for (int m = m0; m < m1; ++m) {
+        float px = 0.f, py = 0.f;
+        if (active) {
+            const float* md = modes + m * mode_stride + (int64_t)(i * NP) * ef + col;
+#pragma unroll
+            for (int r = 0; r < NP; ++r) {
+                const float v = md[(int64_t)r * ef];
+                px = fmaf(jxr[r], v, px);
+                py = fmaf(jyr[r], v, py);
+            }
+        }
+#pragma unroll
+        for (int off = NP / 2; off > 0; off >>= 1) {
+            px += __shfl_xor(px, off, kWave);
+            py += __shfl_xor(py, off, kWave);
+        }
+        if (sl >= 0) {
+            im[(int64_t)m * 2 * n_valid + sl] = scale * px;
+            im[(int64_t)m * 2 * n_valid + n_valid + sl] = scale * py;
+        }
     }
 }
 
@@ -813,6 +1036,7 @@ struct shslopec_plan {
     int32_t* d_idx = nullptr;
     double* d_w = nullptr;     // xw | yw | mw
     double* d_mom = nullptr;   // [max_frames][n_sub][4]
+    double* d_factor = nullptr;  // [max_frames][n_sub]: 1/tot or 0 (dim-subaperture cut-off)
 };
 
 namespace {
@@ -1102,6 +1326,7 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
     memcpy(w.data() + 2 * npp, mw, npp * sizeof(double));
     if ((e = hipMalloc((void**)&p->d_w, w.size() * sizeof(double))) != hipSuccess ||
         (e = hipMalloc((void**)&p->d_mom, (size_t)max_frames * n_sub * 4 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void**)&p->d_factor, (size_t)max_frames * n_sub * sizeof(double))) != hipSuccess ||
         (e = hipMemcpy(p->d_w, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
     if (block_origin) {
@@ -1154,6 +1379,7 @@ int shslopec_plan_destroy(shslopec_plan* p) {
     (void)hipFree(p->d_idx);
     (void)hipFree(p->d_w);
     (void)hipFree(p->d_mom);
+    (void)hipFree(p->d_factor);
     (void)hipFree(p->d_sorted);
     (void)hipFree(p->d_order);
     (void)hipFree(p->d_sep);
@@ -1200,7 +1426,121 @@ int shslopec_forward(shslopec_plan* p, int n_frames, const float* pixels, int64_
     }
     SHWFS_LAUNCH_CHECK();
     hipLaunchKernelGGL(cog_finalize_kernel, dim3(n_frames), dim3(1024), 0, st, p->d_mom, p->n_sub,
-                       slopes, slopes_stride, flux, totals);
+                       slopes, slopes_stride, flux, totals, p->d_factor);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+}  // extern "C"
+
+namespace {
+template <int NP, int M, int TS>
+int launch_vjp(const VjpArgs& a, int n_tiles, int n_frames, int npx, hipStream_t st) {
+    const int lds = TS * (M * NP + 1) * (int)sizeof(float2) + TS * npx * npx * (int)sizeof(float);
+    if (lds > 150 * 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP tile exceeds the LDS budget");
+    hipLaunchKernelGGL((sh_ngs_vjp_kernel<NP, M, TS>), dim3(n_tiles, n_frames), dim3(TS * NP), lds, st, a);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+using VjpLauncher = int (*)(const VjpArgs&, int, int, int, hipStream_t);
+VjpLauncher pick_vjp(int np, int M, int* ts) {
+    if (np == 8 && M == 16) { *ts = 32; return launch_vjp<8, 16, 32>; }
+    if (np == 6 && M == 16) { *ts = 42; return launch_vjp<6, 16, 42>; }
+    if (np == 4 && M == 8) { *ts = 64; return launch_vjp<4, 8, 64>; }
+    if (np == 16 && M == 32) { *ts = 16; return launch_vjp<16, 32, 16>; }
+    return nullptr;
+}
+}  // namespace
+
+extern "C" {
+
+int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase,
+              int64_t ph_stride, double S0, const float* grad_ccd, int64_t gccd_stride, float* grad_phase,
+              int64_t gph_stride, void* stream) {
+    if (!p || !A || !phase || !grad_ccd || !grad_phase) return set_error(SHWFS_ERR_ARG, "null argument");
+    const shwfs_geom& g = p->g;
+    if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the NGS path");
+    if (!g.parseval)
+        return set_error(SHWFS_ERR_UNSUPPORTED, "VJP needs the Parseval normalisation (no FoV cut, all-ones mask)");
+    if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
+    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
+    const int64_t ccd_px = (int64_t)g.n_lenses * g.subap_npx * g.n_lenses * g.subap_npx;
+    if (n_frames > 1 && ((a_stride != 0 && a_stride < ef_px) || (ph_stride != 0 && ph_stride < ef_px) ||
+                         gccd_stride < ccd_px || gph_stride < ef_px))
+        return set_error(SHWFS_ERR_ARG, "frame stride too small");
+    int ts = 0;
+    VjpLauncher L = pick_vjp(g.np_sub, g.spec_size, &ts);
+    if (!L) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP instantiated for (np_sub, spec_size) in {(4,8),(6,16),(8,16),(16,32)}");
+    DeviceGuard dg(p->device);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int amp_frames = (a_stride == 0) ? 1 : n_frames;
+    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
+                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp);
+    SHWFS_LAUNCH_CHECK();
+    VjpArgs a{};
+    a.A = A; a.a_stride = a_stride; a.phase = phase; a.ph_stride = ph_stride;
+    a.gccd = grad_ccd; a.gccd_stride = gccd_stride; a.gphase = grad_phase; a.gph_stride = gph_stride;
+    a.roww = p->d_roww; a.colw = p->d_colw; a.amp = p->d_amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
+    a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.n_sub = p->n_sub;
+    a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0; a.pixel_area = g.pixel_area;
+    a.norm_const = g.norm_const;
+    const int n_tiles = (p->n_sub + ts - 1) / ts;
+    return L(a, n_tiles, n_frames, g.subap_npx, st);
+}
+
+int shslopec_vjp(shslopec_plan* p, int n_frames, const float* pixels, int64_t pix_stride, float thr, int thr_mode,
+                 const float* grad_slopes, int64_t gsl_stride, float* grad_pixels, void* stream) {
+    if (!p || !pixels || !grad_slopes || !grad_pixels) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_frames <= 0 || n_frames > p->max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
+    if (thr_mode != 0 && thr_mode != 1) return set_error(SHWFS_ERR_ARG, "thr_mode must be 0 or 1");
+    const int64_t frame = (int64_t)p->frame_h * p->frame_w;
+    if (n_frames > 1 && (pix_stride < frame || gsl_stride < 2 * (int64_t)p->n_sub))
+        return set_error(SHWFS_ERR_ARG, "frame stride too small");
+    DeviceGuard dg(p->device);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int npp = p->np_sub * p->np_sub;
+    const double* xw = p->d_w;
+    const double* yw = p->d_w + npp;
+    const double* mw = p->d_w + 2 * npp;
+    constexpr int kSubPerBlock = 4;
+    dim3 grid((p->n_sub + kSubPerBlock - 1) / kSubPerBlock, n_frames);
+    hipLaunchKernelGGL(cog_moments_kernel, grid, dim3(kSubPerBlock * kWave), 0, st, pixels, pix_stride,
+                       p->d_origin, p->d_idx, p->n_sub, p->np_sub, p->frame_w, xw, yw, mw, thr, thr_mode, p->d_mom);
+    SHWFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cog_finalize_kernel, dim3(n_frames), dim3(1024), 0, st, p->d_mom, p->n_sub,
+                       (float*)nullptr, (int64_t)0, (double*)nullptr, (double*)nullptr, p->d_factor);
+    SHWFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(256), dim3(256), 0, st, grad_pixels,
+                       (int64_t)(n_frames - 1) * frame + frame);
+    SHWFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cog_vjp_kernel, grid, dim3(kSubPerBlock * kWave), 0, st, pixels, pix_stride, p->d_origin,
+                       p->d_idx, p->n_sub, p->np_sub, p->frame_w, xw, yw, mw, thr, thr_mode, p->d_mom,
+                       p->d_factor, grad_slopes, gsl_stride, grad_pixels, frame);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
+                           const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float scale,
+                           float* im, void* stream) {
+    if (!jx || !jy || !modes || !slot || !im) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_lenses <= 0 || np_sub <= 0 || n_modes <= 0 || n_valid <= 0 || ef_size != n_lenses * np_sub)
+        return set_error(SHWFS_ERR_ARG, "inconsistent sizes");
+    if (ef_size > 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "pupil wider than 1024 px");
+    if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int threads = (ef_size + 63) / 64 * 64;
+    const int mpb = 8;
+    dim3 grid(n_lenses, (n_modes + mpb - 1) / mpb);
+#define IMJ(NPV) hipLaunchKernelGGL(im_from_jacobian_kernel<NPV>, grid, dim3(threads), 0, st, jx, jy, ef_size, \
+                                    n_lenses, modes, mode_stride, n_modes, mpb, slot, n_valid, scale, im)
+    switch (np_sub) {
+        case 4: IMJ(4); break;
+        case 8: IMJ(8); break;
+        case 16: IMJ(16); break;
+        default: return set_error(SHWFS_ERR_UNSUPPORTED, "interaction matrix kernel instantiated for np_sub 4, 8, 16");
+    }
+#undef IMJ
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }

@@ -1,0 +1,161 @@
+"""SHSensor: a batched SH + ShSlopec pipeline on preallocated device buffers, with its adjoint.
+
+This is the array-level API under the drop-in processing objects: many frames per launch
+(push-pull fields, sequential screens), vector-Jacobian products of the slopes with respect to
+the phase, and interaction matrices built from them.
+
+Slopes follow SubapData order (column-major subaperture blocks, sh_subap_calibrator.py:56-78),
+[sx..., sy...] (slopes.py:27-32).  Phases are in nm (ElectricField.phaseInNm).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from specula_amd import _native
+from specula_amd.lib.subaps import detect_subaps
+from specula_amd.processing_objects.sh import SHPlan
+from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weights
+
+
+class SHSensor:
+    def __init__(self, geom, pupil, idxs=None, energy_th=0.5, max_frames=1, device=0, thr=0.0, variant=0):
+        self.geom = geom
+        self.device = torch.device(f'cuda:{device}')
+        self.max_frames = max_frames
+        self.plan = SHPlan(geom, max_frames=max_frames, device=device, variant=variant)
+        ef, ccd = geom.ef_size, geom.ccd_side
+        self.A = torch.as_tensor(np.asarray(pupil, dtype=np.float32), device=self.device).contiguous()
+        self.thr = float(thr)
+        if idxs is None:
+            flat = torch.zeros((1, ef, ef), dtype=torch.float32, device=self.device)
+            img = torch.empty((1, ccd, ccd), dtype=torch.float32, device=self.device)
+            self.plan.forward(self.A, flat, img, 1.0, n_frames=1)
+            idxs, dmap = detect_subaps(img[0].cpu().numpy(), geom.n_lenses, energy_th)
+            self.display_map = dmap
+        self.idxs = np.asarray(idxs, dtype=np.int64)
+        self.n_sub = self.idxs.shape[0]
+        self.weights = compute_xy_weights(geom.subap_npx)
+        self.slopec = SlopecPlan(self.idxs, (ccd, ccd), self.weights, max_frames=max_frames, device=device)
+        self.ccd = torch.empty((max_frames, ccd, ccd), dtype=torch.float32, device=self.device)
+        self.slopes_buf = torch.empty((max_frames, 2 * self.n_sub), dtype=torch.float32, device=self.device)
+        self.flux = torch.empty((max_frames, self.n_sub), dtype=torch.float64, device=self.device)
+        self.totals = torch.empty((max_frames, 2), dtype=torch.float64, device=self.device)
+        self._gccd = None
+        self._slot = None
+
+    # ------------------------------------------------------------------ forward
+    def forward(self, phase, S0=1.0, stream=None):
+        """phase: (F, ef, ef) float32 device tensor -> slopes view (F, 2 n_sub) (buffers reused)."""
+        F = phase.shape[0]
+        if F > self.max_frames:
+            raise ValueError(f'{F} frames > max_frames {self.max_frames}')
+        self.plan.forward(self.A, phase, self.ccd[:F], S0, n_frames=F, stream=stream)
+        self.slopec.forward(self.ccd[:F], self.slopes_buf[:F], self.flux[:F], self.totals[:F], thr=self.thr,
+                            n_frames=F, stream=stream)
+        return self.slopes_buf[:F]
+
+    # ------------------------------------------------------------------ adjoint
+    def vjp(self, phase, grad_slopes, stream=None):
+        """dL/dphase for cotangents grad_slopes (C, 2 n_sub) at ``phase`` (ef, ef) or (C, ef, ef)."""
+        ef, ccd = self.geom.ef_size, self.geom.ccd_side
+        C = grad_slopes.shape[0]
+        if C > self.max_frames:
+            raise ValueError(f'{C} cotangents > max_frames {self.max_frames}')
+        shared = phase.dim() == 2 or phase.shape[0] == 1
+        ph = phase.reshape(-1, ef, ef).contiguous()
+        ph_fwd = ph[:1].expand(C, ef, ef).contiguous() if shared else ph
+        self.forward(ph_fwd, stream=stream)                 # pixels of the current state
+        if self._gccd is None or self._gccd.shape[0] < C:
+            self._gccd = torch.empty((self.max_frames, ccd, ccd), dtype=torch.float32, device=self.device)
+        gs = grad_slopes.to(torch.float32).contiguous()
+        gph = torch.empty((C, ef, ef), dtype=torch.float32, device=self.device)
+        lib = _native.lib()
+        st = _native.stream_handle(stream)
+        _native.check(lib.shslopec_vjp(self.slopec._h, C, _native.ptr(self.ccd), ccd * ccd, self.thr, 0,
+                                       _native.ptr(gs), 2 * self.n_sub, _native.ptr(self._gccd), st),
+                      'shslopec_vjp')
+        _native.check(lib.shwfs_vjp(self.plan._h, C, _native.ptr(self.A), 0, _native.ptr(ph),
+                                    0 if shared else ef * ef, 1.0, _native.ptr(self._gccd), ccd * ccd,
+                                    _native.ptr(gph), ef * ef, st), 'shwfs_vjp')
+        return gph
+
+    def jacobian_maps(self, phase, stream=None):
+        """(Jx, Jy): ef x ef maps of d slope_x / d phase and d slope_y / d phase of each pixel's own
+        subaperture (block-diagonal Jacobian, two VJPs)."""
+        g = torch.zeros((2, 2 * self.n_sub), dtype=torch.float32, device=self.device)
+        g[0, :self.n_sub] = 1.0
+        g[1, self.n_sub:] = 1.0
+        gph = self.vjp(phase, g, stream=stream)
+        return gph[0], gph[1]
+
+    def slot_map(self):
+        if self._slot is None:
+            n, npx, W = self.geom.n_lenses, self.geom.subap_npx, self.geom.ccd_side
+            slot = -np.ones(n * n, dtype=np.int32)
+            org = self.idxs[:, 0]
+            i, j = (org // W) // npx, (org % W) // npx
+            slot[i * n + j] = np.arange(self.n_sub, dtype=np.int32)
+            self._slot = torch.tensor(slot, device=self.device)
+        return self._slot
+
+    def interaction_matrix(self, phase, modes, scale=2.0, jac=None, stream=None):
+        """IM (K, 2 n_sub) = scale * J . modes, push-pull convention for scale = 2."""
+        jx, jy = jac if jac is not None else self.jacobian_maps(phase, stream=stream)
+        K = modes.shape[0]
+        ef = self.geom.ef_size
+        im = torch.empty((K, 2 * self.n_sub), dtype=torch.float32, device=self.device)
+        modes = modes.contiguous()
+        _native.check(_native.lib().shwfs_im_from_jacobian(
+            self.geom.n_lenses, self.geom.ovs_np_sub, ef, _native.ptr(jx), _native.ptr(jy), K,
+            _native.ptr(modes), ef * ef, _native.ptr(self.slot_map()), self.n_sub, float(scale),
+            _native.ptr(im), _native.stream_handle(stream)), 'shwfs_im_from_jacobian')
+        return im
+
+    def pushpull_interaction_matrix(self, phase, modes, amp, chunk=None, stream=None):
+        """Reference convention (im_rec_calibrator.py:58-63): IM[m] = (s(phi + a m) - s(phi - a m)) / a,
+        evaluated in batches of 2*chunk sensor frames."""
+        K = modes.shape[0]
+        chunk = chunk or max(1, self.max_frames // 2)
+        ef = self.geom.ef_size
+        im = torch.empty((K, 2 * self.n_sub), dtype=torch.float32, device=self.device)
+        buf = torch.empty((2 * chunk, ef, ef), dtype=torch.float32, device=self.device)
+        ph = phase.reshape(1, ef, ef)
+        for m0 in range(0, K, chunk):
+            m1 = min(K, m0 + chunk)
+            c = m1 - m0
+            torch.add(ph, modes[m0:m1], alpha=amp, out=buf[:c])
+            torch.add(ph, modes[m0:m1], alpha=-amp, out=buf[c:2 * c])
+            s = self.forward(buf[:2 * c], stream=stream)
+            torch.sub(s[:c], s[c:2 * c], out=im[m0:m1])
+            im[m0:m1] /= amp
+        return im
+
+
+class SHSlopes(torch.autograd.Function):
+    """slopes = SHSensor.forward(phase) with the hand-written HIP adjoint as backward.
+
+    phase: (ef, ef) or (F, ef, ef) float32 device tensor [nm]; returns (F, 2 n_sub)."""
+
+    @staticmethod
+    def forward(ctx, phase, sensor):
+        ph = phase.reshape(-1, sensor.geom.ef_size, sensor.geom.ef_size).to(torch.float32).contiguous()
+        ctx.sensor = sensor
+        ctx.save_for_backward(ph)
+        ctx.in_shape = phase.shape
+        return sensor.forward(ph).clone()
+
+    @staticmethod
+    def backward(ctx, grad_slopes):
+        (ph,) = ctx.saved_tensors
+        sensor = ctx.sensor
+        F = ph.shape[0]
+        if F == 1:
+            g = sensor.vjp(ph[0], grad_slopes.reshape(1, -1))
+        else:
+            g = torch.cat([sensor.vjp(ph[f], grad_slopes[f:f + 1]) for f in range(F)])
+        return g.reshape(ctx.in_shape), None
+
+
+def sh_slopes(phase, sensor):
+    return SHSlopes.apply(phase, sensor)
