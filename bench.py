@@ -29,6 +29,8 @@ def parse():
     ap.add_argument('--frames', type=int, default=100)
     ap.add_argument('--cpu-seconds', type=float, default=15.0,
                     help='budget for the CPU baseline sample (0 disables it)')
+    ap.add_argument('--extras', type=int, default=1,
+                    help='1: also time the LGS sensor and the 1000-mode interaction matrix')
     return ap.parse_args()
 
 
@@ -82,6 +84,64 @@ def hip_elapsed_ms(a, b):
     if h.hipEventSynchronize(b) != 0 or h.hipEventElapsedTime(ctypes.byref(ms), a, b) != 0:
         raise RuntimeError('hipEventElapsedTime failed')
     return ms.value
+
+
+def timed(fn, reps, sync):
+    fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / reps
+
+
+def bench_lgs(torch, dev):
+    """lgs_40x40: 40x40 subapertures of 16 px, N=128, elongated Na kernels; 20-frame batches."""
+    from specula_amd import workloads
+    from specula_amd.processing_objects.sh import SHPlan
+    wl = workloads.lgs_40x40()
+    g = wl.geometry()
+    kern = workloads.lgs_kernel(g)
+    F = 20
+    plan = SHPlan(g, lgs=True, max_frames=F, device=0, kernels_normalised=kern.kernels_normalised())
+    plan.set_kernels(torch.tensor(kern.lag_spectra(g.ovs_np_sub), device=dev))
+    A = torch.tensor(wl.pupil().astype(np.float32), device=dev)
+    ph = torch.tensor(wl.frames(F).astype(np.float32), device=dev)
+    out = torch.empty((F, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
+    dt = timed(lambda: plan.forward(A, ph, out, 1.0, n_frames=F), 10, torch.cuda.synchronize)
+    kbytes = g.n_lenses ** 2 * (2 * g.ovs_np_sub - 1) ** 2 * 8
+    return {'workload': 'lgs_40x40', 'frames_per_s': round(F / dt, 1), 'ms_per_frame': round(dt * 1e3 / F, 5),
+            'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
+
+
+def bench_im(torch, dev, n_modes=1000):
+    """Interaction matrix of n_modes smooth modes on elt_ngs_80x80: two VJPs + the Jacobian
+    contraction, against batched push-pull (2 n_modes sensor frames) on the same GPU."""
+    from specula_amd import workloads
+    from specula_amd.lib.modes import fourier_modes
+    from specula_amd.sensor import SHSensor
+    wl = workloads.elt_ngs_80x80()
+    sen = SHSensor(wl.geometry(), wl.pupil(), max_frames=200, device=0)
+    ph = torch.tensor(wl.frames(1)[0].astype(np.float32), device=dev)
+    modes = fourier_modes(wl.pupil(), n_modes, device=dev)
+    sync = torch.cuda.synchronize
+    box = {}
+
+    def adj():
+        box['adj'] = sen.interaction_matrix(ph, modes)
+
+    def pp():
+        box['pp'] = sen.pushpull_interaction_matrix(ph, modes, amp=50.0, chunk=100)
+
+    t_adj = timed(adj, 5, sync)
+    t_pp = timed(pp, 2, sync)
+    rel = float((box['adj'] - box['pp']).norm() / box['pp'].norm())
+    pp5 = sen.pushpull_interaction_matrix(ph, modes, amp=5.0, chunk=100)
+    rel5 = float((box['adj'] - pp5).norm() / pp5.norm())
+    return {'workload': 'elt_ngs_80x80', 'modes': n_modes, 'slopes': 2 * sen.n_sub,
+            'adjoint_ms': round(t_adj * 1e3, 3), 'pushpull_ms': round(t_pp * 1e3, 3),
+            'speedup': round(t_pp / t_adj, 2), 'rel_diff_pushpull_50nm': rel, 'rel_diff_pushpull_5nm': rel5}
 
 
 def _cpu_model():
@@ -184,6 +244,9 @@ def main():
                      'kernel': 'sh_ngs_mfma_kernel<8>',
                      'bytes_per_launch': bytes_per_frame * F},
     }
+    if args.extras:
+        result['lgs'] = bench_lgs(torch, dev)
+        result['interaction_matrix'] = bench_im(torch, dev)
     if args.cpu_seconds > 0:
         result['cpu_baseline'] = cpu_baseline(wl, A, frames, idxs, args.cpu_seconds)
     print(json.dumps(result), flush=True)
