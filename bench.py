@@ -144,6 +144,35 @@ def bench_im(torch, dev, n_modes=1000):
             'speedup': round(t_pp / t_adj, 2), 'rel_diff_pushpull_50nm': rel, 'rel_diff_pushpull_5nm': rel5}
 
 
+def bench_multi(torch, dev, reps=20):
+    """Uneven batch: 6 LGS 40x40 (N 128), NGS 10x10 / 20x20 (N 32) and 80x80 (N 64), plus 2000
+    push-pull fields of the 80x80 sensor, in one persistent SH launch + CoG launches captured in one
+    HIP graph; grid/order from the autotuner."""
+    from specula_amd import autotune, workloads
+    entries = workloads.multi_sensor_entries(n_lgs=6, n_pushpull=2000)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    batch, rep = autotune.tune_batch(entries, grids=[0, 2 * cus, 8 * cus], orders=(0, 2), reps=3, use_cache=False)
+    batch.capture()
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.replay()
+        lat.append(batch.latencies_us())       # synchronises
+    dt = (time.perf_counter() - t0) / reps
+    lat = np.array(lat)
+    frames = sum(e.n_frames for e in entries)
+    subaps = sum(e.n_frames * e.plan.geom.n_lenses ** 2 for e in entries)
+    per = {e.name: {'p50_us': round(float(np.percentile(lat[:, k], 50)), 1),
+                    'p99_us': round(float(np.percentile(lat[:, k], 99)), 1)} for k, e in enumerate(entries)}
+    return {'sensors': len(entries), 'frames_per_replay': frames, 'ms_per_replay': round(dt * 1e3, 3),
+            'frames_per_s': round(frames / dt, 1), 'subap_ffts_per_s': round(subaps / dt, 1),
+            'autotune': {'grid': rep['grid'], 'order': rep['order'],
+                         'candidates_ms': {f"{c['grid']}/{c['order']}": round(c['seconds'] * 1e3, 3)
+                                           for c in rep['candidates']}},
+            'forward_completion_us': per}
+
+
 def _cpu_model():
     try:
         for line in open('/proc/cpuinfo'):
@@ -247,6 +276,7 @@ def main():
     if args.extras:
         result['lgs'] = bench_lgs(torch, dev)
         result['interaction_matrix'] = bench_im(torch, dev)
+        result['multi_sensor'] = bench_multi(torch, dev)
     if args.cpu_seconds > 0:
         result['cpu_baseline'] = cpu_baseline(wl, A, frames, idxs, args.cpu_seconds)
     print(json.dumps(result), flush=True)

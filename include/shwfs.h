@@ -136,6 +136,25 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
                            int n_modes, const float* modes, int64_t mode_stride, const int32_t* slot,
                            int n_valid, float scale, float* im, void* stream);
 
+/* ---- multi-sensor batch -----------------------------------------------------------------
+ * No reference counterpart (the reference triggers one SH object at a time, each optionally a
+ * captured CUDA graph, base_processing_obj.py:114-155).  One persistent launch runs every
+ * (sensor, frame, subaperture tile) of several Parseval-normalised sensors of uneven shapes
+ * (NGS MFMA np 8 plans and LGS np 16 / N 128 plans).  All buffers are fixed at creation (graph
+ * friendly): per sensor A (ef^2, static), phase (n_frames x ef^2), ccd (n_frames x ccd^2), S0.
+ * grid_blocks: persistent blocks (0: one per work item); order: 0 heaviest tiles first,
+ * 1 sensor by sensor, 2 round robin over sensors. */
+typedef struct shwfs_batch shwfs_batch;
+int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n_frames,
+                       const float* const* A, const float* const* phase, float* const* ccd,
+                       const double* S0, int grid_blocks, int order, int device, shwfs_batch** out);
+int shwfs_batch_forward(shwfs_batch* batch, void* stream);
+/* Per-sensor completion time [us] of the last shwfs_batch_forward, from its start (synchronises
+ * the stream). */
+int shwfs_batch_latencies(shwfs_batch* batch, double* us, void* stream);
+int shwfs_batch_info(shwfs_batch* batch, int32_t* n_items, int32_t* grid);
+int shwfs_batch_destroy(shwfs_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

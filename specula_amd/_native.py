@@ -17,6 +17,8 @@ EXPORTS = (
     'shwfs_set_kernels', 'shwfs_forward',
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian',
+    'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
+    'shwfs_batch_destroy',
 )
 
 
@@ -56,6 +58,11 @@ def _declare(lib):
         'shslopec_vjp': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P]),
         'shwfs_vjp': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P, _I64, _P]),
         'shwfs_im_from_jacobian': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P]),
+        'shwfs_batch_create': (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_P)]),
+        'shwfs_batch_forward': (_I, [_P, _P]),
+        'shwfs_batch_latencies': (_I, [_P, _P, _P]),
+        'shwfs_batch_info': (_I, [_P, _P, _P]),
+        'shwfs_batch_destroy': (_I, [_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

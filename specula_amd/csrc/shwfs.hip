@@ -329,20 +329,27 @@ sh_ngs_forward_kernel(NgsArgs g) {
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 template <int NP>
-__global__ void __launch_bounds__(256)
-sh_ngs_mfma_kernel(NgsArgs g) {
-    constexpr int M = 16;
-    constexpr int SW = 64 / NP;
-    constexpr int RS = 20;             // words per (q, ky) row: 16 used + 4 pad (conflict-free b128 rows)
-    constexpr int QS = M * RS + 16;    // words per subaperture: +16 rotates banks between subapertures
+struct NgsMfmaShape {
+    static constexpr int M = 16;
+    static constexpr int SW = 64 / NP;
+    static constexpr int RS = 20;             // words per (q, ky) row: 16 used + 4 pad (conflict-free b128 rows)
+    static constexpr int QS = M * RS + 16;    // words per subaperture: +16 rotates banks between subapertures
+    static constexpr int LDS_F4 = 4 * SW * QS / 4;   // float4 per block (4 waves)
+};
+
+// One tile = 4 waves x SW subapertures of frame f.  `lds` holds NgsMfmaShape<NP>::LDS_F4 float4.
+template <int NP>
+__device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f, float4* lds, double* red) {
+    using Sh = NgsMfmaShape<NP>;
+    constexpr int M = Sh::M;
+    constexpr int SW = Sh::SW;
+    constexpr int RS = Sh::RS;
+    constexpr int QS = Sh::QS;
     static_assert(NP <= 8, "M = 16 path needs np_sub <= 8");
-    __shared__ float4 lds_all[4][SW * QS / 4];
-    __shared__ double red[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    float2* g1 = reinterpret_cast<float2*>(lds_all[w]);
-    float* sS = reinterpret_cast<float*>(lds_all[w]);
-    const int f = blockIdx.y;
-    const int sub0 = (blockIdx.x * 4 + w) * SW;
+    float2* g1 = reinterpret_cast<float2*>(lds + w * (SW * QS / 4));
+    float* sS = reinterpret_cast<float*>(lds + w * (SW * QS / 4));
+    const int sub0 = (tile * 4 + w) * SW;
     const int n = g.n_lenses;
 
     {   // phase A: column transforms
@@ -454,8 +461,16 @@ sh_ngs_mfma_kernel(NgsArgs g) {
     }
     if (!g.parseval) {
         bsum = block_sum(bsum, red);
-        if (threadIdx.x == 0) g.bpart[(int64_t)f * g.n_tiles + blockIdx.x] = bsum;
+        if (threadIdx.x == 0) g.bpart[(int64_t)f * g.n_tiles + tile] = bsum;
     }
+}
+
+template <int NP>
+__global__ void __launch_bounds__(256)
+sh_ngs_mfma_kernel(NgsArgs g) {
+    __shared__ float4 lds_all[NgsMfmaShape<NP>::LDS_F4];
+    __shared__ double red[4];
+    ngs_mfma_tile<NP>(g, blockIdx.x, blockIdx.y, lds_all, red);
 }
 
 // ---------------------------------------------------------------- LGS forward (sh.py:381-394)
@@ -468,17 +483,19 @@ sh_ngs_mfma_kernel(NgsArgs g) {
 //   3: task (s, lx)  forward FFT over ky, D = F * K_s, V[Y] = sum_my P[Y][my] D[my]
 //   4: task (s, Y, half)  B[Y][X] = Re sum_lx V[Y][lx] Q[X][lx]
 template <int NP, int M, int TS>
-__global__ void __launch_bounds__(256)
-sh_lgs_forward_kernel(NgsArgs g) {
+struct LgsShape {
+    static constexpr int L = 2 * NP - 1;
+    static constexpr int E1 = M * NP, E2 = L * M, E3 = 16 * L;
+    static constexpr int LDS_F2 = TS * (E1 > E2 ? (E1 > E3 ? E1 : E3) : (E2 > E3 ? E2 : E3));
+};
+
+template <int NP, int M, int TS>
+__device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, float2* buf, double* red) {
     constexpr int L = 2 * NP - 1;
     static_assert(L <= M, "lag grid must fit the spectrum grid");
-    extern __shared__ float4 lds_raw[];
-    float2* buf = reinterpret_cast<float2*>(lds_raw);
-    __shared__ double red[4];
     const int t = threadIdx.x;
-    const int f = blockIdx.y;
     const int n = g.n_lenses;
-    const int sub0 = blockIdx.x * TS;
+    const int sub0 = tile * TS;
     const float2* Pc = reinterpret_cast<const float2*>(g.roww);   // npx_pad x L complex
     const float2* Qc = reinterpret_cast<const float2*>(g.colw);
 
@@ -617,8 +634,16 @@ sh_lgs_forward_kernel(NgsArgs g) {
     }
     if (!g.parseval) {
         bsum = block_sum(bsum, red);
-        if (t == 0) g.bpart[(int64_t)f * g.n_tiles + blockIdx.x] = bsum;
+        if (t == 0) g.bpart[(int64_t)f * g.n_tiles + tile] = bsum;
     }
+}
+
+template <int NP, int M, int TS>
+__global__ void __launch_bounds__(256)
+sh_lgs_forward_kernel(NgsArgs g) {
+    extern __shared__ float4 lds_raw[];
+    __shared__ double red[4];
+    lgs_tile<NP, M, TS>(g, blockIdx.x, blockIdx.y, reinterpret_cast<float2*>(lds_raw), red);
 }
 
 // Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
@@ -998,6 +1023,43 @@ __global__ void im_from_jacobian_kernel(const float* __restrict__ jx, const floa
     }
 }
 
+
+// ---------------------------------------------------------------- multi-sensor persistent batch
+// One launch processes a work list of (sensor, frame, tile) items of uneven shapes: NGS tiles
+// (np 8, MFMA binning, 32 subapertures) and LGS tiles (np 16, N 128, 8 subapertures).  Blocks
+// stride over the list; LDS is sized for the larger body.  Per-sensor completion times are
+// recorded with the constant-rate wall clock for latency statistics.
+struct BatchItem {
+    int32_t sensor, frame, tile, kind;   // kind 0: NGS MFMA np 8, 1: LGS np 16 / M 32
+};
+
+constexpr int kBatchLdsBytes =
+    (NgsMfmaShape<8>::LDS_F4 * 16 > LgsShape<16, 32, 8>::LDS_F2 * 8) ? NgsMfmaShape<8>::LDS_F4 * 16
+                                                                        : LgsShape<16, 32, 8>::LDS_F2 * 8;
+
+__global__ void batch_stamp_kernel(unsigned long long* __restrict__ ts, int n_sensors) {
+    if (threadIdx.x == 0) ts[0] = (unsigned long long)wall_clock64();
+    for (int s = threadIdx.x; s < n_sensors; s += blockDim.x) ts[1 + s] = 0ull;
+}
+
+__global__ void __launch_bounds__(256)
+sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict__ items, int n_items,
+                unsigned long long* __restrict__ ts) {
+    extern __shared__ float4 lds_raw[];
+    __shared__ double red[4];
+    for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+        __syncthreads();                               // LDS of the previous item is free
+        const BatchItem w = items[it];
+        const NgsArgs& g = sensors[w.sensor];
+        if (w.kind == 0)
+            ngs_mfma_tile<8>(g, w.tile, w.frame, lds_raw, red);
+        else
+            lgs_tile<16, 32, 8>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(&ts[1 + w.sensor], (unsigned long long)wall_clock64());
+    }
+}
+
 }  // namespace
 
 // ============================================================================ plans
@@ -1079,9 +1141,7 @@ using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
 
 template <int NP, int M, int TS>
 int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
-    constexpr int L = 2 * NP - 1;
-    constexpr int e1 = M * NP, e2 = L * M, e3 = 16 * L;
-    constexpr int lds = TS * (e1 > e2 ? (e1 > e3 ? e1 : e3) : (e2 > e3 ? e2 : e3)) * (int)sizeof(float2);
+    constexpr int lds = LgsShape<NP, M, TS>::LDS_F2 * (int)sizeof(float2);
     static_assert(lds <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
     dim3 grid(p->n_tiles, n_frames);
     hipLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3(256), lds, st, a);
@@ -1542,6 +1602,155 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
     }
 #undef IMJ
     SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+}  // extern "C"
+
+struct shwfs_batch {
+    int device = 0;
+    int n_sensors = 0;
+    int n_items = 0;
+    int grid = 0;
+    NgsArgs* d_args = nullptr;
+    BatchItem* d_items = nullptr;
+    double* d_amp = nullptr;          // [n_sensors][2]
+    double* d_amp_part = nullptr;
+    unsigned* d_counter = nullptr;
+    unsigned long long* d_ts = nullptr;   // [1 + n_sensors]
+};
+
+extern "C" {
+
+int shwfs_batch_destroy(shwfs_batch* b) {
+    if (!b) return SHWFS_OK;
+    DeviceGuard dg(b->device);
+    (void)hipFree(b->d_args);
+    (void)hipFree(b->d_items);
+    (void)hipFree(b->d_amp);
+    (void)hipFree(b->d_amp_part);
+    (void)hipFree(b->d_counter);
+    (void)hipFree(b->d_ts);
+    delete b;
+    return SHWFS_OK;
+}
+
+int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n_frames,
+                       const float* const* A, const float* const* phase, float* const* ccd, const double* S0,
+                       int grid_blocks, int order, int device, shwfs_batch** out) {
+    if (n_sensors <= 0 || !plans || !n_frames || !A || !phase || !ccd || !S0 || !out)
+        return set_error(SHWFS_ERR_ARG, "null argument");
+    if (order < 0 || order > 2) return set_error(SHWFS_ERR_ARG, "order must be 0, 1 or 2");
+    std::vector<NgsArgs> args(n_sensors);
+    std::vector<BatchItem> items;
+    std::vector<int> cost(n_sensors);
+    for (int s = 0; s < n_sensors; ++s) {
+        const shwfs_plan* p = plans[s];
+        if (!p || !A[s] || !phase[s] || !ccd[s]) return set_error(SHWFS_ERR_ARG, "null sensor entry");
+        if (p->device != device) return set_error(SHWFS_ERR_ARG, "plan on another device");
+        const shwfs_geom& g = p->g;
+        int kind;
+        if (!g.lgs && p->mfma && g.np_sub == 8 && g.spec_size == 16) kind = 0;
+        else if (g.lgs && g.np_sub == 16 && g.spec_size == 32) kind = 1;
+        else return set_error(SHWFS_ERR_UNSUPPORTED, "batch supports NGS MFMA np 8 and LGS np 16 plans");
+        if (!g.parseval) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs Parseval-normalised sensors");
+        if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan without kernels");
+        if (n_frames[s] <= 0) return set_error(SHWFS_ERR_ARG, "non-positive frame count");
+        const int ts = (kind == 0) ? 4 * NgsMfmaShape<8>::SW : 8;
+        const int tiles = (p->n_sub + ts - 1) / ts;
+        cost[s] = kind == 0 ? 1 : 4;
+        NgsArgs& a = args[s];
+        a = NgsArgs{};
+        a.A = A[s]; a.a_stride = 0; a.phase = phase[s]; a.ph_stride = (int64_t)g.ef_size * g.ef_size;
+        a.ccd = ccd[s];
+        a.ccd_stride = (int64_t)g.n_lenses * g.subap_npx * g.n_lenses * g.subap_npx;
+        a.roww = p->d_roww; a.colw = p->d_colw; a.amp_fstep = 0; a.bpart = nullptr;
+        a.kspec = reinterpret_cast<const float2*>(p->d_kspec);
+        a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
+        a.n_sub = p->n_sub; a.n_tiles = tiles; a.parseval = 1;
+        a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0[s];
+        a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
+        for (int f = 0; f < n_frames[s]; ++f)
+            for (int t = 0; t < tiles; ++t) items.push_back(BatchItem{s, f, t, kind});
+    }
+    if (order == 0)
+        std::stable_sort(items.begin(), items.end(),
+                         [&](const BatchItem& x, const BatchItem& y) { return cost[x.sensor] > cost[y.sensor]; });
+    else if (order == 2) {   // round robin across sensors
+        std::vector<std::vector<BatchItem>> per(n_sensors);
+        for (auto& it : items) per[it.sensor].push_back(it);
+        items.clear();
+        for (size_t r = 0;; ++r) {
+            bool any = false;
+            for (int s = 0; s < n_sensors; ++s)
+                if (r < per[s].size()) { items.push_back(per[s][r]); any = true; }
+            if (!any) break;
+        }
+    }
+    DeviceGuard dg(device);
+    auto* b = new shwfs_batch();
+    b->device = device;
+    b->n_sensors = n_sensors;
+    b->n_items = (int)items.size();
+    b->grid = grid_blocks > 0 ? std::min(grid_blocks, b->n_items) : b->n_items;
+    auto cleanup = [&](int code) { shwfs_batch_destroy(b); return code; };
+    hipError_t e;
+    const int amp_blocks = 64;
+    if ((e = hipMalloc((void**)&b->d_args, n_sensors * sizeof(NgsArgs))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_items, items.size() * sizeof(BatchItem))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_amp, n_sensors * 2 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_amp_part, n_sensors * amp_blocks * 2 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_counter, n_sensors * sizeof(unsigned))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_ts, (n_sensors + 1) * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMemset(b->d_counter, 0, n_sensors * sizeof(unsigned))) != hipSuccess)
+        return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
+    // the pupils are static for the batch: amplitude statistics once, here
+    for (int s = 0; s < n_sensors; ++s) {
+        const int64_t px = (int64_t)plans[s]->g.ef_size * plans[s]->g.ef_size;
+        hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, px,
+                           b->d_amp_part + (size_t)s * amp_blocks * 2, b->d_counter + s, b->d_amp + 2 * s);
+        args[s].amp = b->d_amp + 2 * s;
+    }
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
+        (e = hipMemcpy(b->d_args, args.data(), n_sensors * sizeof(NgsArgs), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice)) != hipSuccess)
+        return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e)));
+    *out = b;
+    return SHWFS_OK;
+}
+
+int shwfs_batch_forward(shwfs_batch* b, void* stream) {
+    if (!b) return set_error(SHWFS_ERR_ARG, "null batch");
+    DeviceGuard dg(b->device);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, st, b->d_ts, b->n_sensors);
+    SHWFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(sh_batch_kernel, dim3(b->grid), dim3(256), kBatchLdsBytes, st, b->d_args, b->d_items,
+                       b->n_items, b->d_ts);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+int shwfs_batch_latencies(shwfs_batch* b, double* us, void* stream) {
+    if (!b || !us) return set_error(SHWFS_ERR_ARG, "null argument");
+    DeviceGuard dg(b->device);
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    std::vector<unsigned long long> ts(b->n_sensors + 1);
+    SHWFS_HIP_CHECK(hipMemcpyAsync(ts.data(), b->d_ts, ts.size() * sizeof(unsigned long long),
+                                   hipMemcpyDeviceToHost, st));
+    SHWFS_HIP_CHECK(hipStreamSynchronize(st));
+    int rate_khz = 0;
+    SHWFS_HIP_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, b->device));
+    if (rate_khz <= 0) return set_error(SHWFS_ERR_HIP, "wall clock rate unavailable");
+    for (int s = 0; s < b->n_sensors; ++s)
+        us[s] = (ts[1 + s] >= ts[0]) ? (double)(ts[1 + s] - ts[0]) * 1e3 / rate_khz : -1.0;
+    return SHWFS_OK;
+}
+
+int shwfs_batch_info(shwfs_batch* b, int32_t* n_items, int32_t* grid) {
+    if (!b || !n_items || !grid) return set_error(SHWFS_ERR_ARG, "null argument");
+    *n_items = b->n_items;
+    *grid = b->grid;
     return SHWFS_OK;
 }
 

@@ -102,3 +102,56 @@ def lgs_kernel(geom, launcher_pos=(20.0, 0.0, 0.0), spot_size=1.0, zfocus=90e3):
                           target_device_idx=-1, precision=1)
     k.prepare_for_sh(sodium_altitude=LGS_LAYERS_M.tolist(), sodium_intensity=sodium_profile().tolist())
     return k
+
+
+def scao_ngs_10x10():
+    """10x10 subapertures of 8 px on an 80 px, 4 m pupil, N = 32, 8 CCD px (4:1)."""
+    pitch = 4.0 / 80
+    ld = _lambda_over_d(750, 8 * pitch)
+    kw = dict(wavelengthInNm=750, subap_wanted_fov=8 * ld, sensor_pxscale=ld,
+              subap_on_diameter=10, subap_npx=8, fov_ovs_coeff=1.0)
+    return Workload('scao_ngs_10x10', 80, pitch, 0.14, kw, ld / 4, seed=1)
+
+
+def multi_sensor_entries(n_lgs=6, n_pushpull=2000, amp_nm=50.0, device=0, with_slopes=True):
+    """The uneven batch: n_lgs LGS 40x40 sensors (N 128, launchers on a 20 m circle, own seeded
+    screens), NGS 10x10 / 20x20 (N 32) and 80x80 (N 64) sensors, and n_pushpull push-pull
+    fields (phi0 +- a m_k, 50 nm RMS smooth modes) for the 80x80 sensor."""
+    import torch
+    from specula_amd.batch import make_entry
+    from specula_amd.lib.modes import fourier_modes
+    from specula_amd.processing_objects.sh import SHPlan
+    entries = []
+    dev = torch.device(f'cuda:{device}')
+    for k in range(n_lgs):
+        wl = lgs_40x40()
+        g = wl.geometry()
+        ang = 2 * np.pi * k / max(n_lgs, 1)
+        kern = lgs_kernel(g, launcher_pos=(20.0 * np.cos(ang), 20.0 * np.sin(ang), 0.0))
+        plan = SHPlan(g, lgs=True, max_frames=1, device=device, kernels_normalised=kern.kernels_normalised())
+        plan.set_kernels(torch.tensor(kern.lag_spectra(g.ovs_np_sub), device=dev))
+        e = make_entry(f'lgs40_{k}', plan, wl.pupil(), 1, with_slopes=with_slopes, device=device)
+        wl.seed = 100 + k
+        e.phase.copy_(torch.tensor(wl.frames(1).astype(np.float32), device=dev))
+        entries.append(e)
+    for j, mk in enumerate((scao_ngs_10x10, scao_ngs_20x20, elt_ngs_80x80)):
+        wl = mk()
+        g = wl.geometry()
+        e = make_entry(wl.name, SHPlan(g, max_frames=1, device=device), wl.pupil(), 1, with_slopes=with_slopes,
+                       device=device)
+        wl.seed = 100 + n_lgs + j
+        e.phase.copy_(torch.tensor(wl.frames(1).astype(np.float32), device=dev))
+        entries.append(e)
+    if n_pushpull:
+        wl = elt_ngs_80x80()
+        g = wl.geometry()
+        e = make_entry('elt80_pushpull', SHPlan(g, max_frames=n_pushpull, device=device), wl.pupil(), n_pushpull,
+                       with_slopes=with_slopes, device=device)
+        half = n_pushpull // 2
+        modes = fourier_modes(wl.pupil(), half, device=dev)
+        phi0 = entries[-1].phase[0]
+        torch.add(phi0, modes, alpha=amp_nm, out=e.phase[:half])
+        torch.add(phi0, modes, alpha=-amp_nm, out=e.phase[half:2 * half])
+        del modes
+        entries.append(e)
+    return entries
