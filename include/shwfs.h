@@ -84,6 +84,11 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
  * No reference counterpart (measurement support). */
 int shwfs_plan_set_timing_events(shwfs_plan* plan, void* start, void* stop);
 
+/* Launch-shape knob of the NGS MFMA path: each block streams `fpb` consecutive frames of its
+ * subaperture tile, prefetching the next frame's pupil samples (1, 2, 4 or 8; default 4).
+ * Results do not depend on it.  No reference counterpart (tuning). */
+int shwfs_plan_set_frames_per_block(shwfs_plan* plan, int fpb);
+
 /* ShSlopec plan.  Either `block_origin` (n_sub linear pixel offsets of square np_sub^2
  * blocks, row stride frame_w) or `idx` (n_sub x np_sub^2 linear pixel indices, int32) must
  * be given (host pointers).  xw, yw, mw: np_sub^2 weights (host, FP64): the x/y CoG weights

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'lib', 'libs
 # every symbol declared in include/shwfs.h
 EXPORTS = (
     'shwfs_version', 'shwfs_last_error',
-    'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events',
+    'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events', 'shwfs_plan_set_frames_per_block',
     'shwfs_set_kernels', 'shwfs_forward',
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian',
@@ -50,6 +50,7 @@ def _declare(lib):
         'shwfs_plan_create': (_I, [ctypes.POINTER(ShwfsGeom), _P, _P, _I, ctypes.POINTER(_P)]),
         'shwfs_plan_destroy': (_I, [_P]),
         'shwfs_plan_set_timing_events': (_I, [_P, _P, _P]),
+        'shwfs_plan_set_frames_per_block': (_I, [_P, _I]),
         'shwfs_set_kernels': (_I, [_P, _P, _P]),
         'shwfs_forward': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P]),
         'shslopec_plan_create': (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _I, ctypes.POINTER(_P)]),

@@ -338,8 +338,34 @@ struct NgsMfmaShape {
 };
 
 // One tile = 4 waves x SW subapertures of frame f.  `lds` holds NgsMfmaShape<NP>::LDS_F4 float4.
+// Global loads of one tile (kept separate so that a streaming caller can issue the next
+// frame's loads before computing the current one).
 template <int NP>
-__device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f, float4* lds, double* red) {
+__device__ __forceinline__ void ngs_mfma_load(const NgsArgs& g, int tile, int f, float (&av)[NP], float (&pv)[NP]) {
+    using Sh = NgsMfmaShape<NP>;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int q = lane / NP, x = lane - (lane / NP) * NP;
+    const int sub = (tile * 4 + w) * Sh::SW + q;
+    if (q < Sh::SW && sub < g.n_sub) {
+        const int n = g.n_lenses;
+        const int i = sub / n, j = sub - (sub / n) * n;
+        const int64_t off = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x;
+        const float* ar = g.A + f * g.a_stride + off;
+        const float* pr = g.phase + f * g.ph_stride + off;
+#pragma unroll
+        for (int y = 0; y < NP; ++y) {
+            av[y] = ar[(int64_t)y * g.ef_size];
+            pv[y] = pr[(int64_t)y * g.ef_size];
+        }
+    } else {
+#pragma unroll
+        for (int y = 0; y < NP; ++y) { av[y] = 0.f; pv[y] = 0.f; }
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void ngs_mfma_compute(const NgsArgs& g, int tile, int f, const float (&av)[NP],
+                                                 const float (&phv)[NP], float4* lds, double* red) {
     using Sh = NgsMfmaShape<NP>;
     constexpr int M = Sh::M;
     constexpr int SW = Sh::SW;
@@ -354,23 +380,11 @@ __device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f,
 
     {   // phase A: column transforms
         const int q = lane / NP, x = lane - (lane / NP) * NP;
-        const int sub = sub0 + q;
         float2 e[NP];
-        if (q < SW && sub < g.n_sub) {
-            const int i = sub / n, j = sub - (sub / n) * n;
-            const int64_t off = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x;
-            const float* ar = g.A + f * g.a_stride + off;
-            const float* pr = g.phase + f * g.ph_stride + off;
-            const double ax = g.a0 + x;
-            const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
+        const double ax = g.a0 + x;
+        const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
 #pragma unroll
-            for (int y = 0; y < NP; ++y)
-                e[y] = field_sample_rev(ar[(int64_t)y * g.ef_size], pr[(int64_t)y * g.ef_size], k_rev, tilt_rev,
-                                        ax + g.a0 + y);
-        } else {
-#pragma unroll
-            for (int y = 0; y < NP; ++y) e[y] = make_float2(0.f, 0.f);
-        }
+        for (int y = 0; y < NP; ++y) e[y] = field_sample_rev(av[y], phv[y], k_rev, tilt_rev, ax + g.a0 + y);
         float2 G[M];
         fft_pruned<M, NP, false>(e, G);
         if (q < SW) {
@@ -437,6 +451,7 @@ __device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f,
     float* ccdf = g.ccd + f * g.ccd_stride;
     if (64 % (4 * SW) == 0 && g.npx == 16 && nq == SW) {
         // lane -> fixed (subaperture q, float4 column x4); CCD rows Y = lane/32 + 2t
+        const float fscale = (float)scale;
         const int x4 = lane & 3, q = (lane >> 2) % SW;
         const int sub = sub0 + q;
         const int i = sub / n, j = sub - (sub / n) * n;
@@ -445,8 +460,8 @@ __device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f,
         constexpr int YSTEP = 64 / (4 * SW);
         for (int Y = (lane >> 2) / SW; Y < 16; Y += YSTEP) {
             const float4 v = *reinterpret_cast<const float4*>(src + Y * RS);
-            *reinterpret_cast<float4*>(base + (int64_t)Y * W) =
-                make_float4((float)(v.x * scale), (float)(v.y * scale), (float)(v.z * scale), (float)(v.w * scale));
+            const floatx4 o = {v.x * fscale, v.y * fscale, v.z * fscale, v.w * fscale};
+            __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(base + (int64_t)Y * W));
         }
     } else if (nq > 0) {
         const int per = nq * g.npx * g.npx;
@@ -466,11 +481,39 @@ __device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f,
 }
 
 template <int NP>
+__device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f, float4* lds, double* red) {
+    float av[NP], pv[NP];
+    ngs_mfma_load<NP>(g, tile, f, av, pv);
+    ngs_mfma_compute<NP>(g, tile, f, av, pv, lds, red);
+}
+
+template <int NP>
 __global__ void __launch_bounds__(256)
 sh_ngs_mfma_kernel(NgsArgs g) {
     __shared__ float4 lds_all[NgsMfmaShape<NP>::LDS_F4];
     __shared__ double red[4];
     ngs_mfma_tile<NP>(g, blockIdx.x, blockIdx.y, lds_all, red);
+}
+
+// Streaming form: one block walks FPB consecutive frames of its tile and issues the next frame's
+// pupil loads before transforming the current one, so HBM latency overlaps the FFT/MFMA work.
+template <int NP, int FPB>
+__global__ void __launch_bounds__(256)
+sh_ngs_mfma_stream_kernel(NgsArgs g, int n_frames) {
+    __shared__ float4 lds_all[NgsMfmaShape<NP>::LDS_F4];
+    __shared__ double red[4];
+    const int tile = blockIdx.x;
+    const int f0 = blockIdx.y * FPB;
+    const int f1 = min(n_frames, f0 + FPB);
+    float av[NP], pv[NP], an[NP], pn[NP];
+    ngs_mfma_load<NP>(g, tile, f0, av, pv);
+    for (int f = f0; f < f1; ++f) {
+        if (f + 1 < f1) ngs_mfma_load<NP>(g, tile, f + 1, an, pn);
+        if (f > f0) __syncthreads();                     // LDS of the previous frame is free
+        ngs_mfma_compute<NP>(g, tile, f, av, pv, lds_all, red);
+#pragma unroll
+        for (int y = 0; y < NP; ++y) { av[y] = an[y]; pv[y] = pn[y]; }
+    }
 }
 
 // ---------------------------------------------------------------- LGS forward (sh.py:381-394)
@@ -1074,6 +1117,7 @@ struct shwfs_plan {
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
     bool kernels_set = false;      // LGS: kernel spectra uploaded
+    int fpb = 4;                   // frames per block of the streaming MFMA kernel (1: no streaming)
     int L = 0;                     // LGS lag-grid size 2 np - 1
     float* d_roww = nullptr;
     float* d_colw = nullptr;
@@ -1131,8 +1175,17 @@ int ngs_tile_size(int np) { return 256 / np; }
 
 template <int NP>
 int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
-    dim3 grid(p->n_tiles, n_frames);
-    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP>), grid, dim3(256), 0, st, a);
+    const int fpb = p->fpb;
+    if (fpb <= 1 || n_frames == 1) {
+        hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP>), dim3(p->n_tiles, n_frames), dim3(256), 0, st, a);
+    } else {
+        dim3 grid(p->n_tiles, (n_frames + fpb - 1) / fpb);
+        switch (fpb) {
+            case 2: hipLaunchKernelGGL((sh_ngs_mfma_stream_kernel<NP, 2>), grid, dim3(256), 0, st, a, n_frames); break;
+            case 4: hipLaunchKernelGGL((sh_ngs_mfma_stream_kernel<NP, 4>), grid, dim3(256), 0, st, a, n_frames); break;
+            default: hipLaunchKernelGGL((sh_ngs_mfma_stream_kernel<NP, 8>), grid, dim3(256), 0, st, a, n_frames); break;
+        }
+    }
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -1280,6 +1333,13 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_bpart);
     delete p;
+    return SHWFS_OK;
+}
+
+int shwfs_plan_set_frames_per_block(shwfs_plan* p, int fpb) {
+    if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
+    if (fpb != 1 && fpb != 2 && fpb != 4 && fpb != 8) return set_error(SHWFS_ERR_ARG, "frames per block must be 1, 2, 4 or 8");
+    p->fpb = fpb;
     return SHWFS_OK;
 }
 

@@ -74,6 +74,11 @@ class SHPlan:
                       'shwfs_plan_create')
         self._h = h
 
+    def set_frames_per_block(self, fpb):
+        """Streaming depth of the NGS MFMA kernel (1, 2, 4 or 8 frames per block)."""
+        _native.check(_native.lib().shwfs_plan_set_frames_per_block(self._h, int(fpb)), 'set_frames_per_block')
+        self.fpb = int(fpb)
+
     def set_kernels(self, kspec, stream=None):
         """kspec: (n_sub, L, L) complex64 device tensor of kernel spectra on the lag grid."""
         n_sub = self.geom.n_lenses ** 2

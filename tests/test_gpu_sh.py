@@ -169,3 +169,20 @@ def test_slopec_parity_elt():
                                   dict(exp_weight=1.5)])
 def test_slopec_modes(opts):
     _slopec_case(workloads.scao_ngs_20x20(), **opts)
+
+
+@pytest.mark.parametrize('fpb', [1, 2, 4, 8])
+def test_streaming_frames_per_block_is_exact(fpb):
+    wl = workloads.scao_ngs_20x20()
+    g = wl.geometry()
+    A = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
+    ph = torch.tensor(wl.frames(7).astype(np.float32), device=DEV)
+    ref = torch.empty((7, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan = SHPlan(g, max_frames=7, device=0, variant=2)
+    plan.set_frames_per_block(1)
+    plan.forward(A, ph, ref, 1.0, n_frames=7)
+    out = torch.empty_like(ref)
+    plan.set_frames_per_block(fpb)
+    plan.forward(A, ph, out, 1.0, n_frames=7)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
