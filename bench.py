@@ -24,11 +24,16 @@ FP32_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--frames', type=int, default=100)
     ap.add_argument('--cpu-seconds', type=float, default=15.0,
                     help='budget for the CPU baseline sample (0 disables it)')
+    ap.add_argument('--unfused', action='store_true',
+                    help='run ShSlopec as its own pass instead of in the SH kernel epilogue')
+    ap.add_argument('--fpb', type=int, default=0, help='frames per block of the SH kernel (0: plan default)')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='also run two 256 MiB device copies (reference traffic for rocprofv3 counters)')
     ap.add_argument('--extras', type=int, default=1,
                     help='1: also time the LGS sensor and the 1000-mode interaction matrix')
     return ap.parse_args()
@@ -203,6 +208,8 @@ def main():
     frames = wl.frames(F).astype(np.float32)
 
     plan = SHPlan(g, max_frames=F, device=0)
+    if args.fpb > 0:
+        plan.set_frames_per_block(args.fpb)
     At = torch.tensor(A, device=dev)
     ph = torch.tensor(frames, device=dev)
     ccd = torch.empty((F, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
@@ -220,15 +227,21 @@ def main():
 
     stream = torch.cuda.current_stream()
     ev = [(hip_event(), hip_event()) for _ in range(args.steps)]
+    # CoG fused into the SH kernel epilogue (one pass; the CCD frames are still written)
+    fused = (not args.unfused) and plan.bind_slopec(sl_plan)
 
     def step(k=None):
         if k is not None:
             _native.check(_native.lib().shwfs_plan_set_timing_events(plan._h, ev[k][0], ev[k][1]),
                           'set_timing_events')
-        plan.forward(At, ph, ccd, 1.0, n_frames=F, stream=stream)
+        if fused:
+            plan.forward_slopes(At, ph, ccd, 1.0, slopes, flux, totals, n_frames=F, stream=stream)
+        else:
+            plan.forward(At, ph, ccd, 1.0, n_frames=F, stream=stream)
         if k is not None:
             _native.check(_native.lib().shwfs_plan_set_timing_events(plan._h, None, None), 'set_timing_events')
-        sl_plan.forward(ccd, slopes, flux, totals, n_frames=F, stream=stream)
+        if not fused:
+            sl_plan.forward(ccd, slopes, flux, totals, n_frames=F, stream=stream)
 
     for _ in range(args.warmup):
         step()
@@ -239,11 +252,20 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 
+    if args.copy_probe:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     kern_ms = float(np.mean([hip_elapsed_ms(a, b) for a, b in ev]))
     ms_per_step = elapsed * 1e3 / args.steps
     fps = F * args.steps / elapsed
     px = g.ef_size * g.ef_size
     bytes_per_frame = 4 * (2 * px + g.ccd_side * g.ccd_side)      # A + phase read, CCD written
+    if fused:
+        bytes_per_frame += 32 * n_sub                              # FP64 CoG moments written
     achieved = bytes_per_frame * F / (kern_ms * 1e-3) / 1e9
     N = g.fft_size
     fft_flop = 5 * N * N * np.log2(N * N) * g.n_lenses ** 2       # 5 N^2 log2(N^2) per subaperture
@@ -270,7 +292,8 @@ def main():
         'fft_fp32_peak_frac': round(fft_flop * F / (kern_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-                     'kernel': 'sh_ngs_mfma_kernel<8>',
+                     'kernel': f'sh_ngs_mfma_kernel<8, true, 16, {1 if fused else 0}>',
+                     'fused_cog': bool(fused), 'frames_per_block': plan.fpb,
                      'bytes_per_launch': bytes_per_frame * F},
     }
     if args.extras:

@@ -85,7 +85,7 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
 int shwfs_plan_set_timing_events(shwfs_plan* plan, void* start, void* stop);
 
 /* Launch-shape knob of the NGS MFMA path: each block streams `fpb` consecutive frames of its
- * subaperture tile, prefetching the next frame's pupil samples (1, 2, 4 or 8; default 4).
+ * subaperture tile, prefetching the next frame's pupil samples (1 to 64; default 4).
  * Results do not depend on it.  No reference counterpart (tuning). */
 int shwfs_plan_set_frames_per_block(shwfs_plan* plan, int fpb);
 
@@ -112,6 +112,25 @@ int shslopec_forward(shslopec_plan* plan, int n_frames,
                      float thr, int thr_mode,
                      float* slopes, int64_t slopes_frame_stride,
                      double* flux, double* totals, void* stream);
+
+/* Fused SH forward + ShSlopec: the centre of gravity of each valid subaperture is taken from
+ * the binned CCD tile while it is still on chip, so the CCD frames are written once and never
+ * read back.  Results equal shwfs_forward followed by shslopec_forward on the CCD frames
+ * (same outputs and conventions as those two calls; the moment sums are FP64).
+ * shwfs_plan_bind_slopec checks that the pair can be fused: NGS MFMA plan with np_sub 8,
+ * Parseval normalisation and 8 or 16 px per subaperture; ShSlopec plan made from block origins
+ * lying on the lenslet grid of this CCD, with the default separable weights.  It returns
+ * SHWFS_ERR_UNSUPPORTED otherwise (then call the two functions).  The bound ShSlopec plan must
+ * outlive its use; bind NULL to unbind.  Replaces the SH -> ShSlopec hand-over of one loop step
+ * (processing_objects/sh.py:348-428 followed by sh_slopec.py:308-438). */
+int shwfs_plan_bind_slopec(shwfs_plan* plan, shslopec_plan* slopec);
+int shwfs_forward_slopes(shwfs_plan* plan, int n_frames,
+                         const float* A, int64_t a_frame_stride,
+                         const float* phase_nm, int64_t phase_frame_stride,
+                         double S0, float* ccd, int64_t ccd_frame_stride,
+                         float thr, int thr_mode,
+                         float* slopes, int64_t slopes_frame_stride,
+                         double* flux, double* totals, void* stream);
 
 /* ---- adjoint (vector-Jacobian products) ------------------------------------------------
  * No reference counterpart: the reference builds interaction matrices by push-pull

@@ -11,11 +11,14 @@ import os
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'lib', 'libshwfs.so')
 
 # every symbol declared in include/shwfs.h
+SHWFS_OK, SHWFS_ERR_ARG, SHWFS_ERR_HIP, SHWFS_ERR_UNSUPPORTED = 0, -1, -2, -3   # include/shwfs.h
+
 EXPORTS = (
     'shwfs_version', 'shwfs_last_error',
     'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events', 'shwfs_plan_set_frames_per_block',
     'shwfs_set_kernels', 'shwfs_forward',
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
+    'shwfs_plan_bind_slopec', 'shwfs_forward_slopes',
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian',
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
@@ -56,6 +59,8 @@ def _declare(lib):
         'shslopec_plan_create': (_I, [_I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _I, ctypes.POINTER(_P)]),
         'shslopec_plan_destroy': (_I, [_P]),
         'shslopec_forward': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
+        'shwfs_plan_bind_slopec': (_I, [_P, _P]),
+        'shwfs_forward_slopes': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
         'shslopec_vjp': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P]),
         'shwfs_vjp': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P, _I64, _P]),
         'shwfs_im_from_jacobian': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P]),

@@ -104,9 +104,12 @@ __device__ __forceinline__ float2 field_sample_rev(float a, float ph, double k_r
 // Per frame: sum(A) (masked area, electric_field.py:87-89) and sum(A^2), FP64, deterministic.
 // Last-block-done pattern: every block writes a partial, the last one to arrive reduces the
 // partials in fixed order and resets the arrival counter (so graph replays need no memset).
+// With `scale_out`, the Parseval flux scale S0 pixel_area sum(A) / (norm_const sum(A^2)) of the
+// frame is also written (as float, the factor the SH kernels apply), `scale_num` = S0 pixel_area.
 __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, int64_t n_px,
                                  double* __restrict__ part, unsigned* __restrict__ counter,
-                                 double* __restrict__ out) {
+                                 double* __restrict__ out, double scale_num, double norm_const,
+                                 float* __restrict__ scale_out) {
     __shared__ double red[16];
     __shared__ bool last;
     const int f = blockIdx.y;
@@ -159,6 +162,7 @@ __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, 
         if (threadIdx.x == 0) {
             out[2 * f] = t1;
             out[2 * f + 1] = t2;
+            if (scale_out) scale_out[f] = (float)(scale_num * t1 / (norm_const * t2));
             counter[f] = 0u;
         }
     }
@@ -176,8 +180,20 @@ struct NgsArgs {
     const float* colw;   // npx_pad x M   (Q)
     const double* amp;   // [frame][2]: sum(A), sum(A^2)     (frame index f * amp_fstep)
     int amp_fstep;
-    double* bpart;       // [frame][n_tiles]: sum of unscaled B   (non-Parseval mode)
+    const float* fscale; // Parseval flux scale per amplitude frame (frame index f * scale_fstep)
+    int scale_fstep;
+    double* bpart;       // [frame][n_parts]: partial sums of unscaled B   (non-Parseval mode)
     const float2* kspec; // LGS: [n_sub][L][L] kernel spectra on the lag grid (L = 2 np - 1)
+    const int32_t* slot; // fused CoG: [n_sub] slope index of each subaperture, -1 if not valid
+    double* mom;         // fused CoG: [frame][n_valid][4] flux, thresholded total, x and y sums
+    double* tpart;       // fused CoG: [frame][n_parts][2] per-wave sums of flux and thresholded total
+    const float* cogw;   // fused CoG: x weights [npx], then y weights [npx]
+    float thr;           // fused CoG threshold
+    int n_valid;         // fused CoG: number of valid subapertures
+    int vec;             // pupil rows may be read as float4
+    int store_vec;       // CCD rows may be written as float4
+    int cog;             // fused CoG mode: 0 off, 1 subtract-and-clamp, 2 pedestal
+    int n_parts;         // partial sums per frame in bpart
     int n_lenses, ef_size, npx, npx_pad, n_sub, n_tiles, parseval;
     double k, tilt, a0, S0, pixel_area, norm_const;
 };
@@ -310,210 +326,417 @@ sh_ngs_forward_kernel(NgsArgs g) {
     }
     if (!g.parseval) {
         bsum = block_sum(bsum, red);
-        if (t == 0) g.bpart[(int64_t)f * g.n_tiles + tile] = bsum;
+        if (t == 0) g.bpart[(int64_t)f * g.n_parts + tile] = bsum;
     }
 }
 
 // ---------------------------------------------------------------- NGS forward, MFMA binning (M = 16)
-// One wavefront owns SW = 64/NP consecutive subapertures (a block holds 4 independent waves):
-//   A: lane (q, x)  -> E along column x of subaperture q, pruned FFT_16 over y    -> G1[q][ky][x]
-//   B: lane (q, ky) -> pruned FFT_16 over x of row ky, |.|^2                      -> S[q][ky][kx]
-//   C: per subaperture, B = P S Q^T as 2 x 4 v_mfma_f32_16x16x4f32:
-//        T = S Q^T  with the K index permuted as k = 4*(lane/16) + step, so that each lane reads
-//                   one float4 of S and one float4 of Q;
-//        B = P T    where T, left in the accumulator layout (lane l, reg r = T[4(l/16)+r][l%16]),
-//                   is directly the B operand of step r under the same K permutation.
-//      The result lands as lane l, reg r = B[Y = 4(l/16)+r][X = l%16].
-// G1 (complex, 8 per row) and S (real, 16 per row) occupy the same 64 bytes per (q, ky) row, and
-// each row is read and rewritten by the same lane, so they share one LDS buffer.
+// One wavefront owns SW = 64/NP consecutive subapertures and its own LDS region; the waves of a
+// block never synchronise with each other.
+//   A: lane (q, y)  -> E along pupil row y of subaperture q, pruned FFT_16 over x   -> R[q][kx][y]
+//   B: lane (q, kx) -> autocorrelation over y of column kx (2np-1 lag reals)        -> C^T[q][kx][j]
+//      (the power spectrum S = A C along y, A the cosine/sine synthesis, so P S = (P A) C: the
+//      row operator uploaded for this kernel is P~ = P A)
+//   C: per subaperture, B = P~ C Q^T as 2 x 4 v_mfma_f32_16x16x4f32:
+//        T = C^T P~^T with the K index permuted as k = 4*(lane/16) + step, so that each lane reads
+//                     one float4 of C^T (an LDS row chunk) and one float4 of P~;
+//        D = Q T      where T, left in the accumulator layout (lane l, reg r = T[4(l/16)+r][l%16]),
+//                     is directly the B operand of step r under the same K permutation.
+//      D = B^T, so lane l ends with the float4 B[Y = l%16][X = 4(l/16) .. 4(l/16)+3]: one row chunk,
+//      written back over the consumed S^T chunk it came from.
+//   D: coalesced CCD stores of whole subaperture row segments, and (fused ShSlopec) the FP64
+//      centre-of-gravity moments of each subaperture from the same registers.
+// R (np complex per row) and S^T / B (16 reals per row) share the 64 bytes of an LDS row.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 template <int NP>
 struct NgsMfmaShape {
     static constexpr int M = 16;
-    static constexpr int SW = 64 / NP;
-    static constexpr int RS = 20;             // words per (q, ky) row: 16 used + 4 pad (conflict-free b128 rows)
-    static constexpr int QS = M * RS + 16;    // words per subaperture: +16 rotates banks between subapertures
-    static constexpr int LDS_F4 = 4 * SW * QS / 4;   // float4 per block (4 waves)
+    static constexpr int SW = 64 / NP;               // subapertures per wave
+    static constexpr int QS = M * M + 16;            // LDS words per subaperture (16 rows x 16 + 16 pad)
+    static constexpr int WAVE_F4 = SW * QS / 4;      // float4 per wave
+    static constexpr int WPB = 4;                    // waves per block
+    static constexpr int LDS_F4 = WPB * WAVE_F4;     // float4 per block
+    static constexpr int RPL = (SW * M + 63) / 64;   // phase-B rows per lane
 };
 
-// One tile = 4 waves x SW subapertures of frame f.  `lds` holds NgsMfmaShape<NP>::LDS_F4 float4.
-// Global loads of one tile (kept separate so that a streaming caller can issue the next
-// frame's loads before computing the current one).
-template <int NP>
-__device__ __forceinline__ void ngs_mfma_load(const NgsArgs& g, int tile, int f, float (&av)[NP], float (&pv)[NP]) {
-    using Sh = NgsMfmaShape<NP>;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int q = lane / NP, x = lane - (lane / NP) * NP;
-    const int sub = (tile * 4 + w) * Sh::SW + q;
-    if (q < Sh::SW && sub < g.n_sub) {
-        const int n = g.n_lenses;
-        const int i = sub / n, j = sub - (sub / n) * n;
-        const int64_t off = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x;
-        const float* ar = g.A + f * g.a_stride + off;
-        const float* pr = g.phase + f * g.ph_stride + off;
-#pragma unroll
-        for (int y = 0; y < NP; ++y) {
-            av[y] = ar[(int64_t)y * g.ef_size];
-            pv[y] = pr[(int64_t)y * g.ef_size];
-        }
-    } else {
-#pragma unroll
-        for (int y = 0; y < NP; ++y) { av[y] = 0.f; pv[y] = 0.f; }
-    }
+// Chunk swizzle of LDS row r: 4-word chunk c of row r lives at chunk c ^ lds_swz(r).  With the
+// 16-word subaperture pad (272 = 16 mod 64) every access pattern of the kernel is free of bank
+// conflicts for 32 and for 64 banks: 16-byte reads of one chunk of 8/16 consecutive rows, 8-byte
+// column writes of 2/4 consecutive subapertures, 16-byte row-segment reads of 2/4 subapertures.
+__host__ __device__ constexpr int lds_swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+// CCD store layout of the fast path (npx = 4, 8 or 16 with np = 8): lane -> (subaperture q, float4
+// column x4) and CCD rows Y = h, h + YSTEP, ... of that subaperture.
+template <int NP, int NPX>
+struct NgsStoreShape {
+    static constexpr int SW = 64 / NP;
+    static constexpr int X4 = NPX > 0 ? NPX / 4 : 1;
+    static constexpr int LPR = X4 * SW;                          // lanes per row group
+    static constexpr int YSTEP = NPX > 0 ? 64 / LPR : 1;
+    static constexpr int NY = NPX > 0 ? (NPX + YSTEP - 1) / YSTEP : 1;   // rows per lane
+};
+
+// max(v, 0) on the bit pattern (one integer max, no NaN canonicalisation): non-negative floats
+// order like their int32 patterns and every negative float (sign bit set) maps to +0.
+__device__ __forceinline__ float clamp0(float v) { return __int_as_float(max(__float_as_int(v), 0)); }
+
+constexpr int kYwWord = 256;   // first padding word of subaperture 0 in a wave's LDS region
+
+// Hand-over between lanes of one wave through LDS: a wave's LDS operations execute in order, so
+// wavefront-scope fences around a wave barrier (no s_barrier) are enough.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Subapertures owned by one wave.
+struct NgsWave {
+    int sub0;     // first subaperture (row-major lenslet index)
+    int nq;       // how many of the SW exist
+    int i0, j0;   // lenslet row and column of sub0
+};
+
+__device__ __forceinline__ void wave_sub_rc(const NgsWave& wv, int n, int q, int& i, int& j) {
+    j = wv.j0 + q;
+    i = wv.i0;
+    while (j >= n) { j -= n; ++i; }
 }
 
 template <int NP>
-__device__ __forceinline__ void ngs_mfma_compute(const NgsArgs& g, int tile, int f, const float (&av)[NP],
-                                                 const float (&phv)[NP], float4* lds, double* red) {
+__device__ __forceinline__ NgsWave ngs_wave(const NgsArgs& g, int wave_id) {
+    NgsWave wv;
+    wv.sub0 = wave_id * NgsMfmaShape<NP>::SW;
+    wv.nq = max(0, min(NgsMfmaShape<NP>::SW, g.n_sub - wv.sub0));
+    wv.i0 = wv.sub0 / g.n_lenses;
+    wv.j0 = wv.sub0 - wv.i0 * g.n_lenses;
+    return wv;
+}
+
+// Pupil samples of lane (q, y): row y of subaperture q (A and phase).  Kept separate from the
+// transform so that a caller can issue the next frame's loads before transforming this one.
+// `off`: pupil offset of this lane's row segment, -1 for lanes without a subaperture.  `fs`: the
+// frame's flux scale (Parseval mode), loaded with the samples so that its latency is hidden too.
+template <int NP, bool PARSEVAL>
+__device__ __forceinline__ void ngs_mfma_load(const NgsArgs& g, int off, int f, float (&av)[NP], float (&pv)[NP],
+                                              float& fs) {
+    fs = PARSEVAL ? g.fscale[f * g.scale_fstep] : 1.f;
+    if (off >= 0) {
+        const float* ar = (g.A + f * g.a_stride) + (unsigned)off;
+        const float* pr = (g.phase + f * g.ph_stride) + (unsigned)off;
+        if constexpr (NP % 4 == 0) {
+            if (g.vec) {
+#pragma unroll
+                for (int c = 0; c < NP / 4; ++c) {
+                    const float4 a4 = reinterpret_cast<const float4*>(ar)[c];
+                    const float4 p4 = reinterpret_cast<const float4*>(pr)[c];
+                    av[4 * c] = a4.x; av[4 * c + 1] = a4.y; av[4 * c + 2] = a4.z; av[4 * c + 3] = a4.w;
+                    pv[4 * c] = p4.x; pv[4 * c + 1] = p4.y; pv[4 * c + 2] = p4.z; pv[4 * c + 3] = p4.w;
+                }
+                return;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < NP; ++x) { av[x] = ar[x]; pv[x] = pr[x]; }
+    } else {
+#pragma unroll
+        for (int x = 0; x < NP; ++x) { av[x] = 0.f; pv[x] = 0.f; }
+    }
+}
+
+// Per-wave constants that do not change from frame to frame.
+template <int NP, int NPX, int COG>
+struct NgsWaveConst {
+    floatx4 pv, qv;       // binning operator chunks of this lane (P row l%16, Q row l%16, chunk l/16)
+    // fused CoG y weights live in the 16 padding words of subaperture 0 (never touched by the
+    // transform): LDS words kYwWord .. kYwWord + npx - 1
+    int cword;            // LDS word of this lane's phase-C chunk (without the subaperture offset)
+    int ld_off;           // pupil offset of this lane's row segment (-1: no subaperture)
+    int st_off;           // fast store layout: CCD offset of this lane's first float4
+    float xw[4];          // fused CoG: x weights of this lane's 4 columns
+    int slot;             // fused CoG: slope index of this lane's subaperture (-1: not valid)
+};
+
+// Transform one frame of the wave's subapertures and store it (plus CoG moments when COG != 0:
+// 1 = subtract-and-clamp threshold, 2 = pedestal threshold, 3 = no threshold (clamp at zero),
+// sh_slopec.py:380-387).
+template <int NP, bool PARSEVAL, int NPX, int COG>
+__device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& wv, int wave_id, int f,
+                                               const float (&av)[NP], const float (&phv)[NP], float fs, float* sw,
+                                               const NgsWaveConst<NP, NPX, COG>& wc) {
     using Sh = NgsMfmaShape<NP>;
     constexpr int M = Sh::M;
     constexpr int SW = Sh::SW;
-    constexpr int RS = Sh::RS;
     constexpr int QS = Sh::QS;
     static_assert(NP <= 8, "M = 16 path needs np_sub <= 8");
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    float2* g1 = reinterpret_cast<float2*>(lds + w * (SW * QS / 4));
-    float* sS = reinterpret_cast<float*>(lds + w * (SW * QS / 4));
-    const int sub0 = (tile * 4 + w) * SW;
+    const int lane = threadIdx.x & 63;
     const int n = g.n_lenses;
 
-    {   // phase A: column transforms
-        const int q = lane / NP, x = lane - (lane / NP) * NP;
+    {   // A: row transforms of the field
+        const int q = lane / NP, y = lane - (lane / NP) * NP;
+        const double k_rev = g.k * (1.0 / kTwoPi), t_rev = g.tilt * (1.0 / kTwoPi);
+        const double cy = t_rev * (2.0 * g.a0 + y);
         float2 e[NP];
-        const double ax = g.a0 + x;
-        const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
 #pragma unroll
-        for (int y = 0; y < NP; ++y) e[y] = field_sample_rev(av[y], phv[y], k_rev, tilt_rev, ax + g.a0 + y);
-        float2 G[M];
-        fft_pruned<M, NP, false>(e, G);
+        for (int x = 0; x < NP; ++x) {
+            double rv = fma((double)phv[x], k_rev, -fma(t_rev, (double)x, cy));
+            rv -= rint(rv);
+            const float rf = (float)rv;
+            e[x] = make_float2(av[x] * __builtin_amdgcn_cosf(rf), av[x] * __builtin_amdgcn_sinf(rf));
+        }
+        float2 R[M];
+        fft_pruned<M, NP, false>(e, R);
         if (q < SW) {
+            const int base = q * QS + 2 * (y & 1);
+            int cw[4];
 #pragma unroll
-            for (int ky = 0; ky < M; ++ky) g1[(q * QS + ky * RS) / 2 + x] = G[ky];
+            for (int s = 0; s < 4; ++s) cw[s] = base + (((y >> 1) ^ s) << 2);
+#pragma unroll
+            for (int kx = 0; kx < M; ++kx)
+                *reinterpret_cast<float2*>(sw + cw[lds_swz(kx)] + (kx << 4)) = R[kx];
         }
     }
-    __syncthreads();
+    wave_lds_sync();
 
-    for (int r = lane; r < SW * M; r += 64) {   // phase B: row transforms and power spectrum
-        const int rowoff = (r / M) * QS + (r % M) * RS;
-        float2 row[NP];
+#pragma unroll 1
+    for (int it = 0; it < Sh::RPL; ++it) {   // B: column transforms and power spectrum
+        const int rho = lane + 64 * it;
+        if ((SW * M) % 64 != 0 && rho >= SW * M) break;
+        const int q = rho >> 4, kx = rho & 15;
+        const int rowb = (q * QS + (kx << 4)) | (lds_swz(kx) << 2);
+        constexpr int NC = (NP + 1) / 2;          // float4 chunks holding the np complex samples
+        float2 col[2 * NC];
 #pragma unroll
-        for (int x = 0; x < NP; ++x) row[x] = g1[rowoff / 2 + x];
-        float2 G[M];
-        fft_pruned<M, NP, false>(row, G);
-        float4* srow = reinterpret_cast<float4*>(sS + rowoff);
+        for (int c = 0; c < NC; ++c) {
+            const float4 v = *reinterpret_cast<const float4*>(sw + (rowb ^ (c << 2)));
+            col[2 * c] = make_float2(v.x, v.y);
+            col[2 * c + 1] = make_float2(v.z, v.w);
+        }
+        // autocorrelation along y of column kx: C[d] = sum_y R[y] conj(R[y-d]), d = 0 .. np-1.
+        // |FFT_16 over y|^2 = C[0] + 2 Re sum_d C[d] W^(ky d) is linear in these 2np-1 reals, so the
+        // cosine/sine synthesis is folded into the row operator (P~ = P A on the host).
+        float c[M];
 #pragma unroll
-        for (int c = 0; c < M / 4; ++c)
-            srow[c] = make_float4(G[4 * c].x * G[4 * c].x + G[4 * c].y * G[4 * c].y,
-                                  G[4 * c + 1].x * G[4 * c + 1].x + G[4 * c + 1].y * G[4 * c + 1].y,
-                                  G[4 * c + 2].x * G[4 * c + 2].x + G[4 * c + 2].y * G[4 * c + 2].y,
-                                  G[4 * c + 3].x * G[4 * c + 3].x + G[4 * c + 3].y * G[4 * c + 3].y);
+        for (int j = 0; j < M; ++j) c[j] = 0.f;
+#pragma unroll
+        for (int y = 0; y < NP; ++y) c[0] = fmaf(col[y].x, col[y].x, fmaf(col[y].y, col[y].y, c[0]));
+#pragma unroll
+        for (int d = 1; d < NP; ++d) {
+            float cr = 0.f, ci = 0.f;
+#pragma unroll
+            for (int y = d; y < NP; ++y) {
+                cr = fmaf(col[y].x, col[y - d].x, fmaf(col[y].y, col[y - d].y, cr));
+                ci = fmaf(col[y].y, col[y - d].x, fmaf(-col[y].x, col[y - d].y, ci));
+            }
+            c[d] = cr;
+            c[NP - 1 + d] = ci;
+        }
+#pragma unroll
+        for (int q4 = 0; q4 < M / 4; ++q4)
+            *reinterpret_cast<float4*>(sw + (rowb ^ (q4 << 2))) =
+                make_float4(c[4 * q4], c[4 * q4 + 1], c[4 * q4 + 2], c[4 * q4 + 3]);
     }
-    __syncthreads();
+    wave_lds_sync();
 
-    // phase C: binning on the matrix cores
-    const int i16 = lane & 15, g4 = lane >> 4;
-    const floatx4 qv = *reinterpret_cast<const floatx4*>(g.colw + i16 * M + 4 * g4);
-    const floatx4 pv = *reinterpret_cast<const floatx4*>(g.roww + i16 * M + 4 * g4);
-    double scale = 1.0;
-    if (g.parseval) {
-        const double* am = g.amp + (int64_t)f * g.amp_fstep;
-        scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
+    // C: binning on the matrix cores; the frame's flux scale is folded into Q (Parseval mode)
+    floatx4 qv = wc.qv;
+    if constexpr (PARSEVAL) qv *= fs;
+    auto bin_one = [&](int q) {
+        floatx4* chunk = reinterpret_cast<floatx4*>(sw + q * QS + wc.cword);
+        const floatx4 sv = *chunk;
+        floatx4 t = {0.f, 0.f, 0.f, 0.f};
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[0], wc.pv[0], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[1], wc.pv[1], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[2], wc.pv[2], t, 0, 0, 0);
+        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[3], wc.pv[3], t, 0, 0, 0);
+        floatx4 d = {0.f, 0.f, 0.f, 0.f};
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[0], t[0], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[1], t[1], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[2], t[2], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_16x16x4f32(qv[3], t[3], d, 0, 0, 0);
+        *chunk = d;
+    };
+    if (wv.nq == SW) {
+#pragma unroll
+        for (int q = 0; q < SW; ++q) bin_one(q);
+    } else {
+        for (int q = 0; q < wv.nq; ++q) bin_one(q);
     }
+    wave_lds_sync();
+
+    // D: stores (+ moments)
+    float* ccdf = g.ccd + f * g.ccd_stride;
     const int W = n * g.npx;
     double bsum = 0.0;
-    for (int q = 0; q < SW; ++q) {
-        const int sub = sub0 + q;
-        if (sub >= g.n_sub) break;                       // wave-uniform
-        const floatx4 sv = *reinterpret_cast<const floatx4*>(sS + q * QS + i16 * RS + 4 * g4);
-        floatx4 t = {0.f, 0.f, 0.f, 0.f};
-        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[0], qv[0], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[1], qv[1], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[2], qv[2], t, 0, 0, 0);
-        t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[3], qv[3], t, 0, 0, 0);
-        floatx4 b = {0.f, 0.f, 0.f, 0.f};
-        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[0], t[0], b, 0, 0, 0);
-        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[1], t[1], b, 0, 0, 0);
-        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[2], t[2], b, 0, 0, 0);
-        b = __builtin_amdgcn_mfma_f32_16x16x4f32(pv[3], t[3], b, 0, 0, 0);
-        // stage the 16 x 16 tile over this subaperture's (already consumed) spectrum in LDS;
-        // rows/columns >= npx are exactly zero (zero-padded operators)
-        float* bt = sS + q * QS;
+    if constexpr (NPX > 0) {
+        using St = NgsStoreShape<NP, NPX>;
+        constexpr int X4 = St::X4, LPR = St::LPR, YSTEP = St::YSTEP, NY = St::NY;
+        static_assert(NPX % 4 == 0 && 64 % LPR == 0, "row groups must tile the wave");
+        const int x4 = lane % X4, q = (lane / X4) % SW, h = lane / LPR;
+        const bool live = q < wv.nq;
+        (void)x4; (void)W;
+        const int dst0 = wc.st_off;
+        // this lane's 4 x NY pixels: flux, thresholded column sums, y moment (FP32 partials)
+        float l_f = 0.f, l_y = 0.f, cw0 = 0.f, cw1 = 0.f, cw2 = 0.f, cw3 = 0.f;
+        float* ccdw = ccdf + dst0;
+        if (live) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            bt[(4 * g4 + r) * RS + i16] = b[r];
-            if (!g.parseval && 4 * g4 + r < g.npx && i16 < g.npx) bsum += (double)b[r];
+            for (int t = 0; t < NY; ++t) {
+                const int Y = h + YSTEP * t;
+                if (NPX % YSTEP != 0 && Y >= NPX) break;
+                const float4 v = *reinterpret_cast<const float4*>(sw + q * QS + (Y << 4) + ((x4 ^ lds_swz(Y)) << 2));
+                const floatx4 o = {v.x, v.y, v.z, v.w};
+                __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(ccdw + (unsigned)(YSTEP * t * W)));
+                if constexpr (!PARSEVAL) bsum += (double)((v.x + v.y) + (v.z + v.w));
+                if constexpr (COG != 0) {
+                    float w0, w1, w2, w3;
+                    if constexpr (COG == 1) {          // subtract the threshold, clamp at zero
+                        w0 = clamp0(v.x - g.thr); w1 = clamp0(v.y - g.thr);
+                        w2 = clamp0(v.z - g.thr); w3 = clamp0(v.w - g.thr);
+                    } else if constexpr (COG == 2) {   // pedestal
+                        w0 = v.x < g.thr ? 0.f : v.x; w1 = v.y < g.thr ? 0.f : v.y;
+                        w2 = v.z < g.thr ? 0.f : v.z; w3 = v.w < g.thr ? 0.f : v.w;
+                    } else {                           // no threshold: clamp at zero only
+                        w0 = clamp0(v.x); w1 = clamp0(v.y); w2 = clamp0(v.z); w3 = clamp0(v.w);
+                    }
+                    const float yv = sw[kYwWord + Y];   // y weight of row Y (kept in LDS padding)
+                    l_f += (v.x + v.y) + (v.z + v.w);
+                    l_y = fmaf((w0 + w1) + (w2 + w3), yv, l_y);
+                    cw0 += w0; cw1 += w1; cw2 += w2; cw3 += w3;
+                }
+            }
         }
-    }
-    __syncthreads();
-    // coalesced stores: consecutive lanes walk X, then subapertures, then CCD rows
-    const int nq = min(SW, g.n_sub - sub0);
-    float* ccdf = g.ccd + f * g.ccd_stride;
-    if (64 % (4 * SW) == 0 && g.npx == 16 && nq == SW) {
-        // lane -> fixed (subaperture q, float4 column x4); CCD rows Y = lane/32 + 2t
-        const float fscale = (float)scale;
-        const int x4 = lane & 3, q = (lane >> 2) % SW;
-        const int sub = sub0 + q;
-        const int i = sub / n, j = sub - (sub / n) * n;
-        float* base = ccdf + (int64_t)(i * 16) * W + j * 16 + 4 * x4;
-        const float* src = sS + q * QS + 4 * x4;
-        constexpr int YSTEP = 64 / (4 * SW);
-        for (int Y = (lane >> 2) / SW; Y < 16; Y += YSTEP) {
-            const float4 v = *reinterpret_cast<const float4*>(src + Y * RS);
-            const floatx4 o = {v.x * fscale, v.y * fscale, v.z * fscale, v.w * fscale};
-            __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(base + (int64_t)Y * W));
+        if constexpr (COG != 0) {
+            // FP32 partials of 4 x NY pixels per lane; FP64 across lanes (and subapertures)
+            const float l_t = (cw0 + cw1) + (cw2 + cw3);
+            const float l_x = fmaf(cw3, wc.xw[3], fmaf(cw2, wc.xw[2], fmaf(cw1, wc.xw[1], cw0 * wc.xw[0])));
+            double m_f = l_f, m_t = l_t, m_x = l_x, m_y = l_y;
+#pragma unroll
+            for (int o = 1; o < X4; o <<= 1) {
+                m_f += __shfl_xor(m_f, o, kWave); m_t += __shfl_xor(m_t, o, kWave);
+                m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
+            }
+#pragma unroll
+            for (int o = LPR; o < 64; o <<= 1) {
+                m_f += __shfl_xor(m_f, o, kWave); m_t += __shfl_xor(m_t, o, kWave);
+                m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
+            }
+            const bool writer = live && lane < LPR && x4 == 0 && wc.slot >= 0;
+            if (writer) {
+                double* m = g.mom + ((int64_t)f * g.n_valid + wc.slot) * 4;
+                *reinterpret_cast<double2*>(m) = make_double2(m_f, m_t);
+                *reinterpret_cast<double2*>(m + 2) = make_double2(m_x, m_y);
+            }
+            // the wave's share of the frame totals (flux, thresholded total), fixed order
+            double p_f = writer ? m_f : 0.0, p_t = writer ? m_t : 0.0;
+#pragma unroll
+            for (int o = X4; o < LPR; o <<= 1) {
+                p_f += __shfl_xor(p_f, o, kWave);
+                p_t += __shfl_xor(p_t, o, kWave);
+            }
+            if (lane == 0)
+                *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + wave_id) * 2) = make_double2(p_f, p_t);
         }
-    } else if (nq > 0) {
-        const int per = nq * g.npx * g.npx;
+    } else {
+        const int npx = g.npx;
+        const int per = wv.nq * npx * npx;
         for (int e = lane; e < per; e += 64) {
-            const int X = e % g.npx, t2 = e / g.npx;
-            const int q = t2 % nq, Y = t2 / nq;
-            const int sub = sub0 + q;
-            const int i = sub / n, j = sub - (sub / n) * n;
-            ccdf[(int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X] =
-                (float)((double)sS[q * QS + Y * RS + X] * scale);
+            const int X = e % npx, t2 = e / npx;
+            const int q = t2 % wv.nq, Y = t2 / wv.nq;
+            int i, j;
+            wave_sub_rc(wv, n, q, i, j);
+            const float v = sw[q * QS + (Y << 4) + (((X >> 2) ^ lds_swz(Y)) << 2) + (X & 3)];
+            if constexpr (!PARSEVAL) bsum += (double)v;
+            ccdf[(i * npx + Y) * W + j * npx + X] = v;
         }
     }
-    if (!g.parseval) {
-        bsum = block_sum(bsum, red);
-        if (threadIdx.x == 0) g.bpart[(int64_t)f * g.n_tiles + tile] = bsum;
+    if constexpr (!PARSEVAL) {
+        bsum = wave_sum(bsum);
+        if (lane == 0) g.bpart[(int64_t)f * g.n_parts + wave_id] = bsum;
     }
 }
 
-template <int NP>
-__device__ __forceinline__ void ngs_mfma_tile(const NgsArgs& g, int tile, int f, float4* lds, double* red) {
-    float av[NP], pv[NP];
-    ngs_mfma_load<NP>(g, tile, f, av, pv);
-    ngs_mfma_compute<NP>(g, tile, f, av, pv, lds, red);
-}
-
-template <int NP>
-__global__ void __launch_bounds__(256)
-sh_ngs_mfma_kernel(NgsArgs g) {
-    __shared__ float4 lds_all[NgsMfmaShape<NP>::LDS_F4];
-    __shared__ double red[4];
-    ngs_mfma_tile<NP>(g, blockIdx.x, blockIdx.y, lds_all, red);
-}
-
-// Streaming form: one block walks FPB consecutive frames of its tile and issues the next frame's
-// pupil loads before transforming the current one, so HBM latency overlaps the FFT/MFMA work.
-template <int NP, int FPB>
-__global__ void __launch_bounds__(256)
-sh_ngs_mfma_stream_kernel(NgsArgs g, int n_frames) {
-    __shared__ float4 lds_all[NgsMfmaShape<NP>::LDS_F4];
-    __shared__ double red[4];
-    const int tile = blockIdx.x;
-    const int f0 = blockIdx.y * FPB;
-    const int f1 = min(n_frames, f0 + FPB);
-    float av[NP], pv[NP], an[NP], pn[NP];
-    ngs_mfma_load<NP>(g, tile, f0, av, pv);
-    for (int f = f0; f < f1; ++f) {
-        if (f + 1 < f1) ngs_mfma_load<NP>(g, tile, f + 1, an, pn);
-        if (f > f0) __syncthreads();                     // LDS of the previous frame is free
-        ngs_mfma_compute<NP>(g, tile, f, av, pv, lds_all, red);
+template <int NP, int NPX, int COG>
+__device__ __forceinline__ NgsWaveConst<NP, NPX, COG> ngs_wave_const(const NgsArgs& g, const NgsWave& wv,
+                                                                    float* sw) {
+    constexpr int M = NgsMfmaShape<NP>::M;
+    const int lane = threadIdx.x & 63;
+    const int i16 = lane & 15, g4 = lane >> 4;
+    NgsWaveConst<NP, NPX, COG> wc;
+    wc.pv = *reinterpret_cast<const floatx4*>(g.roww + i16 * M + 4 * g4);
+    wc.qv = *reinterpret_cast<const floatx4*>(g.colw + i16 * M + 4 * g4);
+    wc.cword = (i16 << 4) + ((g4 ^ lds_swz(i16)) << 2);
+    wc.slot = -1;
+    {
+        const int q = lane / NP, y = lane - (lane / NP) * NP;
+        wc.ld_off = -1;
+        if (q < wv.nq) {
+            int i, j;
+            wave_sub_rc(wv, g.n_lenses, q, i, j);
+            wc.ld_off = (i * NP + y) * g.ef_size + j * NP;
+        }
+    }
+    wc.st_off = 0;
+    if constexpr (NPX > 0) {
+        using St = NgsStoreShape<NP, NPX>;
+        const int x4 = lane % St::X4, q = (lane / St::X4) % St::SW, h = lane / St::LPR;
+        int i, j;
+        wave_sub_rc(wv, g.n_lenses, q, i, j);
+        wc.st_off = (i * NPX + h) * (g.n_lenses * NPX) + j * NPX + 4 * x4;
+    }
+    if constexpr (COG != 0 && NPX > 0) {
+        using St = NgsStoreShape<NP, NPX>;
+        constexpr int SW = St::SW, X4 = St::X4, LPR = St::LPR, YSTEP = St::YSTEP, NY = St::NY;
+        const int x4 = lane % X4, q = (lane / X4) % SW, h = lane / LPR;
 #pragma unroll
-        for (int y = 0; y < NP; ++y) { av[y] = an[y]; pv[y] = pn[y]; }
+        for (int c = 0; c < 4; ++c) wc.xw[c] = g.cogw[4 * x4 + c];
+        if (lane < NPX) sw[kYwWord + lane] = g.cogw[NPX + lane];
+        wave_lds_sync();
+        (void)YSTEP; (void)NY; (void)h;
+        if (q < wv.nq) wc.slot = g.slot[wv.sub0 + q];
     }
+    return wc;
+}
+
+// Frames [f0, f1) of the wave's subapertures; the next frame's pupil loads are issued before the
+// current frame is transformed, so HBM latency overlaps the FFT / MFMA work.
+template <int NP, bool PARSEVAL, int NPX, int COG>
+__device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, int f0, int f1, float* sw) {
+    const NgsWave wv = ngs_wave<NP>(g, wave_id);
+    if (wv.nq == 0) {                     // past the last subaperture: empty partial sums only
+        if ((threadIdx.x & 63) == 0)
+            for (int f = f0; f < f1; ++f) {
+                if constexpr (!PARSEVAL) g.bpart[(int64_t)f * g.n_parts + wave_id] = 0.0;
+                if constexpr (COG != 0)
+                    *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + wave_id) * 2) = make_double2(0.0, 0.0);
+            }
+        return;
+    }
+    const NgsWaveConst<NP, NPX, COG> wc = ngs_wave_const<NP, NPX, COG>(g, wv, sw);
+    float av[NP], pv[NP], an[NP], pn[NP], fs, fn = 1.f;
+    ngs_mfma_load<NP, PARSEVAL>(g, wc.ld_off, f0, av, pv, fs);
+    for (int f = f0; f < f1; ++f) {
+        if (f + 1 < f1) ngs_mfma_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, an, pn, fn);
+        if (f > f0) wave_lds_sync();
+        ngs_mfma_frame<NP, PARSEVAL, NPX, COG>(g, wv, wave_id, f, av, pv, fs, sw, wc);
+#pragma unroll
+        for (int x = 0; x < NP; ++x) { av[x] = an[x]; pv[x] = pn[x]; }
+        fs = fn;
+    }
+}
+
+// grid = (n_tiles, ceil(n_frames / fpb)); block = 4 independent waves.
+template <int NP, bool PARSEVAL, int NPX, int COG>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 8 ? 4 : 3, 8)))
+sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
+    using Sh = NgsMfmaShape<NP>;
+    __shared__ float4 lds_all[Sh::LDS_F4];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int f0 = blockIdx.y * fpb;
+    const int f1 = min(n_frames, f0 + fpb);
+    ngs_mfma_frames<NP, PARSEVAL, NPX, COG>(g, blockIdx.x * Sh::WPB + w, f0, f1,
+                                            reinterpret_cast<float*>(lds_all + w * Sh::WAVE_F4));
 }
 
 // ---------------------------------------------------------------- LGS forward (sh.py:381-394)
@@ -677,7 +900,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     }
     if (!g.parseval) {
         bsum = block_sum(bsum, red);
-        if (t == 0) g.bpart[(int64_t)f * g.n_tiles + tile] = bsum;
+        if (t == 0) g.bpart[(int64_t)f * g.n_parts + tile] = bsum;
     }
 }
 
@@ -691,13 +914,13 @@ sh_lgs_forward_kernel(NgsArgs g) {
 
 // Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
 __global__ void sh_rescale_kernel(float* __restrict__ ccd, int64_t ccd_stride, int64_t n_px,
-                                  const double* __restrict__ bpart, int n_tiles,
+                                  const double* __restrict__ bpart, int n_parts,
                                   const double* __restrict__ amp, int amp_fstep,
                                   double S0, double pixel_area) {
     __shared__ double red[16];
     const int f = blockIdx.y;
     double t = 0.0;
-    for (int b = threadIdx.x; b < n_tiles; b += blockDim.x) t += bpart[(int64_t)f * n_tiles + b];
+    for (int b = threadIdx.x; b < n_parts; b += blockDim.x) t += bpart[(int64_t)f * n_parts + b];
     t = block_sum(t, red);
     const double scale = S0 * pixel_area * amp[(int64_t)f * amp_fstep] / t;
     float* c = ccd + f * ccd_stride;
@@ -842,6 +1065,42 @@ __global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
         if (flux) flux[(int64_t)f * n_sub + s] = m[4 * s];
     }
     if (totals && threadIdx.x == 0) {
+        totals[2 * f] = sf;
+        totals[2 * f + 1] = sf / n_sub;
+    }
+}
+
+
+// Fused-path variant: the frame totals come as per-wave partial sums from the SH kernel, so every
+// block reduces them (same fixed order in every block) and handles its own slice of subapertures.
+// grid = (ceil(n_sub / blockDim), n_frames).
+__global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const double* __restrict__ tpart,
+                                          int n_parts, int n_sub, float* __restrict__ slopes,
+                                          int64_t slopes_stride, double* __restrict__ flux,
+                                          double* __restrict__ totals, double* __restrict__ factor_out) {
+    __shared__ double red[16];
+    const int f = blockIdx.y;
+    const double* tp = tpart + (int64_t)f * n_parts * 2;
+    double sf = 0.0, st = 0.0;
+    for (int b = threadIdx.x; b < n_parts; b += blockDim.x) {
+        sf += tp[2 * b];
+        st += tp[2 * b + 1];
+    }
+    sf = block_sum(sf, red);
+    st = block_sum(st, red);
+    const double limit = 1.0 / ((st / n_sub) * 1e-3);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n_sub) {
+        const double* m = mom + ((int64_t)f * n_sub + s) * 4;
+        double factor = 1.0 / m[1];
+        if (factor > limit) factor = 0.0;
+        factor_out[(int64_t)f * n_sub + s] = factor;
+        float* sl = slopes + f * slopes_stride;
+        sl[s] = (float)(m[2] * factor);
+        sl[n_sub + s] = (float)(m[3] * factor);
+        flux[(int64_t)f * n_sub + s] = m[0];
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
         totals[2 * f] = sf;
         totals[2 * f + 1] = sf / n_sub;
     }
@@ -1094,9 +1353,17 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
         __syncthreads();                               // LDS of the previous item is free
         const BatchItem w = items[it];
         const NgsArgs& g = sensors[w.sensor];
-        if (w.kind == 0)
-            ngs_mfma_tile<8>(g, w.tile, w.frame, lds_raw, red);
-        else
+        if (w.kind == 0) {
+            const int wv = threadIdx.x >> 6;
+            float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
+            const int wave_id = w.tile * NgsMfmaShape<8>::WPB + wv;
+            if (g.npx == 16 && g.store_vec)
+                ngs_mfma_frames<8, true, 16, 0>(g, wave_id, w.frame, w.frame + 1, sw);
+            else if (g.npx == 8 && g.store_vec)
+                ngs_mfma_frames<8, true, 8, 0>(g, wave_id, w.frame, w.frame + 1, sw);
+            else
+                ngs_mfma_frames<8, true, 0, 0>(g, wave_id, w.frame, w.frame + 1, sw);
+        } else
             lgs_tile<16, 32, 8>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
         __syncthreads();
         if (threadIdx.x == 0) atomicMax(&ts[1 + w.sensor], (unsigned long long)wall_clock64());
@@ -1113,6 +1380,7 @@ struct shwfs_plan {
     int npx_pad = 0;
     int ts = 0;
     int n_tiles = 0;
+    int n_parts = 0;               // non-Parseval partial sums per frame
     int amp_blocks = 0;
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
@@ -1120,12 +1388,18 @@ struct shwfs_plan {
     int fpb = 4;                   // frames per block of the streaming MFMA kernel (1: no streaming)
     int L = 0;                     // LGS lag-grid size 2 np - 1
     float* d_roww = nullptr;
+    float* d_rowlag = nullptr;     // MFMA path: row operator in the lag domain, P~ = P A (16 x 16)
     float* d_colw = nullptr;
     float* d_kspec = nullptr;
     double* d_amp = nullptr;       // [max_frames][2]
+    float* d_scale = nullptr;      // [max_frames] Parseval flux scale
     double* d_amp_part = nullptr;  // [max_frames][amp_blocks][2]
     unsigned* d_counter = nullptr; // [max_frames]
-    double* d_bpart = nullptr;     // [max_frames][n_tiles]
+    double* d_bpart = nullptr;     // [max_frames][n_parts]
+    double* d_tpart = nullptr;     // [max_frames][n_parts][2] fused CoG frame-total partials
+    shslopec_plan* cog = nullptr;  // bound ShSlopec plan of the fused forward (shwfs_plan_bind_slopec)
+    int32_t* d_slot = nullptr;     // [n_sub] slope index per subaperture (-1: not valid)
+    float* d_cogw = nullptr;       // [2 npx] CoG x weights, y weights
     hipEvent_t ev_start = nullptr; // optional timing hook around the main kernel
     hipEvent_t ev_stop = nullptr;
 };
@@ -1135,6 +1409,8 @@ struct shslopec_plan {
     int n_sub = 0, np_sub = 0, frame_h = 0, frame_w = 0, max_frames = 0;
     bool rows_path = false;        // cog_rows_kernel (separable weights, square blocks)
     bool vec4 = false;
+    std::vector<int32_t> h_origin; // block origins (host copy; empty for index-list plans)
+    std::vector<double> h_sep;     // xv | yv of the separable weights (rows_path only)
     int32_t* d_sorted = nullptr;   // row-major sorted block origins
     int32_t* d_order = nullptr;    // subaperture index of each sorted slot
     double* d_sep = nullptr;       // xv | yv
@@ -1173,21 +1449,50 @@ int launch_ngs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
 
 int ngs_tile_size(int np) { return 256 / np; }
 
-template <int NP>
-int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
-    const int fpb = p->fpb;
-    if (fpb <= 1 || n_frames == 1) {
-        hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP>), dim3(p->n_tiles, n_frames), dim3(256), 0, st, a);
-    } else {
-        dim3 grid(p->n_tiles, (n_frames + fpb - 1) / fpb);
-        switch (fpb) {
-            case 2: hipLaunchKernelGGL((sh_ngs_mfma_stream_kernel<NP, 2>), grid, dim3(256), 0, st, a, n_frames); break;
-            case 4: hipLaunchKernelGGL((sh_ngs_mfma_stream_kernel<NP, 4>), grid, dim3(256), 0, st, a, n_frames); break;
-            default: hipLaunchKernelGGL((sh_ngs_mfma_stream_kernel<NP, 8>), grid, dim3(256), 0, st, a, n_frames); break;
-        }
-    }
+template <int NP, bool PARSEVAL, int NPX, int COG>
+int launch_mfma(const NgsArgs& a, int n_tiles, int n_frames, int fpb, hipStream_t st) {
+    dim3 grid(n_tiles, (n_frames + fpb - 1) / fpb);
+    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG>), grid, dim3(64 * NgsMfmaShape<NP>::WPB), 0, st,
+                       a, n_frames, fpb);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
+}
+
+// Fast CCD store layout for np 8 with 8 or 16 px per subaperture (aligned CCD), else the generic one.
+inline int mfma_store_npx(int np, const NgsArgs& a) {
+    if (np != 8 || !a.store_vec) return 0;
+    return (a.npx == 16 || a.npx == 8) ? a.npx : 0;
+}
+
+template <int NP>
+int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
+    const int fpb = (n_frames == 1) ? 1 : p->fpb;
+    const int T = p->n_tiles;
+    if constexpr (NP == 8) {
+        const int npx = mfma_store_npx(8, a);
+        if (a.cog) {   // fused CoG: Parseval scale and the fast store layout (checked at bind time)
+            if (!a.parseval || npx == 0) return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs the fast Parseval path");
+            const int mode = (a.cog == 1 && a.thr == 0.f) ? 3 : a.cog;   // 3: no threshold
+            if (npx == 16) return mode == 3 ? launch_mfma<8, true, 16, 3>(a, T, n_frames, fpb, st)
+                                : mode == 1 ? launch_mfma<8, true, 16, 1>(a, T, n_frames, fpb, st)
+                                            : launch_mfma<8, true, 16, 2>(a, T, n_frames, fpb, st);
+            return mode == 3 ? launch_mfma<8, true, 8, 3>(a, T, n_frames, fpb, st)
+                 : mode == 1 ? launch_mfma<8, true, 8, 1>(a, T, n_frames, fpb, st)
+                             : launch_mfma<8, true, 8, 2>(a, T, n_frames, fpb, st);
+        }
+        if (a.parseval) {
+            if (npx == 16) return launch_mfma<8, true, 16, 0>(a, T, n_frames, fpb, st);
+            if (npx == 8) return launch_mfma<8, true, 8, 0>(a, T, n_frames, fpb, st);
+            return launch_mfma<8, true, 0, 0>(a, T, n_frames, fpb, st);
+        }
+        if (npx == 16) return launch_mfma<8, false, 16, 0>(a, T, n_frames, fpb, st);
+        if (npx == 8) return launch_mfma<8, false, 8, 0>(a, T, n_frames, fpb, st);
+        return launch_mfma<8, false, 0, 0>(a, T, n_frames, fpb, st);
+    } else {
+        if (a.cog) return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs np_sub 8");
+        return a.parseval ? launch_mfma<NP, true, 0, 0>(a, T, n_frames, fpb, st)
+                          : launch_mfma<NP, false, 0, 0>(a, T, n_frames, fpb, st);
+    }
 }
 
 using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
@@ -1277,7 +1582,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     p->npx_pad = (g.subap_npx + 3) / 4 * 4;
     p->mfma = use_mfma;
     p->L = 2 * g.np_sub - 1;
-    p->amp_blocks = 64;
+    p->amp_blocks = 256;
     const int M = g.spec_size;
     std::vector<float> rw, cw;
     if (g.lgs) {
@@ -1301,20 +1606,47 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
             }
     }
     p->n_tiles = (p->n_sub + p->ts - 1) / p->ts;
+    p->n_parts = (!g.lgs && use_mfma) ? p->n_tiles * NgsMfmaShape<8>::WPB : p->n_tiles;
     auto cleanup = [&](int code) { shwfs_plan_destroy(p); return code; };
     hipError_t e = hipSuccess;
 #define PLAN_ALLOC(ptr, bytes)                                                                  \
     e = hipMalloc((void**)&(ptr), (bytes));                                                     \
     if (e != hipSuccess) return cleanup(set_error(SHWFS_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)));
     PLAN_ALLOC(p->d_roww, rw.size() * sizeof(float));
+    std::vector<float> rl;
+    if (!g.lgs && use_mfma) {
+        // P~[Y][j]: j = 0 -> C[0]; j = d -> Re C[d]; j = np-1+d -> Im C[d]   (d = 1 .. np-1), with
+        // |G[ky]|^2 = C[0] + 2 sum_d (Re C[d] cos(2 pi ky d/M) + Im C[d] sin(2 pi ky d/M))
+        rl.assign((size_t)16 * M, 0.f);
+        for (int Y = 0; Y < g.subap_npx; ++Y) {
+            const double* pr = row_w + (size_t)Y * M;
+            double c0 = 0.0;
+            for (int ky = 0; ky < M; ++ky) c0 += pr[ky];
+            rl[(size_t)Y * M] = (float)c0;
+            for (int d = 1; d < g.np_sub; ++d) {
+                double cr = 0.0, ci = 0.0;
+                for (int ky = 0; ky < M; ++ky) {
+                    const double th = 2.0 * M_PI * (double)((ky * d) % M) / M;
+                    cr += 2.0 * pr[ky] * std::cos(th);
+                    ci += 2.0 * pr[ky] * std::sin(th);
+                }
+                rl[(size_t)Y * M + d] = (float)cr;
+                rl[(size_t)Y * M + g.np_sub - 1 + d] = (float)ci;
+            }
+        }
+        PLAN_ALLOC(p->d_rowlag, rl.size() * sizeof(float));
+    }
     PLAN_ALLOC(p->d_colw, cw.size() * sizeof(float));
     PLAN_ALLOC(p->d_amp, (size_t)g.max_frames * 2 * sizeof(double));
+    PLAN_ALLOC(p->d_scale, (size_t)g.max_frames * sizeof(float));
     PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
     PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
-    PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_tiles * sizeof(double));
+    PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_parts * sizeof(double));
     if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
 #undef PLAN_ALLOC
-    if ((e = hipMemcpy(p->d_roww, rw.data(), rw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+    if ((!rl.empty() &&
+         (e = hipMemcpy(p->d_rowlag, rl.data(), rl.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) ||
+        (e = hipMemcpy(p->d_roww, rw.data(), rw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(p->d_colw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemset(p->d_counter, 0, (size_t)g.max_frames * sizeof(unsigned))) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e)));
@@ -1326,19 +1658,24 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     if (!p) return SHWFS_OK;
     DeviceGuard dg(p->device);
     (void)hipFree(p->d_roww);
+    (void)hipFree(p->d_rowlag);
     (void)hipFree(p->d_colw);
     (void)hipFree(p->d_kspec);
     (void)hipFree(p->d_amp);
+    (void)hipFree(p->d_scale);
     (void)hipFree(p->d_amp_part);
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_bpart);
+    (void)hipFree(p->d_slot);
+    (void)hipFree(p->d_tpart);
+    (void)hipFree(p->d_cogw);
     delete p;
     return SHWFS_OK;
 }
 
 int shwfs_plan_set_frames_per_block(shwfs_plan* p, int fpb) {
     if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
-    if (fpb != 1 && fpb != 2 && fpb != 4 && fpb != 8) return set_error(SHWFS_ERR_ARG, "frames per block must be 1, 2, 4 or 8");
+    if (fpb < 1 || fpb > 64) return set_error(SHWFS_ERR_ARG, "frames per block must be in [1, 64]");
     p->fpb = fpb;
     return SHWFS_OK;
 }
@@ -1361,9 +1698,20 @@ int shwfs_set_kernels(shwfs_plan* p, const float* kspec, void* stream) {
     return SHWFS_OK;
 }
 
-int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
-                  const float* phase, int64_t ph_stride, double S0, float* ccd,
-                  int64_t ccd_stride, void* stream) {
+}  // extern "C"
+
+namespace {
+inline bool aligned16(const void* ptr) { return (reinterpret_cast<uintptr_t>(ptr) & 15u) == 0; }
+
+// Fused CoG request of one forward call (nullptr: plain SH forward).
+struct CogCall {
+    float thr;
+    int thr_mode;
+};
+
+int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase,
+                    int64_t ph_stride, double S0, float* ccd, int64_t ccd_stride, const CogCall* cog,
+                    hipStream_t st) {
     if (!p || !A || !phase || !ccd) return set_error(SHWFS_ERR_ARG, "null argument");
     if (n_frames <= 0 || n_frames > p->g.max_frames)
         return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
@@ -1375,26 +1723,39 @@ int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
     if (n_frames > 1 && ccd_stride < ccd_side * ccd_side)
         return set_error(SHWFS_ERR_ARG, "ccd frame stride too small");
     DeviceGuard dg(p->device);
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
 
     // amplitude statistics: one pass per distinct amplitude frame
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
     hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp);
+                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, S0 * g.pixel_area,
+                       g.norm_const, g.parseval ? p->d_scale : nullptr);
     SHWFS_LAUNCH_CHECK();
 
     NgsArgs a{};
+    a.fscale = p->d_scale; a.scale_fstep = (a_stride == 0) ? 0 : 1;
     a.A = A; a.a_stride = a_stride;
     a.phase = phase; a.ph_stride = ph_stride;
     a.ccd = ccd; a.ccd_stride = ccd_stride;
-    a.roww = p->d_roww; a.colw = p->d_colw;
+    a.roww = (p->mfma && !g.lgs) ? p->d_rowlag : p->d_roww; a.colw = p->d_colw;
     a.amp = p->d_amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
-    a.bpart = p->d_bpart;
+    a.bpart = p->d_bpart; a.n_parts = p->n_parts;
     a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
     a.n_sub = p->n_sub; a.n_tiles = p->n_tiles; a.parseval = g.parseval;
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0;
     a.S0 = S0; a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
     a.kspec = reinterpret_cast<const float2*>(p->d_kspec);
+    a.vec = (g.ef_size % 4 == 0) && aligned16(A) && aligned16(phase) && (a_stride % 4 == 0) &&
+            (n_frames == 1 || ph_stride % 4 == 0);
+    a.store_vec = aligned16(ccd) && (ccd_side % 4 == 0) && (n_frames == 1 || ccd_stride % 4 == 0);
+    if (cog) {
+        a.cog = cog->thr_mode + 1;
+        a.thr = cog->thr;
+        a.slot = p->d_slot;
+        a.mom = p->cog->d_mom;
+        a.cogw = p->d_cogw;
+        a.tpart = p->d_tpart;
+        a.n_valid = p->cog->n_sub;
+    }
     if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan: call shwfs_set_kernels first");
     int lts = 0;
     NgsLauncher L = g.lgs ? pick_lgs(g.np_sub, g.spec_size, &lts)
@@ -1405,10 +1766,89 @@ int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
     if (p->ev_stop) SHWFS_HIP_CHECK(hipEventRecord(p->ev_stop, st));
     if (!g.parseval) {
         hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, n_frames), dim3(256), 0, st, ccd, ccd_stride,
-                           ccd_side * ccd_side, p->d_bpart, p->n_tiles, p->d_amp,
+                           ccd_side * ccd_side, p->d_bpart, p->n_parts, p->d_amp,
                            (a_stride == 0) ? 0 : 2, S0, g.pixel_area);
         SHWFS_LAUNCH_CHECK();
     }
+    return SHWFS_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
+                  const float* phase, int64_t ph_stride, double S0, float* ccd,
+                  int64_t ccd_stride, void* stream) {
+    return sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, ccd, ccd_stride, nullptr,
+                           reinterpret_cast<hipStream_t>(stream));
+}
+
+int shwfs_plan_bind_slopec(shwfs_plan* p, shslopec_plan* c) {
+    if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
+    DeviceGuard dg(p->device);
+    p->cog = nullptr;
+    if (!c) return SHWFS_OK;
+    const shwfs_geom& g = p->g;
+    const int n = g.n_lenses, npx = g.subap_npx, W = n * npx;
+    if (g.lgs || !p->mfma || g.np_sub != 8 || !g.parseval || (npx != 8 && npx != 16))
+        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs an NGS MFMA plan with np_sub 8, Parseval "
+                                                "normalisation and 8 or 16 px per subaperture");
+    if (c->device != p->device) return set_error(SHWFS_ERR_ARG, "plans on different devices");
+    if (!c->rows_path || c->np_sub != npx || c->frame_h != W || c->frame_w != W)
+        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs square subaperture blocks of the SH pixel "
+                                                "grid with separable weights (the ShSlopec default)");
+    std::vector<int32_t> slot((size_t)p->n_sub, -1);
+    for (int s = 0; s < c->n_sub; ++s) {
+        const int32_t o = c->h_origin[s];
+        const int r0 = o / W, c0 = o % W;
+        if (r0 % npx != 0 || c0 % npx != 0)
+            return set_error(SHWFS_ERR_UNSUPPORTED, "subaperture block not aligned to the lenslet grid");
+        const int sub = (r0 / npx) * n + c0 / npx;
+        if (slot[sub] >= 0) return set_error(SHWFS_ERR_UNSUPPORTED, "two subapertures share one lenslet block");
+        slot[sub] = s;
+    }
+    std::vector<float> w(2 * (size_t)npx);
+    for (int k = 0; k < 2 * npx; ++k) w[k] = (float)c->h_sep[k];
+    if (!p->d_slot) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_slot, (size_t)p->n_sub * sizeof(int32_t)));
+    if (!p->d_cogw) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_cogw, 2 * (size_t)npx * sizeof(float)));
+    if (!p->d_tpart)
+        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_tpart, (size_t)g.max_frames * p->n_parts * 2 * sizeof(double)));
+    SHWFS_HIP_CHECK(hipMemcpy(p->d_slot, slot.data(), slot.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    SHWFS_HIP_CHECK(hipMemcpy(p->d_cogw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    p->cog = c;
+    return SHWFS_OK;
+}
+
+int shwfs_forward_slopes(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
+                         const float* phase, int64_t ph_stride, double S0, float* ccd, int64_t ccd_stride,
+                         float thr, int thr_mode, float* slopes, int64_t slopes_stride, double* flux,
+                         double* totals, void* stream) {
+    if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
+    shslopec_plan* c = p->cog;
+    if (!c) return set_error(SHWFS_ERR_ARG, "no ShSlopec plan bound (shwfs_plan_bind_slopec)");
+    if (!slopes || !flux || !totals) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (thr_mode != 0 && thr_mode != 1) return set_error(SHWFS_ERR_ARG, "thr_mode must be 0 or 1");
+    if (n_frames <= 0 || n_frames > c->max_frames)
+        return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames] of the ShSlopec plan");
+    if (n_frames > 1 && slopes_stride < 2 * (int64_t)c->n_sub)
+        return set_error(SHWFS_ERR_ARG, "slopes frame stride too small");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int64_t ccd_side = (int64_t)p->g.n_lenses * p->g.subap_npx;
+    const bool fast = aligned16(ccd) && (n_frames == 1 || ccd_stride % 4 == 0) && (ccd_side % 4 == 0);
+    if (!fast) {   // unaligned CCD buffer: the two passes give the same result
+        int rc = sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, ccd, ccd_stride, nullptr, st);
+        if (rc) return rc;
+        return shslopec_forward(c, n_frames, ccd, ccd_stride, thr, thr_mode, slopes, slopes_stride, flux,
+                                totals, stream);
+    }
+    const CogCall cc{thr, thr_mode};
+    int rc = sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, ccd, ccd_stride, &cc, st);
+    if (rc) return rc;
+    DeviceGuard dg(p->device);
+    hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 255) / 256, n_frames), dim3(256), 0, st,
+                       c->d_mom, p->d_tpart, p->n_parts, c->n_sub, slopes, slopes_stride, flux, totals,
+                       c->d_factor);
+    SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
 
@@ -1437,6 +1877,7 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
     DeviceGuard dg(device);
     auto* p = new shslopec_plan();
     p->device = device; p->n_sub = n_sub; p->np_sub = np_sub;
+    if (block_origin) p->h_origin.assign(block_origin, block_origin + n_sub);
     p->frame_h = frame_h; p->frame_w = frame_w; p->max_frames = max_frames;
     auto cleanup = [&](int code) { shslopec_plan_destroy(p); return code; };
     hipError_t e;
@@ -1482,6 +1923,7 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
                 return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
             p->rows_path = true;
             p->vec4 = aligned;
+            p->h_sep = sepw;
         }
     } else {
         if ((e = hipMalloc((void**)&p->d_idx, n_sub * npp * sizeof(int32_t))) != hipSuccess ||
@@ -1595,7 +2037,7 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
     hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp);
+                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, 0.0, 0.0, (float*)nullptr);
     SHWFS_LAUNCH_CHECK();
     VjpArgs a{};
     a.A = A; a.a_stride = a_stride; a.phase = phase; a.ph_stride = ph_stride;
@@ -1675,6 +2117,7 @@ struct shwfs_batch {
     NgsArgs* d_args = nullptr;
     BatchItem* d_items = nullptr;
     double* d_amp = nullptr;          // [n_sensors][2]
+    float* d_scale = nullptr;         // [n_sensors]
     double* d_amp_part = nullptr;
     unsigned* d_counter = nullptr;
     unsigned long long* d_ts = nullptr;   // [1 + n_sensors]
@@ -1688,6 +2131,7 @@ int shwfs_batch_destroy(shwfs_batch* b) {
     (void)hipFree(b->d_args);
     (void)hipFree(b->d_items);
     (void)hipFree(b->d_amp);
+    (void)hipFree(b->d_scale);
     (void)hipFree(b->d_amp_part);
     (void)hipFree(b->d_counter);
     (void)hipFree(b->d_ts);
@@ -1724,10 +2168,12 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         a.A = A[s]; a.a_stride = 0; a.phase = phase[s]; a.ph_stride = (int64_t)g.ef_size * g.ef_size;
         a.ccd = ccd[s];
         a.ccd_stride = (int64_t)g.n_lenses * g.subap_npx * g.n_lenses * g.subap_npx;
-        a.roww = p->d_roww; a.colw = p->d_colw; a.amp_fstep = 0; a.bpart = nullptr;
+        a.roww = (kind == 0) ? p->d_rowlag : p->d_roww; a.colw = p->d_colw; a.amp_fstep = 0; a.bpart = nullptr;
         a.kspec = reinterpret_cast<const float2*>(p->d_kspec);
         a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
         a.n_sub = p->n_sub; a.n_tiles = tiles; a.parseval = 1;
+        a.vec = (g.ef_size % 4 == 0) && aligned16(A[s]) && aligned16(phase[s]);
+        a.store_vec = aligned16(ccd[s]) && ((int64_t)g.n_lenses * g.subap_npx) % 4 == 0;
         a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0[s];
         a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
         for (int f = 0; f < n_frames[s]; ++f)
@@ -1755,10 +2201,11 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     b->grid = grid_blocks > 0 ? std::min(grid_blocks, b->n_items) : b->n_items;
     auto cleanup = [&](int code) { shwfs_batch_destroy(b); return code; };
     hipError_t e;
-    const int amp_blocks = 64;
+    const int amp_blocks = 256;
     if ((e = hipMalloc((void**)&b->d_args, n_sensors * sizeof(NgsArgs))) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_items, items.size() * sizeof(BatchItem))) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_amp, n_sensors * 2 * sizeof(double))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_scale, n_sensors * sizeof(float))) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_amp_part, n_sensors * amp_blocks * 2 * sizeof(double))) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_counter, n_sensors * sizeof(unsigned))) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_ts, (n_sensors + 1) * sizeof(unsigned long long))) != hipSuccess ||
@@ -1768,8 +2215,11 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     for (int s = 0; s < n_sensors; ++s) {
         const int64_t px = (int64_t)plans[s]->g.ef_size * plans[s]->g.ef_size;
         hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, px,
-                           b->d_amp_part + (size_t)s * amp_blocks * 2, b->d_counter + s, b->d_amp + 2 * s);
+                           b->d_amp_part + (size_t)s * amp_blocks * 2, b->d_counter + s, b->d_amp + 2 * s,
+                           S0[s] * plans[s]->g.pixel_area, plans[s]->g.norm_const, b->d_scale + s);
         args[s].amp = b->d_amp + 2 * s;
+        args[s].fscale = b->d_scale + s;
+        args[s].scale_fstep = 0;
     }
     if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
         (e = hipMemcpy(b->d_args, args.data(), n_sensors * sizeof(NgsArgs), hipMemcpyHostToDevice)) != hipSuccess ||

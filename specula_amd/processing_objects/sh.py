@@ -61,6 +61,8 @@ class SHPlan:
         g.norm_const = norm if norm is not None else 0.0
         g.pixel_area = geom.pixel_pitch ** 2
         self._geom_c = g
+        self.fpb = 4                                    # frames per block (library default)
+        self._slopec = None
         if lgs:     # complex operators as interleaved (re, im)
             rw = np.ascontiguousarray(np.stack([row.real, row.imag], axis=-1), dtype=np.float64)
             cw = np.ascontiguousarray(np.stack([col.real, col.imag], axis=-1), dtype=np.float64)
@@ -75,7 +77,7 @@ class SHPlan:
         self._h = h
 
     def set_frames_per_block(self, fpb):
-        """Streaming depth of the NGS MFMA kernel (1, 2, 4 or 8 frames per block)."""
+        """Streaming depth of the NGS MFMA kernel (1 to 64 frames per block)."""
         _native.check(_native.lib().shwfs_plan_set_frames_per_block(self._h, int(fpb)), 'set_frames_per_block')
         self.fpb = int(fpb)
 
@@ -88,9 +90,39 @@ class SHPlan:
         _native.check(_native.lib().shwfs_set_kernels(self._h, _native.ptr(kspec), _native.stream_handle(stream)),
                       'shwfs_set_kernels')
 
-    def forward(self, A, phase, out, S0, n_frames=1, a_frame_stride=None, stream=None):
-        """A, phase: (n_frames,) ef x ef float32 device tensors (A may be 2-D = shared);
-        out: (n_frames,) ccd x ccd float32."""
+    def bind_slopec(self, slopec):
+        """Fuse the centre of gravity of a SlopecPlan into this plan's forward
+        (shwfs_plan_bind_slopec).  Returns False when the pair cannot be fused; ``None`` unbinds."""
+        h = slopec._h if slopec is not None else None
+        rc = _native.lib().shwfs_plan_bind_slopec(self._h, h)
+        if rc == _native.SHWFS_ERR_UNSUPPORTED:
+            self._slopec = None
+            return False
+        _native.check(rc, 'shwfs_plan_bind_slopec')
+        self._slopec = slopec      # keeps the bound plan alive
+        return slopec is not None
+
+    def forward_slopes(self, A, phase, out, S0, slopes, flux, totals, thr=0.0, pedestal=False, n_frames=1,
+                       a_frame_stride=None, stream=None):
+        """SH forward plus the bound ShSlopec in one pass (equal to ``forward`` followed by
+        ``SlopecPlan.forward(out, ...)``)."""
+        sc = getattr(self, '_slopec', None)
+        if sc is None:
+            raise RuntimeError('no SlopecPlan bound: call bind_slopec first')
+        a_frame_stride = self._check(A, phase, out, n_frames, a_frame_stride)
+        for t, dt in ((slopes, torch.float32), (flux, torch.float64), (totals, torch.float64)):
+            if t.dtype != dt or not t.is_contiguous() or not t.is_cuda:
+                raise ValueError('slopes float32, flux/totals float64, contiguous, on device')
+        if (slopes.numel() < n_frames * 2 * sc.n_sub or flux.numel() < n_frames * sc.n_sub
+                or totals.numel() < 2 * n_frames):
+            raise ValueError('ShSlopec buffer sizes do not match the plan')
+        ef, ccd = self.geom.ef_size, self.geom.ccd_side
+        _native.check(_native.lib().shwfs_forward_slopes(
+            self._h, n_frames, _native.ptr(A), a_frame_stride, _native.ptr(phase), ef * ef, float(S0),
+            _native.ptr(out), ccd * ccd, float(thr), 1 if pedestal else 0, _native.ptr(slopes), 2 * sc.n_sub,
+            _native.ptr(flux), _native.ptr(totals), _native.stream_handle(stream)), 'shwfs_forward_slopes')
+
+    def _check(self, A, phase, out, n_frames, a_frame_stride):
         for t in (A, phase, out):
             if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
                 raise ValueError('SH buffers must be contiguous float32 device tensors')
@@ -102,6 +134,13 @@ class SHPlan:
             a_frame_stride = 0 if A.numel() == ef * ef else ef * ef
         if A.numel() < (ef * ef if a_frame_stride == 0 else n_frames * ef * ef):
             raise ValueError('amplitude buffer too small')
+        return a_frame_stride
+
+    def forward(self, A, phase, out, S0, n_frames=1, a_frame_stride=None, stream=None):
+        """A, phase: (n_frames,) ef x ef float32 device tensors (A may be 2-D = shared);
+        out: (n_frames,) ccd x ccd float32."""
+        a_frame_stride = self._check(A, phase, out, n_frames, a_frame_stride)
+        ef, ccd = self.geom.ef_size, self.geom.ccd_side
         _native.check(_native.lib().shwfs_forward(
             self._h, n_frames, _native.ptr(A), a_frame_stride, _native.ptr(phase), ef * ef,
             float(S0), _native.ptr(out), ccd * ccd, _native.stream_handle(stream)), 'shwfs_forward')

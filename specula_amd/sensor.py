@@ -43,16 +43,24 @@ class SHSensor:
         self.totals = torch.empty((max_frames, 2), dtype=torch.float64, device=self.device)
         self._gccd = None
         self._slot = None
+        # CoG fused into the SH kernel epilogue when the geometry allows it (same results)
+        self.fused = self.plan.bind_slopec(self.slopec)
 
     # ------------------------------------------------------------------ forward
-    def forward(self, phase, S0=1.0, stream=None):
-        """phase: (F, ef, ef) float32 device tensor -> slopes view (F, 2 n_sub) (buffers reused)."""
+    def forward(self, phase, S0=1.0, stream=None, two_pass=False):
+        """phase: (F, ef, ef) float32 device tensor -> slopes view (F, 2 n_sub) (buffers reused).
+        The CoG runs in the SH kernel epilogue when the geometry allows it; ``two_pass`` forces
+        the separate ShSlopec pass (FP64 sums per pixel instead of per 32-pixel partial)."""
         F = phase.shape[0]
         if F > self.max_frames:
             raise ValueError(f'{F} frames > max_frames {self.max_frames}')
-        self.plan.forward(self.A, phase, self.ccd[:F], S0, n_frames=F, stream=stream)
-        self.slopec.forward(self.ccd[:F], self.slopes_buf[:F], self.flux[:F], self.totals[:F], thr=self.thr,
-                            n_frames=F, stream=stream)
+        if self.fused and not two_pass:
+            self.plan.forward_slopes(self.A, phase, self.ccd[:F], S0, self.slopes_buf[:F], self.flux[:F],
+                                     self.totals[:F], thr=self.thr, n_frames=F, stream=stream)
+        else:
+            self.plan.forward(self.A, phase, self.ccd[:F], S0, n_frames=F, stream=stream)
+            self.slopec.forward(self.ccd[:F], self.slopes_buf[:F], self.flux[:F], self.totals[:F], thr=self.thr,
+                                n_frames=F, stream=stream)
         return self.slopes_buf[:F]
 
     # ------------------------------------------------------------------ adjoint

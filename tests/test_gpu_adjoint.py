@@ -42,7 +42,9 @@ def test_vjp_matches_fp64_finite_differences():
 
 def test_vjp_matches_gpu_finite_differences_50_directions():
     """ELT 80x80 sensor: 50 seeded directions, central differences (1e-3 rad) of the FP32 forward,
-    all 100 perturbed frames in one batched launch."""
+    all 100 perturbed frames in one batched launch.  The differences use the two-pass CoG (FP64
+    sums per pixel): at this step size the FP32 partial sums of the fused epilogue add rounding
+    noise of the same order as the smallest directional derivatives."""
     wl = workloads.elt_ngs_80x80()
     sen = make(wl, 100)
     ph = torch.tensor(wl.frames(1)[0].astype(np.float32), device=DEV)
@@ -52,14 +54,18 @@ def test_vjp_matches_gpu_finite_differences_50_directions():
     k = 2 * np.pi / sen.geom.wavelengthInNm
     h = 1e-3 / k
     frames = torch.cat([ph + h * V, ph - h * V]).contiguous()
-    s = sen.forward(frames).double()
+    s = sen.forward(frames, two_pass=True).double()
     g = torch.randn(2 * sen.n_sub, generator=gen).to(DEV)
     fd = ((s[:50] - s[50:]) @ g.double()) / (2 * h)
     grad = sen.vjp(ph, g[None].float())[0]
     adj = (V * grad).sum(dim=(1, 2)).double()
     rel = ((adj - fd).abs() / fd.abs().clamp_min(1e-30)).cpu().numpy()
-    print(f'50 directions: median rel err {np.median(rel):.2e}, max {rel.max():.2e}', flush=True)
-    assert np.median(rel) < 1e-2 and rel.max() < 5e-2
+    # error against the typical derivative size: a direction whose derivative happens to be
+    # small is limited by the FP32 noise of the differences, not by the adjoint
+    err = ((adj - fd).abs() / fd.pow(2).mean().sqrt()).cpu().numpy()
+    print(f'50 directions: median rel err {np.median(rel):.2e}, max {rel.max():.2e}; '
+          f'max err / rms(fd) {err.max():.2e}', flush=True)
+    assert np.median(rel) < 1e-2 and err.max() < 1e-2
 
 
 def test_autograd_function():

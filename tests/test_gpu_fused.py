@@ -1,0 +1,119 @@
+"""SH forward with the ShSlopec centre of gravity fused into the kernel epilogue
+(shwfs_forward_slopes) against the two-pass path (shwfs_forward + shslopec_forward) and the
+FP64 oracle."""
+import numpy as np
+import pytest
+import torch
+
+from helpers import check_slopes
+from oracle import sh_oracle as O
+from specula_amd import workloads
+from specula_amd.lib.subaps import detect_subaps
+from specula_amd.processing_objects.sh import SHPlan
+from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weights
+from specula_amd.sensor import SHSensor
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+
+
+def _setup(wl, F):
+    g = wl.geometry()
+    A = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
+    plan = SHPlan(g, max_frames=F, device=0)
+    flat = torch.zeros((1, g.ef_size, g.ef_size), dtype=torch.float32, device=DEV)
+    img = torch.empty((1, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan.forward(A, flat, img, 1.0)
+    idxs, _ = detect_subaps(img[0].cpu().numpy(), g.n_lenses, 0.5)
+    sc = SlopecPlan(idxs, (g.ccd_side, g.ccd_side), compute_xy_weights(g.subap_npx), max_frames=F, device=0)
+    return g, A, plan, sc, idxs
+
+
+def _buffers(g, n_sub, F):
+    return (torch.empty((F, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV),
+            torch.empty((F, 2 * n_sub), dtype=torch.float32, device=DEV),
+            torch.empty((F, n_sub), dtype=torch.float64, device=DEV),
+            torch.empty((F, 2), dtype=torch.float64, device=DEV))
+
+
+@pytest.mark.parametrize('name', ['elt_ngs_80x80', 'scao_ngs_20x20'])
+@pytest.mark.parametrize('thr_rel,pedestal', [(0.0, False), (0.3, False), (0.3, True)])
+def test_fused_equals_two_pass(name, thr_rel, pedestal):
+    wl = getattr(workloads, name)()
+    F = 5                              # with 4 frames per block: one full and one partial group
+    g, A, plan, sc, idxs = _setup(wl, F)
+    n_sub = idxs.shape[0]
+    print(f'fused case {name} thr_rel={thr_rel} pedestal={pedestal} n_sub={n_sub}', flush=True)
+    ph = torch.tensor(wl.frames(F).astype(np.float32), device=DEV)
+    ccd2, s2, f2, t2 = _buffers(g, n_sub, F)
+    plan.forward(A, ph, ccd2, 1.0, n_frames=F)
+    thr = float(thr_rel * ccd2.mean()) if thr_rel > 0 else 0.0
+    sc.forward(ccd2, s2, f2, t2, thr=thr, pedestal=pedestal, n_frames=F)
+    assert plan.bind_slopec(sc)
+    ccd1, s1, f1, t1 = _buffers(g, n_sub, F)
+    plan.forward_slopes(A, ph, ccd1, 1.0, s1, f1, t1, thr=thr, pedestal=pedestal, n_frames=F)
+    torch.cuda.synchronize()
+    assert torch.equal(ccd1, ccd2)                       # same kernel arithmetic for the image
+    # moments: FP32 partials of 4 x 8 pixels per lane (fused) vs FP64 per pixel (two-pass)
+    torch.testing.assert_close(s1, s2, rtol=0, atol=2e-6)
+    torch.testing.assert_close(f1, f2, rtol=2e-6, atol=0)
+    torch.testing.assert_close(t1, t2, rtol=1e-6, atol=0)
+    out = ccd1[0].double().cpu().numpy()
+    ref = O.shslopec(out, idxs, thr_value=thr if thr > 0 else -1, thr_pedestal=pedestal)
+    check_slopes(s1[0].cpu().numpy(), ref['slopes'], g.subap_npx)
+    np.testing.assert_allclose(f1[0].cpu().numpy(), ref['flux'], rtol=1e-5, atol=1e-7 * ref['flux'].max())
+
+
+def test_fused_frames_per_block_and_repeat():
+    """Streaming depth does not change the fused result; a second call overwrites the first."""
+    wl = workloads.scao_ngs_20x20()
+    F = 7
+    g, A, plan, sc, idxs = _setup(wl, F)
+    n_sub = idxs.shape[0]
+    assert plan.bind_slopec(sc)
+    ph = torch.tensor(wl.frames(F).astype(np.float32), device=DEV)
+    ref = _buffers(g, n_sub, F)
+    plan.set_frames_per_block(1)
+    plan.forward_slopes(A, ph, ref[0], 1.0, *ref[1:], n_frames=F)
+    for fpb in (2, 3, 8, 16):
+        out = _buffers(g, n_sub, F)
+        plan.set_frames_per_block(fpb)
+        plan.forward_slopes(A, ph, out[0], 1.0, *out[1:], n_frames=F)
+        plan.forward_slopes(A, ph, out[0], 1.0, *out[1:], n_frames=F)
+        torch.cuda.synchronize()
+        for a, b in zip(out, ref):
+            assert torch.equal(a, b), fpb
+
+
+def test_fused_bind_rejects_unsupported_pairs():
+    wl = workloads.scao_ngs_20x20()
+    g, A, plan, sc, idxs = _setup(wl, 1)
+    # index-list addressing (permuted pixel order) cannot be fused
+    perm = np.random.default_rng(0).permutation(idxs.shape[1])
+    w = {k: v.ravel()[perm].reshape(v.shape) for k, v in compute_xy_weights(g.subap_npx).items()}
+    sc_idx = SlopecPlan(idxs[:, perm], (g.ccd_side, g.ccd_side), w, max_frames=1, device=0)
+    assert not sc_idx.block_mode
+    assert plan.bind_slopec(sc_idx) is False
+    # register-GEMV plan
+    plan_v1 = SHPlan(g, max_frames=1, device=0, variant=1)
+    assert plan_v1.bind_slopec(sc) is False
+    assert plan.bind_slopec(sc) is True
+    assert plan.bind_slopec(None) is False
+    with pytest.raises(RuntimeError):
+        s = torch.empty((1, 2 * idxs.shape[0]), dtype=torch.float32, device=DEV)
+        plan.forward_slopes(A, A[None].contiguous(), torch.empty((1, g.ccd_side, g.ccd_side), device=DEV), 1.0,
+                            s, torch.empty((1, idxs.shape[0]), dtype=torch.float64, device=DEV),
+                            torch.empty((1, 2), dtype=torch.float64, device=DEV))
+
+
+def test_sensor_uses_fused_path():
+    wl = workloads.elt_ngs_80x80()
+    sen = SHSensor(wl.geometry(), wl.pupil(), max_frames=3, device=0)
+    assert sen.fused
+    ph = torch.tensor(wl.frames(3).astype(np.float32), device=DEV)
+    s = sen.forward(ph).clone()
+    ccd, s2, f2, t2 = _buffers(sen.geom, sen.n_sub, 3)
+    sen.plan.forward(sen.A, ph, ccd, 1.0, n_frames=3)
+    sen.slopec.forward(ccd, s2, f2, t2, n_frames=3)
+    torch.cuda.synchronize()
+    torch.testing.assert_close(s, s2, rtol=0, atol=2e-6)

@@ -40,6 +40,7 @@ def run_case(kw, ef, pitch, obs, seed, fov_res=0.0, n_frames=1, S0=1.0, variant=
     for k in range(n_frames):
         ref = O.sh_forward(A, frames[k].astype(np.float64), go, S0=S0)
         st = check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
+        print(f'  frame {k}: total flux rel err {st["total"]:.2e}, normalised image err {st["img"]:.2e}', flush=True)
     return st, plan
 
 
@@ -65,6 +66,16 @@ def test_cut_mask_lcm_parity(kw, variant):
     assert not plan.parseval
 
 
+@pytest.mark.parametrize('npx,px', [(8, 0.25), (16, 0.125)])
+def test_np8_cut_fast_store_parity(npx, px):
+    """np 8 with a FoV cut (N=40, cut 8): the MFMA kernel's non-Parseval partial sums with the
+    row-segment CCD stores of 8 and 16 px subapertures, 3 frames streamed through one block."""
+    kw = dict(wavelengthInNm=600, subap_wanted_fov=1.6 if npx == 8 else 2.0, sensor_pxscale=px,
+              subap_on_diameter=20, subap_npx=npx, fov_ovs_coeff=1.0)
+    st, plan = run_case(kw, 160, 0.05, 0.14, seed=7, n_frames=3, variant=2)
+    assert not plan.parseval and plan.geom.ovs_np_sub == 8 and plan.geom.cutpixels == 8
+
+
 @pytest.mark.parametrize('variant', [1, 2])
 def test_elt_ngs_80x80_parity(variant):
     wl = workloads.elt_ngs_80x80()
@@ -76,7 +87,9 @@ def test_elt_ngs_80x80_parity(variant):
     go = oracle_geometry(wl)
     for k in range(2):
         ref = O.sh_forward(A, frames[k].astype(np.float64), go)
-        check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
+        st = check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
+        print(f'  ELT frame {k} variant {variant}: total flux rel err {st["total"]:.2e}, '
+              f'normalised image err {st["img"]:.2e}', flush=True)
 
 
 def test_batch_equals_single_and_stride():

@@ -97,6 +97,54 @@ def test_binning_operators_reproduce_oracle(case):
     assert (norm is not None) == (g.cutpixels == 0 and g.fp_mask.all())
 
 
+def lag_row_operator(row, np_sub):
+    """P~ = P A: columns [C0, Re C1..C_{np-1}, Im C1..C_{np-1}] of the lag-domain power spectrum."""
+    npx, M = row.shape
+    out = np.zeros((npx, M))
+    ky = np.arange(M)
+    out[:, 0] = row.sum(axis=1)
+    for d in range(1, np_sub):
+        th = 2 * np.pi * ((ky * d) % M) / M
+        out[:, d] = 2 * row @ np.cos(th)
+        out[:, np_sub - 1 + d] = 2 * row @ np.sin(th)
+    return out
+
+
+@pytest.mark.parametrize('case', range(len(GEOMS)))
+def test_lag_domain_binning_reproduces_oracle(case):
+    """MFMA-kernel formulation: FFT_16 along x, autocorrelation along y (2np-1 lag reals), then
+    B = (P A) C Q^T with the cosine/sine synthesis A folded into the row operator."""
+    ef, pitch, kw = GEOMS[case]
+    g = sh_geometry(ef, pitch, **kw)
+    row, col, M, norm = build_operators(g)
+    p = g.ovs_np_sub
+    if M != 16 or p > 8:
+        pytest.skip('lag-domain path is the M = 16 kernel')
+    go = O.sh_geometry(ef_size=ef, pixel_pitch=pitch, **kw)
+    A = annular_pupil(ef, 0.2)
+    ph = von_karman_opd(256, pitch, seed=case)[:ef, :ef]
+    ref = O.sh_forward(A, ph, go, S0=1.0)
+    rl = lag_row_operator(row, p)
+    n, N, npx = g.n_lenses, g.fft_size, g.subap_npx
+    a = np.arange(p) + (-p) // 2
+    tilt = np.exp(-1j * np.pi * (a[None, :] + a[:, None]) / N)
+    k = 2 * np.pi / g.wavelengthInNm
+    out = np.zeros((n * npx, n * npx))
+    for i in range(n):
+        for j in range(n):
+            sl = np.s_[i * p:(i + 1) * p, j * p:(j + 1) * p]
+            R = np.fft.fft(A[sl] * np.exp(1j * k * ph[sl]) * tilt, n=M, axis=1)   # R[y][kx]
+            C = np.zeros((M, M))
+            C[0] = (np.abs(R) ** 2).sum(axis=0)
+            for d in range(1, p):
+                c = (R[d:] * np.conj(R[:p - d])).sum(axis=0)
+                C[d], C[p - 1 + d] = c.real, c.imag
+            out[i * npx:(i + 1) * npx, j * npx:(j + 1) * npx] = rl @ C @ col.T
+    phot = g.pixel_pitch ** 2 * A.sum()
+    out *= phot / (norm * (A ** 2).sum()) if norm is not None else phot / out.sum()
+    np.testing.assert_allclose(out, ref, rtol=0, atol=1e-12 * ref.max())
+
+
 def test_spec_size():
     assert [spec_size(n) for n in (4, 5, 6, 8, 9, 16, 17)] == [8, 16, 16, 16, 32, 32, 64]
 
