@@ -229,6 +229,8 @@ def main():
     ev = [(hip_event(), hip_event()) for _ in range(args.steps)]
     # CoG fused into the SH kernel epilogue (one pass; the CCD frames are still written)
     fused = (not args.unfused) and plan.bind_slopec(sl_plan)
+    # the telescope pupil is static: its statistics (masked area, Parseval scale) once
+    plan.set_static_pupil(At, 1.0)
 
     def step(k=None):
         if k is not None:

@@ -40,6 +40,7 @@ typedef struct shwfs_geom {
     int32_t parseval;      /* 1: sum of the binned image = norm_const * sum(A^2) exactly      */
     int32_t max_frames;    /* largest n_frames accepted by shwfs_forward                      */
     int32_t variant;       /* kernel choice: 0 auto, 1 register GEMV binning, 2 MFMA binning  */
+                           /* (FP32), 3 MFMA binning from split bf16 pieces (hi + lo)        */
     double  k_rad_per_nm;  /* 2*pi/lambda (electric_field.py:65)                              */
     double  tilt_rad;      /* pi/N: half-pixel tilt coefficient (sh.py:469-477)               */
     double  tilt_a0;       /* first tilt abscissa: -ceil(np_sub/2)                            */
@@ -77,6 +78,15 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
                   const float* A, int64_t a_frame_stride,
                   const float* phase_nm, int64_t phase_frame_stride,
                   double S0, float* ccd, int64_t ccd_frame_stride, void* stream);
+
+/* Register a pupil amplitude that does not change between calls: its statistics (sum(A),
+ * sum(A^2) and the Parseval flux scale for S0) are computed once here, on `stream`, and later
+ * shwfs_forward / shwfs_forward_slopes / shwfs_vjp calls with the same A pointer, frame stride and
+ * S0 reuse them instead of recomputing them per call.  The caller must register A again after
+ * changing its contents; A = NULL forgets the registration.  No reference counterpart: SH
+ * recomputes the masked area from in_ef.A on every trigger (processing_objects/sh.py:422-428). */
+int shwfs_plan_set_pupil(shwfs_plan* plan, const float* A, int64_t a_frame_stride, int n_frames,
+                         double S0, void* stream);
 
 /* Optional timing hook: when both events are non-null (hipEvent_t), shwfs_forward records
  * `start` immediately before and `stop` immediately after the main SH kernel launch, on the

@@ -1,7 +1,7 @@
 """Measurement-driven choice of launch shapes.
 
-* ``tune_plan_variant``: register-GEMV versus MFMA binning for an NGS plan (the matrix cores are
-  used only when they measure faster).
+* ``tune_plan_variant``: register-GEMV, FP32 MFMA or bf16-split MFMA binning for an NGS plan (the
+  matrix cores are used only when they measure faster).
 * ``tune_batch``: persistent grid size and work-list order of a MultiSensorBatch.
 
 Results are cached in a JSON file keyed by device name and workload signature
@@ -69,7 +69,7 @@ def tune_plan_variant(geom, pupil, n_frames=32, device=0, use_cache=True):
     ph = torch.zeros((n_frames, geom.ef_size, geom.ef_size), dtype=torch.float32, device=dev)
     out = torch.empty((n_frames, geom.ccd_side, geom.ccd_side), dtype=torch.float32, device=dev)
     times = {}
-    for v in (1, 2):
+    for v in (1, 2, 3):
         try:
             plan = SHPlan(geom, max_frames=n_frames, device=device, variant=v)
         except RuntimeError:

@@ -81,6 +81,21 @@ __device__ double block_sum(double v, double* red) {
     return s;
 }
 
+// Block-wide FP64 sums of two values at once (one pair of barriers); `red` needs blockDim/32
+// doubles.  Results valid in every thread, fixed summation order.
+__device__ double2 block_sum2(double a, double b, double* red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+    const int nw = (blockDim.x + kWave - 1) / kWave;
+    __syncthreads();
+    if (lane == 0) { red[2 * wid] = a; red[2 * wid + 1] = b; }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+    for (int w = 0; w < nw; ++w) { sa += red[2 * w]; sb += red[2 * w + 1]; }
+    return make_double2(sa, sb);
+}
+
 // Phase of one pupil sample: (2 pi / lambda) * phi - (pi / N) * (ax + ay), reduced to
 // [-pi, pi] in FP64 before the FP32 sincos (the unreduced phase reaches tens of radians).
 __device__ __forceinline__ float2 field_sample(float a, float ph, double k, double tilt, double ax_ay) {
@@ -110,7 +125,7 @@ __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, 
                                  double* __restrict__ part, unsigned* __restrict__ counter,
                                  double* __restrict__ out, double scale_num, double norm_const,
                                  float* __restrict__ scale_out) {
-    __shared__ double red[16];
+    __shared__ double red[32];
     __shared__ bool last;
     const int f = blockIdx.y;
     const float* a = A + f * a_stride;
@@ -137,12 +152,11 @@ __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, 
             s1 += v; s2 += v * v;
         }
     }
-    s1 = block_sum(s1, red);
-    s2 = block_sum(s2, red);
+    const double2 s12 = block_sum2(s1, s2, red);
     double* p = part + ((int64_t)f * gridDim.x + blockIdx.x) * 2;
     if (threadIdx.x == 0) {
-        p[0] = s1;
-        p[1] = s2;
+        p[0] = s12.x;
+        p[1] = s12.y;
         __threadfence();
         unsigned prev = atomicAdd(&counter[f], 1u);
         last = (prev == gridDim.x - 1);
@@ -157,8 +171,9 @@ __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, 
             t2 += pf[2 * b + 1];
         }
         // per-thread strided partials, then fixed-order block reduction
-        t1 = block_sum(t1, red);
-        t2 = block_sum(t2, red);
+        const double2 t12 = block_sum2(t1, t2, red);
+        t1 = t12.x;
+        t2 = t12.y;
         if (threadIdx.x == 0) {
             out[2 * f] = t1;
             out[2 * f + 1] = t2;
@@ -348,6 +363,27 @@ sh_ngs_forward_kernel(NgsArgs g) {
 //      centre-of-gravity moments of each subaperture from the same registers.
 // R (np complex per row) and S^T / B (16 reals per row) share the 64 bytes of an LDS row.
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// Split of four floats into bf16 pieces [hi(4) | lo(4)]: hi = x rounded to bf16, lo = the
+// residual x - hi (exact in FP32) rounded to bf16, so |x - hi - lo| <= 2^-17 |x| (unbiased).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8 split_bf16(floatx4 v) {
+    const bf16x2 h01 = {(__bf16)v[0], (__bf16)v[1]};     // one packed conversion per pair
+    const bf16x2 h23 = {(__bf16)v[2], (__bf16)v[3]};
+    const unsigned u01 = __builtin_bit_cast(unsigned, h01), u23 = __builtin_bit_cast(unsigned, h23);
+    const bf16x2 l01 = {(__bf16)(v[0] - __uint_as_float(u01 << 16)), (__bf16)(v[1] - __uint_as_float(u01 & 0xffff0000u))};
+    const bf16x2 l23 = {(__bf16)(v[2] - __uint_as_float(u23 << 16)), (__bf16)(v[3] - __uint_as_float(u23 & 0xffff0000u))};
+    const uintx4 r = {u01, u23, __builtin_bit_cast(unsigned, l01), __builtin_bit_cast(unsigned, l23)};
+    return __builtin_bit_cast(bf16x8, r);
+}
+
+// [lo | hi] from [hi | lo]
+__device__ __forceinline__ bf16x8 swap_halves(bf16x8 v) {
+    return __builtin_shufflevector(v, v, 4, 5, 6, 7, 0, 1, 2, 3);
+}
 
 template <int NP>
 struct NgsMfmaShape {
@@ -449,6 +485,7 @@ __device__ __forceinline__ void ngs_mfma_load(const NgsArgs& g, int off, int f, 
 template <int NP, int NPX, int COG>
 struct NgsWaveConst {
     floatx4 pv, qv;       // binning operator chunks of this lane (P row l%16, Q row l%16, chunk l/16)
+    bf16x8 pb_hl, pb_lh;  // BF3: [P_hi | P_lo] and [P_lo | P_hi] of pv
     // fused CoG y weights live in the 16 padding words of subaperture 0 (never touched by the
     // transform): LDS words kYwWord .. kYwWord + npx - 1
     int cword;            // LDS word of this lane's phase-C chunk (without the subaperture offset)
@@ -461,7 +498,7 @@ struct NgsWaveConst {
 // Transform one frame of the wave's subapertures and store it (plus CoG moments when COG != 0:
 // 1 = subtract-and-clamp threshold, 2 = pedestal threshold, 3 = no threshold (clamp at zero),
 // sh_slopec.py:380-387).
-template <int NP, bool PARSEVAL, int NPX, int COG>
+template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& wv, int wave_id, int f,
                                                const float (&av)[NP], const float (&phv)[NP], float fs, float* sw,
                                                const NgsWaveConst<NP, NPX, COG>& wc) {
@@ -542,9 +579,30 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
     // C: binning on the matrix cores; the frame's flux scale is folded into Q (Parseval mode)
     floatx4 qv = wc.qv;
     if constexpr (PARSEVAL) qv *= fs;
+    bf16x8 qa_hl, qa_lh;                 // BF3: [Q_hi | Q_lo], [Q_lo | Q_hi] of the scaled Q chunk
+    if constexpr (BF3) {
+        qa_hl = split_bf16(qv);
+        qa_lh = swap_halves(qa_hl);
+    }
     auto bin_one = [&](int q) {
         floatx4* chunk = reinterpret_cast<floatx4*>(sw + q * QS + wc.cword);
         const floatx4 sv = *chunk;
+        if constexpr (BF3) {
+            // FP32-level products from bf16 pieces x = x_hi + x_lo: the K slots of a lane hold
+            // [hi(4) | lo(4)] of its four j (or kx) values, so [x_hi | x_lo] . [y_hi | y_lo] and
+            // [x_hi | x_lo] . [y_lo | y_hi] (two 16x16x32 products) give all four cross terms of
+            // (x_hi + x_lo)(y_hi + y_lo); what remains is the 2^-17 representation error.
+            const bf16x8 a = split_bf16(sv);                         // [C_hi | C_lo]
+            floatx4 t = {0.f, 0.f, 0.f, 0.f};
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wc.pb_hl, t, 0, 0, 0);   // C_hi P_hi + C_lo P_lo
+            t = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wc.pb_lh, t, 0, 0, 0);   // C_hi P_lo + C_lo P_hi
+            const bf16x8 b = split_bf16(t);                          // [T_hi | T_lo]
+            floatx4 d = {0.f, 0.f, 0.f, 0.f};
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qa_hl, b, d, 0, 0, 0);      // Q_hi T_hi + Q_lo T_lo
+            d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(qa_lh, b, d, 0, 0, 0);      // Q_lo T_hi + Q_hi T_lo
+            *chunk = d;
+            return;
+        }
         floatx4 t = {0.f, 0.f, 0.f, 0.f};
         t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[0], wc.pv[0], t, 0, 0, 0);
         t = __builtin_amdgcn_mfma_f32_16x16x4f32(sv[1], wc.pv[1], t, 0, 0, 0);
@@ -667,6 +725,8 @@ __device__ __forceinline__ NgsWaveConst<NP, NPX, COG> ngs_wave_const(const NgsAr
     wc.pv = *reinterpret_cast<const floatx4*>(g.roww + i16 * M + 4 * g4);
     wc.qv = *reinterpret_cast<const floatx4*>(g.colw + i16 * M + 4 * g4);
     wc.cword = (i16 << 4) + ((g4 ^ lds_swz(i16)) << 2);
+    wc.pb_hl = split_bf16(wc.pv);
+    wc.pb_lh = swap_halves(wc.pb_hl);
     wc.slot = -1;
     {
         const int q = lane / NP, y = lane - (lane / NP) * NP;
@@ -701,7 +761,7 @@ __device__ __forceinline__ NgsWaveConst<NP, NPX, COG> ngs_wave_const(const NgsAr
 
 // Frames [f0, f1) of the wave's subapertures; the next frame's pupil loads are issued before the
 // current frame is transformed, so HBM latency overlaps the FFT / MFMA work.
-template <int NP, bool PARSEVAL, int NPX, int COG>
+template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, int f0, int f1, float* sw) {
     const NgsWave wv = ngs_wave<NP>(g, wave_id);
     if (wv.nq == 0) {                     // past the last subaperture: empty partial sums only
@@ -719,7 +779,7 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
     for (int f = f0; f < f1; ++f) {
         if (f + 1 < f1) ngs_mfma_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, an, pn, fn);
         if (f > f0) wave_lds_sync();
-        ngs_mfma_frame<NP, PARSEVAL, NPX, COG>(g, wv, wave_id, f, av, pv, fs, sw, wc);
+        ngs_mfma_frame<NP, PARSEVAL, NPX, COG, BF3>(g, wv, wave_id, f, av, pv, fs, sw, wc);
 #pragma unroll
         for (int x = 0; x < NP; ++x) { av[x] = an[x]; pv[x] = pn[x]; }
         fs = fn;
@@ -727,7 +787,7 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
 }
 
 // grid = (n_tiles, ceil(n_frames / fpb)); block = 4 independent waves.
-template <int NP, bool PARSEVAL, int NPX, int COG>
+template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 8 ? 4 : 3, 8)))
 sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
     using Sh = NgsMfmaShape<NP>;
@@ -735,7 +795,7 @@ sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int f0 = blockIdx.y * fpb;
     const int f1 = min(n_frames, f0 + fpb);
-    ngs_mfma_frames<NP, PARSEVAL, NPX, COG>(g, blockIdx.x * Sh::WPB + w, f0, f1,
+    ngs_mfma_frames<NP, PARSEVAL, NPX, COG, BF3>(g, blockIdx.x * Sh::WPB + w, f0, f1,
                                             reinterpret_cast<float*>(lds_all + w * Sh::WAVE_F4));
 }
 
@@ -1078,23 +1138,25 @@ __global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const 
                                           int n_parts, int n_sub, float* __restrict__ slopes,
                                           int64_t slopes_stride, double* __restrict__ flux,
                                           double* __restrict__ totals, double* __restrict__ factor_out) {
-    __shared__ double red[16];
+    __shared__ double red[32];
     const int f = blockIdx.y;
     const double* tp = tpart + (int64_t)f * n_parts * 2;
     double sf = 0.0, st = 0.0;
     for (int b = threadIdx.x; b < n_parts; b += blockDim.x) {
-        sf += tp[2 * b];
-        st += tp[2 * b + 1];
+        const double2 v = *reinterpret_cast<const double2*>(tp + 2 * b);
+        sf += v.x;
+        st += v.y;
     }
-    sf = block_sum(sf, red);
-    st = block_sum(st, red);
+    const double2 sft = block_sum2(sf, st, red);
+    sf = sft.x;
+    st = sft.y;
     const double limit = 1.0 / ((st / n_sub) * 1e-3);
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s < n_sub) {
         const double* m = mom + ((int64_t)f * n_sub + s) * 4;
         double factor = 1.0 / m[1];
         if (factor > limit) factor = 0.0;
-        factor_out[(int64_t)f * n_sub + s] = factor;
+        if (factor_out) factor_out[(int64_t)f * n_sub + s] = factor;
         float* sl = slopes + f * slopes_stride;
         sl[s] = (float)(m[2] * factor);
         sl[n_sub + s] = (float)(m[3] * factor);
@@ -1358,11 +1420,11 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
             float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
             const int wave_id = w.tile * NgsMfmaShape<8>::WPB + wv;
             if (g.npx == 16 && g.store_vec)
-                ngs_mfma_frames<8, true, 16, 0>(g, wave_id, w.frame, w.frame + 1, sw);
+                ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, w.frame, w.frame + 1, sw);
             else if (g.npx == 8 && g.store_vec)
-                ngs_mfma_frames<8, true, 8, 0>(g, wave_id, w.frame, w.frame + 1, sw);
+                ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, w.frame, w.frame + 1, sw);
             else
-                ngs_mfma_frames<8, true, 0, 0>(g, wave_id, w.frame, w.frame + 1, sw);
+                ngs_mfma_frames<8, true, 0, 0, false>(g, wave_id, w.frame, w.frame + 1, sw);
         } else
             lgs_tile<16, 32, 8>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
         __syncthreads();
@@ -1384,6 +1446,7 @@ struct shwfs_plan {
     int amp_blocks = 0;
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
+    bool bf3 = false;              // MFMA binning from bf16 pieces (three-term split, variant 3)
     bool kernels_set = false;      // LGS: kernel spectra uploaded
     int fpb = 4;                   // frames per block of the streaming MFMA kernel (1: no streaming)
     int L = 0;                     // LGS lag-grid size 2 np - 1
@@ -1393,6 +1456,10 @@ struct shwfs_plan {
     float* d_kspec = nullptr;
     double* d_amp = nullptr;       // [max_frames][2]
     float* d_scale = nullptr;      // [max_frames] Parseval flux scale
+    const float* static_A = nullptr;   // pupil whose statistics are held in d_amp / d_scale
+    int64_t static_a_stride = 0;
+    int static_frames = 0;
+    double static_S0 = 0.0;
     double* d_amp_part = nullptr;  // [max_frames][amp_blocks][2]
     unsigned* d_counter = nullptr; // [max_frames]
     double* d_bpart = nullptr;     // [max_frames][n_parts]
@@ -1449,11 +1516,11 @@ int launch_ngs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
 
 int ngs_tile_size(int np) { return 256 / np; }
 
-template <int NP, bool PARSEVAL, int NPX, int COG>
+template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 int launch_mfma(const NgsArgs& a, int n_tiles, int n_frames, int fpb, hipStream_t st) {
     dim3 grid(n_tiles, (n_frames + fpb - 1) / fpb);
-    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG>), grid, dim3(64 * NgsMfmaShape<NP>::WPB), 0, st,
-                       a, n_frames, fpb);
+    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG, BF3>), grid, dim3(64 * NgsMfmaShape<NP>::WPB), 0,
+                       st, a, n_frames, fpb);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -1464,35 +1531,40 @@ inline int mfma_store_npx(int np, const NgsArgs& a) {
     return (a.npx == 16 || a.npx == 8) ? a.npx : 0;
 }
 
-template <int NP>
-int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
-    const int fpb = (n_frames == 1) ? 1 : p->fpb;
-    const int T = p->n_tiles;
+template <int NP, bool BF3>
+int launch_ngs_mfma_prec(const NgsArgs& a, int T, int n_frames, int fpb, hipStream_t st) {
     if constexpr (NP == 8) {
         const int npx = mfma_store_npx(8, a);
         if (a.cog) {   // fused CoG: Parseval scale and the fast store layout (checked at bind time)
             if (!a.parseval || npx == 0) return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs the fast Parseval path");
             const int mode = (a.cog == 1 && a.thr == 0.f) ? 3 : a.cog;   // 3: no threshold
-            if (npx == 16) return mode == 3 ? launch_mfma<8, true, 16, 3>(a, T, n_frames, fpb, st)
-                                : mode == 1 ? launch_mfma<8, true, 16, 1>(a, T, n_frames, fpb, st)
-                                            : launch_mfma<8, true, 16, 2>(a, T, n_frames, fpb, st);
-            return mode == 3 ? launch_mfma<8, true, 8, 3>(a, T, n_frames, fpb, st)
-                 : mode == 1 ? launch_mfma<8, true, 8, 1>(a, T, n_frames, fpb, st)
-                             : launch_mfma<8, true, 8, 2>(a, T, n_frames, fpb, st);
+            if (npx == 16) return mode == 3 ? launch_mfma<8, true, 16, 3, BF3>(a, T, n_frames, fpb, st)
+                                : mode == 1 ? launch_mfma<8, true, 16, 1, BF3>(a, T, n_frames, fpb, st)
+                                            : launch_mfma<8, true, 16, 2, BF3>(a, T, n_frames, fpb, st);
+            return mode == 3 ? launch_mfma<8, true, 8, 3, BF3>(a, T, n_frames, fpb, st)
+                 : mode == 1 ? launch_mfma<8, true, 8, 1, BF3>(a, T, n_frames, fpb, st)
+                             : launch_mfma<8, true, 8, 2, BF3>(a, T, n_frames, fpb, st);
         }
         if (a.parseval) {
-            if (npx == 16) return launch_mfma<8, true, 16, 0>(a, T, n_frames, fpb, st);
-            if (npx == 8) return launch_mfma<8, true, 8, 0>(a, T, n_frames, fpb, st);
-            return launch_mfma<8, true, 0, 0>(a, T, n_frames, fpb, st);
+            if (npx == 16) return launch_mfma<8, true, 16, 0, BF3>(a, T, n_frames, fpb, st);
+            if (npx == 8) return launch_mfma<8, true, 8, 0, BF3>(a, T, n_frames, fpb, st);
+            return launch_mfma<8, true, 0, 0, BF3>(a, T, n_frames, fpb, st);
         }
-        if (npx == 16) return launch_mfma<8, false, 16, 0>(a, T, n_frames, fpb, st);
-        if (npx == 8) return launch_mfma<8, false, 8, 0>(a, T, n_frames, fpb, st);
-        return launch_mfma<8, false, 0, 0>(a, T, n_frames, fpb, st);
+        if (npx == 16) return launch_mfma<8, false, 16, 0, BF3>(a, T, n_frames, fpb, st);
+        if (npx == 8) return launch_mfma<8, false, 8, 0, BF3>(a, T, n_frames, fpb, st);
+        return launch_mfma<8, false, 0, 0, BF3>(a, T, n_frames, fpb, st);
     } else {
         if (a.cog) return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs np_sub 8");
-        return a.parseval ? launch_mfma<NP, true, 0, 0>(a, T, n_frames, fpb, st)
-                          : launch_mfma<NP, false, 0, 0>(a, T, n_frames, fpb, st);
+        return a.parseval ? launch_mfma<NP, true, 0, 0, BF3>(a, T, n_frames, fpb, st)
+                          : launch_mfma<NP, false, 0, 0, BF3>(a, T, n_frames, fpb, st);
     }
+}
+
+template <int NP>
+int launch_ngs_mfma(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
+    const int fpb = (n_frames == 1) ? 1 : p->fpb;
+    return p->bf3 ? launch_ngs_mfma_prec<NP, true>(a, p->n_tiles, n_frames, fpb, st)
+                  : launch_ngs_mfma_prec<NP, false>(a, p->n_tiles, n_frames, fpb, st);
 }
 
 using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
@@ -1558,7 +1630,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         return set_error(SHWFS_ERR_ARG, "ef_size must equal n_lenses * np_sub");
     if (g.spec_size < 2 * g.np_sub - 1)
         return set_error(SHWFS_ERR_ARG, "spec_size must be >= 2*np_sub-1");
-    if (g.variant < 0 || g.variant > 2) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1 or 2");
+    if (g.variant < 0 || g.variant > 3) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1, 2 or 3");
     NgsLauncher mf = nullptr, v1 = nullptr, lg = nullptr;
     int lgs_ts = 0;
     bool use_mfma = false;
@@ -1569,7 +1641,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     } else {
         mf = pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx);
         v1 = pick_ngs(g.np_sub, g.spec_size);
-        use_mfma = (g.variant == 2) || (g.variant == 0 && mf != nullptr);
+        use_mfma = (g.variant >= 2) || (g.variant == 0 && mf != nullptr);
         if (use_mfma && !mf) return set_error(SHWFS_ERR_UNSUPPORTED, "MFMA variant needs spec_size 16, np_sub 5..8, subap_npx <= 16");
         if (!use_mfma && !v1)
             return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
@@ -1581,6 +1653,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     p->n_sub = g.n_lenses * g.n_lenses;
     p->npx_pad = (g.subap_npx + 3) / 4 * 4;
     p->mfma = use_mfma;
+    p->bf3 = use_mfma && g.variant == 3;
     p->L = 2 * g.np_sub - 1;
     p->amp_blocks = 256;
     const int M = g.spec_size;
@@ -1724,12 +1797,18 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
         return set_error(SHWFS_ERR_ARG, "ccd frame stride too small");
     DeviceGuard dg(p->device);
 
-    // amplitude statistics: one pass per distinct amplitude frame
+    // amplitude statistics: one pass per distinct amplitude frame, unless this is the pupil
+    // registered with shwfs_plan_set_pupil (statistics already on the device)
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
-    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, S0 * g.pixel_area,
-                       g.norm_const, g.parseval ? p->d_scale : nullptr);
-    SHWFS_LAUNCH_CHECK();
+    const bool cached = A == p->static_A && a_stride == p->static_a_stride && S0 == p->static_S0 &&
+                        amp_frames <= p->static_frames;
+    if (!cached) {
+        p->static_A = nullptr;
+        hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
+                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, S0 * g.pixel_area,
+                           g.norm_const, g.parseval ? p->d_scale : nullptr);
+        SHWFS_LAUNCH_CHECK();
+    }
 
     NgsArgs a{};
     a.fscale = p->d_scale; a.scale_fstep = (a_stride == 0) ? 0 : 1;
@@ -1781,6 +1860,27 @@ int shwfs_forward(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride,
                   int64_t ccd_stride, void* stream) {
     return sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, ccd, ccd_stride, nullptr,
                            reinterpret_cast<hipStream_t>(stream));
+}
+
+int shwfs_plan_set_pupil(shwfs_plan* p, const float* A, int64_t a_stride, int n_frames, double S0, void* stream) {
+    if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
+    p->static_A = nullptr;
+    if (!A) return SHWFS_OK;
+    const shwfs_geom& g = p->g;
+    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
+    if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
+    if (a_stride != 0 && a_stride < ef_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
+    const int amp_frames = (a_stride == 0) ? 1 : n_frames;
+    DeviceGuard dg(p->device);
+    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), A, a_stride, ef_px, p->d_amp_part, p->d_counter,
+                       p->d_amp, S0 * g.pixel_area, g.norm_const, g.parseval ? p->d_scale : nullptr);
+    SHWFS_LAUNCH_CHECK();
+    p->static_A = A;
+    p->static_a_stride = a_stride;
+    p->static_frames = amp_frames;
+    p->static_S0 = S0;
+    return SHWFS_OK;
 }
 
 int shwfs_plan_bind_slopec(shwfs_plan* p, shslopec_plan* c) {
@@ -1847,7 +1947,7 @@ int shwfs_forward_slopes(shwfs_plan* p, int n_frames, const float* A, int64_t a_
     DeviceGuard dg(p->device);
     hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 255) / 256, n_frames), dim3(256), 0, st,
                        c->d_mom, p->d_tpart, p->n_parts, c->n_sub, slopes, slopes_stride, flux, totals,
-                       c->d_factor);
+                       (double*)nullptr);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2036,9 +2136,12 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
     DeviceGuard dg(p->device);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
-    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                       A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, 0.0, 0.0, (float*)nullptr);
-    SHWFS_LAUNCH_CHECK();
+    if (!(A == p->static_A && a_stride == p->static_a_stride && amp_frames <= p->static_frames)) {
+        p->static_A = nullptr;       // d_amp now describes this A
+        hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
+                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, 0.0, 0.0, (float*)nullptr);
+        SHWFS_LAUNCH_CHECK();
+    }
     VjpArgs a{};
     a.A = A; a.a_stride = a_stride; a.phase = phase; a.ph_stride = ph_stride;
     a.gccd = grad_ccd; a.gccd_stride = gccd_stride; a.gphase = grad_phase; a.gph_stride = gph_stride;

@@ -26,7 +26,8 @@ class SHPlan:
 
     def __init__(self, geom, lgs=False, max_frames=1, device=0, variant=0, kernels_normalised=True):
         """variant: 0 picks the kernel automatically, 1 forces register-GEMV binning, 2 forces
-        MFMA binning (spec_size 16, np_sub 5..8, subap_npx <= 16).
+        FP32 MFMA binning (spec_size 16, np_sub 5..8, subap_npx <= 16), 3 MFMA binning from split
+        bf16 pieces (x = hi + lo, all four cross terms; relative error ~2^-17 per operand).
         lgs: per-subaperture kernel correlation; spectra come from ``set_kernels``.  With
         ``kernels_normalised`` (unit-sum kernels) the Parseval normalisation applies."""
         row, col, M, norm = build_operators(geom, lgs=lgs)
@@ -89,6 +90,28 @@ class SHPlan:
             raise ValueError(f'kernel spectra must be a contiguous complex64 ({n_sub}, {self.L}, {self.L}) tensor')
         _native.check(_native.lib().shwfs_set_kernels(self._h, _native.ptr(kspec), _native.stream_handle(stream)),
                       'shwfs_set_kernels')
+
+    def set_static_pupil(self, A, S0=1.0, n_frames=1, a_frame_stride=None, stream=None):
+        """Register an amplitude that stays constant (shwfs_plan_set_pupil): its statistics are
+        computed once and reused by forward calls with the same tensor, stride and S0.  Register
+        again after modifying ``A``; ``None`` forgets it."""
+        if A is None:
+            _native.check(_native.lib().shwfs_plan_set_pupil(self._h, None, 0, 1, 0.0, None), 'set_pupil')
+            self._static_key = None
+            return
+        ef = self.geom.ef_size
+        if A.dtype != torch.float32 or not A.is_contiguous() or not A.is_cuda:
+            raise ValueError('pupil must be a contiguous float32 device tensor')
+        if a_frame_stride is None:
+            a_frame_stride = 0 if A.numel() == ef * ef else ef * ef
+        _native.check(_native.lib().shwfs_plan_set_pupil(self._h, _native.ptr(A), a_frame_stride, n_frames,
+                                                         float(S0), _native.stream_handle(stream)), 'set_pupil')
+        self._static_key = (A.data_ptr(), A._version, a_frame_stride, float(S0))
+
+    def static_pupil_is(self, A, S0, a_frame_stride):
+        """True when ``A`` (same storage, not modified since) is the registered static pupil."""
+        key = getattr(self, '_static_key', None)
+        return key is not None and key == (A.data_ptr(), A._version, a_frame_stride, float(S0))
 
     def bind_slopec(self, slopec):
         """Fuse the centre of gravity of a SlopecPlan into this plan's forward
@@ -322,6 +345,9 @@ class SH(BaseProcessingObj):
 
     def trigger_code(self):
         ef, A, ph = self._inputs_fp32()
+        if not self.use_graph and not self._plan.static_pupil_is(A, ef.S0, 0):
+            # pupil statistics once per distinct amplitude (storage and in-place version)
+            self._plan.set_static_pupil(A, ef.S0, stream=self.stream)
         self._plan.forward(A, ph, self._out_i.i, ef.S0, n_frames=1, stream=self.stream)
 
     def trigger(self):
@@ -330,6 +356,7 @@ class SH(BaseProcessingObj):
             key = (A.data_ptr(), ph.data_ptr(), float(ef.S0), self._out_i.i.data_ptr())
             if key != self._graph_key:
                 self._keep = (A, ph)
+                self._plan.set_static_pupil(None)   # the graph recomputes them on every replay
                 self.capture_stream()
                 self._graph_key = key
         super().trigger()

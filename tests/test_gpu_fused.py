@@ -17,10 +17,10 @@ pytestmark = pytest.mark.gpu
 DEV = 'cuda:0'
 
 
-def _setup(wl, F):
+def _setup(wl, F, variant=0):
     g = wl.geometry()
     A = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
-    plan = SHPlan(g, max_frames=F, device=0)
+    plan = SHPlan(g, max_frames=F, device=0, variant=variant)
     flat = torch.zeros((1, g.ef_size, g.ef_size), dtype=torch.float32, device=DEV)
     img = torch.empty((1, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
     plan.forward(A, flat, img, 1.0)
@@ -36,14 +36,15 @@ def _buffers(g, n_sub, F):
             torch.empty((F, 2), dtype=torch.float64, device=DEV))
 
 
+@pytest.mark.parametrize('variant', [0, 3])
 @pytest.mark.parametrize('name', ['elt_ngs_80x80', 'scao_ngs_20x20'])
 @pytest.mark.parametrize('thr_rel,pedestal', [(0.0, False), (0.3, False), (0.3, True)])
-def test_fused_equals_two_pass(name, thr_rel, pedestal):
+def test_fused_equals_two_pass(name, thr_rel, pedestal, variant):
     wl = getattr(workloads, name)()
     F = 5                              # with 4 frames per block: one full and one partial group
-    g, A, plan, sc, idxs = _setup(wl, F)
+    g, A, plan, sc, idxs = _setup(wl, F, variant)
     n_sub = idxs.shape[0]
-    print(f'fused case {name} thr_rel={thr_rel} pedestal={pedestal} n_sub={n_sub}', flush=True)
+    print(f'fused case {name} thr_rel={thr_rel} pedestal={pedestal} variant={variant} n_sub={n_sub}', flush=True)
     ph = torch.tensor(wl.frames(F).astype(np.float32), device=DEV)
     ccd2, s2, f2, t2 = _buffers(g, n_sub, F)
     plan.forward(A, ph, ccd2, 1.0, n_frames=F)

@@ -44,7 +44,7 @@ def run_case(kw, ef, pitch, obs, seed, fov_res=0.0, n_frames=1, S0=1.0, variant=
     return st, plan
 
 
-@pytest.mark.parametrize('variant', [1, 2])
+@pytest.mark.parametrize('variant', [1, 2, 3])
 def test_scao_ngs_20x20_parity(variant):
     wl = workloads.scao_ngs_20x20()
     st, plan = run_case(wl.sh_kwargs, wl.ef_size, wl.pixel_pitch, wl.obstruction, wl.seed, n_frames=3, S0=7.5,
@@ -60,23 +60,24 @@ def test_scao_ngs_20x20_parity(variant):
     dict(wavelengthInNm=500, subap_wanted_fov=1.2, sensor_pxscale=0.2, subap_on_diameter=20, subap_npx=6,
          fov_ovs_coeff=1.0),    # N=24, cut 10, 14 -> 6 LCM rebin
 ])
-@pytest.mark.parametrize('variant', [1, 2])
+@pytest.mark.parametrize('variant', [1, 2, 3])
 def test_cut_mask_lcm_parity(kw, variant):
     st, plan = run_case(kw, 120, 0.05, 0.1, seed=5, variant=variant)
     assert not plan.parseval
 
 
+@pytest.mark.parametrize('variant', [2, 3])
 @pytest.mark.parametrize('npx,px', [(8, 0.25), (16, 0.125)])
-def test_np8_cut_fast_store_parity(npx, px):
+def test_np8_cut_fast_store_parity(npx, px, variant):
     """np 8 with a FoV cut (N=40, cut 8): the MFMA kernel's non-Parseval partial sums with the
     row-segment CCD stores of 8 and 16 px subapertures, 3 frames streamed through one block."""
     kw = dict(wavelengthInNm=600, subap_wanted_fov=1.6 if npx == 8 else 2.0, sensor_pxscale=px,
               subap_on_diameter=20, subap_npx=npx, fov_ovs_coeff=1.0)
-    st, plan = run_case(kw, 160, 0.05, 0.14, seed=7, n_frames=3, variant=2)
+    st, plan = run_case(kw, 160, 0.05, 0.14, seed=7, n_frames=3, variant=variant)
     assert not plan.parseval and plan.geom.ovs_np_sub == 8 and plan.geom.cutpixels == 8
 
 
-@pytest.mark.parametrize('variant', [1, 2])
+@pytest.mark.parametrize('variant', [1, 2, 3])
 def test_elt_ngs_80x80_parity(variant):
     wl = workloads.elt_ngs_80x80()
     g = wl.geometry()
@@ -199,3 +200,57 @@ def test_streaming_frames_per_block_is_exact(fpb):
     plan.forward(A, ph, out, 1.0, n_frames=7)
     torch.cuda.synchronize()
     assert torch.equal(out, ref)
+
+
+def test_static_pupil_registration():
+    """shwfs_plan_set_pupil: registered statistics give the same image as per-call statistics;
+    a different amplitude (other storage or re-registered contents) is not served from it."""
+    wl = workloads.scao_ngs_20x20()
+    g = wl.geometry()
+    A = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
+    ph = torch.tensor(wl.frames(2).astype(np.float32), device=DEV)
+    plan = SHPlan(g, max_frames=2, device=0)
+    ref = torch.empty((2, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan.forward(A, ph, ref, 2.0, n_frames=2)
+    plan.set_static_pupil(A, 2.0)
+    out = torch.empty_like(ref)
+    plan.forward(A, ph, out, 2.0, n_frames=2)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    A2 = A.clone()
+    A2[: g.ef_size // 2] *= 0.5                      # other storage: recomputed per call
+    ref2 = torch.empty_like(ref)
+    plan.forward(A2, ph, ref2, 2.0, n_frames=2)
+    plan.forward(A, ph, out, 2.0, n_frames=2)      # back to A: recomputed, registration dropped
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref) and not torch.equal(ref2, ref)
+    assert abs(float(ref2.sum()) / float(ref.sum()) - float(A2.sum()) / float(A.sum())) < 1e-5
+    plan.set_static_pupil(A2, 2.0)
+    plan.forward(A2, ph, out, 2.0, n_frames=2)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref2)
+
+
+def test_sh_object_follows_in_place_pupil_change():
+    """The drop-in SH keys its pupil statistics on the tensor's storage and in-place version."""
+    from specula_amd.data_objects.electric_field import ElectricField
+    from specula_amd.processing_objects.sh import SH
+    wl = workloads.scao_ngs_20x20()
+    sh = SH(target_device_idx=0, **wl.sh_kwargs)
+    ef = ElectricField(wl.ef_size, wl.ef_size, wl.pixel_pitch, S0=1.0, target_device_idx=0)
+    ef.A[:] = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
+    ef.phaseInNm[:] = torch.tensor(wl.frames(1)[0].astype(np.float32), device=DEV)
+    sh.inputs['in_ef'].set(ef)
+    sh.setup()
+    totals = []
+    for t, scale in enumerate((1.0, 0.5, 0.5)):
+        if scale != 1.0 and t == 1:
+            ef.A.mul_(scale)                         # in place: same storage, new version
+        ef.generation_time = t + 1
+        sh.check_ready(t + 1)
+        sh.trigger()
+        sh.post_trigger()
+        torch.cuda.synchronize()
+        totals.append(float(sh.outputs['out_i'].i.sum()))
+    masked = float(torch.tensor(wl.pupil()).sum()) * wl.pixel_pitch ** 2
+    np.testing.assert_allclose(totals, [masked, 0.5 * masked, 0.5 * masked], rtol=1e-5)
