@@ -18,6 +18,12 @@ import time
 import numpy as np
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the main kernel, measured with rocprofv3 --pmc FETCH_SIZE and then
+# WRITE_SIZE (KiB per dispatch) and scaled by what the same counters read for a 256 MiB device copy
+# (fetch x2.0, write x1.0); command and raw figures: profiles/pmc_traffic_elt_ngs_80x80.md.
+# Keyed by (CoG fused, frames per launch, default frames per block).  The static pupil is fetched
+# about once per launch, so the traffic is below the algorithmic bytes (pupil counted per frame).
+MEASURED_TRAFFIC = {(True, 100, True): 837_200_000}
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -293,8 +299,9 @@ def main():
         'fft_gflops_nominal': round(fft_flop * F / (kern_ms * 1e-3) / 1e9, 1),
         'fft_fp32_peak_frac': round(fft_flop * F / (kern_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': round(achieved / HBM_PEAK_GBS, 4), 'traffic': None,
-                     'kernel': f'sh_ngs_mfma_kernel<8, true, 16, {1 if fused else 0}>',
+                     'frac': round(achieved / HBM_PEAK_GBS, 4),
+                     'traffic': MEASURED_TRAFFIC.get((bool(fused), F, args.fpb in (0, 4))),
+                     'kernel': f'sh_ngs_mfma_kernel<8, true, 16, {3 if fused else 0}, false>',
                      'fused_cog': bool(fused), 'frames_per_block': plan.fpb,
                      'bytes_per_launch': bytes_per_frame * F},
     }
