@@ -1344,45 +1344,56 @@ sh_ngs_vjp_kernel(VjpArgs g) {
 // one subaperture, so two VJPs (cotangent 1 on every x slope, then on every y slope) give
 // Jx, Jy (pupil maps, slope per nm) and IM[m][slot] = scale * sum_{p in subap} J[p] mode_m[p]
 // (scale 2 reproduces the push-pull convention (s(+a) - s(-a))/a of im_rec_calibrator.py:58-63).
-// Block = one strip of NP pupil rows x a chunk of modes; one thread per pupil column.
+// Block = one strip of NP pupil rows x a chunk of modes; one thread per pupil column.  Only the
+// columns of valid subapertures read the modes (the Jacobian is zero elsewhere), and MU modes are
+// loaded before they are reduced so that each wave keeps several row segments in flight.
 template <int NP>
 __global__ void im_from_jacobian_kernel(const float* __restrict__ jx, const float* __restrict__ jy, int ef,
                                         int n_lenses, const float* __restrict__ modes, int64_t mode_stride,
                                         int n_modes, int modes_per_block, const int32_t* __restrict__ slot,
                                         int n_valid, float scale, float* __restrict__ im) {
+    constexpr int MU = 4;
     const int i = blockIdx.x;
     const int col = threadIdx.x;
-    const bool active = col < ef;
+    const int j = col / NP, xin = col - (col / NP) * NP;
+    const int sl = (col < ef) ? slot[i * n_lenses + j] : -1;
+    const bool valid = sl >= 0;
     float jxr[NP], jyr[NP];
 #pragma unroll
     for (int r = 0; r < NP; ++r) {
         const int64_t o = (int64_t)(i * NP + r) * ef + col;
-        jxr[r] = active ? jx[o] : 0.f;
-        jyr[r] = active ? jy[o] : 0.f;
+        jxr[r] = valid ? jx[o] : 0.f;
+        jyr[r] = valid ? jy[o] : 0.f;
     }
-    const int j = col / NP, xin = col - j * NP;
-    const int sl = (active && xin == 0) ? slot[i * n_lenses + j] : -1;
     const int m0 = blockIdx.y * modes_per_block;
     const int m1 = min(n_modes, m0 + modes_per_block);
-    for (int m = m0; m < m1; ++m) {
-        float px = 0.f, py = 0.f;
-        if (active) {
-            const float* md = modes + m * mode_stride + (int64_t)(i * NP) * ef + col;
+    const float* base = modes + (int64_t)(i * NP) * ef + col;
+    for (int mb = m0; mb < m1; mb += MU) {
+        float v[MU][NP];
+#pragma unroll
+        for (int u = 0; u < MU; ++u) {
+            const bool ld = valid && mb + u < m1;
+            const float* md = base + (int64_t)(mb + u) * mode_stride;
+#pragma unroll
+            for (int r = 0; r < NP; ++r) v[u][r] = ld ? md[(int64_t)r * ef] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < MU; ++u) {
+            float px = 0.f, py = 0.f;
 #pragma unroll
             for (int r = 0; r < NP; ++r) {
-                const float v = md[(int64_t)r * ef];
-                px = fmaf(jxr[r], v, px);
-                py = fmaf(jyr[r], v, py);
+                px = fmaf(jxr[r], v[u][r], px);
+                py = fmaf(jyr[r], v[u][r], py);
             }
-        }
 #pragma unroll
-        for (int off = NP / 2; off > 0; off >>= 1) {
-            px += __shfl_xor(px, off, kWave);
-            py += __shfl_xor(py, off, kWave);
-        }
-        if (sl >= 0) {
-            im[(int64_t)m * 2 * n_valid + sl] = scale * px;
-            im[(int64_t)m * 2 * n_valid + n_valid + sl] = scale * py;
+            for (int off = NP / 2; off > 0; off >>= 1) {
+                px += __shfl_xor(px, off, kWave);
+                py += __shfl_xor(py, off, kWave);
+            }
+            if (valid && xin == 0 && mb + u < m1) {
+                im[(int64_t)(mb + u) * 2 * n_valid + sl] = scale * px;
+                im[(int64_t)(mb + u) * 2 * n_valid + n_valid + sl] = scale * py;
+            }
         }
     }
 }
@@ -2195,7 +2206,7 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
     if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int threads = (ef_size + 63) / 64 * 64;
-    const int mpb = 8;
+    const int mpb = 16;
     dim3 grid(n_lenses, (n_modes + mpb - 1) / mpb);
 #define IMJ(NPV) hipLaunchKernelGGL(im_from_jacobian_kernel<NPV>, grid, dim3(threads), 0, st, jx, jy, ef_size, \
                                     n_lenses, modes, mode_stride, n_modes, mpb, slot, n_valid, scale, im)
