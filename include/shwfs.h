@@ -88,6 +88,18 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
 int shwfs_plan_set_pupil(shwfs_plan* plan, const float* A, int64_t a_frame_stride, int n_frames,
                          double S0, void* stream);
 
+/* Field resampling for fov_ovs != 1, phase rotation or shift (SH.setup, sh.py:448-458, and the
+ * interpolation block of SH.trigger_code, sh.py:350-360).  After this call shwfs_forward and
+ * shwfs_forward_slopes take A and phase_nm on the input grid (ef_in x ef_in; the plan's
+ * ef_size is the resampled side): the phase is extended beyond the pupil edge with the sources
+ * ext_src1 / ext_src2 (host int32 maps, ef_in^2, -1 = none; pixel p takes phase[src1], or
+ * 2 phase[src1] - phase[src2] when src2 >= 0, extrapolate_edge_pixel.py:4-29), then A and the
+ * phase are resampled bilinearly at input coordinates y = a0 Y + a1 X + a2, x = a3 Y + a4 X + a5
+ * (clamped; affine = host double[6]) of output pixel (Y, X) (Interp2D, interp2d.py:48-122).
+ * The masked area stays that of the input amplitude.  NULL maps: no extrapolation. */
+int shwfs_plan_set_resampling(shwfs_plan* plan, int ef_in, const int32_t* ext_src1, const int32_t* ext_src2,
+                              const double* affine);
+
 /* Optional timing hook: when both events are non-null (hipEvent_t), shwfs_forward records
  * `start` immediately before and `stop` immediately after the main SH kernel launch, on the
  * call's stream, so that the caller can time that kernel alone.  Pass nulls to disable.

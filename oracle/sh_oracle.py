@@ -11,8 +11,9 @@ index arithmetic, so that it reproduces the reference's numbers.  Deviations:
 * ``cutpixels == 0``: the reference slice ``psf[:, c//2:-c//2]`` is empty and raises
   (sh.py:405,410).  Here the slice end is ``N - ceil(c/2)``, which equals the
   reference's end for every ``c >= 1`` and is ``N`` for ``c == 0``.
-* Only the ``fov_ovs == 1`` path without rotation/shift is restated (no Interp2D,
-  no edge extrapolation); ``sh_forward`` raises otherwise.
+* The interpolated path (fov_ovs != 1, rotation or shift: sh.py:350-360, 450-460) is
+  restated with FP64 sampling coordinates (the reference builds them in float32 when
+  precision is single).
 
 Pinning (what checks each function; see tests/test_oracle_pins.py):
   make_xy / make_mask ....... closed forms + reference docstring examples (make_xy.py:93-119)
@@ -27,6 +28,14 @@ Pinning (what checks each function; see tests/test_oracle_pins.py):
                               1-px tilt slope 2/(npx-1) (test_sh_slopec.py:109-114)
   detect_subaps ............. idx layout / ordering of scao_subaps_n8_th0.5_ref.fits
   lgs_map_sh ................ IDL golden lgs_map_sh_ref.fits (test_convolution_kernel.py:175-276)
+  extrapolate_edge_pixel_mat_define / extrapolate_edge_pixel
+                              IDL goldens extrapol_array_doExt2Pix{False,True}.fits and
+                              extrapolated{1,2}.fits (test_extrapol_mat_define.py,
+                              test_extrapolate_edge_pixel.py)
+  interp2d .................. IDL golden ref_phase.fits (test_interp2d.py:19-43), exact on
+                              affine functions, no reads outside the input
+  sh_forward (interpolated) . flat-wavefront slopes of scao_sn_n8_th0.5_ref.fits (fov_ovs 2),
+                              reduces to the plain path for an identity resampling
 """
 import math
 
@@ -86,7 +95,8 @@ def lenslet(n_lenses):
 def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_pxscale,
                 subap_on_diameter, subap_npx, FoVres30mas=False, squaremask=True,
                 fov_ovs_coeff=0.0, do_not_double_fov_ovs=False, set_fov_res_to_turbpxsc=False,
-                fov_resolution_arcsec=None):
+                fov_resolution_arcsec=None, xShiftPhInPixel=0, yShiftPhInPixel=0,
+                aXShiftPhInPixel=0, aYShiftPhInPixel=0, rotAnglePhInDeg=0, aRotAnglePhInDeg=0):
     """Restates SH.__init__ (sh.py:43-110), set_in_ef (sh.py:112-237) and the geometry part of
     calc_trigger_geometry (sh.py:239-282).  Angles in arcsec as in the YAML keys.
     ``fov_resolution_arcsec`` mirrors assigning ``sh._fov_resolution_arcsec`` before setup()."""
@@ -189,7 +199,11 @@ def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_p
                          f'is not divisible by  {n_lenses} subapertures.')
 
     # calc_trigger_geometry (sh.py:239-282)
-    wf1_pitch = pixel_pitch / fov_ovs if fov_ovs != 1 else pixel_pitch   # :254-259
+    rot0 = rotAnglePhInDeg + aRotAnglePhInDeg
+    shift0 = np.round(np.array([xShiftPhInPixel + aXShiftPhInPixel,
+                                yShiftPhInPixel + aYShiftPhInPixel]) * fov_ovs).astype(int)
+    resample = fov_ovs != 1 or rot0 != 0 or np.sum(np.abs(shift0)) != 0
+    wf1_pitch = pixel_pitch / fov_ovs if resample else pixel_pitch    # :254-259
     fp4_pixel_pitch = wavelengthInNm / 1e9 / (wf1_pitch * fft_size)      # :268
     fov_complete = fft_size * fp4_pixel_pitch
     sensor_subap_fov = _sensor_pxscale * subap_npx
@@ -200,7 +214,15 @@ def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_p
         raise ValueError(f'sensor FoV exceeds the FFT FoV (cutpixels={cutpixels})')
     fp_mask = make_mask(fft_size, diaratio=_subap_wanted_fov / fov_complete, square=squaremask)  # :282
 
+    # phase resampling (calc_trigger_geometry, sh.py:247-254; setup, sh.py:448-452)
+    rot = rotAnglePhInDeg + aRotAnglePhInDeg
+    xy_shift = np.round(np.array([xShiftPhInPixel + aXShiftPhInPixel,
+                                  yShiftPhInPixel + aYShiftPhInPixel]) * fov_ovs).astype(int)
+    interpolate = fov_ovs != 1 or rot != 0 or np.sum(np.abs(xy_shift)) != 0
+
     return dict(ef_size=ef_size, pixel_pitch=pixel_pitch, wavelengthInNm=wavelengthInNm,
+                interpolate=bool(interpolate), rot_deg=rot, xy_shift=xy_shift,
+                ef_ovs=int(ef_size * fov_ovs), wf1_pitch=wf1_pitch,
                 n_lenses=n_lenses, np_sub=_np_sub, ovs_np_sub=ovs_np_sub, fov_ovs=fov_ovs,
                 scale_ovs=scale_ovs, fft_size=fft_size, cutpixels=cutpixels, cutsize=cutsize,
                 subap_npx=subap_npx, ccd_side=subap_npx * n_lenses, fp_mask=fp_mask,
@@ -253,16 +275,19 @@ def toccd(a, newshape, set_total=None):
 
 # --------------------------------------------------------------------------- SH forward
 
-def sh_forward(A, phase_nm, geom, S0=1.0, kernels=None, return_psfimage=False):
-    """SH.trigger_code + post_trigger (sh.py:348-428) in FP64, fov_ovs == 1 path.
+def sh_forward(A, phase_nm, geom, S0=1.0, kernels=None, return_psfimage=False, ext_mask=None):
+    """SH.trigger_code + post_trigger (sh.py:348-428) in FP64.
 
     ``kernels``: (n_lenses**2, N, N) complex array as produced by ``process_kernels``
     (ConvolutionKernel.kernels), or None for the NGS path.
+    ``ext_mask``: with resampling, the amplitude the edge-extrapolation maps were built from
+    (in_ef.A at set-up, sh.py:453); None = ``A``.
     Returns the CCD intensity (ccd_side x ccd_side) float64."""
-    if geom['fov_ovs'] != 1:
-        raise NotImplementedError('oracle restates the fov_ovs == 1 path only')
     A = np.asarray(A, dtype=np.float64)
     phase_nm = np.asarray(phase_nm, dtype=np.float64)
+    A_in = A
+    if geom.get('interpolate', False):                              # sh.py:350-360
+        A, phase_nm = resample_field(A, phase_nm, geom, ext_mask)
     n = geom['n_lenses']
     p = geom['ovs_np_sub']
     N = geom['fft_size']
@@ -293,11 +318,124 @@ def sh_forward(A, phase_nm, geom, S0=1.0, kernels=None, return_psfimage=False):
         psfimage[i * cs:(i + 1) * cs, :] = cut.swapaxes(0, 1).reshape(-1, n * cs)   # sh.py:410-413
     ccd_side = geom['ccd_side']
     out = toccd(psfimage, (ccd_side, ccd_side))                      # sh.py:420
-    phot = S0 * (geom['pixel_pitch'] ** 2) * np.sum(A)               # sh.py:426; electric_field.py:87-89
+    phot = S0 * (geom['pixel_pitch'] ** 2) * np.sum(A_in)            # sh.py:426 (in_ef, not wf1); electric_field.py:87-89
     out = out * (phot / out.sum())                                   # sh.py:427
     if return_psfimage:
         return out, psfimage
     return out
+
+
+# --------------------------------------------------------------------------- phase resampling
+
+def extrapolate_edge_pixel_mat_define(mask, do_ext_2_pix=False):
+    """extrapolate_edge_pixel_mat_define.py:5-140 (loop for loop).  Returns int32 maps of the
+    source pixel (linear index) for 1-pixel and 2-pixel linear extrapolation (-1: none), the
+    indices of the pixels to fill, and the positions among them with a 2-pixel source."""
+    smask = mask.shape
+    idx_mask = np.where(mask)
+    float_mask = mask.astype(float)
+    s1 = np.full(float_mask.shape, -1.0)
+    s2 = np.full(float_mask.shape, -1.0)
+    lin = np.zeros_like(float_mask)
+    lin[idx_mask] = np.ravel_multi_index(idx_mask, smask)
+    find1 = (np.roll(float_mask, 1, axis=0) + np.roll(float_mask, -1, axis=0) +     # :26-28
+             np.roll(float_mask, 1, axis=1) + np.roll(float_mask, -1, axis=1)) * (1.0 - float_mask)
+    find2 = (np.roll(float_mask, 2, axis=0) + np.roll(float_mask, -2, axis=0) +     # :31-33
+             np.roll(float_mask, 2, axis=1) + np.roll(float_mask, -2, axis=1)) * (1.0 - float_mask)
+    idx1 = np.where(find1 > 0)
+    idx2 = np.where((find2 > 0) & (find1 == 0))
+
+    def visit(idx, step):                 # one pixel, the four directions in the reference order
+        r, c = idx
+        test = -1
+        near, far = step, step + 1
+        arr = s1 if step == 1 else s2
+        # (row offset, col offset) for "y+", "y-", "x+", "x-" (:45-79 / :93-127)
+        for dr, dc, ok_near, ok_far in (
+                (0, 1, c + near < smask[1] - 1, c + far < smask[1] - 1),
+                (0, -1, c - near > 0, c - far > 0),
+                (1, 0, r + near < smask[0] - 1, r + far < smask[0] - 1),
+                (-1, 0, r - near > 0, r - far > 0)):
+            if ok_near and arr[idx] == -1:
+                if float_mask[r + dr * near, c + dc * near] > 0:
+                    if ok_far and float_mask[r + dr * far, c + dc * far] > 0:
+                        s2[idx] = lin[r + dr * far, c + dc * far]
+                        s1[idx] = lin[r + dr * near, c + dc * near]
+                    else:
+                        test = lin[r + dr * near, c + dc * near]
+        if s1[idx] == -1 and test >= 0:                                     # :81-82
+            s1[idx] = test
+
+    for idx in zip(*idx1):
+        visit(idx, 1)
+    if do_ext_2_pix:
+        for idx in zip(*idx2):
+            visit(idx, 2)
+    idx_1pix = np.where(s1 >= 0)                                            # :134-135
+    idx_2pix = np.where(s2[idx_1pix] >= 0)
+    return s1.astype(np.int32), s2.astype(np.int32), idx_1pix, idx_2pix
+
+
+def extrapolate_edge_pixel(phase, s1, s2, idx_1pix, idx_2pix):
+    """extrapolate_edge_pixel.py:4-29: fill the pixels just outside the pupil with the value of
+    the nearest pupil pixel, or the linear extrapolation of the two nearest along one axis."""
+    out = np.array(phase, dtype=np.float64, copy=True)
+    flat = out.ravel().copy()
+    v1 = flat[s1[idx_1pix]]
+    v2 = flat[s2[idx_1pix]]
+    v1[idx_2pix] = 2 * v1[idx_2pix] - v2[idx_2pix]
+    out[idx_1pix] = v1
+    return out
+
+
+def interp2d_coords(input_shape, output_shape, rot_deg=0.0, row_shift=0.0, col_shift=0.0):
+    """Interp2D.__init__ sampling points, interp2d.py:48-93 (default grid): (yy, xx) in input
+    pixel coordinates, clamped to the input."""
+    yy, xx = np.mgrid[0:output_shape[0], 0:output_shape[1]].astype(np.float64)
+    yy *= (input_shape[0] - 1) / output_shape[0]                            # :64
+    xx *= (input_shape[1] - 1) / output_shape[1]
+    if rot_deg != 0:                                                         # :73-81
+        yc = input_shape[0] / 2 - 0.5
+        xc = input_shape[1] / 2 - 0.5
+        cos_ = np.cos(rot_deg * 3.1415 / 180.0)
+        sin_ = np.sin(rot_deg * 3.1415 / 180.0)
+        xxr = (xx - xc) * cos_ - (yy - yc) * sin_
+        yyr = (xx - xc) * sin_ + (yy - yc) * cos_
+        xx, yy = xxr + xc, yyr + yc
+    if row_shift != 0 or col_shift != 0:                                     # :83-85
+        yy = yy + row_shift
+        xx = xx + col_shift
+    yy = np.clip(yy, 0, input_shape[0] - 1)                                  # :87-90
+    xx = np.clip(xx, 0, input_shape[1] - 1)
+    return yy, xx
+
+
+def interp2d(value, yy, xx):
+    """Bilinear interpolation on the integer grid (Interp2D.interpolate CPU path, interp2d.py:
+    117-122: scipy RegularGridInterpolator, method 'linear'); points on the last grid line use
+    the last cell with weight 1."""
+    value = np.asarray(value, dtype=np.float64)
+    n0, n1 = value.shape
+    y = np.asarray(yy, dtype=np.float64)
+    x = np.asarray(xx, dtype=np.float64)
+    i0 = np.clip(np.floor(y).astype(np.int64), 0, n0 - 2)
+    j0 = np.clip(np.floor(x).astype(np.int64), 0, n1 - 2)
+    ty, tx = y - i0, x - j0
+    return ((1 - ty) * (1 - tx) * value[i0, j0] + (1 - ty) * tx * value[i0, j0 + 1] +
+            ty * (1 - tx) * value[i0 + 1, j0] + ty * tx * value[i0 + 1, j0 + 1])
+
+
+def resample_field(A, phase_nm, geom, ext_mask=None):
+    """SH.trigger_code interpolation block, sh.py:350-360, with the setup of sh.py:448-458:
+    edge extrapolation of the phase (mask = in_ef.A at set-up, 2-pixel extension), then A and
+    the extrapolated phase resampled on the ef*fov_ovs grid (rotation and rounded shifts)."""
+    ef = A.shape[0]
+    mask = A if ext_mask is None else np.asarray(ext_mask, dtype=np.float64)
+    s1, s2, i1, i2 = extrapolate_edge_pixel_mat_define(mask, do_ext_2_pix=True)
+    ph = extrapolate_edge_pixel(phase_nm, s1, s2, i1, i2)
+    out = (geom['ef_ovs'], geom['ef_ovs'])
+    yy, xx = interp2d_coords((ef, ef), out, geom['rot_deg'], geom['xy_shift'][0], geom['xy_shift'][1])
+    return interp2d(A, yy, xx), interp2d(ph, yy, xx)
 
 
 # --------------------------------------------------------------------------- ShSlopec

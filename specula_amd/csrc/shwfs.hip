@@ -121,10 +121,13 @@ __device__ __forceinline__ float2 field_sample_rev(float a, float ph, double k_r
 // partials in fixed order and resets the arrival counter (so graph replays need no memset).
 // With `scale_out`, the Parseval flux scale S0 pixel_area sum(A) / (norm_const sum(A^2)) of the
 // frame is also written (as float, the factor the SH kernels apply), `scale_num` = S0 pixel_area.
+// With `B` (resampled field) the square sum is taken over B instead (sum(A) stays the masked area
+// of the input field, sh.py:426).
 __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, int64_t n_px,
                                  double* __restrict__ part, unsigned* __restrict__ counter,
                                  double* __restrict__ out, double scale_num, double norm_const,
-                                 float* __restrict__ scale_out) {
+                                 float* __restrict__ scale_out, const float* __restrict__ B,
+                                 int64_t b_stride, int64_t n_b) {
     __shared__ double red[32];
     __shared__ bool last;
     const int f = blockIdx.y;
@@ -150,6 +153,14 @@ __global__ void amp_stats_kernel(const float* __restrict__ A, int64_t a_stride, 
              q += (int64_t)gridDim.x * blockDim.x) {
             double v = a[q];
             s1 += v; s2 += v * v;
+        }
+    }
+    if (B) {
+        s2 = 0.0;
+        const float* b = B + f * b_stride;
+        for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_b; q += (int64_t)gridDim.x * blockDim.x) {
+            const double v = b[q];
+            s2 += v * v;
         }
     }
     const double2 s12 = block_sum2(s1, s2, red);
@@ -989,6 +1000,45 @@ __global__ void sh_rescale_kernel(float* __restrict__ ccd, int64_t ccd_stride, i
         c[q] = (float)((double)c[q] * scale);
 }
 
+// ---------------------------------------------------------------- field resampling (fov_ovs != 1)
+// SH.trigger_code, sh.py:350-360: the phase is first extended 1-2 pixels beyond the pupil edge
+// (extrapolate_edge_pixel.py:4-29: pixel p with ext[p].x >= 0 takes v[ext.x], or
+// 2 v[ext.x] - v[ext.y] when ext.y >= 0), then A and phase are resampled bilinearly on the
+// ef_out grid (Interp2D default grid with rotation and shift, interp2d.py:48-122): output (Y, X)
+// samples the input at (ay.x Y + ay.y X + ay.z, ax.x Y + ax.y X + ax.z), clamped to the input.
+struct ResampleMap {
+    double yy, yx, y0, xy, xx, x0;
+};
+
+__global__ void resample_kernel(const float* __restrict__ in, int64_t in_stride, const int2* __restrict__ ext,
+                                float* __restrict__ out, int64_t out_stride, int ef_in, int ef_out,
+                                ResampleMap m) {
+    const int f = blockIdx.y;
+    const float* v = in + f * in_stride;
+    float* o = out + f * out_stride;
+    const int64_t n_out = (int64_t)ef_out * ef_out;
+    const double lim = ef_in - 1;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_out; q += (int64_t)gridDim.x * blockDim.x) {
+        const int Y = (int)(q / ef_out), X = (int)(q - (int64_t)Y * ef_out);
+        const double y = fmin(fmax(m.yy * Y + m.yx * X + m.y0, 0.0), lim);
+        const double x = fmin(fmax(m.xy * Y + m.xx * X + m.x0, 0.0), lim);
+        const int i0 = min((int)floor(y), ef_in - 2), j0 = min((int)floor(x), ef_in - 2);
+        const float ty = (float)(y - i0), tx = (float)(x - j0);
+        float c[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int p = (i0 + (k >> 1)) * ef_in + j0 + (k & 1);
+            float val = v[p];
+            if (ext) {
+                const int2 e = ext[p];
+                if (e.x >= 0) val = e.y >= 0 ? 2.f * v[e.x] - v[e.y] : v[e.x];
+            }
+            c[k] = val;
+        }
+        o[q] = (1.f - ty) * ((1.f - tx) * c[0] + tx * c[1]) + ty * ((1.f - tx) * c[2] + tx * c[3]);
+    }
+}
+
 // ---------------------------------------------------------------- ShSlopec CoG
 // One wavefront per subaperture: FP64 moments (flux, thresholded total, x and y numerators).
 __global__ void cog_moments_kernel(const float* __restrict__ pix, int64_t pix_stride,
@@ -1467,6 +1517,11 @@ struct shwfs_plan {
     float* d_kspec = nullptr;
     double* d_amp = nullptr;       // [max_frames][2]
     float* d_scale = nullptr;      // [max_frames] Parseval flux scale
+    int ef_in = 0;                 // resampling: side of the input field (0: no resampling)
+    ResampleMap rmap{};
+    int2* d_ext = nullptr;         // [ef_in^2] edge-extrapolation sources of the phase
+    float* d_A_rs = nullptr;       // [max_frames][ef_size^2] resampled amplitude
+    float* d_ph_rs = nullptr;      // [max_frames][ef_size^2] resampled phase
     const float* static_A = nullptr;   // pupil whose statistics are held in d_amp / d_scale
     int64_t static_a_stride = 0;
     int static_frames = 0;
@@ -1752,6 +1807,9 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_bpart);
     (void)hipFree(p->d_slot);
     (void)hipFree(p->d_tpart);
+    (void)hipFree(p->d_ext);
+    (void)hipFree(p->d_A_rs);
+    (void)hipFree(p->d_ph_rs);
     (void)hipFree(p->d_cogw);
     delete p;
     return SHWFS_OK;
@@ -1793,6 +1851,37 @@ struct CogCall {
     int thr_mode;
 };
 
+int launch_resample(const shwfs_plan* p, const float* in, int64_t in_stride, const int2* ext, float* out,
+                    int n_frames, hipStream_t st) {
+    const int64_t n_out = (int64_t)p->g.ef_size * p->g.ef_size;
+    const int blocks = (int)std::min<int64_t>((n_out + 255) / 256, 2048);
+    hipLaunchKernelGGL(resample_kernel, dim3(blocks, n_frames), dim3(256), 0, st, in, in_stride, ext, out, n_out,
+                       p->ef_in, p->g.ef_size, p->rmap);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+// Statistics of the amplitude frames (masked area, square sum, Parseval scale); with resampling
+// the amplitude is first resampled (d_A_rs) and the square sum is taken there.
+int pupil_stats(shwfs_plan* p, const float* A, int64_t a_stride, int amp_frames, double S0, hipStream_t st) {
+    const shwfs_geom& g = p->g;
+    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
+    const float* B = nullptr;
+    int64_t b_stride = 0, n_in = ef_px;
+    if (p->ef_in) {
+        n_in = (int64_t)p->ef_in * p->ef_in;
+        int rc = launch_resample(p, A, a_stride, nullptr, p->d_A_rs, amp_frames, st);
+        if (rc) return rc;
+        B = p->d_A_rs;
+        b_stride = ef_px;
+    }
+    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st, A, a_stride, n_in,
+                       p->d_amp_part, p->d_counter, p->d_amp, S0 * g.pixel_area, g.norm_const,
+                       g.parseval ? p->d_scale : nullptr, B, b_stride, ef_px);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
 int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase,
                     int64_t ph_stride, double S0, float* ccd, int64_t ccd_stride, const CogCall* cog,
                     hipStream_t st) {
@@ -1801,9 +1890,10 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
         return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
     const shwfs_geom& g = p->g;
     const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
+    const int64_t in_px = p->ef_in ? (int64_t)p->ef_in * p->ef_in : ef_px;   // input field size
     const int64_t ccd_side = (int64_t)g.n_lenses * g.subap_npx;
-    if (a_stride != 0 && a_stride < ef_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
-    if (n_frames > 1 && ph_stride < ef_px) return set_error(SHWFS_ERR_ARG, "phase frame stride too small");
+    if (a_stride != 0 && a_stride < in_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
+    if (n_frames > 1 && ph_stride < in_px) return set_error(SHWFS_ERR_ARG, "phase frame stride too small");
     if (n_frames > 1 && ccd_stride < ccd_side * ccd_side)
         return set_error(SHWFS_ERR_ARG, "ccd frame stride too small");
     DeviceGuard dg(p->device);
@@ -1815,10 +1905,16 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
                         amp_frames <= p->static_frames;
     if (!cached) {
         p->static_A = nullptr;
-        hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, S0 * g.pixel_area,
-                           g.norm_const, g.parseval ? p->d_scale : nullptr);
-        SHWFS_LAUNCH_CHECK();
+        int rc = pupil_stats(p, A, a_stride, amp_frames, S0, st);
+        if (rc) return rc;
+    }
+    if (p->ef_in) {   // resampled field: the SH kernels read d_A_rs / d_ph_rs
+        int rc = launch_resample(p, phase, ph_stride, p->d_ext, p->d_ph_rs, n_frames, st);
+        if (rc) return rc;
+        A = p->d_A_rs;
+        a_stride = (a_stride == 0) ? 0 : ef_px;
+        phase = p->d_ph_rs;
+        ph_stride = ef_px;
     }
 
     NgsArgs a{};
@@ -1878,19 +1974,49 @@ int shwfs_plan_set_pupil(shwfs_plan* p, const float* A, int64_t a_stride, int n_
     p->static_A = nullptr;
     if (!A) return SHWFS_OK;
     const shwfs_geom& g = p->g;
-    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
+    const int64_t in_px = p->ef_in ? (int64_t)p->ef_in * p->ef_in : (int64_t)g.ef_size * g.ef_size;
     if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
-    if (a_stride != 0 && a_stride < ef_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
+    if (a_stride != 0 && a_stride < in_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
     DeviceGuard dg(p->device);
-    hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), A, a_stride, ef_px, p->d_amp_part, p->d_counter,
-                       p->d_amp, S0 * g.pixel_area, g.norm_const, g.parseval ? p->d_scale : nullptr);
-    SHWFS_LAUNCH_CHECK();
+    int rc = pupil_stats(p, A, a_stride, amp_frames, S0, reinterpret_cast<hipStream_t>(stream));
+    if (rc) return rc;
     p->static_A = A;
     p->static_a_stride = a_stride;
     p->static_frames = amp_frames;
     p->static_S0 = S0;
+    return SHWFS_OK;
+}
+
+int shwfs_plan_set_resampling(shwfs_plan* p, int ef_in, const int32_t* ext_src1, const int32_t* ext_src2,
+                              const double* affine) {
+    if (!p || !affine) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (ef_in < 2) return set_error(SHWFS_ERR_ARG, "input field side must be >= 2");
+    if ((ext_src1 == nullptr) != (ext_src2 == nullptr)) return set_error(SHWFS_ERR_ARG, "give both source maps or neither");
+    const int64_t n_in = (int64_t)ef_in * ef_in;
+    std::vector<int2> ext;
+    if (ext_src1) {
+        ext.resize(n_in);
+        for (int64_t q = 0; q < n_in; ++q) {   // host-side bounds check of every gather index
+            if (ext_src1[q] < -1 || ext_src1[q] >= n_in || ext_src2[q] < -1 || ext_src2[q] >= n_in ||
+                (ext_src1[q] < 0 && ext_src2[q] >= 0))
+                return set_error(SHWFS_ERR_ARG, "edge-extrapolation source index out of range");
+            ext[q] = make_int2(ext_src1[q], ext_src2[q]);
+        }
+    }
+    const shwfs_geom& g = p->g;
+    const size_t rs_bytes = (size_t)g.max_frames * g.ef_size * g.ef_size * sizeof(float);
+    DeviceGuard dg(p->device);
+    (void)hipFree(p->d_ext); p->d_ext = nullptr;
+    if (!p->d_A_rs) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_A_rs, rs_bytes));
+    if (!p->d_ph_rs) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_ph_rs, rs_bytes));
+    if (!ext.empty()) {
+        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_ext, n_in * sizeof(int2)));
+        SHWFS_HIP_CHECK(hipMemcpy(p->d_ext, ext.data(), n_in * sizeof(int2), hipMemcpyHostToDevice));
+    }
+    p->rmap = ResampleMap{affine[0], affine[1], affine[2], affine[3], affine[4], affine[5]};
+    p->ef_in = ef_in;
+    p->static_A = nullptr;
     return SHWFS_OK;
 }
 
@@ -2133,6 +2259,7 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
     if (!p || !A || !phase || !grad_ccd || !grad_phase) return set_error(SHWFS_ERR_ARG, "null argument");
     const shwfs_geom& g = p->g;
     if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the NGS path");
+    if (p->ef_in) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available without field resampling (fov_ovs 1)");
     if (!g.parseval)
         return set_error(SHWFS_ERR_UNSUPPORTED, "VJP needs the Parseval normalisation (no FoV cut, all-ones mask)");
     if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
@@ -2150,7 +2277,8 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
     if (!(A == p->static_A && a_stride == p->static_a_stride && amp_frames <= p->static_frames)) {
         p->static_A = nullptr;       // d_amp now describes this A
         hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, 0.0, 0.0, (float*)nullptr);
+                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, 0.0, 0.0, (float*)nullptr,
+                           (const float*)nullptr, (int64_t)0, (int64_t)0);
         SHWFS_LAUNCH_CHECK();
     }
     VjpArgs a{};
@@ -2272,6 +2400,7 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         else if (g.lgs && g.np_sub == 16 && g.spec_size == 32) kind = 1;
         else return set_error(SHWFS_ERR_UNSUPPORTED, "batch supports NGS MFMA np 8 and LGS np 16 plans");
         if (!g.parseval) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs Parseval-normalised sensors");
+        if (p->ef_in) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs sensors without field resampling");
         if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan without kernels");
         if (n_frames[s] <= 0) return set_error(SHWFS_ERR_ARG, "non-positive frame count");
         const int ts = (kind == 0) ? 4 * NgsMfmaShape<8>::SW : 8;
@@ -2330,7 +2459,8 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         const int64_t px = (int64_t)plans[s]->g.ef_size * plans[s]->g.ef_size;
         hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, px,
                            b->d_amp_part + (size_t)s * amp_blocks * 2, b->d_counter + s, b->d_amp + 2 * s,
-                           S0[s] * plans[s]->g.pixel_area, plans[s]->g.norm_const, b->d_scale + s);
+                           S0[s] * plans[s]->g.pixel_area, plans[s]->g.norm_const, b->d_scale + s,
+                           (const float*)nullptr, (int64_t)0, (int64_t)0);
         args[s].amp = b->d_amp + 2 * s;
         args[s].fscale = b->d_scale + s;
         args[s].scale_fstep = 0;

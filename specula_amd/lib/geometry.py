@@ -34,6 +34,8 @@ class SHGeometry:
     sensor_pxscale: float    # [rad] (rewritten by set_in_ef, sh.py:217)
     turbulence_pxscale: float
     fp_mask: np.ndarray = field(repr=False)
+    rot_deg: float = 0.0     # phase rotation (rotAnglePhInDeg + aRotAnglePhInDeg)
+    xy_shift: tuple = (0, 0) # rounded (x, y) shifts in resampled pixels (sh.py:249-252)
 
     @property
     def ccd_side(self):
@@ -41,7 +43,13 @@ class SHGeometry:
 
     @property
     def needs_interpolation(self):
-        return self.fov_ovs != 1
+        """SH resamples the field before the transforms (sh.py:254, 452)."""
+        return self.fov_ovs != 1 or self.rot_deg != 0 or (abs(self.xy_shift[0]) + abs(self.xy_shift[1])) != 0
+
+    @property
+    def ef_ovs(self):
+        """Side of the resampled field (sh.py:449)."""
+        return int(self.ef_size * self.fov_ovs)
 
 
 def _lens_size(n_lenses):
@@ -75,7 +83,8 @@ def _search_fov_resolution(turb_px, sensor_px_arcsec, wanted_fov_arcsec, subap_n
 def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_pxscale,
                 subap_on_diameter, subap_npx, FoVres30mas=False, squaremask=True, fov_ovs_coeff=0.0,
                 do_not_double_fov_ovs=False, set_fov_res_to_turbpxsc=False,
-                fov_resolution_arcsec=0.0):
+                fov_resolution_arcsec=0.0, xShiftPhInPixel=0, yShiftPhInPixel=0, aXShiftPhInPixel=0,
+                aYShiftPhInPixel=0, rotAnglePhInDeg=0, aRotAnglePhInDeg=0):
     """Geometry for an ElectricField of side ``ef_size`` (angles in arcsec, as the YAML keys).
     ``fov_resolution_arcsec`` != 0 plays the role of a pre-set ``SH._fov_resolution_arcsec``."""
     n = int(subap_on_diameter)
@@ -134,7 +143,10 @@ def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_p
         raise ValueError(f'ERROR: interpolated input phase size {ef_size} * {round(fov_ovs)} is not '
                          f'divisible by  {n} subapertures.')
 
-    wf1_pitch = pixel_pitch / fov_ovs if fov_ovs != 1 else pixel_pitch
+    rot = rotAnglePhInDeg + aRotAnglePhInDeg                                   # sh.py:247-252
+    shift = np.round(np.array([xShiftPhInPixel + aXShiftPhInPixel,
+                               yShiftPhInPixel + aYShiftPhInPixel]) * fov_ovs).astype(int)
+    wf1_pitch = pixel_pitch / fov_ovs
     fp4 = wavelengthInNm / 1e9 / (wf1_pitch * fft_size)
     fov_complete = fft_size * fp4
     cut = int(np.round((fov_complete - sensor_px_rad * subap_npx) / fp4) / 2 * 2)
@@ -146,4 +158,5 @@ def sh_geometry(ef_size, pixel_pitch, wavelengthInNm, subap_wanted_fov, sensor_p
                       scale_ovs=scale_ovs, fft_size=fft_size, cutpixels=cut, cutsize=fft_size - cut,
                       subap_npx=subap_npx, fp4_pixel_pitch=fp4, fov_complete=fov_complete,
                       fov_resolution_arcsec=fov_res, sensor_pxscale=sensor_px_rad,
-                      turbulence_pxscale=turb_px, fp_mask=mask)
+                      turbulence_pxscale=turb_px, fp_mask=mask, rot_deg=float(rot),
+                      xy_shift=(int(shift[0]), int(shift[1])))

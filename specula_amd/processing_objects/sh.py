@@ -18,6 +18,7 @@ from specula_amd.data_objects.intensity import Intensity
 from specula_amd.data_objects.lenslet import Lenslet
 from specula_amd.data_objects.laser_launch_telescope import LaserLaunchTelescope
 from specula_amd.lib.binning import build_operators
+from specula_amd.lib.resample import edge_extrapolation_maps, resampling_map
 from specula_amd.lib.geometry import sh_geometry
 
 
@@ -47,7 +48,7 @@ class SHPlan:
         self.col_op = col
         g = _native.ShwfsGeom()
         g.n_lenses = geom.n_lenses
-        g.ef_size = geom.ef_size
+        g.ef_size = geom.ef_ovs if geom.needs_interpolation else geom.ef_size   # field the kernels see
         g.np_sub = geom.ovs_np_sub
         g.fft_size = geom.fft_size
         g.spec_size = M
@@ -62,6 +63,7 @@ class SHPlan:
         g.norm_const = norm if norm is not None else 0.0
         g.pixel_area = geom.pixel_pitch ** 2
         self._geom_c = g
+        self.resampled = False
         self.fpb = 4                                    # frames per block (library default)
         self._slopec = None
         if lgs:     # complex operators as interleaved (re, im)
@@ -76,6 +78,25 @@ class SHPlan:
                                             cw.ctypes.data_as(ctypes.c_void_p), device, ctypes.byref(h)),
                       'shwfs_plan_create')
         self._h = h
+
+    def set_resampling(self, pupil):
+        """Field resampling for fov_ovs != 1, rotation or shift (shwfs_plan_set_resampling): edge-
+        extrapolation sources from the pupil mask (sh.py:452-458) and Interp2D's sampling map
+        (sh.py:450).  Forward calls then take fields on the input grid."""
+        g = self.geom
+        mask = np.asarray(pupil.cpu().numpy() if hasattr(pupil, 'cpu') else pupil)
+        if mask.shape != (g.ef_size, g.ef_size):
+            raise ValueError(f'pupil must be {g.ef_size} x {g.ef_size}')
+        src1, src2 = edge_extrapolation_maps(mask, two_pixel=True)
+        aff = np.ascontiguousarray(resampling_map((g.ef_size, g.ef_size), (g.ef_ovs, g.ef_ovs), g.rot_deg,
+                                                  g.xy_shift[0], g.xy_shift[1]), dtype=np.float64)
+        s1 = np.ascontiguousarray(src1, dtype=np.int32)
+        s2 = np.ascontiguousarray(src2, dtype=np.int32)
+        vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)   # noqa: E731
+        _native.check(_native.lib().shwfs_plan_set_resampling(self._h, g.ef_size, vp(s1), vp(s2), vp(aff)),
+                      'shwfs_plan_set_resampling')
+        self._static_key = None
+        self.resampled = True
 
     def set_frames_per_block(self, fpb):
         """Streaming depth of the NGS MFMA kernel (1 to 64 frames per block)."""
@@ -254,7 +275,10 @@ class SH(BaseProcessingObj):
                         squaremask=self._squaremask, fov_ovs_coeff=self._fov_ovs_coeff,
                         do_not_double_fov_ovs=self._do_not_double_fov_ovs,
                         set_fov_res_to_turbpxsc=self._set_fov_res_to_turbpxsc,
-                        fov_resolution_arcsec=self._fov_resolution_arcsec)
+                        fov_resolution_arcsec=self._fov_resolution_arcsec,
+                        xShiftPhInPixel=self._xShiftPhInPixel, yShiftPhInPixel=self._yShiftPhInPixel,
+                        aXShiftPhInPixel=self._aXShiftPhInPixel, aYShiftPhInPixel=self._aYShiftPhInPixel,
+                        rotAnglePhInDeg=self._rotAnglePhInDeg, aRotAnglePhInDeg=self._aRotAnglePhInDeg)
         self.geometry = g
         self._fov_resolution_arcsec = g.fov_resolution_arcsec
         self._np_sub = g.np_sub
@@ -266,25 +290,22 @@ class SH(BaseProcessingObj):
         self._fp_mask = g.fp_mask
         self._sensor_pxscale = g.sensor_pxscale
 
-    def _check_supported(self):
-        rot = self._rotAnglePhInDeg + self._aRotAnglePhInDeg
-        shift = np.round(np.array([self._xShiftPhInPixel + self._aXShiftPhInPixel,
-                                   self._yShiftPhInPixel + self._aYShiftPhInPixel]) * self._fov_ovs)
-        if self._fov_ovs != 1 or rot != 0 or np.sum(np.abs(shift)) != 0:
-            raise NotImplementedError(
-                f'SH geometry needs field interpolation (fov_ovs={self._fov_ovs}, rotation={rot}, '
-                f'shift={shift.tolist()}); set fov_ovs_coeff=1.0 so that fov_ovs == 1')
+    def _make_plan(self, **kw):
+        plan = SHPlan(self.geometry, max_frames=1, device=self.target_device_idx, **kw)
+        if self.geometry.needs_interpolation:
+            # extrapolation sources from the pupil at set-up time, as the reference (sh.py:452-453)
+            plan.set_resampling(self.in_ef.A.float())
+        return plan
 
     def setup(self):
         super().setup()
         in_ef = self.inputs['in_ef'].get(target_device_idx=self.target_device_idx)
         self.set_in_ef(in_ef)
-        self._check_supported()
         self.in_ef = in_ef
         self._kernelobj = None
         self._kernel_version = None
         if self._laser_launch_tel is None:
-            self._plan = SHPlan(self.geometry, lgs=False, max_frames=1, device=self.target_device_idx)
+            self._plan = self._make_plan(lgs=False)
         else:
             self._make_kernel_object(in_ef)
         self.build_stream(allow_parallel=False, capture=False)
@@ -326,8 +347,7 @@ class SH(BaseProcessingObj):
         if self._kernelobj.version != self._kernel_version:
             normalised = self._kernelobj.kernels_normalised()
             if self._plan is None or self._plan.parseval != normalised:
-                self._plan = SHPlan(self.geometry, lgs=True, max_frames=1, device=self.target_device_idx,
-                                    kernels_normalised=normalised)
+                self._plan = self._make_plan(lgs=True, kernels_normalised=normalised)
                 self._graph_key = None
             self._kspec = torch.tensor(self._kernelobj.lag_spectra(self.geometry.ovs_np_sub), device=self.device)
             self._plan.set_kernels(self._kspec, stream=self.stream)

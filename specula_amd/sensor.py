@@ -26,6 +26,8 @@ class SHSensor:
         self.plan = SHPlan(geom, max_frames=max_frames, device=device, variant=variant)
         ef, ccd = geom.ef_size, geom.ccd_side
         self.A = torch.as_tensor(np.asarray(pupil, dtype=np.float32), device=self.device).contiguous()
+        if geom.needs_interpolation:
+            self.plan.set_resampling(np.asarray(pupil))
         self.thr = float(thr)
         if idxs is None:
             flat = torch.zeros((1, ef, ef), dtype=torch.float32, device=self.device)

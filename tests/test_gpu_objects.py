@@ -101,10 +101,10 @@ def test_setup_errors():
     with pytest.raises(ValueError):
         sh.setup()                                     # in_ef not set (base_processing_obj.py:175-177)
     sh2 = SH(wavelengthInNm=500, subap_wanted_fov=3, sensor_pxscale=0.5, subap_on_diameter=20,
-             subap_npx=6, target_device_idx=0)         # default fov_ovs_coeff -> fov_ovs == 2
-    sh2.inputs['in_ef'].set(ElectricField(120, 120, 0.05, S0=1, target_device_idx=0))
-    with pytest.raises(NotImplementedError):
-        sh2.setup()
+             subap_npx=6, target_device_idx=0)
+    sh2.inputs['in_ef'].set(ElectricField(120, 100, 0.05, S0=1, target_device_idx=0))
+    with pytest.raises(ValueError):
+        sh2.setup()                                    # non-square field
     with pytest.raises(ValueError):
         sh2.inputs['in_ef'].set(Pixels(4, 4, target_device_idx=0))
 

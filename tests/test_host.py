@@ -221,3 +221,56 @@ def test_von_karman_structure_function():
         d = np.mean((ph[:, lag:] - ph[:, :-lag]) ** 2)
         expect = 6.88 * (lag * pitch / r0) ** (5 / 3)
         assert 0.6 < d / expect < 1.4
+
+
+# ---------------------------------------------------------------- field resampling (host side)
+
+def test_edge_extrapolation_maps_match_golden_and_oracle():
+    from specula_amd.lib.resample import edge_extrapolation_maps
+    golden = os.path.join(os.path.dirname(__file__), 'golden')
+    mask = fits.read(os.path.join(golden, 'mask.fits'))[0][0]
+    for two, name in ((False, 'extrapol_array_doExt2PixFalse.fits'), (True, 'extrapol_array_doExt2PixTrue.fits')):
+        s1, s2 = edge_extrapolation_maps(mask, two_pixel=two)
+        h = fits.read(os.path.join(golden, name))
+        np.testing.assert_array_equal(s1, h[0][0].astype(np.int32))
+        np.testing.assert_array_equal(s2, h[1][0].astype(np.int32))
+    rng = np.random.default_rng(3)
+    for k in range(10):   # masks touching the borders exercise the periodic neighbour test
+        m = (rng.random((23, 29)) < 0.45).astype(float)
+        a1, a2, _, _ = O.extrapolate_edge_pixel_mat_define(m, do_ext_2_pix=True)
+        b1, b2 = edge_extrapolation_maps(m, two_pixel=True)
+        np.testing.assert_array_equal(a1, b1)
+        np.testing.assert_array_equal(a2, b2)
+    for k in range(6):    # grey amplitudes: (1 - A) > 0 inside the pupil pulls interior pixels in
+        m = (rng.random((23, 29)) < 0.5) * rng.choice([0.5, 1.0, 0.25], size=(23, 29))
+        a1, a2, _, _ = O.extrapolate_edge_pixel_mat_define(m, do_ext_2_pix=True)
+        b1, b2 = edge_extrapolation_maps(m, two_pixel=True)
+        np.testing.assert_array_equal(a1, b1)
+        np.testing.assert_array_equal(a2, b2)
+    half = 0.5 * fits.read(os.path.join(golden, 'mask.fits'))[0][0]
+    b1, _ = edge_extrapolation_maps(half, two_pixel=True)
+    assert np.count_nonzero((b1 >= 0) & (half > 0)) > 0     # interior pixels of a grey pupil selected
+
+
+@pytest.mark.parametrize('rot,rs,cs', [(0.0, 0, 0), (12.5, 2, -3), (-40.0, 0, 1)])
+def test_resampling_map_matches_oracle_coordinates(rot, rs, cs):
+    from specula_amd.lib.resample import resampling_map
+    yy, xx = O.interp2d_coords((40, 48), (90, 100), rot, rs, cs)
+    c = resampling_map((40, 48), (90, 100), rot, rs, cs)
+    Y, X = np.mgrid[0:90, 0:100]
+    np.testing.assert_allclose(np.clip(c[0] * Y + c[1] * X + c[2], 0, 39), yy, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(np.clip(c[3] * Y + c[4] * X + c[5], 0, 47), xx, rtol=0, atol=1e-12)
+
+
+def test_geometry_rotation_shift_keys():
+    kw = dict(wavelengthInNm=600, subap_wanted_fov=4.0, sensor_pxscale=0.5, subap_on_diameter=8, subap_npx=8)
+    g = sh_geometry(64, 0.015625, **kw)
+    go = O.sh_geometry(ef_size=64, pixel_pitch=0.015625, **kw)
+    assert g.needs_interpolation and go['interpolate'] and g.ef_ovs == go['ef_ovs'] == 128
+    kw1 = dict(kw, fov_ovs_coeff=1.0, xShiftPhInPixel=0.6, rotAnglePhInDeg=5.0, aRotAnglePhInDeg=1.0)
+    g = sh_geometry(64, 0.015625, **kw1)
+    go = O.sh_geometry(ef_size=64, pixel_pitch=0.015625, **kw1)
+    assert g.fov_ovs == go['fov_ovs'] == 1 and g.needs_interpolation
+    assert g.rot_deg == go['rot_deg'] == 6.0 and tuple(g.xy_shift) == tuple(go['xy_shift']) == (1, 0)
+    g = sh_geometry(64, 0.015625, **dict(kw, fov_ovs_coeff=1.0))
+    assert not g.needs_interpolation

@@ -292,3 +292,82 @@ def test_process_kernels_index_and_normalisation():
         for j in range(3):
             np.testing.assert_allclose(k[j * 3 + i], np.fft.ifft2(real[i * 3 + j] / real[i * 3 + j].sum()))
             np.testing.assert_allclose(k[j * 3 + i][0, 0], 1 / 64)
+
+
+# ---------------------------------------------------------------- field resampling (fov_ovs != 1)
+
+def _golden(name, hdu=0):
+    return fits.read(os.path.join(GOLDEN, name))[hdu][0]
+
+
+@pytest.mark.parametrize('two_pixel,name', [(False, 'extrapol_array_doExt2PixFalse.fits'),
+                                            (True, 'extrapol_array_doExt2PixTrue.fits')])
+def test_extrapolation_maps_match_idl_golden(two_pixel, name):
+    """test_extrapol_mat_define.py: source maps of the 160x160 pupil mask, both variants."""
+    s1, s2, _, _ = O.extrapolate_edge_pixel_mat_define(_golden('mask.fits'), do_ext_2_pix=two_pixel)
+    np.testing.assert_array_equal(s1, _golden(name, 0).astype(np.int32))
+    np.testing.assert_array_equal(s2, _golden(name, 1).astype(np.int32))
+
+
+@pytest.mark.parametrize('maps,ref', [('extrapol_array_doExt2PixFalse.fits', 'extrapolated1.fits'),
+                                      ('extrapol_array_doExt2PixTrue.fits', 'extrapolated2.fits')])
+def test_extrapolate_edge_pixel_matches_idl_golden(maps, ref):
+    """test_extrapolate_edge_pixel.py (float32 goldens: agreement to their rounding)."""
+    m1, m2 = _golden(maps, 0).astype(int), _golden(maps, 1).astype(int)
+    i1 = np.where(m1 >= 0)
+    i2 = np.where(m2[i1] >= 0)
+    out = O.extrapolate_edge_pixel(_golden('phase.fits'), m1, m2, i1, i2)
+    want = _golden(ref)
+    np.testing.assert_allclose(out, want, rtol=1e-7, atol=0)
+    # the extrapolated ring really changed: pixels outside the mask now carry values
+    assert np.count_nonzero(out != _golden('phase.fits')) > 100
+
+
+def test_interp2d_matches_idl_golden():
+    """test_interp2d.py:19-43: explicit sampling points from make_xy; rtol 4e-2 with the one
+    pixel the reference also exempts."""
+    phase = _golden('input_phase.fits')
+    ref = _golden('ref_phase.fits')
+    xx, yy = O.make_xy(480, 479.84003 / 2.)
+    xx1 = np.clip(xx + 240.5 + 0.0930127, 0, phase.shape[1] - 1)
+    yy1 = np.clip(yy + 240.5, 0, phase.shape[0] - 1)
+    out = O.interp2d(phase, yy1, xx1)
+    out[394, 365] = ref[394, 365]
+    np.testing.assert_allclose(out, ref, rtol=4e-2)
+
+
+def test_interp2d_exact_on_affine_and_in_bounds():
+    """Bilinear sampling reproduces a + b y + c x exactly (fixes the weights and the corner
+    order); the default grid with a rotation never reads outside the input (the reference's
+    NaN-padding boundary test, test_interp2d.py:47-62)."""
+    y, x = np.mgrid[0:23, 0:31].astype(float)
+    f = 1.5 - 0.25 * y + 0.75 * x
+    yy, xx = O.interp2d_coords(f.shape, (50, 40), rot_deg=17.0, row_shift=1, col_shift=-2)
+    np.testing.assert_allclose(O.interp2d(f, yy, xx), 1.5 - 0.25 * yy + 0.75 * xx, rtol=0, atol=1e-12)
+    big = np.full((20, 20), np.nan)
+    big[:10] = 1.0
+    yy, xx = O.interp2d_coords((10, 20), (5, 5), rot_deg=45)
+    assert not np.isnan(O.interp2d(big[:10], yy, xx)).any()
+    # default grid without rotation: k (n_in - 1) / n_out along each axis (interp2d.py:61-65)
+    yy, xx = O.interp2d_coords((10, 20), (4, 8))
+    np.testing.assert_allclose(yy[:, 0], np.arange(4) * 9 / 4)
+    np.testing.assert_allclose(xx[0], np.arange(8) * 19 / 8)
+
+
+def test_interpolated_sh_reproduces_scao_fixtures():
+    """params_scao_sh_test*.yml geometry (64 px pupil, 8x8 subapertures, fov_ovs 2, N 32, cut 24):
+    the flat-wavefront image selects the 52 subapertures of scao_subaps_n8_th0.5_ref.fits and
+    gives the near-zero slopes of scao_sn_n8_th0.5_ref.fits."""
+    g = O.sh_geometry(ef_size=64, pixel_pitch=0.015625, wavelengthInNm=600, subap_wanted_fov=4.0,
+                      sensor_pxscale=0.5, subap_on_diameter=8, subap_npx=8)
+    assert (g['fov_ovs'], g['ovs_np_sub'], g['fft_size'], g['cutpixels'], g['interpolate']) == (2, 16, 32, 24, True)
+    A = O.make_mask(64, 0.0, 1.0).astype(float)
+    img = O.sh_forward(A, np.zeros((64, 64)), g)
+    idxs_ref = fits.read(os.path.join(GOLDEN, 'scao_subaps_n8_th0.5_ref.fits'))[1][0].T
+    idxs, _ = O.detect_subaps(img, 8, 0.5)
+    np.testing.assert_array_equal(idxs, idxs_ref)
+    sn = fits.read(os.path.join(GOLDEN, 'scao_sn_n8_th0.5_ref.fits'))[1][0]
+    s = O.shslopec(img, idxs_ref)['slopes']
+    np.testing.assert_allclose(s, sn, rtol=0, atol=2e-8)
+    # flux normalisation uses the input pupil's masked area (sh.py:426), not the resampled one
+    np.testing.assert_allclose(img.sum(), A.sum() * 0.015625 ** 2, rtol=1e-12)
