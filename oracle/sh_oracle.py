@@ -603,3 +603,85 @@ def gaussian_kernels_for_sh(geom, spot_size, positive_shift_tt=True):
     real = lgs_map_sh(n, geom['pixel_pitch'] * geom['ef_size'], 0, 90e3, [0], profz=[1.0],
                       fwhmb=spot_size, ps=pxscale, ssp=geom['fft_size'], overs=1, theta=lgs_tt)
     return process_kernels(real, n, n, return_fft=True)
+
+
+# --------------------------------------------------------------------------- modal DM, push-pull IM
+
+def zernike_degree(j):
+    """ZernikeGenerator.degree, zernike_generator.py:141-151 (Noll order)."""
+    n = int(np.ceil(0.5 * (np.sqrt(8 * j + 1) - 3)))                      # :404-421
+    cn = n * (n + 1) / 2 + 1
+    if n % 2 == 0:
+        m = int(j - cn + 1) // 2 * 2
+    else:
+        m = int(j - cn) // 2 * 2 + 1
+    return n, m
+
+
+def zernike_map(j, dim):
+    """ZernikeGenerator(dim).getZernike(j), zernike_generator.py:83-99,153-190,192-246, FP64:
+    pixel-centre coordinates of a unit disk of diameter ``dim``, the factorial sum for R_n^m."""
+    sz = math.ceil(dim)
+    c = sz / 2
+    Y, X = (np.mgrid[0.5:sz + 0.5:1, 0.5:sz + 0.5:1] - c) / (dim / 2)
+    rho = np.sqrt(X ** 2 + Y ** 2)
+    theta = np.arctan2(Y, X)
+    n, m = zernike_degree(j)
+    if n == 0 and m == 0:
+        R = np.ones_like(rho)
+    else:
+        R = np.zeros_like(rho)
+        for s in range((n - abs(m)) // 2 + 1):
+            R = R + ((-1) ** s * math.factorial(n - s) /
+                     (math.factorial(s) * math.factorial((n + abs(m)) // 2 - s) *
+                      math.factorial((n - abs(m)) // 2 - s))) * rho ** (n - 2 * s)
+    NC = np.sqrt(2 * (n + 1))
+    if m == 0:
+        return np.sqrt(0.5) * NC * R
+    return NC * R * (np.cos(m * theta) if j % 2 == 0 else np.sin(m * theta))
+
+
+def compute_zern_ifunc(dim, nzern, obsratio=0.0, diaratio=1.0):
+    """compute_zern_ifunc.py:6-29 with make_orto_modes (utils.py:51-83): Zernike 2..nzern+1 on
+    the make_mask pixels, Householder QR (numpy.linalg.qr), mean removed, unit std.
+    Returns (ifunc (nzern, n_px), mask (dim, dim) float)."""
+    mask = make_mask(dim, obsratio, diaratio).astype(float)
+    idx = np.where(mask)
+    z = np.array([zernike_map(j, dim)[idx] for j in range(2, nzern + 2)])
+    q, _ = np.linalg.qr(z.T)
+    q = q.T
+    q = q - np.mean(q, axis=1, keepdims=True)
+    q = q / np.std(q, axis=1, keepdims=True)
+    return q, mask
+
+
+def dm_phase(ifunc, mask, cmd, sign=-1):
+    """DM.trigger_code, dm.py:72-86 (no m2c): phase[idx] = (sign * cmd) @ ifunc, zero elsewhere."""
+    out = np.zeros(mask.shape)
+    out[np.where(mask)] = (sign * np.asarray(cmd, dtype=np.float64)) @ ifunc
+    return out
+
+
+def pushpull_time_history(n_modes, vect_amplitude):
+    """modal_pushpull_signal.py:35-65 (ncycles 1, nsamples 1): rows +a_m, -a_m for each mode."""
+    th = np.zeros((2 * n_modes, n_modes))
+    for i in range(n_modes):
+        th[2 * i:2 * i + 2, i] = [vect_amplitude[i], -vect_amplitude[i]]
+    return th
+
+
+def accumulate_intmat(slopes_hist, commands_hist, n_modes):
+    """ImRecCalibrator.trigger_code, im_rec_calibrator.py:42-71: IM[mode] += slopes / command of
+    the non-zero command of each step."""
+    im = np.zeros((n_modes, slopes_hist.shape[1]))
+    for s, c in zip(slopes_hist, commands_hist):
+        nz = np.nonzero(c)[0]
+        if len(nz) > 0 and nz[0] < n_modes:
+            im[nz[0]] += s / c[nz[0]]
+    return im
+
+
+def recmat_fits_data(im):
+    """Intmat.generate_rec (intmat.py:74-87: pinv) as Recmat.save writes it (recmat.py:40-50:
+    the transpose), i.e. the (n_modes, n_slopes) array of the file's extension 1."""
+    return np.linalg.pinv(im).T

@@ -1,0 +1,95 @@
+"""Drop-in DM (modal deformable mirror) processing object, specula/processing_objects/dm.py:10-96.
+
+The command -> phase projection ``phase[idx] = (sign * cmd) @ ifunc`` is one device GEMV per
+trigger (GEMM for batches of command vectors, ``project``) against the influence functions laid
+out on the full pupil grid (zero outside the mask), so the product lands directly in the layer's
+phase array with no scatter pass.
+"""
+import numpy as np
+import torch
+
+from specula_amd.base import BaseProcessingObj, BaseValue, InputValue
+from specula_amd.data_objects.ifunc import IFunc
+from specula_amd.data_objects.layer import Layer
+
+
+class DM(BaseProcessingObj):
+    def __init__(self, simul_params=None, height: float = 0.0, ifunc: IFunc = None, m2c=None, type_str: str = None,
+                 nmodes: int = None, nzern: int = None, start_mode: int = None, input_offset: int = 0,
+                 idx_modes=None, npixels: int = None, obsratio: float = None, diaratio: float = None,
+                 pupilstop=None, sign: int = -1, pixel_pitch: float = None,
+                 target_device_idx: int = None, precision: int = None):
+        super().__init__(target_device_idx=target_device_idx, precision=precision)
+        if simul_params is not None:
+            pixel_pitch = simul_params.pixel_pitch
+        if pixel_pitch is None:
+            raise ValueError('DM needs simul_params (or pixel_pitch)')
+        self.pixel_pitch = pixel_pitch
+        mask = None
+        if pupilstop is not None:
+            mask = pupilstop.A
+            if npixels is None:
+                npixels = mask.shape[0]
+        if ifunc is None:
+            ifunc = IFunc(type_str=type_str, mask=mask, npixels=npixels, obsratio=obsratio, diaratio=diaratio,
+                          nzern=nzern, nmodes=nmodes, start_mode=start_mode, idx_modes=idx_modes,
+                          target_device_idx=target_device_idx, precision=precision)
+        self._ifunc = ifunc
+        ny, nx = ifunc.mask_inf_func.shape
+        n_if = ifunc.size[0]
+        self.if_commands = torch.zeros(n_if, dtype=self.dtype, device=self.device)
+        self.layer = Layer(ny, nx, pixel_pitch, height, target_device_idx=target_device_idx, precision=precision)
+        self.layer.A = ifunc.mask_inf_func.clone()
+        # influence functions on the full grid: GEMM output is the phase array itself
+        self._ifunc_grid = ifunc.modes_on_grid().reshape(n_if, ny * nx).contiguous()
+        if m2c is not None:
+            m = m2c.m2c if hasattr(m2c, 'm2c') else m2c
+            self.m2c = torch.as_tensor(np.asarray(m), dtype=self.dtype, device=self.device)
+            self.m2c_commands = torch.zeros(self.m2c.shape[1], dtype=self.dtype, device=self.device)
+        else:
+            self.m2c = None
+            self.m2c_commands = None
+        self.input_offset = input_offset
+        self.nmodes = nmodes
+        self.sign = sign
+        self.inputs['in_command'] = InputValue(type=BaseValue)
+        self.outputs['out_layer'] = self.layer
+
+    @property
+    def ifunc(self):
+        return self._ifunc.influence_function
+
+    def _commands(self, cmd):
+        cmd = torch.as_tensor(cmd, dtype=self.dtype, device=self.device)
+        if self.nmodes is not None:
+            cmd = cmd[..., self.input_offset:self.input_offset + self.nmodes]
+        if self.m2c is not None:
+            full = torch.zeros(cmd.shape[:-1] + (self.m2c.shape[1],), dtype=self.dtype, device=self.device)
+            full[..., :cmd.shape[-1]] = cmd
+            cmd = full @ self.m2c.T
+        return cmd
+
+    def trigger_code(self):
+        """dm.py:72-86."""
+        cmd = self._commands(self.local_inputs['in_command'].value)
+        self.if_commands.zero_()
+        self.if_commands[:cmd.shape[-1]] = self.sign * cmd
+        torch.mm(self.if_commands.view(1, -1), self._ifunc_grid, out=self.layer.phaseInNm.view(1, -1))
+        self.layer.generation_time = self.current_time
+
+    def project(self, commands, out=None, base=None):
+        """Phases (F, ny, nx) of F command vectors (F, n_cmd) in one GEMM: ``base`` (ny, nx) plus
+        the DM surface of each command, as trigger_code would produce them one at a time."""
+        cmd = self._commands(commands)
+        F = cmd.shape[0]
+        ny, nx = self.layer.A.shape
+        c = torch.zeros((F, self._ifunc_grid.shape[0]), dtype=self.dtype, device=self.device)
+        c[:, :cmd.shape[-1]] = self.sign * cmd
+        if out is None:
+            out = torch.empty((F, ny, nx), dtype=self.dtype, device=self.device)
+        flat = out.view(F, ny * nx)
+        if base is None:
+            torch.matmul(c, self._ifunc_grid, out=flat)
+        else:
+            torch.addmm(base.reshape(1, ny * nx).to(self.dtype), c, self._ifunc_grid, out=flat)
+        return out

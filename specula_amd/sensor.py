@@ -19,7 +19,11 @@ from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weig
 
 
 class SHSensor:
-    def __init__(self, geom, pupil, idxs=None, energy_th=0.5, max_frames=1, device=0, thr=0.0, variant=0):
+    def __init__(self, geom, pupil, idxs=None, energy_th=0.5, max_frames=1, device=0, thr=0.0, variant=0,
+                 extrapolation_mask=None):
+        """extrapolation_mask: with field resampling, the amplitude the edge-extrapolation maps are
+        built from (SH.setup reads in_ef.A, sh.py:453; inside a simulation that is the not yet
+        propagated all-ones field, so no pixel is extrapolated).  None = ``pupil``."""
         self.geom = geom
         self.device = torch.device(f'cuda:{device}')
         self.max_frames = max_frames
@@ -27,7 +31,7 @@ class SHSensor:
         ef, ccd = geom.ef_size, geom.ccd_side
         self.A = torch.as_tensor(np.asarray(pupil, dtype=np.float32), device=self.device).contiguous()
         if geom.needs_interpolation:
-            self.plan.set_resampling(np.asarray(pupil))
+            self.plan.set_resampling(np.asarray(pupil if extrapolation_mask is None else extrapolation_mask))
         self.thr = float(thr)
         if idxs is None:
             flat = torch.zeros((1, ef, ef), dtype=torch.float32, device=self.device)

@@ -371,3 +371,71 @@ def test_interpolated_sh_reproduces_scao_fixtures():
     np.testing.assert_allclose(s, sn, rtol=0, atol=2e-8)
     # flux normalisation uses the input pupil's masked area (sh.py:426), not the resampled one
     np.testing.assert_allclose(img.sum(), A.sum() * 0.015625 ** 2, rtol=1e-12)
+
+
+# ---------------------------------------------------------------- modal DM and push-pull calibration
+
+SCAO_SH = dict(wavelengthInNm=600, subap_wanted_fov=4.0, sensor_pxscale=0.5, subap_on_diameter=8, subap_npx=8)
+
+
+def test_zernike_closed_forms_and_orthonormality():
+    """Noll's table (Noll 1976, table 1) at the pixel centres; (1/pi) integral Z_i Z_j = delta_ij."""
+    dim = 48
+    c = dim / 2
+    Y, X = (np.mgrid[0.5:dim + 0.5:1, 0.5:dim + 0.5:1] - c) / (dim / 2)
+    r, t = np.hypot(X, Y), np.arctan2(Y, X)
+    table = {1: np.ones_like(r), 2: 2 * X, 3: 2 * Y, 4: math.sqrt(3) * (2 * r ** 2 - 1),
+             5: math.sqrt(6) * r ** 2 * np.sin(2 * t), 6: math.sqrt(6) * r ** 2 * np.cos(2 * t),
+             7: math.sqrt(8) * (3 * r ** 3 - 2 * r) * np.sin(t), 8: math.sqrt(8) * (3 * r ** 3 - 2 * r) * np.cos(t),
+             9: math.sqrt(8) * r ** 3 * np.sin(3 * t), 10: math.sqrt(8) * r ** 3 * np.cos(3 * t),
+             11: math.sqrt(5) * (6 * r ** 4 - 6 * r ** 2 + 1)}
+    for j, z in table.items():
+        np.testing.assert_allclose(O.zernike_map(j, dim), z, rtol=0, atol=1e-12, err_msg=f'Z{j}')
+    big = 400
+    disk = O.make_mask(big, 0.0, 1.0).astype(bool)
+    Z = np.array([O.zernike_map(j, big)[disk] for j in range(1, 16)])
+    G = Z @ Z.T / disk.sum()
+    np.testing.assert_allclose(G, np.eye(15), atol=2e-2)
+
+
+def test_zernike_ifunc_normalised_modes():
+    """compute_zern_ifunc: mean-free, unit std rows spanning the same space as the raw modes."""
+    ifunc, mask = O.compute_zern_ifunc(64, 10, obsratio=0.1)
+    np.testing.assert_allclose(ifunc.mean(axis=1), 0, atol=1e-12)
+    np.testing.assert_allclose(ifunc.std(axis=1), 1, rtol=1e-12)
+    idx = np.where(mask)
+    raw = np.array([O.zernike_map(j, 64)[idx] for j in range(2, 12)])
+    # mode k lies in span(raw modes 1..k, piston): residual of a least-squares fit is ~0
+    for k in range(10):
+        basis = np.vstack([raw[:k + 1], np.ones(raw.shape[1])]).T
+        coef, *_ = np.linalg.lstsq(basis, ifunc[k], rcond=None)
+        assert np.abs(basis @ coef - ifunc[k]).max() < 1e-9
+    # tip (Z2 = 2x) keeps its sign through the Householder QR on this pupil
+    assert np.corrcoef(ifunc[0], raw[0])[0, 1] > 0.999
+
+
+def scao_pushpull_oracle():
+    go = O.sh_geometry(ef_size=64, pixel_pitch=0.015625, **SCAO_SH)
+    idxs = fits.read(os.path.join(GOLDEN, 'scao_subaps_n8_th0.5_ref.fits'))[1][0].T
+    ifunc, dmask = O.compute_zern_ifunc(64, 40, obsratio=0.1, diaratio=None)
+    A = O.make_mask(64, None, 1.0).astype(float) * dmask        # pupilstop x DM layer amplitude
+    th = O.pushpull_time_history(40, [50.0] * 40)
+    slopes = []
+    for cmd in th:
+        ph = O.dm_phase(ifunc, dmask, cmd, sign=1)
+        # the SH was set up on the not yet propagated (all-ones) field: no edge extrapolation
+        img = O.sh_forward(A, ph, go, S0=1.0, ext_mask=np.ones((64, 64)))
+        slopes.append(O.shslopec(img, idxs)['slopes'])
+    return O.accumulate_intmat(np.array(slopes), th, 40), th, np.array(slopes)
+
+
+def test_pushpull_calibration_reproduces_scao_rec_fixture():
+    """test_sh_calibration.py + params_scao_sh_test{,_rec}.yml: 40 Zernike modes at 50 nm pushed
+    and pulled through SH (fov_ovs 2, cut 24) and ShSlopec; pinv of the IM as Recmat.save writes
+    it equals scao_rec_n8_th0.5_ref.fits to the test's decimal=3 (|d| < 1.5e-3)."""
+    im, _, _ = scao_pushpull_oracle()
+    rec = O.recmat_fits_data(im)
+    ref = fits.read(os.path.join(GOLDEN, 'scao_rec_n8_th0.5_ref.fits'))[1][0]
+    assert rec.shape == ref.shape == (40, 104)
+    np.testing.assert_array_almost_equal(rec, ref, decimal=3)
+    assert np.abs(rec - ref).max() < 1e-4                 # observed 2.6e-5 (the file is FP32)
