@@ -7,9 +7,10 @@ index transposition ``kernels[j*dimx+i] <- real_kernels[i*dimx+j]``.  The HIP SH
 these spectra on the (2 np - 1)^2 lag grid: ``lag_spectra(np_sub)`` gathers them.
 
 Kernels are recomputed when the SHA1 of the parameters changes (convolution_kernel.py:161-174,
-307-330); unlike the reference, no FITS cache file is written to the working directory.
+307-330); the FITS cache (v1.1 files named by the hash) is used only when ``cache_dir`` is set.
 """
 import hashlib
+import os
 import json
 
 import numpy as np
@@ -97,6 +98,7 @@ class ConvolutionKernel(BaseDataObj):
         self.real_kernels = None
         self.kernels = None
         self._kernel_fn = None
+        self.cache_dir = None     # directory of cached kernel FITS files (None: no files)
         self.version = 0          # bumped each time the kernels are recomputed
 
     def calculate_focus(self):
@@ -165,7 +167,9 @@ class ConvolutionKernel(BaseDataObj):
         self.version += 1
 
     def prepare_for_sh(self, sodium_altitude=None, sodium_intensity=None, current_time=None):
-        """Recompute only when the parameter hash changes (convolution_kernel.py:307-330)."""
+        """Recompute only when the parameter hash changes (convolution_kernel.py:307-330).  With
+        ``cache_dir`` set, kernels are loaded from / saved to ``<cache_dir>/<hash name>.fits`` as
+        the reference does in its working directory; without it nothing is written."""
         if sodium_altitude is not None:
             self.zlayer = sodium_altitude
         if sodium_intensity is not None:
@@ -173,9 +177,51 @@ class ConvolutionKernel(BaseDataObj):
         fn = self.build()
         if fn != self._kernel_fn:
             self._kernel_fn = fn
-            self.calculate_lgs_map()
+            path = os.path.join(self.cache_dir, fn + '.fits') if self.cache_dir else None
+            if path and os.path.exists(path):
+                self.restore(path, kernel_obj=self, return_fft=self.return_fft)
+            else:
+                self.calculate_lgs_map()
+                if path:
+                    self.save(path)
         if current_time is not None:
             self.generation_time = current_time
+
+    def save(self, filename):
+        """convolution_kernel.py:276-305: VERSION 1.1 header (PXSCALE, DIM, OVERSAMP, POSTT,
+        SPOTSIZE, DIMX, DIMY) and the real (unnormalised) kernels in extension 1."""
+        from specula_amd.lib import fits
+        hdr = {'VERSION': 1.1, 'PXSCALE': float(self.pxscale), 'DIM': int(self.dimension),
+               'OVERSAMP': int(self.oversampling), 'POSTT': bool(self.positive_shift_tt),
+               'SPOTSIZE': float(self.spot_size), 'DIMX': int(self.dimx), 'DIMY': int(self.dimy)}
+        dt = np.float32 if self.precision == 1 else np.float64
+        fits.write(filename, [(None, hdr), (np.asarray(self.real_kernels, dtype=dt), {})])
+
+    @staticmethod
+    def restore(filename, target_device_idx=None, kernel_obj=None, return_fft=False):
+        """convolution_kernel.py:332-380."""
+        from specula_amd.lib import fits
+        hdus = fits.read(filename)
+        hdr = hdus[0][1]
+        if hdr.get('VERSION') != 1.1:
+            raise ValueError(f"Unknown version {hdr.get('VERSION')}. Only version=1.1 is supported")
+        if kernel_obj is None:
+            kernel_obj = ConvolutionKernel(dimx=hdr['DIMX'], dimy=hdr['DIMY'], pxscale=hdr['PXSCALE'],
+                                           pupil_size_m=0.0, dimension=hdr['DIM'], launcher_pos=[0.0, 0.0, 0.0],
+                                           launcher_size=hdr['SPOTSIZE'], oversampling=hdr['OVERSAMP'],
+                                           positive_shift_tt=hdr['POSTT'], target_device_idx=target_device_idx)
+            kernel_obj.spot_size = hdr['SPOTSIZE']
+        else:
+            if kernel_obj.dimx != hdr['DIMX'] or kernel_obj.dimy != hdr['DIMY']:
+                raise ValueError('Provided kernel object dimensions do not match the FITS file dimensions')
+            kernel_obj.pxscale = hdr['PXSCALE']
+            kernel_obj.dimension = hdr['DIM']
+            kernel_obj.oversampling = hdr['OVERSAMP']
+            kernel_obj.positive_shift_tt = hdr['POSTT']
+            kernel_obj.spot_size = hdr['SPOTSIZE']
+        kernel_obj.real_kernels = np.asarray(hdus[1][0], dtype=np.float64)
+        kernel_obj.process_kernels(return_fft=return_fft)
+        return kernel_obj
 
     def lag_spectra(self, np_sub):
         """(n_sub, L, L) complex64 with L = 2 np - 1: kernels[s][m_y mod N][m_x mod N] for

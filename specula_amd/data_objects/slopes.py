@@ -1,3 +1,4 @@
+import numpy as np
 import torch
 
 from specula_amd.base import BaseDataObj
@@ -56,3 +57,32 @@ class Slopes(BaseDataObj):
 
     def indy(self):
         return self.indicesY
+
+    def save(self, filename, hdr=None):
+        """slopes.py:125-135: VERSION 2 / INTRLVD (and the optional tags) over a 2-element dummy,
+        the slope vector in extension 1."""
+        from specula_amd.lib import fits
+        h = dict(hdr or {})
+        h['VERSION'] = 2
+        h['INTRLVD'] = int(self.interleave)
+        if getattr(self, 'pupdata_tag', None) is not None:
+            h['PUPD_TAG'] = self.pupdata_tag
+        if getattr(self, 'subapdata_tag', None) is not None:
+            h['SUBAP_TAG'] = self.subapdata_tag
+        fits.write(filename, [(np.zeros(2), h), (self.slopes.cpu().numpy(), {})])
+
+    @staticmethod
+    def restore(filename, target_device_idx=None):
+        """slopes.py:141-156.  A missing PUPD_TAG card (SH slope files, e.g. slope nulls, are
+        written without one) restores as an empty tag instead of failing."""
+        from specula_amd.lib import fits
+        hdus = fits.read(filename)
+        hdr = hdus[0][1]
+        version = int(hdr['VERSION'])
+        if version > 2:
+            raise ValueError(f'Error: unknown version {version} in file {filename}')
+        s = Slopes(slopes=hdus[1][0].copy(), interleave=bool(hdr.get('INTRLVD', 0)),
+                   target_device_idx=target_device_idx)
+        if version >= 2:
+            s.pupdata_tag = str(hdr.get('PUPD_TAG', '')).strip()
+        return s

@@ -123,3 +123,37 @@ def test_im_rec_calibrator_accumulates_and_writes_fixture_rec(tmp_path):
         again.inputs['in_slopes'].set(sl)
         again.inputs['in_commands'].set(cmd)
         again.setup()
+
+
+def test_slopes_fits_restore_fixture_and_round_trip(tmp_path):
+    from specula_amd.data_objects.slopes import Slopes
+    sn = Slopes.restore(os.path.join(GOLDEN, 'scao_sn_n8_th0.5_ref.fits'), target_device_idx=-1)
+    raw = fits.read(os.path.join(GOLDEN, 'scao_sn_n8_th0.5_ref.fits'))[1][0]
+    np.testing.assert_array_equal(sn.slopes.numpy(), raw)
+    assert sn.size == 104 and not sn.interleave and sn.pupdata_tag == ''
+    s = Slopes(slopes=np.arange(10, dtype=np.float32), interleave=True, target_device_idx=-1)
+    s.pupdata_tag = 'pup'
+    s.save(str(tmp_path / 's.fits'))
+    b = Slopes.restore(str(tmp_path / 's.fits'), target_device_idx=-1)
+    np.testing.assert_array_equal(b.slopes.numpy(), s.slopes.numpy())
+    assert b.interleave and b.pupdata_tag == 'pup'
+    torch.testing.assert_close(b.xslopes, torch.tensor([0., 2, 4, 6, 8]))
+
+
+def test_convolution_kernel_fits_cache_round_trip(tmp_path):
+    from specula_amd.data_objects.convolution_kernel import ConvolutionKernel
+    kw = dict(dimx=4, dimy=4, pxscale=0.3, pupil_size_m=8.0, dimension=32, launcher_pos=[5.0, 5.0, 0.0],
+              launcher_size=1.0, target_device_idx=-1)
+    k = ConvolutionKernel(**kw)
+    k.cache_dir = str(tmp_path)
+    k.prepare_for_sh(sodium_altitude=[88e3, 90e3, 92e3], sodium_intensity=[0.2, 0.6, 0.2])
+    files = list(tmp_path.glob('ConvolutionKernel*.fits'))
+    assert len(files) == 1
+    hdr = fits.read(str(files[0]))[0][1]
+    assert hdr['VERSION'] == 1.1 and hdr['DIM'] == 32 and hdr['POSTT'] is True
+    k2 = ConvolutionKernel(**kw)
+    k2.cache_dir = str(tmp_path)
+    k2.prepare_for_sh(sodium_altitude=[88e3, 90e3, 92e3], sodium_intensity=[0.2, 0.6, 0.2])   # loads the file
+    np.testing.assert_allclose(k2.kernels, k.kernels, rtol=0, atol=1e-7)   # FP32 file
+    k3 = ConvolutionKernel.restore(str(files[0]), target_device_idx=-1, return_fft=True)
+    np.testing.assert_allclose(k3.kernels, k.kernels, rtol=0, atol=1e-7)
