@@ -201,6 +201,44 @@ int shwfs_batch_latencies(shwfs_batch* batch, double* us, void* stream);
 int shwfs_batch_info(shwfs_batch* batch, int32_t* n_items, int32_t* grid);
 int shwfs_batch_destroy(shwfs_batch* batch);
 
+/* ---- detector post-step (CCD.trigger_code, processing_objects/ccd.py:198-280) -------------
+ * Noise sources use a counter-based generator keyed by their seed and indexed by
+ * (frame * n_pixels + pixel, counter, draw): results are reproducible for a given counter, and
+ * statistically (not bit-) equivalent to numpy's default_rng streams of the reference. */
+#define SHCCD_PHOTON         0x001  /* photon_noise: Poisson, then ADU quantisation            */
+#define SHCCD_READOUT        0x002  /* readout_noise: + readout_level * N(0,1)                 */
+#define SHCCD_EXCESS         0x004  /* excess_noise: EM gamma of shape delta*x, scale emccd    */
+#define SHCCD_DARK           0x008  /* darkcurrent_noise                                       */
+#define SHCCD_BACKGROUND     0x010  /* background_noise                                        */
+#define SHCCD_CIC            0x020  /* cic_noise: + Bernoulli(cic_level)                       */
+#define SHCCD_KEEP_BIAS      0x040  /* _keep_ADU_bias                                          */
+#define SHCCD_BG_REMOVE_AVG  0x080  /* _bg_remove_average                                      */
+#define SHCCD_KEEP_DARK      0x100  /* _do_not_remove_dark                                     */
+
+typedef struct shccd_params {
+    int32_t in_rows, in_cols;      /* integrated frame = size * binning (ccd.py:134-136)     */
+    int32_t out_rows, out_cols;    /* pixel frame = size // binning                          */
+    int32_t binning;
+    int32_t flags;                 /* SHCCD_* bits                                           */
+    double qe;                     /* quantum_eff (apply_qe, ccd.py:275-280)                 */
+    double background_level, darkcurrent_level, readout_level, cic_level;
+    double adu_gain, adu_bias, emccd_gain, excess_delta;
+    uint64_t photon_seed, readout_seed, excess_seed, cic_seed;
+} shccd_params;
+
+/* Intensity.sum(in_i, factor) into the integration buffer (ccd.py:200-205):
+ * integrated[f] += in_i[f] * factor over n_px pixels per frame. */
+int shccd_integrate(int n_frames, const float* in_i, int64_t in_stride, double factor, float* integrated,
+                    int64_t int_stride, int64_t n_px, void* stream);
+/* Readout (apply_binning, apply_qe, apply_noise, ccd.py:207-273) of n_frames frames into
+ * `pixels`.  Input: the integration buffer (read, then zeroed as ccd.py:217 does) and/or the
+ * current intensity `in_i * factor` (the last integration step fused into the readout; either
+ * pointer may be NULL, not both).  pixel_gains: out_rows x out_cols or NULL.  `counter`
+ * selects the random numbers of this readout (the caller advances it). */
+int shccd_readout(const shccd_params* params, int n_frames, float* integrated, int64_t int_stride,
+                  const float* in_i, int64_t in_stride, double factor, const float* pixel_gains,
+                  uint64_t counter, float* pixels, int64_t out_stride, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -685,3 +685,54 @@ def recmat_fits_data(im):
     """Intmat.generate_rec (intmat.py:74-87: pinv) as Recmat.save writes it (recmat.py:40-50:
     the transpose), i.e. the (n_modes, n_slopes) array of the file's extension 1."""
     return np.linalg.pinv(im).T
+
+
+# --------------------------------------------------------------------------- CCD
+
+def ccd_readout(integrated, binning=1, quantum_eff=1.0, photon_noise=False, readout_noise=False,
+                excess_noise=False, darkcurrent_noise=False, background_noise=False, cic_noise=False,
+                readout_level=0.0, darkcurrent_level=0.0, background_level=0.0, cic_level=0.0,
+                ADU_gain=8, ADU_bias=400, emccd_gain=1, excess_delta=1.0, pixelGains=None,
+                keep_ADU_bias=False, bg_remove_average=False, do_not_remove_dark=False,
+                photon_rng=None, readout_rng=None, cic_rng=None, excess_rng=None, dtype=np.float32):
+    """CCD.apply_binning + apply_qe + apply_noise, ccd.py:220-273, with numpy generators and the
+    reference's dtype flow (float32 frame; int64 after Poisson, so an added readout noise is
+    truncated by ``astype``; float64 after the ADU conversion).  ``excess_noise`` uses
+    numpy's gamma in place of scipy.stats.gamma.rvs (same distribution)."""
+    if excess_noise:                                                   # ccd.py:104-106
+        emccd_gain = 400
+        ADU_gain = 1 / 20
+    frame = np.array(integrated, dtype=dtype)
+    if binning > 1:                                                    # :264-268
+        tot = np.sum(frame)
+        h, w = frame.shape[0] // binning, frame.shape[1] // binning
+        frame = frame.reshape(h, binning, w, binning).sum(axis=(1, 3))
+        frame = frame * binning ** 2 * (tot / np.sum(frame))
+    if quantum_eff != 1:                                               # :276-277
+        frame *= dtype(quantum_eff) if dtype is np.float32 else quantum_eff
+    if background_noise or darkcurrent_noise:                          # :222-223
+        frame += (background_level + darkcurrent_level)
+    if cic_noise:                                                      # :228-229
+        frame = frame + cic_rng.binomial(1, cic_level, frame.shape)
+    if photon_noise:                                                   # :231-232
+        frame = photon_rng.poisson(frame)
+    if excess_noise:                                                   # :234-235
+        frame = 1.0 / excess_delta * excess_rng.gamma(excess_delta * frame, emccd_gain)
+    if readout_noise:                                                  # :237-239
+        ron = readout_rng.standard_normal(size=frame.size)
+        frame += (ron.reshape(frame.shape) * readout_level).astype(frame.dtype)
+    if pixelGains is not None:                                         # :241-242 (in place there,
+        frame = frame * pixelGains                                     # which numpy refuses for int64)
+    if photon_noise:                                                   # :249-262
+        frame = np.round(frame * ADU_gain) + ADU_bias
+        frame = np.where(frame < 0, 0, frame)
+        if not keep_ADU_bias:
+            frame = frame - ADU_bias
+        frame = frame / ADU_gain
+        if excess_noise:
+            frame = frame / emccd_gain
+        if darkcurrent_noise and not do_not_remove_dark:
+            frame = frame - darkcurrent_level
+        if bg_remove_average and not do_not_remove_dark:
+            frame = frame - background_level
+    return frame

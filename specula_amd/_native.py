@@ -22,7 +22,24 @@ EXPORTS = (
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian',
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
+    'shccd_integrate', 'shccd_readout',
 )
+
+SHCCD_PHOTON, SHCCD_READOUT, SHCCD_EXCESS, SHCCD_DARK, SHCCD_BACKGROUND, SHCCD_CIC = 1, 2, 4, 8, 16, 32
+SHCCD_KEEP_BIAS, SHCCD_BG_REMOVE_AVG, SHCCD_KEEP_DARK = 64, 128, 256
+
+
+class ShccdParams(ctypes.Structure):
+    """Mirror of ``shccd_params`` in include/shwfs.h."""
+    _fields_ = [
+        ('in_rows', ctypes.c_int32), ('in_cols', ctypes.c_int32), ('out_rows', ctypes.c_int32),
+        ('out_cols', ctypes.c_int32), ('binning', ctypes.c_int32), ('flags', ctypes.c_int32),
+        ('qe', ctypes.c_double), ('background_level', ctypes.c_double), ('darkcurrent_level', ctypes.c_double),
+        ('readout_level', ctypes.c_double), ('cic_level', ctypes.c_double), ('adu_gain', ctypes.c_double),
+        ('adu_bias', ctypes.c_double), ('emccd_gain', ctypes.c_double), ('excess_delta', ctypes.c_double),
+        ('photon_seed', ctypes.c_uint64), ('readout_seed', ctypes.c_uint64), ('excess_seed', ctypes.c_uint64),
+        ('cic_seed', ctypes.c_uint64),
+    ]
 
 
 class ShwfsGeom(ctypes.Structure):
@@ -71,6 +88,9 @@ def _declare(lib):
         'shwfs_batch_latencies': (_I, [_P, _P, _P]),
         'shwfs_batch_info': (_I, [_P, _P, _P]),
         'shwfs_batch_destroy': (_I, [_P]),
+        'shccd_integrate': (_I, [_I, _P, _I64, _D, _P, _I64, _I64, _P]),
+        'shccd_readout': (_I, [ctypes.POINTER(ShccdParams), _I, _P, _I64, _P, _I64, _D, _P, ctypes.c_uint64, _P,
+                               _I64, _P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

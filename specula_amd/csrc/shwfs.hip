@@ -42,6 +42,9 @@ int set_error(int code, const std::string& msg) {
 }
 }  // namespace
 
+// error reporting shared with the library's other translation units
+int shwfs_internal_set_error(int code, const char* msg) { return set_error(code, msg); }
+
 #define SHWFS_HIP_CHECK(expr)                                                              \
     do {                                                                                   \
         hipError_t e_ = (expr);                                                            \

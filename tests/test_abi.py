@@ -14,7 +14,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 def declared():
     text = open(HEADER).read()
     text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
-    return set(re.findall(r'\b((?:shwfs|shslopec)_[a-z_]+)\s*\(', text))
+    return set(re.findall(r'\b((?:shwfs|shslopec|shccd)_[a-z_]+)\s*\(', text))
 
 
 def test_library_exports_header_symbols():
@@ -29,6 +29,24 @@ def test_library_exports_header_symbols():
 def test_geom_struct_layout():
     # int32 x 10 then 5 doubles, no padding surprises
     assert ctypes.sizeof(_native.ShwfsGeom) == 10 * 4 + 5 * 8
+    # int32 x 6, 9 doubles, 4 uint64
+    assert ctypes.sizeof(_native.ShccdParams) == 6 * 4 + 9 * 8 + 4 * 8
+
+
+def test_ccd_argument_errors_are_reported():
+    lib = _native.load_library()
+    p = _native.ShccdParams(in_rows=8, in_cols=8, out_rows=4, out_cols=4, binning=3, flags=0, qe=1.0)
+    rc = lib.shccd_readout(ctypes.byref(p), 1, None, 0, ctypes.c_void_p(16), 64, 1.0, None, 0,
+                           ctypes.c_void_p(16), 16, None)
+    assert rc == -1 and b'in = out * binning' in lib.shwfs_last_error()
+    p.binning = 2
+    p.flags = _native.SHCCD_PHOTON
+    p.adu_gain = 0.0
+    rc = lib.shccd_readout(ctypes.byref(p), 1, None, 0, ctypes.c_void_p(16), 64, 1.0, None, 0,
+                           ctypes.c_void_p(16), 16, None)
+    assert rc == -1 and b'ADU gain' in lib.shwfs_last_error()
+    rc = lib.shccd_integrate(1, None, 0, 1.0, None, 0, 16, None)
+    assert rc == -1
 
 
 def test_argument_errors_are_reported():

@@ -62,11 +62,10 @@ def test_batched_pushpull_reproduces_scao_rec():
 
 
 def test_drop_in_objects_calibration_loop(tmp_path):
-    """DM -> field -> SH -> ShSlopec -> ImRecCalibrator objects stepped as LoopControl does (SH set
-    up on the still all-ones field), rec file written at finalize equals the fixture."""
+    """DM -> field -> SH -> CCD -> ShSlopec -> ImRecCalibrator objects stepped as LoopControl does
+    (SH set up on the still all-ones field), rec file written at finalize equals the fixture."""
     from specula_amd.base import BaseValue
     from specula_amd.data_objects.electric_field import ElectricField
-    from specula_amd.data_objects.pixels import Pixels
     from specula_amd.data_objects.subap_data import SubapData
     from specula_amd.processing_objects.im_rec_calibrator import ImRecCalibrator
     from specula_amd.processing_objects.sh import SH
@@ -79,17 +78,25 @@ def test_drop_in_objects_calibration_loop(tmp_path):
     ef = ElectricField(64, 64, PITCH, S0=1.0, target_device_idx=0)
     sh = SH(target_device_idx=0, **SCAO_SH)
     sh.inputs['in_ef'].set(ef)
-    pix = Pixels(64, 64, target_device_idx=0)
+    from specula_amd.data_objects.simul_params import SimulParams
+    from specula_amd.processing_objects.ccd import CCD
+    # detector of params_scao_sh_test.yml with the _rec override (no photon / readout noise)
+    ccd = CCD(SimulParams(pixel_pupil=64, pixel_pitch=PITCH, time_step=0.001), size=[64, 64], dt=0.001, bandw=300,
+              photon_noise=False, readout_noise=False, readout_level=1.0, quantum_eff=0.3, target_device_idx=0)
+    ccd.inputs['in_i'].set(sh.outputs['out_i'])
+    pix = ccd.outputs['out_pixels']
     sd = SubapData.restore(os.path.join(GOLDEN, 'scao_subaps_n8_th0.5_ref.fits'), target_device_idx=0)
     slopec = ShSlopec(sd, target_device_idx=0)
     slopec.inputs['in_pixels'].set(pix)
     cal = ImRecCalibrator(nmodes=40, data_dir=str(tmp_path), rec_tag='rec', target_device_idx=0)
     cal.inputs['in_slopes'].set(slopec.outputs['out_slopes'])
     cal.inputs['in_commands'].set(cmd)
-    for obj in (dm, sh, slopec, cal):
+    for obj in (dm, sh, ccd, slopec, cal):
         obj.setup()
     th = O.pushpull_time_history(40, [50.0] * 40)
-    for t, c in enumerate(th, start=1):
+    step = ccd.seconds_to_t(0.001)                   # LoopControl advances by the time step in ns
+    for k, c in enumerate(th, start=1):
+        t = k * step
         cmd.value = torch.tensor(c, dtype=torch.float32, device=DEV)
         cmd.generation_time = t
         dm.check_ready(t); dm.trigger(); dm.post_trigger()
@@ -97,8 +104,8 @@ def test_drop_in_objects_calibration_loop(tmp_path):
         ef.phaseInNm.copy_(dm.layer.phaseInNm)
         ef.generation_time = t
         sh.check_ready(t); sh.trigger(); sh.post_trigger()
-        pix.pixels = sh.outputs['out_i'].i           # 1:1 detector without noise: slopes scale-free
-        pix.generation_time = t
+        ccd.check_ready(t); ccd.trigger(); ccd.post_trigger()
+        assert pix.generation_time == t
         slopec.check_ready(t); slopec.trigger(); slopec.post_trigger()
         cal.check_ready(t); cal.trigger(); cal.post_trigger()
     cal.finalize()

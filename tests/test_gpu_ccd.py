@@ -1,0 +1,162 @@
+"""CCD readout on the GPU (shccd_readout / the drop-in CCD object) against the oracle's restatement
+of ccd.py:198-273: exact without noise, distributional with noise (the generators differ: the
+kernel's streams are counter-based, the oracle's are numpy's default_rng as in the reference).
+Every sample set is fixed by its seed and counter, so the statistical checks are deterministic."""
+import numpy as np
+import pytest
+import torch
+from scipy import stats
+
+from oracle import sh_oracle as O
+from specula_amd import _native
+from specula_amd.processing_objects.ccd import CCD, readout_frames
+
+pytestmark = pytest.mark.gpu
+DEV = 'cuda:0'
+
+
+def params(rows, cols, binning=1, **kw):
+    from specula_amd.data_objects.simul_params import SimulParams
+    ccd = CCD(SimulParams(time_step=0.001), size=[rows, cols], dt=0.001, bandw=1.0, binning=binning,
+              target_device_idx=0, **kw)
+    return ccd.params()
+
+
+def run(p, frames, counter=0, factor=1.0, gains=None):
+    x = torch.as_tensor(np.ascontiguousarray(frames, dtype=np.float32), device=DEV)
+    if x.dim() == 2:
+        x = x[None]
+    out = torch.empty((x.shape[0], p.out_rows, p.out_cols), dtype=torch.float32, device=DEV)
+    readout_frames(p, x.contiguous(), out, factor=factor, counter=counter, pixel_gains=gains)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def chi2_two_sample(a, b, min_count=20):
+    """Chi-square homogeneity test of two integer-valued samples (bins with enough counts)."""
+    lo, hi = int(min(a.min(), b.min())), int(max(a.max(), b.max()))
+    ha = np.bincount((a - lo).astype(np.int64), minlength=hi - lo + 1)
+    hb = np.bincount((b - lo).astype(np.int64), minlength=hi - lo + 1)
+    keep = (ha + hb) >= min_count
+    tab = np.vstack([np.append(ha[keep], ha[~keep].sum()), np.append(hb[keep], hb[~keep].sum())])
+    tab = tab[:, tab.sum(axis=0) > 0]
+    return stats.chi2_contingency(tab)[1]
+
+
+@pytest.mark.parametrize('binning', [1, 2])
+def test_noiseless_readout_matches_oracle(binning):
+    rng = np.random.default_rng(0)
+    img = (rng.random((64, 96)) * 1e3).astype(np.float32)
+    p = params(64, 96, binning=binning, quantum_eff=0.3)
+    out = run(p, img, factor=0.25)[0]
+    ref = O.ccd_readout((img * np.float32(0.25)).astype(np.float32), binning=binning, quantum_eff=0.3)
+    np.testing.assert_allclose(out, ref, rtol=2e-6)
+
+
+@pytest.mark.parametrize('lam', [0.3, 2.5, 9.9, 10.0, 37.0, 850.0, 2.5e5])
+def test_poisson_counts_follow_poisson_law(lam):
+    print(f'poisson lambda={lam}', flush=True)
+    p = params(512, 512, photon_noise=True)
+    out = run(p, np.full((512, 512), lam, np.float32))[0].ravel()
+    n = out.size
+    assert np.all(out == np.round(out)) and out.min() >= 0
+    assert abs(out.mean() - lam) < 5 * np.sqrt(lam / n)
+    assert abs(out.var() / lam - 1) < 5 * np.sqrt(2.0 / n) + 1e-3
+    # goodness of fit against the Poisson pmf over the central bins
+    k = out.astype(np.int64)
+    lo, hi = int(stats.poisson.ppf(1e-4, lam)), int(stats.poisson.ppf(1 - 1e-4, lam))
+    edges = np.unique(np.linspace(lo, hi + 1, min(hi - lo + 2, 60)).astype(np.int64))
+    obs = np.histogram(k, bins=edges)[0]
+    cdf = stats.poisson.cdf(edges - 1, lam)
+    exp = np.diff(cdf) * n
+    sel = exp > 20
+    chi2 = ((obs[sel] - exp[sel]) ** 2 / exp[sel]).sum()
+    pval = stats.chi2.sf(chi2, sel.sum() - 1)
+    # same law as numpy's sampler (the reference's generator)
+    ref = np.random.default_rng(1).poisson(lam, size=n)
+    p2 = chi2_two_sample(k, ref)
+    print(f'  mean {out.mean():.6g} var {out.var():.6g}; GOF p={pval:.3g}; vs numpy p={p2:.3g}', flush=True)
+    assert pval > 1e-4 and p2 > 1e-4
+
+
+def test_readout_noise_float_and_truncated():
+    p = params(512, 512, readout_noise=True, readout_level=2.0)
+    out = run(p, np.zeros((512, 512), np.float32))[0].ravel()
+    ks = stats.kstest(out / 2.0, 'norm')
+    print(f'readout: mean {out.mean():.3g} std {out.std():.5g} KS p={ks.pvalue:.3g}', flush=True)
+    assert ks.pvalue > 1e-4
+    # with photon noise the frame is integer and the readout noise is truncated toward zero
+    p = params(512, 512, photon_noise=True, readout_noise=True, readout_level=1.5)
+    out = run(p, np.zeros((512, 512), np.float32))[0].ravel()
+    assert np.all(out == np.trunc(out))
+    ref = O.ccd_readout(np.zeros((512, 512), np.float32), photon_noise=True, readout_noise=True,
+                        readout_level=1.5, photon_rng=np.random.default_rng(1),
+                        readout_rng=np.random.default_rng(2)).ravel()
+    pv = chi2_two_sample(out, ref)
+    print(f'truncated readout: P(0) {np.mean(out == 0):.4f} vs oracle {np.mean(ref == 0):.4f}, p={pv:.3g}', flush=True)
+    assert pv > 1e-4
+
+
+def test_excess_cic_dark_and_adu_paths_match_oracle_distribution():
+    lam = np.full((400, 400), 6.0, np.float32)
+    kw = dict(photon_noise=True, excess_noise=True, readout_noise=True, readout_level=0.5)
+    out = run(params(400, 400, **kw), lam)[0].ravel()
+    ref = O.ccd_readout(lam, **kw, photon_rng=np.random.default_rng(1), readout_rng=np.random.default_rng(2),
+                        excess_rng=np.random.default_rng(3)).ravel()
+    q = 20 / 400            # EMCCD output quantum: 1/(ADU_gain * emccd_gain)
+    pv = chi2_two_sample(np.round(out / q), np.round(ref / q))
+    print(f'excess: mean {out.mean():.4g} vs {ref.mean():.4g}, p={pv:.3g}', flush=True)
+    assert pv > 1e-4
+    kw = dict(photon_noise=True, cic_noise=True, cic_level=0.2, darkcurrent_noise=True, darkcurrent_level=3.0,
+              background_noise=True, background_level=1.0)
+    out = run(params(400, 400, **kw), np.full((400, 400), 4.0, np.float32))[0].ravel()
+    ref = O.ccd_readout(np.full((400, 400), 4.0, np.float32), **kw, photon_rng=np.random.default_rng(1),
+                        cic_rng=np.random.default_rng(5)).ravel()
+    pv = chi2_two_sample(np.round(out * 8), np.round(ref * 8))
+    print(f'cic+dark+bg: mean {out.mean():.4g} vs {ref.mean():.4g} (expect {4 + 0.2 + 3 + 1 - 3:.4g}), p={pv:.3g}',
+          flush=True)
+    assert pv > 1e-4 and abs(out.mean() - 5.2) < 0.05
+
+
+def test_counter_reproducibility_and_batch_independence():
+    p = params(128, 128, photon_noise=True, readout_noise=True, readout_level=1.0)
+    lam = np.full((3, 128, 128), 50.0, np.float32)
+    a = run(p, lam, counter=7)
+    b = run(p, lam, counter=7)
+    c = run(p, lam, counter=8)
+    np.testing.assert_array_equal(a, b)
+    assert np.mean(a == c) < 0.2
+    assert np.mean(a[0] == a[1]) < 0.2                    # frames of a batch draw different numbers
+
+
+def test_ccd_object_integrates_over_dt_and_resets():
+    from specula_amd.data_objects.intensity import Intensity
+    from specula_amd.data_objects.simul_params import SimulParams
+    ccd = CCD(SimulParams(time_step=0.001), size=[32, 32], dt=0.003, bandw=300, quantum_eff=0.5,
+              target_device_idx=0)
+    inp = Intensity(32, 32, target_device_idx=0)
+    ccd.inputs['in_i'].set(inp)
+    ccd.setup()
+    rng = np.random.default_rng(3)
+    frames = [rng.random((32, 32)).astype(np.float32) for _ in range(6)]
+    step = ccd.seconds_to_t(0.001)
+    outs = []
+    for k, fr in enumerate(frames):
+        t = (k + 1) * step
+        inp.i[:] = torch.tensor(fr, device=DEV)
+        inp.generation_time = t
+        ccd.check_ready(t); ccd.trigger(); ccd.post_trigger()
+        outs.append((ccd.outputs['out_pixels'].generation_time == t, ccd.outputs['out_pixels'].pixels.cpu().numpy().copy()))
+    # ccd.py:207: readout when (t + loop_dt - dt) % dt == 0 -> t = 2, 5 ms (steps 2 and 5, 1-based... )
+    read_steps = [k for k, (fresh, _) in enumerate(outs) if fresh]
+    expect = [k for k in range(6) if ((k + 1) * step + step - 3 * step) % (3 * step) == 0]
+    assert read_steps == expect
+    f32 = np.float32(0.001 * 300)
+    last = 0
+    for k in read_steps:
+        acc = np.zeros((32, 32), np.float32)
+        for j in range(last, k + 1):
+            acc += frames[j] * f32
+        last = k + 1
+        np.testing.assert_allclose(outs[k][1], acc * np.float32(0.5), rtol=2e-6)
+    assert float(ccd.outputs['integrated_i'].i.abs().max()) == 0.0 or read_steps[-1] != 5

@@ -439,3 +439,37 @@ def test_pushpull_calibration_reproduces_scao_rec_fixture():
     assert rec.shape == ref.shape == (40, 104)
     np.testing.assert_array_almost_equal(rec, ref, decimal=3)
     assert np.abs(rec - ref).max() < 1e-4                 # observed 2.6e-5 (the file is FP32)
+
+
+# ---------------------------------------------------------------- CCD readout
+
+def test_ccd_oracle_noiseless_binning_and_qe():
+    """apply_binning: block sums times binning^2 (the tot/sum factor is 1); apply_qe scales."""
+    rng = np.random.default_rng(5)
+    img = rng.random((8, 12)).astype(np.float32) * 100
+    out = O.ccd_readout(img, binning=2, quantum_eff=0.3, dtype=np.float64)
+    blocks = img.astype(np.float64).reshape(4, 2, 6, 2).sum(axis=(1, 3))
+    np.testing.assert_allclose(out, blocks * 4 * 0.3, rtol=1e-6)
+    np.testing.assert_array_equal(O.ccd_readout(img), img)
+
+
+def test_ccd_oracle_photon_adu_and_truncated_readout():
+    """Poisson counts quantised to 1/ADU_gain; with photon noise the readout noise is added to an
+    int64 frame, so only its integer part survives; the clamp at -bias."""
+    lam = np.full((200, 200), 25.0, dtype=np.float32)
+    out = O.ccd_readout(lam, photon_noise=True, photon_rng=np.random.default_rng(1))
+    assert np.all(out == np.round(out))                       # integer counts, ADU gain 8
+    assert abs(out.mean() - 25) < 5 * 5 / 200 and abs(out.var() - 25) < 2
+    zero = np.zeros((300, 300), dtype=np.float32)
+    out = O.ccd_readout(zero, photon_noise=True, readout_noise=True, readout_level=1.5,
+                        photon_rng=np.random.default_rng(1), readout_rng=np.random.default_rng(2))
+    assert np.all(out == np.trunc(out))
+    from scipy.stats import norm
+    p0 = norm.cdf(1 / 1.5) - norm.cdf(-1 / 1.5)              # trunc(1.5 Z) == 0
+    assert abs(np.mean(out == 0) - p0) < 5 * np.sqrt(p0 * (1 - p0) / out.size)
+    big = np.full((4, 4), 0.0, dtype=np.float32)
+    out = O.ccd_readout(big, photon_noise=True, readout_noise=True, readout_level=1e4,
+                        photon_rng=np.random.default_rng(3), readout_rng=np.random.default_rng(4))
+    assert out.min() >= -400 / 8                              # clamp of round(x*8)+400 at 0
+    flt = O.ccd_readout(zero, readout_noise=True, readout_level=2.0, readout_rng=np.random.default_rng(2))
+    assert flt.dtype == np.float32 and abs(flt.std() - 2.0) < 0.02 and not np.all(flt == np.trunc(flt))
