@@ -190,6 +190,37 @@ def bench_multi(torch, dev, reps=20):
             'forward_completion_us': per}
 
 
+def bench_ccd(torch, dev, plan, sl_plan, At, ph, ccd, slopes, flux, totals, F, g):
+    """Detector post-step at the ELT frame (1280^2): photon + readout noise (1 e-), QE 0.3, on the
+    SH images scaled to ~1000 photons per subaperture-average pixel; and the noisy loop step
+    SH -> CCD -> ShSlopec (the CoG must see the noisy pixels, so it runs as its own pass)."""
+    from specula_amd.data_objects.simul_params import SimulParams
+    from specula_amd.processing_objects.ccd import CCD, readout_frames
+    det = CCD(SimulParams(time_step=0.001), size=[g.ccd_side, g.ccd_side], dt=0.001, bandw=1.0,
+              photon_noise=True, readout_noise=True, readout_level=1.0, quantum_eff=0.3, target_device_idx=0)
+    p = det.params()
+    plan.forward(At, ph, ccd, 1.0, n_frames=F)
+    factor = float(1000.0 / ccd[0].mean())
+    pix = torch.empty_like(ccd)
+    box = {'n': 0}
+
+    def ro():
+        readout_frames(p, ccd, pix, factor=factor, counter=box['n'])
+        box['n'] += 1
+    t_ro = timed(lambda: [ro() for _ in range(5)], 3, torch.cuda.synchronize) / 5
+    nbytes = 2 * 4 * g.ccd_side ** 2 * F                       # intensity read + pixels written
+
+    def loop():
+        plan.forward(At, ph, ccd, 1.0, n_frames=F)
+        ro()
+        sl_plan.forward(pix, slopes, flux, totals, n_frames=F)
+    t_loop = timed(lambda: [loop() for _ in range(5)], 3, torch.cuda.synchronize) / 5
+    return {'frame': f'{g.ccd_side}x{g.ccd_side}', 'noise': 'photon + readout 1 e-, QE 0.3, ADU 8',
+            'mean_photons_per_px': 300.0, 'readout_frames_per_s': round(F / t_ro, 1),
+            'readout_GBps': round(nbytes / t_ro / 1e9, 1), 'readout_hbm_frac': round(nbytes / t_ro / 1e9 / HBM_PEAK_GBS, 4),
+            'noisy_loop_frames_per_s': round(F / t_loop, 1), 'noisy_loop_ms_per_frame': round(t_loop * 1e3 / F, 5)}
+
+
 def _cpu_model():
     try:
         for line in open('/proc/cpuinfo'):
@@ -325,6 +356,7 @@ def main():
                                         'dtype': 'bf16 hi+lo pieces, f32 accumulation'}
         plan.bind_slopec(sl_plan)
     if args.extras:
+        result['ccd'] = bench_ccd(torch, dev, plan, sl_plan, At, ph, ccd, slopes, flux, totals, F, g)
         result['lgs'] = bench_lgs(torch, dev)
         result['interaction_matrix'] = bench_im(torch, dev)
         result['multi_sensor'] = bench_multi(torch, dev)

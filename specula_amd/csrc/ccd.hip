@@ -19,8 +19,8 @@
 // parity with the reference is distributional.
 //
 // Poisson sampling: multiplication of uniforms (Knuth) for lambda < 10, Hormann's transformed
-// rejection with squeeze (PTRS, Insurance: Mathematics and Economics 12 (1993) 39-45) above,
-// with the acceptance test in FP64.  Gamma: Marsaglia & Tsang (ACM TOMS 26 (2000) 363-372).
+// rejection with squeeze (PTRS, Insurance: Mathematics and Economics 12 (1993) 39-45) above, in
+// FP32 with a cancellation-free log-pmf (the count itself is kept in FP64).  Gamma: Marsaglia & Tsang (ACM TOMS 26 (2000) 363-372).
 
 #include <hip/hip_runtime.h>
 
@@ -53,9 +53,8 @@ __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
     constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
-        const uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
-        c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
+        const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;   // one 32x32->64 multiply each
+        c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
         k0 += W0;
         k1 += W1;
     }
@@ -71,9 +70,11 @@ struct Stream {
     U4 buf{};
     int left = 0;
 
-    __device__ Stream(uint64_t seed, uint64_t px, uint32_t counter)
+    __device__ Stream(uint64_t seed, uint64_t px, uint32_t counter, uint32_t first_draw = 0)
         : k0((uint32_t)seed), k1((uint32_t)(seed >> 32)), px_lo((uint32_t)px), px_hi((uint32_t)(px >> 32)),
-          ctr(counter) {}
+          ctr(counter), draw(first_draw) {}
+    __device__ Stream(uint32_t key0, uint32_t key1, uint64_t px, uint32_t counter, uint32_t first_draw)
+        : k0(key0), k1(key1), px_lo((uint32_t)px), px_hi((uint32_t)(px >> 32)), ctr(counter), draw(first_draw) {}
 
     __device__ __forceinline__ uint32_t next_u32() {
         if (left == 0) {
@@ -114,35 +115,29 @@ __device__ __forceinline__ double log_factorial(double k) {
     return (x - 0.5) * log(x) - x + 0.91893853320467274178 + series;
 }
 
-__device__ double poisson(Stream& s, double lam) {
-    if (!(lam > 0.0)) return 0.0;
-    if (lam < 10.0) {                                     // product of uniforms vs exp(-lam)
-        const double enlam = exp(-lam);
-        double prod = 1.0;
-        int k = 0;
-        for (int it = 0; it < 1024; ++it) {               // P(k > 1023 | lam < 10) = 0 in double
-            prod *= s.uniform_d();
-            if (prod <= enlam) break;
-            ++k;
-        }
-        return (double)k;
-    }
-    const double slam = sqrt(lam), loglam = log(lam);
-    const double b = 0.931 + 2.53 * slam;
-    const double a = -0.059 + 0.02483 * b;
-    const double invalpha = 1.1239 + 1.1328 / (b - 3.4);
-    const double vr = 0.9277 - 3.6224 / (b - 2.0);
-    double k = floor(lam);
-    for (int it = 0; it < 256; ++it) {                    // acceptance > 0.85 per try
-        const double U = s.uniform_d() - 0.5;
-        const double V = s.uniform_d();
-        const double us = 0.5 - fabs(U);
-        k = floor((2.0 * a / us + b) * U + lam + 0.43);
-        if (us >= 0.07 && V <= vr) return k;
-        if (k < 0.0 || (us < 0.013 && V > us)) continue;
-        if (log(V) + log(invalpha) - log(a / (us * us) + b) <= -lam + k * loglam - log_factorial(k)) return k;
-    }
-    return fmax(k, 0.0);
+__device__ __forceinline__ float u01(uint32_t w) { return ((w >> 8) + 1u) * (1.0f / 16777216.0f); }   // (0, 1]
+
+// N(0,1) from two uniform words (Box-Muller; hardware log2 and cos of revolutions).
+__device__ __forceinline__ float normal_from(uint32_t w0, uint32_t w1) {
+    const float r = __builtin_sqrtf(-2.0f * 0.69314718056f * __builtin_amdgcn_logf(u01(w0)));
+    return r * __builtin_amdgcn_cosf(u01(w1));
+}
+
+__device__ __forceinline__ float fast_log(float x) { return 0.69314718056f * __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// log of the Poisson pmf at k >= 32 without cancellation: with k = lam (1 + x) and Stirling's
+// series for log k!,  log p = -lam phi(x) - log(2 pi k)/2 - 1/(12k) + 1/(360k^3),
+// phi(x) = (1 + x) log(1 + x) - x = sum_{n>=2} (-x)^n / (n (n-1))  (series below |x| = 0.1).
+__device__ __forceinline__ float log_pmf_large(double k, float lam, float inv_lam) {
+    const float x = (float)(k - (double)lam) * inv_lam;
+    float phi;
+    if (fabsf(x) < 0.1f)
+        phi = x * x * (0.5f - x * (1.0f / 6 - x * (1.0f / 12 - x * (1.0f / 20 - x * (1.0f / 30 - x * (1.0f / 42 - x * (1.0f / 56)))))));
+    else
+        phi = (1.0f + x) * fast_log(1.0f + x) - x;
+    const float kf = (float)k, r = fast_rcp(kf);
+    return -lam * phi - 0.5f * fast_log(6.283185307f * kf) - r * (1.0f / 12 - r * r * (1.0f / 360));
 }
 
 __device__ double gamma_sample(Stream& s, double shape, double scale) {
@@ -180,16 +175,47 @@ struct ReadoutArgs {
     float* pixels;
     int64_t out_stride;
     int64_t int_stride;       // frame stride of the integrated buffer
+    double inv_adu_gain, inv_emccd_gain;
 };
 
-__global__ void __launch_bounds__(256) ccd_readout_kernel(ReadoutArgs a) {
+// Integrated signal of pixel q of frame f without binning; reads and zeroes the integration
+// buffer (integrated_i.i *= 0, ccd.py:217) and adds the fused last step in_i * factor (FP32, as
+// Intensity.sum).
+__device__ __forceinline__ float raw_pixel(const ReadoutArgs& a, int f, int q) {
+    float acc = 0.0f;
+    if (a.integrated) {
+        float* ip = a.integrated + f * a.int_stride + q;
+        acc = *ip;
+        *ip = 0.0f;
+    }
+    if (a.in_i) acc += a.in_i[f * a.in_stride + q] * (float)a.factor;
+    return acc;
+}
+
+// apply_qe + background/dark (+ CIC in the FULL variants): the expected signal before the photon
+// noise, from the integrated (binned) signal.
+template <bool FULL>
+__device__ __forceinline__ float signal_of(const ReadoutArgs& a, float acc, uint64_t px, uint32_t ctr) {
     const shccd_params& p = a.p;
-    const int f = blockIdx.y;
-    const int64_t n_out = (int64_t)p.out_rows * p.out_cols;
-    const int b = p.binning;
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_out; q += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(q / p.out_cols), c = (int)(q - (int64_t)r * p.out_cols);
-        float acc = 0.0f;
+    if (p.qe != 1.0) acc *= (float)p.qe;
+    if (p.flags & (SHCCD_BACKGROUND | SHCCD_DARK)) acc += (float)(p.background_level + p.darkcurrent_level);
+    if (FULL && (p.flags & SHCCD_CIC)) {
+        Stream s(p.cic_seed, px, ctr);
+        if (p.cic_level > 0.0 && s.uniform_d() <= p.cic_level) acc += 1.0f;
+    }
+    return acc;
+}
+
+// apply_binning + signal_of for pixel q of frame f.
+template <bool FULL>
+__device__ __forceinline__ float load_pixel(const ReadoutArgs& a, int f, int q, uint64_t px, uint32_t ctr) {
+    const shccd_params& p = a.p;
+    float acc = 0.0f;
+    if (!FULL || p.binning == 1) {
+        acc = raw_pixel(a, f, q);
+    } else {
+        const int b = p.binning;
+        const int r = q / p.out_cols, c = q - r * p.out_cols;
         for (int i = 0; i < b; ++i) {
             const int64_t row = (int64_t)(r * b + i) * p.in_cols + (int64_t)c * b;
             for (int j = 0; j < b; ++j) {
@@ -197,49 +223,206 @@ __global__ void __launch_bounds__(256) ccd_readout_kernel(ReadoutArgs a) {
                 if (a.integrated) {
                     float* ip = a.integrated + f * a.int_stride + row + j;
                     v = *ip;
-                    *ip = 0.0f;                                   // integrated_i.i *= 0 (ccd.py:217)
+                    *ip = 0.0f;
                 }
-                if (a.in_i) v += a.in_i[f * a.in_stride + row + j] * (float)a.factor;   // FP32 as Intensity.sum
+                if (a.in_i) v += a.in_i[f * a.in_stride + row + j] * (float)a.factor;
                 acc += v;
             }
         }
-        float x = acc * (float)(b * b);
-        if (p.qe != 1.0) x *= (float)p.qe;
-        const uint64_t px = (uint64_t)f * (uint64_t)n_out + (uint64_t)q;
-        const uint32_t ctr = (uint32_t)a.counter;
-        if (p.flags & (SHCCD_BACKGROUND | SHCCD_DARK)) x += (float)(p.background_level + p.darkcurrent_level);
-        double xd = x;
-        if (p.flags & SHCCD_CIC) {
-            Stream s(p.cic_seed, px, ctr);
-            xd += (s.uniform_d() <= p.cic_level && p.cic_level > 0.0) ? 1.0 : 0.0;
+        acc *= (float)(b * b);
+    }
+    return signal_of<FULL>(a, acc, px, ctr);
+}
+
+// Generator shared by photon and readout noise, keyed by both seeds.  Call 0 of a pixel gives the
+// first Poisson try (words x, y) and the readout normal (words z, w); further tries use calls 1..
+__device__ __forceinline__ uint32_t key0_of(const shccd_params& p) {
+    return (uint32_t)p.photon_seed ^ (uint32_t)(p.readout_seed >> 32);
+}
+__device__ __forceinline__ uint32_t key1_of(const shccd_params& p) { return (uint32_t)p.readout_seed ^ 0x243F6A88u; }
+
+// Everything after the photon noise: EM gain, readout noise (nrm ~ N(0,1)), pixel gains, ADU.
+template <bool FULL>
+__device__ __forceinline__ float finish_value(const ReadoutArgs& a, int q, uint64_t px, uint32_t ctr, double xd,
+                                             bool integer_frame, float nrm) {
+    const shccd_params& p = a.p;
+    if (FULL && (p.flags & SHCCD_EXCESS)) {
+        Stream s(p.excess_seed, px, ctr);
+        xd = gamma_sample(s, p.excess_delta * xd, p.emccd_gain) / p.excess_delta;
+        integer_frame = false;
+    }
+    if (p.flags & SHCCD_READOUT) {
+        const float ron = nrm * (float)p.readout_level;
+        xd += integer_frame ? trunc((double)ron) : (double)ron;
+    }
+    if (FULL && a.gains) xd *= (double)a.gains[q];
+    if (p.flags & SHCCD_PHOTON) {
+        xd = rint(xd * p.adu_gain) + p.adu_bias;                   // round half to even (numpy round)
+        if (xd < 0.0) xd = 0.0;
+        if (!(p.flags & SHCCD_KEEP_BIAS)) xd -= p.adu_bias;
+        xd *= a.inv_adu_gain;           // /ADU_gain: exact for power-of-two gains, else within 1 ulp
+        if (FULL && (p.flags & SHCCD_EXCESS)) xd *= a.inv_emccd_gain;
+        if ((p.flags & SHCCD_DARK) && !(p.flags & SHCCD_KEEP_DARK)) xd -= p.darkcurrent_level;
+        if ((p.flags & SHCCD_BG_REMOVE_AVG) && !(p.flags & SHCCD_KEEP_DARK)) xd -= p.background_level;
+    }
+    return (float)xd;
+}
+
+// Readout without photon noise: groups of 4 pixels per lane (float4 traffic when the frame
+// allows it); one generator call per group gives 4 readout normals (two Box-Muller pairs).
+template <bool FULL>
+__global__ void __launch_bounds__(256) ccd_readout_kernel(ReadoutArgs a, int vec) {
+    const int f = blockIdx.y;
+    const int n_out = a.p.out_rows * a.p.out_cols;
+    const int n_grp = (n_out + 3) >> 2;
+    const uint32_t ctr = (uint32_t)a.counter;
+    const uint32_t k0 = key0_of(a.p), k1 = key1_of(a.p);
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_grp; g += gridDim.x * blockDim.x) {
+        const int q0 = g << 2;
+        float x[4];
+        if (vec) {
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (a.integrated) {
+                float4* ip = reinterpret_cast<float4*>(a.integrated + f * a.int_stride + q0);
+                v = *ip;
+                *ip = make_float4(0.f, 0.f, 0.f, 0.f);             // integrated_i.i *= 0 (ccd.py:217)
+            }
+            if (a.in_i) {
+                const float4 w = *reinterpret_cast<const float4*>(a.in_i + f * a.in_stride + q0);
+                const float fc = (float)a.factor;
+                v.x += w.x * fc; v.y += w.y * fc; v.z += w.z * fc; v.w += w.w * fc;
+            }
+            x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+            const float qe = (float)a.p.qe;
+            const float add = (a.p.flags & (SHCCD_BACKGROUND | SHCCD_DARK)) ? (float)(a.p.background_level + a.p.darkcurrent_level) : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (a.p.qe != 1.0) x[j] *= qe;
+                x[j] += add;
+                if (FULL && (a.p.flags & SHCCD_CIC)) {
+                    Stream s(a.p.cic_seed, (uint64_t)f * (uint64_t)n_out + (uint64_t)(q0 + j), ctr);
+                    if (a.p.cic_level > 0.0 && s.uniform_d() <= a.p.cic_level) x[j] += 1.0f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                x[j] = q0 + j < n_out ? load_pixel<FULL>(a, f, q0 + j, (uint64_t)f * (uint64_t)n_out + (uint64_t)(q0 + j), ctr)
+                                      : 0.0f;
         }
-        bool integer_frame = false;
-        if (p.flags & SHCCD_PHOTON) {
-            Stream s(p.photon_seed, px, ctr);
-            xd = poisson(s, xd);
-            integer_frame = true;
+        float nrm[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.p.flags & SHCCD_READOUT) {
+            const uint64_t gid = (uint64_t)f * (uint64_t)n_grp + (uint64_t)g;
+            const U4 w = philox(U4{(uint32_t)gid, (uint32_t)(gid >> 32), ctr, 0x80000000u}, k0, k1);
+            const float r0 = __builtin_sqrtf(-2.0f * fast_log(u01(w.x))), t0 = u01(w.y);
+            const float r1 = __builtin_sqrtf(-2.0f * fast_log(u01(w.z))), t1 = u01(w.w);
+            nrm[0] = r0 * __builtin_amdgcn_cosf(t0);
+            nrm[1] = r0 * __builtin_amdgcn_sinf(t0);
+            nrm[2] = r1 * __builtin_amdgcn_cosf(t1);
+            nrm[3] = r1 * __builtin_amdgcn_sinf(t1);
         }
-        if (p.flags & SHCCD_EXCESS) {
-            Stream s(p.excess_seed, px, ctr);
-            xd = gamma_sample(s, p.excess_delta * xd, p.emccd_gain) / p.excess_delta;
-            integer_frame = false;
+        float y[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            y[j] = finish_value<FULL>(a, q0 + j < n_out ? q0 + j : q0, (uint64_t)f * (uint64_t)n_out + (uint64_t)(q0 + j), ctr,
+                                (double)x[j], false, nrm[j]);
+        if (vec) {
+            *reinterpret_cast<float4*>(a.pixels + f * a.out_stride + q0) = make_float4(y[0], y[1], y[2], y[3]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (q0 + j < n_out) a.pixels[f * a.out_stride + q0 + j] = y[j];
         }
-        if (p.flags & SHCCD_READOUT) {
-            Stream s(p.readout_seed, px, ctr);
-            const double ron = (double)s.normal() * p.readout_level;
-            xd += integer_frame ? trunc(ron) : (double)(float)ron;
+    }
+}
+
+// Readout with photon noise.  Rejection sampling makes the number of tries per pixel random; so
+// that a wave does not wait for its slowest lane on every pixel, each lane walks its own list of
+// pixels (q, q + stride, ...) and one loop iteration is one try of whatever pixel the lane is
+// on: a lane that accepts moves to its next pixel while the others keep trying.  Every
+// iteration makes exactly one generator call (counter = try index) so that the call is not a
+// divergent branch; try 0 also supplies the readout normal (words z, w).
+template <bool FULL>
+__global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
+    const shccd_params& p = a.p;
+    const int f = blockIdx.y;
+    const int n_out = p.out_rows * p.out_cols;
+    const int stride = gridDim.x * blockDim.x;
+    const uint32_t ctr = (uint32_t)a.counter;
+    const uint32_t k0 = key0_of(p), k1 = key1_of(p);
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    bool fresh = true;
+    uint64_t px = 0;
+    uint32_t wz = 0, ww = 0;
+    float lam = 0.0f, inv_lam = 0.0f, enlam = 0.0f, prod = 1.0f, a_ = 0.0f, b_ = 0.0f, vr = 0.0f;
+    float log_invalpha = 0.0f, log_lam = 0.0f;
+    double k = 0.0;
+    uint32_t tries = 0;
+    float next_raw = 0.0f;                     // signal of the lane's next pixel, loaded one pixel ahead
+    if (!FULL && q < n_out) next_raw = raw_pixel(a, f, q);
+    while (q < n_out) {
+        if (fresh) {
+            px = (uint64_t)f * (uint64_t)n_out + (uint64_t)q;
+            if (FULL) {
+                lam = load_pixel<true>(a, f, q, px, ctr);
+            } else {
+                lam = signal_of<false>(a, next_raw, px, ctr);
+                if (q + stride < n_out) next_raw = raw_pixel(a, f, q + stride);
+            }
+            tries = 0;
+            k = 0.0;
+            prod = 1.0f;
+            const float slam = __builtin_amdgcn_sqrtf(lam);
+            b_ = 0.931f + 2.53f * slam;
+            a_ = -0.059f + 0.02483f * b_;
+            log_invalpha = fast_log(1.1239f + 1.1328f * fast_rcp(b_ - 3.4f));
+            vr = 0.9277f - 3.6224f * fast_rcp(b_ - 2.0f);
+            log_lam = fast_log(lam);
+            inv_lam = fast_rcp(lam);
+            enlam = __builtin_amdgcn_exp2f(-1.44269504089f * lam);
+            fresh = false;
         }
-        if (a.gains) xd *= (double)a.gains[q];
-        if (p.flags & SHCCD_PHOTON) {
-            xd = rint(xd * p.adu_gain) + p.adu_bias;          // round half to even (numpy round)
-            if (xd < 0.0) xd = 0.0;
-            if (!(p.flags & SHCCD_KEEP_BIAS)) xd -= p.adu_bias;
-            xd /= p.adu_gain;
-            if (p.flags & SHCCD_EXCESS) xd /= p.emccd_gain;
-            if ((p.flags & SHCCD_DARK) && !(p.flags & SHCCD_KEEP_DARK)) xd -= p.darkcurrent_level;
-            if ((p.flags & SHCCD_BG_REMOVE_AVG) && !(p.flags & SHCCD_KEEP_DARK)) xd -= p.background_level;
+        const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, tries}, k0, k1);
+        if (tries == 0) {
+            wz = w.z;
+            ww = w.w;
         }
-        a.pixels[f * a.out_stride + q] = (float)xd;
+        const float u1 = u01(w.x), u2 = u01(w.y);
+        ++tries;
+        bool done;
+        if (!(lam > 0.0f)) {
+            k = 0.0;
+            done = true;
+        } else if (lam < 10.0f) {                                  // product of uniforms vs exp(-lam)
+            prod *= u1;
+            done = prod <= enlam;
+            if (!done) {
+                k += 1.0;
+                prod *= u2;
+                done = prod <= enlam;
+                if (!done) k += 1.0;
+            }
+        } else {                                                   // PTRS
+            const float U = u1 - 0.5f, V = u2;
+            const float us = 0.5f - fabsf(U);
+            k = floor((double)((2.0f * a_ * fast_rcp(us) + b_) * U) + (double)lam + 0.43);
+            done = us >= 0.07f && V <= vr;
+            if (!done && !(k < 0.0 || (us < 0.013f && V > us))) {
+                const float lhs = fast_log(V) + log_invalpha - fast_log(a_ * fast_rcp(us * us) + b_);
+                const float rhs = k < 32.0 ? -lam + (float)k * log_lam - (float)kLogFact[(int)k]
+                                           : log_pmf_large(k, lam, inv_lam);
+                done = lhs <= rhs;
+            }
+        }
+        if (!done && tries >= 512u) {                              // never reached in practice
+            k = fmax(k, 0.0);
+            done = true;
+        }
+        if (done) {
+            a.pixels[f * a.out_stride + q] = finish_value<FULL>(a, q, px, ctr, k, true, normal_from(wz, ww));
+            q += stride;
+            fresh = true;
+        }
     }
 }
 
@@ -251,7 +434,10 @@ __global__ void __launch_bounds__(256) ccd_integrate_kernel(const float* __restr
         integrated[f * int_stride + q] += in_i[f * in_stride + q] * (float)factor;
 }
 
-int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
+// blocks for n pixels with at least `per_lane` pixels per lane (capped at 4096 blocks)
+int grid_for(int64_t n, int per_lane) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>((n + 256LL * per_lane - 1) / (256LL * per_lane), 4096));
+}
 
 }  // namespace
 
@@ -263,7 +449,7 @@ int shccd_integrate(int n_frames, const float* in_i, int64_t in_stride, double f
         return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_integrate: bad argument");
     if (n_frames > 1 && (in_stride < n_px || int_stride < n_px))
         return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_integrate: frame stride too small");
-    hipLaunchKernelGGL(ccd_integrate_kernel, dim3(grid_for(n_px), n_frames), dim3(256), 0,
+    hipLaunchKernelGGL(ccd_integrate_kernel, dim3(grid_for(n_px, 1), n_frames), dim3(256), 0,
                        reinterpret_cast<hipStream_t>(stream), in_i, in_stride, factor, integrated, int_stride, n_px);
     CCD_HIP_CHECK(hipGetLastError());
     return SHWFS_OK;
@@ -295,8 +481,30 @@ int shccd_readout(const shccd_params* p, int n_frames, float* integrated, int64_
     a.counter = counter;
     a.pixels = pixels;
     a.out_stride = out_stride;
-    hipLaunchKernelGGL(ccd_readout_kernel, dim3(grid_for(n_out), n_frames), dim3(256), 0,
-                       reinterpret_cast<hipStream_t>(stream), a);
+    a.inv_adu_gain = (p->flags & SHCCD_PHOTON) ? 1.0 / p->adu_gain : 1.0;
+    a.inv_emccd_gain = (p->flags & SHCCD_EXCESS) ? 1.0 / p->emccd_gain : 1.0;
+    if (n_out > (int64_t)INT32_MAX) return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout: frame too large");
+    // FULL variants carry binning, CIC, EM gain and pixel gains; the lean ones the common rest
+    const bool full = p->binning != 1 || (p->flags & (SHCCD_CIC | SHCCD_EXCESS)) || pixel_gains != nullptr;
+    if (p->flags & SHCCD_PHOTON) {
+        if (full)
+            hipLaunchKernelGGL(ccd_photon_kernel<true>, dim3(grid_for(n_out, 32), n_frames), dim3(256), 0,
+                               reinterpret_cast<hipStream_t>(stream), a);
+        else
+            hipLaunchKernelGGL(ccd_photon_kernel<false>, dim3(grid_for(n_out, 32), n_frames), dim3(256), 0,
+                               reinterpret_cast<hipStream_t>(stream), a);
+    } else {
+        auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+        const int vec = p->binning == 1 && n_out % 4 == 0 && al(pixels) && (!integrated || al(integrated)) &&
+                        (!in_i || al(in_i)) &&
+                        (n_frames == 1 || (out_stride % 4 == 0 && int_stride % 4 == 0 && in_stride % 4 == 0));
+        if (full)
+            hipLaunchKernelGGL(ccd_readout_kernel<true>, dim3(grid_for((n_out + 3) / 4, 4), n_frames), dim3(256), 0,
+                               reinterpret_cast<hipStream_t>(stream), a, vec);
+        else
+            hipLaunchKernelGGL(ccd_readout_kernel<false>, dim3(grid_for((n_out + 3) / 4, 4), n_frames), dim3(256), 0,
+                               reinterpret_cast<hipStream_t>(stream), a, vec);
+    }
     CCD_HIP_CHECK(hipGetLastError());
     return SHWFS_OK;
 }
