@@ -177,10 +177,13 @@ int shwfs_vjp(shwfs_plan* plan, int n_frames, const float* A, int64_t a_frame_st
  *   im[m][n_valid + slot] = scale * sum_{p in subaperture} jy[p] * modes[m][p]
  * jx, jy: ef x ef maps (d slope / d phase_nm); modes: n_modes maps of ef x ef (stride
  * mode_stride); slot: n_lenses^2 int32, slope index of each subaperture or -1; scale = 2
- * reproduces the push-pull convention IM = (s(+a) - s(-a)) / a. */
+ * reproduces the push-pull convention IM = (s(+a) - s(-a)) / a.
+ * work: optional device workspace of n_modes * 2 * n_lenses^2 floats followed by n_valid int32
+ * (16-byte aligned); with it the results are written in grid order and reordered into slot
+ * order by a second pass (coalesced stores); NULL writes slot order directly. */
 int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy,
                            int n_modes, const float* modes, int64_t mode_stride, const int32_t* slot,
-                           int n_valid, float scale, float* im, void* stream);
+                           int n_valid, float scale, float* im, void* work, void* stream);
 
 /* ---- multi-sensor batch -----------------------------------------------------------------
  * No reference counterpart (the reference triggers one SH object at a time, each optionally a

@@ -82,7 +82,7 @@ def _declare(lib):
         'shwfs_forward_slopes': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
         'shslopec_vjp': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P]),
         'shwfs_vjp': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P, _I64, _P]),
-        'shwfs_im_from_jacobian': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P]),
+        'shwfs_im_from_jacobian': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P, _P]),
         'shwfs_batch_create': (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_P)]),
         'shwfs_batch_forward': (_I, [_P, _P]),
         'shwfs_batch_latencies': (_I, [_P, _P, _P]),

@@ -1284,28 +1284,37 @@ struct VjpArgs {
 // phase): with G_B = c dL/dI on the subaperture tile,
 //   G_S = P^T G_B Q,   X = G_S conj(F),   H = FFT_M(X) on the np x np support,
 //   dL/dphi = -2 (2 pi / lambda) Im(H E).
-template <int NP, int M, int TS>
+// NPX > 0: CCD pixels per subaperture side fixed at compile time (loops unrolled); 0: g.npx.
+// The binning operators P, Q are staged in LDS after the G_B tiles.
+template <int NP, int M, int TS, int NPX>
 __global__ void __launch_bounds__(TS * NP)
 sh_ngs_vjp_kernel(VjpArgs g) {
     extern __shared__ float4 lds_raw[];
     constexpr int G1S = M * NP + 1;
     constexpr int NT = TS * NP;
     constexpr int CPT = (TS * M + NT - 1) / NT;
+    const int npx = NPX > 0 ? NPX : g.npx;
     float2* buf = reinterpret_cast<float2*>(lds_raw);
     float* gb = reinterpret_cast<float*>(buf + TS * G1S);
-    const int npx2 = g.npx * g.npx;
+    const int npx2 = npx * npx;
+    float* Pl = gb + TS * npx2;                      // P: npx x M
+    float* Ql = Pl + npx * M;                        // Q: npx x M
+    for (int e = threadIdx.x; e < npx * M; e += NT) {
+        Pl[e] = g.roww[e];
+        Ql[e] = g.colw[e];
+    }
     const int t = threadIdx.x, f = blockIdx.y, tile = blockIdx.x, n = g.n_lenses;
     const double* am = g.amp + (int64_t)f * g.amp_fstep;
     const float c = (float)(g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]));
 
     for (int e = t; e < TS * npx2; e += NT) {         // G_B tiles
         const int s = e / npx2, rem = e - s * npx2;
-        const int Y = rem / g.npx, X = rem - Y * g.npx;
+        const int Y = rem / npx, X = rem - Y * npx;
         const int sub = tile * TS + s;
         float v = 0.f;
         if (sub < g.n_sub) {
             const int i = sub / n, j = sub - (sub / n) * n;
-            v = g.gccd[f * g.gccd_stride + (int64_t)(i * g.npx + Y) * (n * g.npx) + j * g.npx + X] * c;
+            v = g.gccd[f * g.gccd_stride + (int64_t)(i * npx + Y) * (n * npx) + j * npx + X] * c;
         }
         gb[s * npx2 + rem] = v;
     }
@@ -1352,11 +1361,25 @@ sh_ngs_vjp_kernel(VjpArgs g) {
 #pragma unroll
             for (int ky = 0; ky < M; ++ky) gs[ky] = 0.f;
             const float* gbs = gb + s * npx2;
-            for (int Y = 0; Y < g.npx; ++Y) {
-                float u = 0.f;
-                for (int X = 0; X < g.npx; ++X) u = fmaf(gbs[Y * g.npx + X], g.colw[X * M + kx], u);
+            if (NPX > 0) {
+                float qc[NPX > 0 ? NPX : 1];
 #pragma unroll
-                for (int ky = 0; ky < M; ++ky) gs[ky] = fmaf(g.roww[Y * M + ky], u, gs[ky]);
+                for (int X = 0; X < NPX; ++X) qc[X] = Ql[X * M + kx];
+#pragma unroll
+                for (int Y = 0; Y < NPX; ++Y) {
+                    float u = 0.f;
+#pragma unroll
+                    for (int X = 0; X < NPX; ++X) u = fmaf(gbs[Y * NPX + X], qc[X], u);
+#pragma unroll
+                    for (int ky = 0; ky < M; ++ky) gs[ky] = fmaf(Pl[Y * M + ky], u, gs[ky]);
+                }
+            } else {
+                for (int Y = 0; Y < npx; ++Y) {
+                    float u = 0.f;
+                    for (int X = 0; X < npx; ++X) u = fmaf(gbs[Y * npx + X], Ql[X * M + kx], u);
+#pragma unroll
+                    for (int ky = 0; ky < M; ++ky) gs[ky] = fmaf(Pl[Y * M + ky], u, gs[ky]);
+                }
             }
             float2 X[M];
 #pragma unroll
@@ -1448,6 +1471,102 @@ __global__ void im_from_jacobian_kernel(const float* __restrict__ jx, const floa
                 im[(int64_t)(mb + u) * 2 * n_valid + n_valid + sl] = scale * py;
             }
         }
+    }
+}
+
+
+// Same contraction with float4 traffic: each lane holds a 4 x 4 pixel block (4 columns of 4
+// rows) of a subaperture, so a subaperture spans (NP/4)^2 consecutive lanes; MU modes in flight.
+// GRID: results go to grid[m][2][n_lenses^2] in subaperture-grid order (consecutive lanes write
+// consecutive words) instead of to the slot-ordered im rows; im_gather_kernel reorders them.
+template <int NP, bool GRID>
+__global__ void __launch_bounds__(1024)
+im_from_jacobian_v4_kernel(const float* __restrict__ jx, const float* __restrict__ jy, int ef, int n_lenses,
+                           const float* __restrict__ modes, int64_t mode_stride, int n_modes, int modes_per_block,
+                           const int32_t* __restrict__ slot, int n_valid, float scale, float* __restrict__ im) {
+    constexpr int MU = 2;
+    constexpr int RS = NP / 4;                     // row blocks per subaperture
+    constexpr int LPS = RS * (NP / 4);             // lanes per subaperture
+    const int i = blockIdx.x;
+    const int t = threadIdx.x;
+    const int rs = t % RS, qd = t / RS;
+    const int col = qd * 4;
+    const int sl = (col < ef) ? slot[i * n_lenses + col / NP] : -1;
+    const bool valid = sl >= 0;
+    const int64_t row0 = (int64_t)(i * NP + rs * 4) * ef + col;
+    float4 jxr[4], jyr[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int64_t o = row0 + (int64_t)r * ef;
+        jxr[r] = valid ? *reinterpret_cast<const float4*>(jx + o) : make_float4(0.f, 0.f, 0.f, 0.f);
+        jyr[r] = valid ? *reinterpret_cast<const float4*>(jy + o) : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    const int m0 = blockIdx.y * modes_per_block;
+    const int m1 = min(n_modes, m0 + modes_per_block);
+    const float* base = modes + row0;
+    for (int mb = m0; mb < m1; mb += MU) {
+        float4 v[MU][4];
+#pragma unroll
+        for (int u = 0; u < MU; ++u) {
+            const bool ld = valid && mb + u < m1;
+            const float* md = base + (int64_t)(mb + u) * mode_stride;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                if (ld) {
+                    const floatx4 w = __builtin_nontemporal_load(reinterpret_cast<const floatx4*>(md + (int64_t)r * ef));
+                    v[u][r] = make_float4(w.x, w.y, w.z, w.w);
+                } else {
+                    v[u][r] = make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MU; ++u) {
+            float px = 0.f, py = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                px = fmaf(jxr[r].x, v[u][r].x, fmaf(jxr[r].y, v[u][r].y, fmaf(jxr[r].z, v[u][r].z, fmaf(jxr[r].w, v[u][r].w, px))));
+                py = fmaf(jyr[r].x, v[u][r].x, fmaf(jyr[r].y, v[u][r].y, fmaf(jyr[r].z, v[u][r].z, fmaf(jyr[r].w, v[u][r].w, py))));
+            }
+#pragma unroll
+            for (int off = LPS / 2; off > 0; off >>= 1) {
+                px += __shfl_xor(px, off, kWave);
+                py += __shfl_xor(py, off, kWave);
+            }
+            if (valid && (t % LPS) == 0 && mb + u < m1) {
+                if (GRID) {
+                    const int64_t nn = (int64_t)n_lenses * n_lenses;
+                    const int64_t gidx = (int64_t)i * n_lenses + col / NP;
+                    im[(int64_t)(mb + u) * 2 * nn + gidx] = scale * px;
+                    im[(int64_t)(mb + u) * 2 * nn + nn + gidx] = scale * py;
+                } else {
+                    im[(int64_t)(mb + u) * 2 * n_valid + sl] = scale * px;
+                    im[(int64_t)(mb + u) * 2 * n_valid + n_valid + sl] = scale * py;
+                }
+            }
+        }
+    }
+}
+
+// inv[slot[g]] = g for every valid subaperture g of the grid.
+__global__ void slot_inverse_kernel(const int32_t* __restrict__ slot, int nn, int32_t* __restrict__ inv) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < nn) {
+        const int s = slot[g];
+        if (s >= 0) inv[s] = g;
+    }
+}
+
+// im[m][s] = grid[m][0][inv[s]], im[m][n_valid + s] = grid[m][1][inv[s]]: coalesced writes.
+__global__ void im_gather_kernel(const float* __restrict__ grid, const int32_t* __restrict__ inv, int nn, int n_valid,
+                                 float* __restrict__ im) {
+    const int m = blockIdx.y;
+    const float* gx = grid + (int64_t)m * 2 * nn;
+    float* row = im + (int64_t)m * 2 * n_valid;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_valid; s += gridDim.x * blockDim.x) {
+        const int g = inv[s];
+        row[s] = gx[g];
+        row[n_valid + s] = gx[nn + g];
     }
 }
 
@@ -2236,20 +2355,26 @@ int shslopec_forward(shslopec_plan* p, int n_frames, const float* pixels, int64_
 }  // extern "C"
 
 namespace {
-template <int NP, int M, int TS>
+template <int NP, int M, int TS, int NPX>
 int launch_vjp(const VjpArgs& a, int n_tiles, int n_frames, int npx, hipStream_t st) {
-    const int lds = TS * (M * NP + 1) * (int)sizeof(float2) + TS * npx * npx * (int)sizeof(float);
+    if (NPX > 0 && npx != NPX) return set_error(SHWFS_ERR_ARG, "VJP instantiation does not match subap_npx");
+    const int lds = TS * (M * NP + 1) * (int)sizeof(float2) + (TS * npx * npx + 2 * npx * M) * (int)sizeof(float);
     if (lds > 150 * 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP tile exceeds the LDS budget");
-    hipLaunchKernelGGL((sh_ngs_vjp_kernel<NP, M, TS>), dim3(n_tiles, n_frames), dim3(TS * NP), lds, st, a);
+    hipLaunchKernelGGL((sh_ngs_vjp_kernel<NP, M, TS, NPX>), dim3(n_tiles, n_frames), dim3(TS * NP), lds, st, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
 using VjpLauncher = int (*)(const VjpArgs&, int, int, int, hipStream_t);
-VjpLauncher pick_vjp(int np, int M, int* ts) {
-    if (np == 8 && M == 16) { *ts = 32; return launch_vjp<8, 16, 32>; }
-    if (np == 6 && M == 16) { *ts = 42; return launch_vjp<6, 16, 42>; }
-    if (np == 4 && M == 8) { *ts = 64; return launch_vjp<4, 8, 64>; }
-    if (np == 16 && M == 32) { *ts = 16; return launch_vjp<16, 32, 16>; }
+VjpLauncher pick_vjp(int np, int M, int npx, int* ts) {
+    if (np == 8 && M == 16) {
+        *ts = 16;
+        if (npx == 16) return launch_vjp<8, 16, 16, 16>;
+        if (npx == 8) return launch_vjp<8, 16, 16, 8>;
+        return launch_vjp<8, 16, 16, 0>;
+    }
+    if (np == 6 && M == 16) { *ts = 42; return launch_vjp<6, 16, 42, 0>; }
+    if (np == 4 && M == 8) { *ts = 64; return launch_vjp<4, 8, 64, 0>; }
+    if (np == 16 && M == 32) { *ts = 16; return launch_vjp<16, 32, 16, 0>; }
     return nullptr;
 }
 }  // namespace
@@ -2272,7 +2397,7 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
                          gccd_stride < ccd_px || gph_stride < ef_px))
         return set_error(SHWFS_ERR_ARG, "frame stride too small");
     int ts = 0;
-    VjpLauncher L = pick_vjp(g.np_sub, g.spec_size, &ts);
+    VjpLauncher L = pick_vjp(g.np_sub, g.spec_size, g.subap_npx, &ts);
     if (!L) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP instantiated for (np_sub, spec_size) in {(4,8),(6,16),(8,16),(16,32)}");
     DeviceGuard dg(p->device);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
@@ -2329,16 +2454,45 @@ int shslopec_vjp(shslopec_plan* p, int n_frames, const float* pixels, int64_t pi
 
 int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
                            const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float scale,
-                           float* im, void* stream) {
+                           float* im, void* work, void* stream) {
     if (!jx || !jy || !modes || !slot || !im) return set_error(SHWFS_ERR_ARG, "null argument");
     if (n_lenses <= 0 || np_sub <= 0 || n_modes <= 0 || n_valid <= 0 || ef_size != n_lenses * np_sub)
         return set_error(SHWFS_ERR_ARG, "inconsistent sizes");
     if (ef_size > 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "pupil wider than 1024 px");
     if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int threads = (ef_size + 63) / 64 * 64;
     const int mpb = 16;
     dim3 grid(n_lenses, (n_modes + mpb - 1) / mpb);
+    const bool v4 = (np_sub == 8 || np_sub == 16) && ef_size % 4 == 0 && mode_stride % 4 == 0 &&
+                    ((reinterpret_cast<uintptr_t>(modes) | reinterpret_cast<uintptr_t>(jx) |
+                      reinterpret_cast<uintptr_t>(jy)) & 15u) == 0;
+    if (v4) {
+        const int threads4 = (ef_size / 4 * (np_sub / 4) + 63) / 64 * 64;
+        const int nn = n_lenses * n_lenses;
+        float* gridbuf = static_cast<float*>(work);
+        int32_t* inv = work ? reinterpret_cast<int32_t*>(gridbuf + (int64_t)n_modes * 2 * nn) : nullptr;
+        if (work) {
+            hipLaunchKernelGGL(slot_inverse_kernel, dim3((nn + 255) / 256), dim3(256), 0, st, slot, nn, inv);
+            SHWFS_LAUNCH_CHECK();
+        }
+#define IMJ4(NPV, G) hipLaunchKernelGGL((im_from_jacobian_v4_kernel<NPV, G>), grid, dim3(threads4), 0, st, jx, jy,    \
+                                        ef_size, n_lenses, modes, mode_stride, n_modes, mpb, slot, n_valid, scale, \
+                                        G ? gridbuf : im)
+        if (np_sub == 8) {
+            if (work) IMJ4(8, true); else IMJ4(8, false);
+        } else {
+            if (work) IMJ4(16, true); else IMJ4(16, false);
+        }
+#undef IMJ4
+        SHWFS_LAUNCH_CHECK();
+        if (work) {
+            hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, n_modes), dim3(256), 0, st, gridbuf, inv,
+                               nn, n_valid, im);
+            SHWFS_LAUNCH_CHECK();
+        }
+        return SHWFS_OK;
+    }
+    const int threads = (ef_size + 63) / 64 * 64;
 #define IMJ(NPV) hipLaunchKernelGGL(im_from_jacobian_kernel<NPV>, grid, dim3(threads), 0, st, jx, jy, ef_size, \
                                     n_lenses, modes, mode_stride, n_modes, mpb, slot, n_valid, scale, im)
     switch (np_sub) {

@@ -49,6 +49,9 @@ class SHSensor:
         self.totals = torch.empty((max_frames, 2), dtype=torch.float64, device=self.device)
         self._gccd = None
         self._slot = None
+        self._im_work = None
+        # the sensor's pupil is fixed: its statistics once (S0 = 1, the forward default)
+        self.plan.set_static_pupil(self.A, 1.0)
         # CoG fused into the SH kernel epilogue when the geometry allows it (same results)
         self.fused = self.plan.bind_slopec(self.slopec)
 
@@ -120,10 +123,14 @@ class SHSensor:
         ef = self.geom.ef_size
         im = torch.empty((K, 2 * self.n_sub), dtype=torch.float32, device=self.device)
         modes = modes.contiguous()
+        # workspace of the grid-order pass: K * 2 * n^2 floats + n_sub int32 (kept between calls)
+        need = K * 2 * self.geom.n_lenses ** 2 + self.n_sub
+        if self._im_work is None or self._im_work.numel() < need:
+            self._im_work = torch.empty(need, dtype=torch.float32, device=self.device)
         _native.check(_native.lib().shwfs_im_from_jacobian(
             self.geom.n_lenses, self.geom.ovs_np_sub, ef, _native.ptr(jx), _native.ptr(jy), K,
             _native.ptr(modes), ef * ef, _native.ptr(self.slot_map()), self.n_sub, float(scale),
-            _native.ptr(im), _native.stream_handle(stream)), 'shwfs_im_from_jacobian')
+            _native.ptr(im), _native.ptr(self._im_work), _native.stream_handle(stream)), 'shwfs_im_from_jacobian')
         return im
 
     def pushpull_interaction_matrix(self, phase, modes, amp, chunk=None, stream=None):
