@@ -151,3 +151,56 @@ def test_resampled_plan_rejects_vjp():
     rc = _native.lib().shwfs_vjp(plan._h, 1, _native.ptr(At), 0, _native.ptr(ph), 64 * 64, 1.0, _native.ptr(gccd),
                                  64 * 64, _native.ptr(gph), 64 * 64, None)
     assert rc == _native.SHWFS_ERR_UNSUPPORTED
+
+
+def _sh_object(**kw):
+    from specula_amd.processing_objects.sh import SH
+    return SH(target_device_idx=0, **kw)
+
+
+def test_reference_sh_flux_at_default_fov_ovs():
+    """test_sh.py:18-42 verbatim (default fov_ovs_coeff -> fov_ovs 2, N 36, cut 10): sum of the
+    intensity equals S0 * masked area (FP32 image, FP64 normalisation: rtol 1e-6 instead of the
+    FP64 test's decimal 7)."""
+    from specula_amd.data_objects.electric_field import ElectricField
+    sh = _sh_object(wavelengthInNm=500, subap_wanted_fov=3, sensor_pxscale=0.5, subap_on_diameter=20, subap_npx=6)
+    ef = ElectricField(120, 120, 0.05, S0=100, target_device_idx=0)
+    ef.generation_time = 1
+    sh.inputs['in_ef'].set(ef)
+    sh.setup()
+    g = sh.geometry
+    assert (g.fov_ovs, g.fft_size, g.cutpixels) == (2, 36, 10)
+    sh.check_ready(1); sh.trigger(); sh.post_trigger()
+    tot = float(sh.outputs['out_i'].i.double().sum())
+    print(f'test_sh flux: {tot:.9g} vs {100 * ef.masked_area():.9g}', flush=True)
+    np.testing.assert_allclose(tot, 100 * ef.masked_area(), rtol=1e-6)
+    go = O.sh_geometry(ef_size=120, pixel_pitch=0.05, wavelengthInNm=500, subap_wanted_fov=3, sensor_pxscale=0.5,
+                       subap_on_diameter=20, subap_npx=6)
+    ref = O.sh_forward(np.ones((120, 120)), np.zeros((120, 120)), go, S0=100, ext_mask=np.ones((120, 120)))
+    check_intensity(sh.outputs['out_i'].i.cpu().numpy(), ref, 20, 6)
+
+
+def test_reference_pixelscale_at_default_fov_ovs():
+    """test_sh.py:44-95 verbatim: a tilt of one CCD pixel shifts the spots by one pixel
+    (decimal 3), with the default fov_ovs 2 resampling."""
+    from specula_amd.data_objects.electric_field import ElectricField
+    pxs, ef_px, pitch, npx = 0.5, 120, 0.05, 6
+    sh = _sh_object(wavelengthInNm=500, subap_wanted_fov=npx * pxs, sensor_pxscale=pxs, subap_on_diameter=20,
+                    subap_npx=npx)
+    ef = ElectricField(ef_px, ef_px, pitch, S0=1, target_device_idx=0)
+    ef.generation_time = 1
+    sh.inputs['in_ef'].set(ef)
+    sh.setup()
+    sh.check_ready(1); sh.trigger(); sh.post_trigger()
+    flat = sh.outputs['out_i'].i.double().cpu().numpy().copy()
+    tilt_value = np.radians(ef_px * pitch * 1 / (60 * 60) * pxs)
+    tilt = np.linspace(-tilt_value / 2, tilt_value / 2, ef_px)
+    ef.phaseInNm[:] = torch.tensor(np.broadcast_to(tilt, (ef_px, ef_px)) * 1e9, dtype=torch.float32, device=DEV)
+    ef.generation_time = 2
+    sh.check_ready(2); sh.trigger(); sh.post_trigger()
+    tilted = sh.outputs['out_i'].i.double().cpu().numpy().copy()
+    shifted = np.roll(flat, (0, 1))
+    shifted[:, ::npx] = 0
+    tilted[:, ::npx] = 0
+    print(f'test_sh pixelscale: max |tilted - shifted| {np.abs(tilted - shifted).max():.3e}', flush=True)
+    np.testing.assert_array_almost_equal(tilted, shifted, decimal=3)
