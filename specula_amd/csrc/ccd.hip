@@ -18,7 +18,7 @@
 // pixel of any readout is reproducible without sequential state.  The streams are not numpy's:
 // parity with the reference is distributional.
 //
-// Poisson sampling: multiplication of uniforms (Knuth) for lambda < 10, Hormann's transformed
+// Poisson sampling: inversion of the CDF (sequential search) for lambda < 10, Hormann's transformed
 // rejection with squeeze (PTRS, Insurance: Mathematics and Economics 12 (1993) 39-45) above, in
 // FP32 with a cancellation-free log-pmf (the count itself is kept in FP64).  Gamma: Marsaglia & Tsang (ACM TOMS 26 (2000) 363-372).
 
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
     bool fresh = true;
     uint64_t px = 0;
     uint32_t wz = 0, ww = 0;
-    float lam = 0.0f, inv_lam = 0.0f, enlam = 0.0f, prod = 1.0f, a_ = 0.0f, b_ = 0.0f, vr = 0.0f;
+    float lam = 0.0f, inv_lam = 0.0f, enlam = 0.0f, a_ = 0.0f, b_ = 0.0f, vr = 0.0f;
     float log_invalpha = 0.0f, log_lam = 0.0f;
     double k = 0.0;
     uint32_t tries = 0;
@@ -371,7 +371,6 @@ __global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
             }
             tries = 0;
             k = 0.0;
-            prod = 1.0f;
             const float slam = __builtin_amdgcn_sqrtf(lam);
             b_ = 0.931f + 2.53f * slam;
             a_ = -0.059f + 0.02483f * b_;
@@ -393,15 +392,19 @@ __global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
         if (!(lam > 0.0f)) {
             k = 0.0;
             done = true;
-        } else if (lam < 10.0f) {                                  // product of uniforms vs exp(-lam)
-            prod *= u1;
-            done = prod <= enlam;
-            if (!done) {
+        } else if (lam < 10.0f) {
+            // inversion of the CDF with one 64-bit uniform: the lane finishes its pixel in this
+            // iteration (the search ends within 100 steps for lam < 10 unless U is within 1e-40
+            // of 1)
+            const double U = ((double)w.x * 4294967296.0 + (double)w.y + 1.0) * (1.0 / 18446744073709551616.0);
+            double pk = (double)enlam, F = pk;
+            k = 0.0;
+            for (int st = 0; st < 100 && U > F; ++st) {
                 k += 1.0;
-                prod *= u2;
-                done = prod <= enlam;
-                if (!done) k += 1.0;
+                pk *= (double)(lam * fast_rcp((float)k));
+                F += pk;
             }
+            done = true;
         } else {                                                   // PTRS
             const float U = u1 - 0.5f, V = u2;
             const float us = 0.5f - fabsf(U);
