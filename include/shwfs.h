@@ -66,7 +66,9 @@ int shwfs_plan_destroy(shwfs_plan* plan);
 
 /* LGS only: per-subaperture kernel spectra sampled on the lag grid, device complex64
  * interleaved, layout [n_lenses^2][2*np_sub-1][2*np_sub-1], subaperture s = row*n + col,
- * lag index (ly + np_sub-1, lx + np_sub-1).  Copied into the plan (device-to-device).
+ * lag index (ly + np_sub-1, lx + np_sub-1).  Copied into the plan (device-to-device).  The
+ * spectra must be those of real kernels (Hermitian: K(-l) = conj K(l)), as ConvolutionKernel's
+ * are; the kernels use only the lx >= 0 half.
  * Replaces ConvolutionKernel.kernels as read at sh.py:383-388. */
 int shwfs_set_kernels(shwfs_plan* plan, const float* kspec_lags, void* stream);
 

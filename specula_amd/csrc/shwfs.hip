@@ -825,13 +825,19 @@ sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
 template <int NP, int M, int TS>
 struct LgsShape {
     static constexpr int L = 2 * NP - 1;
-    static constexpr int E1 = M * NP, E2 = L * M, E3 = 16 * L;
+    static constexpr int LH = NP;                 // lag columns m_x = 0 .. NP-1 (Hermitian half)
+    static constexpr int E1 = M * NP, E2 = LH * M, E3 = 16 * LH;
     static constexpr int LDS_F2 = TS * (E1 > E2 ? (E1 > E3 ? E1 : E3) : (E2 > E3 ? E2 : E3));
 };
 
+// The lag-domain image D = C K is Hermitian on the (m_y, m_x) grid (C is an autocorrelation, K
+// the spectrum of a real kernel), and so are the binning operators (transforms of real ones):
+// Re sum_{m_y, m_x} P[Y][m_y] D[m_y][m_x] Q[X][m_x] only needs the columns m_x >= 0, the
+// m_x > 0 ones counted twice.  Phases 2-4 therefore carry LH = NP lag columns instead of 2NP-1.
 template <int NP, int M, int TS>
 __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, float2* buf, double* red) {
     constexpr int L = 2 * NP - 1;
+    constexpr int LH = LgsShape<NP, M, TS>::LH;
     static_assert(L <= M, "lag grid must fit the spectrum grid");
     const int t = threadIdx.x;
     const int n = g.n_lenses;
@@ -866,9 +872,9 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     }
     __syncthreads();
 
-    // phase 2 (read rows into registers, then overwrite the buffer with lag rows)
+    // phase 2 (read rows into registers, then overwrite the buffer with lag rows m_x >= 0)
     constexpr int T2 = (TS * M + 255) / 256;
-    float2 R[T2][L];
+    float2 R[T2][LH];
 #pragma unroll
     for (int h = 0; h < T2; ++h) {
         const int task = t + 256 * h;
@@ -883,7 +889,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             for (int kx = 0; kx < M; ++kx) G[kx] = make_float2(G[kx].x * G[kx].x + G[kx].y * G[kx].y, 0.f);
             fft_radix2<M, false>(G);
 #pragma unroll
-            for (int li = 0; li < L; ++li) R[h][li] = G[(li - (NP - 1) + M) % M];
+            for (int lh = 0; lh < LH; ++lh) R[h][lh] = G[lh];           // m_x = lh
         }
     }
     __syncthreads();
@@ -893,25 +899,26 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         if (task < TS * M) {
             const int s = task / M, ky = task - (task / M) * M;
 #pragma unroll
-            for (int li = 0; li < L; ++li) buf[(s * L + li) * M + ky] = R[h][li];
+            for (int lh = 0; lh < LH; ++lh) buf[(s * LH + lh) * M + ky] = R[h][lh];
         }
     }
     __syncthreads();
 
     // phase 3: column transforms over ky, kernel multiply, left binning operator
-    constexpr int T3 = (TS * L + 255) / 256;
+    constexpr int T3 = (TS * LH + 255) / 256;
     float2 V[T3][16];
 #pragma unroll
     for (int h = 0; h < T3; ++h) {
         const int task = t + 256 * h;
 #pragma unroll
         for (int Y = 0; Y < 16; ++Y) V[h][Y] = make_float2(0.f, 0.f);
-        if (task < TS * L) {
-            const int s = task / L, li = task - (task / L) * L;
+        if (task < TS * LH) {
+            const int s = task / LH, lh = task - (task / LH) * LH;
+            const int li = lh + NP - 1;                                  // lag index of m_x = lh
             const int sub = sub0 + s;
             float2 F[M];
 #pragma unroll
-            for (int ky = 0; ky < M; ++ky) F[ky] = buf[(s * L + li) * M + ky];
+            for (int ky = 0; ky < M; ++ky) F[ky] = buf[(s * LH + lh) * M + ky];
             fft_radix2<M, false>(F);
             if (sub < g.n_sub) {
                 const float2* kr = g.kspec + (int64_t)sub * L * L + li;
@@ -932,10 +939,10 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 #pragma unroll
     for (int h = 0; h < T3; ++h) {
         const int task = t + 256 * h;
-        if (task < TS * L) {
-            const int s = task / L, li = task - (task / L) * L;
+        if (task < TS * LH) {
+            const int s = task / LH, lh = task - (task / LH) * LH;
 #pragma unroll
-            for (int Y = 0; Y < 16; ++Y) buf[(s * 16 + Y) * L + li] = V[h][Y];
+            for (int Y = 0; Y < 16; ++Y) buf[(s * 16 + Y) * LH + lh] = V[h][Y];
         }
     }
     __syncthreads();
@@ -955,19 +962,24 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         const int sub = sub0 + s;
         if (sub >= g.n_sub) continue;
         const int i = sub / n, j = sub - (sub / n) * n;
-        const float2* vrow = buf + (s * 16 + Y) * L;
-        float2 v[L];
+        const float2* vrow = buf + (s * 16 + Y) * LH;
+        float2 v[LH];
 #pragma unroll
-        for (int li = 0; li < L; ++li) v[li] = vrow[li];
+        for (int lh = 0; lh < LH; ++lh) v[lh] = vrow[lh];
         float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx;
         const int x0 = hx * halfw, x1 = min(g.npx, x0 + halfw);
         for (int X = x0; X < x1; ++X) {
-            float acc = 0.f;
-#pragma unroll
-            for (int li = 0; li < L; ++li) {
-                const float2 qc = Qc[X * L + li];
-                acc = fmaf(v[li].x, qc.x, fmaf(-v[li].y, qc.y, acc));
+            float acc0 = 0.f, acc = 0.f;
+            {
+                const float2 qc = Qc[X * L + NP - 1];                     // m_x = 0: once
+                acc0 = fmaf(v[0].x, qc.x, -v[0].y * qc.y);
             }
+#pragma unroll
+            for (int lh = 1; lh < LH; ++lh) {                             // m_x > 0: twice
+                const float2 qc = Qc[X * L + NP - 1 + lh];
+                acc = fmaf(v[lh].x, qc.x, fmaf(-v[lh].y, qc.y, acc));
+            }
+            acc = fmaf(2.0f, acc, acc0);
             bsum += (double)acc;
             dst[X] = (float)((double)acc * scale);
         }
