@@ -189,9 +189,9 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
 
 /* ---- multi-sensor batch -----------------------------------------------------------------
  * No reference counterpart (the reference triggers one SH object at a time, each optionally a
- * captured CUDA graph, base_processing_obj.py:114-155).  One persistent launch runs every
- * (sensor, frame, subaperture tile) of several Parseval-normalised sensors of uneven shapes
- * (NGS MFMA np 8 plans and LGS np 16 / N 128 plans).  All buffers are fixed at creation (graph
+ * captured CUDA graph, base_processing_obj.py:114-155).  One persistent launch per body type
+ * (LGS np 16 / N 128 tiles, then NGS MFMA np 8 tiles of up to 4 streamed frames) runs every
+ * (sensor, frame, subaperture tile) of several Parseval-normalised sensors of uneven shapes.  All buffers are fixed at creation (graph
  * friendly): per sensor A (ef^2, static), phase (n_frames x ef^2), ccd (n_frames x ccd^2), S0.
  * grid_blocks: persistent blocks (0: one per work item); order: 0 heaviest tiles first,
  * 1 sensor by sensor, 2 round robin over sensors. */

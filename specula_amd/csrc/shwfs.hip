@@ -1569,14 +1569,18 @@ __global__ void slot_inverse_kernel(const int32_t* __restrict__ slot, int nn, in
     }
 }
 
-// im[m][s] = grid[m][0][inv[s]], im[m][n_valid + s] = grid[m][1][inv[s]]: coalesced writes.
+// im[m][s] = grid[m][0][inv[s]], im[m][n_valid + s] = grid[m][1][inv[s]]: coalesced writes; each
+// thread keeps its slot's grid index for kImGatherModes modes.
+constexpr int kImGatherModes = 16;
 __global__ void im_gather_kernel(const float* __restrict__ grid, const int32_t* __restrict__ inv, int nn, int n_valid,
-                                 float* __restrict__ im) {
-    const int m = blockIdx.y;
-    const float* gx = grid + (int64_t)m * 2 * nn;
-    float* row = im + (int64_t)m * 2 * n_valid;
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_valid; s += gridDim.x * blockDim.x) {
-        const int g = inv[s];
+                                 int n_modes, float* __restrict__ im) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n_valid) return;
+    const int g = inv[s];
+    const int m0 = blockIdx.y * kImGatherModes, m1 = min(n_modes, m0 + kImGatherModes);
+    for (int m = m0; m < m1; ++m) {
+        const float* gx = grid + (int64_t)m * 2 * nn;
+        float* row = im + (int64_t)m * 2 * n_valid;
         row[s] = gx[g];
         row[n_valid + s] = gx[nn + g];
     }
@@ -1589,18 +1593,21 @@ __global__ void im_gather_kernel(const float* __restrict__ grid, const int32_t* 
 // stride over the list; LDS is sized for the larger body.  Per-sensor completion times are
 // recorded with the constant-rate wall clock for latency statistics.
 struct BatchItem {
-    int32_t sensor, frame, tile, kind;   // kind 0: NGS MFMA np 8, 1: LGS np 16 / M 32
+    int32_t sensor, frame, tile, nf;     // frames [frame, frame + nf) of one subaperture tile
 };
 
-constexpr int kBatchLdsBytes =
-    (NgsMfmaShape<8>::LDS_F4 * 16 > LgsShape<16, 32, 8>::LDS_F2 * 8) ? NgsMfmaShape<8>::LDS_F4 * 16
-                                                                        : LgsShape<16, 32, 8>::LDS_F2 * 8;
+constexpr int kBatchNgsLds = NgsMfmaShape<8>::LDS_F4 * 16;
+constexpr int kBatchLgsLds = LgsShape<16, 32, 8>::LDS_F2 * 8;
+constexpr int kBatchNgsFpb = 4;          // frames streamed per NGS item (next-frame prefetch)
 
 __global__ void batch_stamp_kernel(unsigned long long* __restrict__ ts, int n_sensors) {
     if (threadIdx.x == 0) ts[0] = (unsigned long long)wall_clock64();
     for (int s = threadIdx.x; s < n_sensors; s += blockDim.x) ts[1 + s] = 0ull;
 }
 
+// One persistent launch per body type (KIND 0: NGS MFMA np 8 tiles, KIND 1: LGS np 16 tiles), so
+// that each runs with its own register and LDS footprint; blocks stride over the item list.
+template <int KIND>
 __global__ void __launch_bounds__(256)
 sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict__ items, int n_items,
                 unsigned long long* __restrict__ ts) {
@@ -1610,18 +1617,19 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
         __syncthreads();                               // LDS of the previous item is free
         const BatchItem w = items[it];
         const NgsArgs& g = sensors[w.sensor];
-        if (w.kind == 0) {
+        if constexpr (KIND == 0) {
             const int wv = threadIdx.x >> 6;
             float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
             const int wave_id = w.tile * NgsMfmaShape<8>::WPB + wv;
             if (g.npx == 16 && g.store_vec)
-                ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, w.frame, w.frame + 1, sw);
+                ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
             else if (g.npx == 8 && g.store_vec)
-                ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, w.frame, w.frame + 1, sw);
+                ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
             else
-                ngs_mfma_frames<8, true, 0, 0, false>(g, wave_id, w.frame, w.frame + 1, sw);
-        } else
+                ngs_mfma_frames<8, true, 0, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+        } else {
             lgs_tile<16, 32, 8>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
+        }
         __syncthreads();
         if (threadIdx.x == 0) atomicMax(&ts[1 + w.sensor], (unsigned long long)wall_clock64());
     }
@@ -2498,8 +2506,8 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
 #undef IMJ4
         SHWFS_LAUNCH_CHECK();
         if (work) {
-            hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, n_modes), dim3(256), 0, st, gridbuf, inv,
-                               nn, n_valid, im);
+            hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, (n_modes + kImGatherModes - 1) / kImGatherModes),
+                               dim3(256), 0, st, gridbuf, inv, nn, n_valid, n_modes, im);
             SHWFS_LAUNCH_CHECK();
         }
         return SHWFS_OK;
@@ -2525,6 +2533,7 @@ struct shwfs_batch {
     int n_sensors = 0;
     int n_items = 0;
     int grid = 0;
+    int n_ngs = 0, n_lgs = 0;         // items of each kind: d_items[0, n_lgs) LGS, then NGS
     NgsArgs* d_args = nullptr;
     BatchItem* d_items = nullptr;
     double* d_amp = nullptr;          // [n_sensors][2]
@@ -2558,7 +2567,7 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     if (order < 0 || order > 2) return set_error(SHWFS_ERR_ARG, "order must be 0, 1 or 2");
     std::vector<NgsArgs> args(n_sensors);
     std::vector<BatchItem> items;
-    std::vector<int> cost(n_sensors);
+    std::vector<int> cost(n_sensors), kinds(n_sensors);
     for (int s = 0; s < n_sensors; ++s) {
         const shwfs_plan* p = plans[s];
         if (!p || !A[s] || !phase[s] || !ccd[s]) return set_error(SHWFS_ERR_ARG, "null sensor entry");
@@ -2588,8 +2597,10 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         a.store_vec = aligned16(ccd[s]) && ((int64_t)g.n_lenses * g.subap_npx) % 4 == 0;
         a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0[s];
         a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
-        for (int f = 0; f < n_frames[s]; ++f)
-            for (int t = 0; t < tiles; ++t) items.push_back(BatchItem{s, f, t, kind});
+        kinds[s] = kind;
+        const int fstep = (kind == 0) ? kBatchNgsFpb : 1;
+        for (int f = 0; f < n_frames[s]; f += fstep)
+            for (int t = 0; t < tiles; ++t) items.push_back(BatchItem{s, f, t, std::min(fstep, n_frames[s] - f)});
     }
     if (order == 0)
         std::stable_sort(items.begin(), items.end(),
@@ -2605,11 +2616,15 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
             if (!any) break;
         }
     }
+    // LGS items first (their own launch), then NGS items; the order above holds within each kind
+    std::stable_partition(items.begin(), items.end(), [&](const BatchItem& x) { return kinds[x.sensor] == 1; });
     DeviceGuard dg(device);
     auto* b = new shwfs_batch();
     b->device = device;
     b->n_sensors = n_sensors;
     b->n_items = (int)items.size();
+    b->n_lgs = (int)std::count_if(items.begin(), items.end(), [&](const BatchItem& x) { return kinds[x.sensor] == 1; });
+    b->n_ngs = b->n_items - b->n_lgs;
     b->grid = grid_blocks > 0 ? std::min(grid_blocks, b->n_items) : b->n_items;
     auto cleanup = [&](int code) { shwfs_batch_destroy(b); return code; };
     hipError_t e;
@@ -2648,9 +2663,16 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, st, b->d_ts, b->n_sensors);
     SHWFS_LAUNCH_CHECK();
-    hipLaunchKernelGGL(sh_batch_kernel, dim3(b->grid), dim3(256), kBatchLdsBytes, st, b->d_args, b->d_items,
-                       b->n_items, b->d_ts);
-    SHWFS_LAUNCH_CHECK();
+    if (b->n_lgs > 0) {
+        hipLaunchKernelGGL(sh_batch_kernel<1>, dim3(std::min(b->grid, b->n_lgs)), dim3(256), kBatchLgsLds, st,
+                           b->d_args, b->d_items, b->n_lgs, b->d_ts);
+        SHWFS_LAUNCH_CHECK();
+    }
+    if (b->n_ngs > 0) {
+        hipLaunchKernelGGL(sh_batch_kernel<0>, dim3(std::min(b->grid, b->n_ngs)), dim3(256), kBatchNgsLds, st,
+                           b->d_args, b->d_items + b->n_lgs, b->n_ngs, b->d_ts);
+        SHWFS_LAUNCH_CHECK();
+    }
     return SHWFS_OK;
 }
 
