@@ -883,13 +883,18 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             float2 row[NP];
 #pragma unroll
             for (int x = 0; x < NP; ++x) row[x] = buf[(s * M + ky) * NP + x];
-            float2 G[M];
-            fft_pruned<M, NP, false>(row, G);
+            // FFT_M(|FFT_M(row)|^2)[m] = M sum_x row[x] conj(row[x + m]) (linear for |m| < NP since
+            // row is zero beyond NP): the lags m_x = 0 .. NP-1 directly, NP (NP + 1) / 2 products
 #pragma unroll
-            for (int kx = 0; kx < M; ++kx) G[kx] = make_float2(G[kx].x * G[kx].x + G[kx].y * G[kx].y, 0.f);
-            fft_radix2<M, false>(G);
+            for (int lh = 0; lh < LH; ++lh) {
+                float re = 0.f, im = 0.f;
 #pragma unroll
-            for (int lh = 0; lh < LH; ++lh) R[h][lh] = G[lh];           // m_x = lh
+                for (int x = 0; x + lh < NP; ++x) {
+                    re = fmaf(row[x].x, row[x + lh].x, fmaf(row[x].y, row[x + lh].y, re));
+                    im = fmaf(row[x].y, row[x + lh].x, fmaf(-row[x].x, row[x + lh].y, im));
+                }
+                R[h][lh] = make_float2(re * (float)M, im * (float)M);      // m_x = lh
+            }
         }
     }
     __syncthreads();
