@@ -1574,18 +1574,49 @@ __global__ void slot_inverse_kernel(const int32_t* __restrict__ slot, int nn, in
     }
 }
 
-// im[m][s] = grid[m][0][inv[s]], im[m][n_valid + s] = grid[m][1][inv[s]]: coalesced writes; each
-// thread keeps its slot's grid index for kImGatherModes modes.
-constexpr int kImGatherModes = 16;
+// im[m][s] = grid[m][0][inv[s]], im[m][n_valid + s] = grid[m][1][inv[s]].  One block per mode:
+// the mode's grid row is read into LDS with coalesced loads, then written in slot order with
+// coalesced stores (the permutation happens in LDS).  Needs 2 nn floats of LDS.
+__global__ void __launch_bounds__(256)
+im_gather_lds_kernel(const float* __restrict__ grid, const int32_t* __restrict__ inv, int nn, int n_valid,
+                     float* __restrict__ im) {
+    extern __shared__ float4 lds_raw[];
+    float* row_l = reinterpret_cast<float*>(lds_raw);
+    const int m = blockIdx.x;
+    const float* gx = grid + (int64_t)m * 2 * nn;
+    if ((nn & 1) == 0) {                               // rows 16-byte aligned: float4 fill, 4 in flight
+        const int n4 = nn >> 1;                        // 2 nn floats = nn / 2 float4
+        const float4* g4 = reinterpret_cast<const float4*>(gx);
+        float4* l4 = reinterpret_cast<float4*>(row_l);
+        for (int e = threadIdx.x; e < n4; e += 4 * blockDim.x) {
+            float4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e + u * (int)blockDim.x < n4) v[u] = g4[e + u * blockDim.x];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e + u * (int)blockDim.x < n4) l4[e + u * blockDim.x] = v[u];
+        }
+    } else {
+        for (int e = threadIdx.x; e < 2 * nn; e += blockDim.x) row_l[e] = gx[e];
+    }
+    __syncthreads();
+    float* row = im + (int64_t)m * 2 * n_valid;
+    for (int s = threadIdx.x; s < n_valid; s += blockDim.x) {
+        const int g = inv[s];
+        row[s] = row_l[g];
+        row[n_valid + s] = row_l[nn + g];
+    }
+}
+
+// Same permutation without LDS (grids too large for it): one thread per slot and mode.
 __global__ void im_gather_kernel(const float* __restrict__ grid, const int32_t* __restrict__ inv, int nn, int n_valid,
-                                 int n_modes, float* __restrict__ im) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_valid) return;
-    const int g = inv[s];
-    const int m0 = blockIdx.y * kImGatherModes, m1 = min(n_modes, m0 + kImGatherModes);
-    for (int m = m0; m < m1; ++m) {
-        const float* gx = grid + (int64_t)m * 2 * nn;
-        float* row = im + (int64_t)m * 2 * n_valid;
+                                 float* __restrict__ im) {
+    const int m = blockIdx.y;
+    const float* gx = grid + (int64_t)m * 2 * nn;
+    float* row = im + (int64_t)m * 2 * n_valid;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_valid; s += gridDim.x * blockDim.x) {
+        const int g = inv[s];
         row[s] = gx[g];
         row[n_valid + s] = gx[nn + g];
     }
@@ -2511,8 +2542,13 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
 #undef IMJ4
         SHWFS_LAUNCH_CHECK();
         if (work) {
-            hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, (n_modes + kImGatherModes - 1) / kImGatherModes),
-                               dim3(256), 0, st, gridbuf, inv, nn, n_valid, n_modes, im);
+            const int lds = 2 * nn * (int)sizeof(float);
+            if (lds <= 64 * 1024)
+                hipLaunchKernelGGL(im_gather_lds_kernel, dim3(n_modes), dim3(256), lds, st, gridbuf, inv, nn, n_valid,
+                                   im);
+            else
+                hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, n_modes), dim3(256), 0, st, gridbuf,
+                                   inv, nn, n_valid, im);
             SHWFS_LAUNCH_CHECK();
         }
         return SHWFS_OK;
