@@ -160,3 +160,62 @@ def test_ccd_object_integrates_over_dt_and_resets():
         last = k + 1
         np.testing.assert_allclose(outs[k][1], acc * np.float32(0.5), rtol=2e-6)
     assert float(ccd.outputs['integrated_i'].i.abs().max()) == 0.0 or read_steps[-1] != 5
+
+
+def test_noisy_loop_slopes_match_oracle_statistics():
+    """SH -> CCD (photon + readout noise) -> ShSlopec with the drop-in objects over 40 frames of the
+    same field: per-subaperture slope mean and spread agree with the oracle chain run with numpy's
+    generators (ccd.py dtype flow), to their sampling errors."""
+    from oracle import sh_oracle as O2
+    from specula_amd import workloads
+    from specula_amd.data_objects.electric_field import ElectricField
+    from specula_amd.data_objects.simul_params import SimulParams
+    from specula_amd.data_objects.subap_data import SubapData
+    from specula_amd.lib.subaps import detect_subaps
+    from specula_amd.processing_objects.sh import SH
+    from specula_amd.processing_objects.sh_slopec import ShSlopec
+    wl = workloads.scao_ngs_20x20()
+    A = wl.pupil()
+    ph = wl.frames(1)[0].astype(np.float32)
+    go = O2.sh_geometry(ef_size=wl.ef_size, pixel_pitch=wl.pixel_pitch, **wl.sh_kwargs)
+    clean = O2.sh_forward(A, ph.astype(np.float64), go, S0=50.0)
+    idxs, dmap = detect_subaps(O2.sh_forward(A, np.zeros_like(A), go), 20, 0.5)
+    sh = SH(**wl.sh_kwargs, target_device_idx=0)
+    if wl.fov_resolution_arcsec:
+        sh._fov_resolution_arcsec = wl.fov_resolution_arcsec
+    ef = ElectricField(wl.ef_size, wl.ef_size, wl.pixel_pitch, S0=50.0, target_device_idx=0)
+    ef.A[:] = torch.tensor(A.astype(np.float32), device=DEV)
+    ef.phaseInNm[:] = torch.tensor(ph, device=DEV)
+    sh.inputs['in_ef'].set(ef)
+    det = CCD(SimulParams(time_step=0.001), size=[160, 160], dt=0.001, bandw=300, photon_noise=True,
+              readout_noise=True, readout_level=2.0, quantum_eff=0.3, target_device_idx=0)
+    det.inputs['in_i'].set(sh.outputs['out_i'])
+    slopec = ShSlopec(SubapData(idxs=idxs, display_map=dmap, nx=20, ny=20, target_device_idx=0), target_device_idx=0)
+    slopec.inputs['in_pixels'].set(det.outputs['out_pixels'])
+    for o in (sh, det, slopec):
+        o.setup()
+    step = det.seconds_to_t(0.001)
+    gpu = []
+    for k in range(1, 41):
+        t = k * step
+        ef.generation_time = t
+        for o in (sh, det, slopec):
+            o.check_ready(t); o.trigger(); o.post_trigger()
+        gpu.append(slopec.outputs['out_slopes'].slopes.cpu().numpy().copy())
+    gpu = np.array(gpu)
+    rp, rr = np.random.default_rng(1), np.random.default_rng(2)
+    ref = []
+    for k in range(40):
+        pix = O2.ccd_readout((clean * np.float32(0.001 * 300)).astype(np.float32), quantum_eff=0.3, photon_noise=True,
+                             readout_noise=True, readout_level=2.0, photon_rng=rp, readout_rng=rr)
+        ref.append(O2.shslopec(np.asarray(pix, np.float64), idxs)['slopes'])
+    ref = np.array(ref)
+    n = len(gpu)
+    diff = np.abs(gpu.mean(0) - ref.mean(0))
+    se = np.sqrt(gpu.var(0) / n + ref.var(0) / n) + 1e-12
+    z = diff / se
+    ratio = np.median(gpu.std(0) / (ref.std(0) + 1e-12))
+    print(f'noisy loop: max z of slope means {z.max():.2f}, median noise ratio {ratio:.3f}, '
+          f'noise rms {ref.std(0).mean():.2e}', flush=True)
+    assert np.mean(z < 3.5) > 0.99 and z.max() < 5.5
+    assert 0.85 < ratio < 1.15
