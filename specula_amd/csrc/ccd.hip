@@ -448,7 +448,7 @@ extern "C" {
 
 int shccd_integrate(int n_frames, const float* in_i, int64_t in_stride, double factor, float* integrated,
                     int64_t int_stride, int64_t n_px, void* stream) {
-    if (!in_i || !integrated || n_frames <= 0 || n_px <= 0)
+    if (!in_i || !integrated || n_frames <= 0 || n_frames > 65535 || n_px <= 0)
         return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_integrate: bad argument");
     if (n_frames > 1 && (in_stride < n_px || int_stride < n_px))
         return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_integrate: frame stride too small");
@@ -461,8 +461,8 @@ int shccd_integrate(int n_frames, const float* in_i, int64_t in_stride, double f
 int shccd_readout(const shccd_params* p, int n_frames, float* integrated, int64_t int_stride, const float* in_i,
                   int64_t in_stride, double factor, const float* pixel_gains, uint64_t counter, float* pixels,
                   int64_t out_stride, void* stream) {
-    if (!p || !pixels || n_frames <= 0 || (!integrated && !in_i))
-        return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout: bad argument");
+    if (!p || !pixels || n_frames <= 0 || n_frames > 65535 || (!integrated && !in_i))
+        return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout: bad argument (null buffers or n_frames outside [1, 65535])");
     if (p->binning < 1 || p->out_rows <= 0 || p->out_cols <= 0 || p->in_rows != p->out_rows * p->binning ||
         p->in_cols != p->out_cols * p->binning)
         return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout: frame sizes must satisfy in = out * binning");

@@ -1868,6 +1868,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
                       int device, shwfs_plan** out) {
     if (!geom || !row_w || !col_w || !out) return set_error(SHWFS_ERR_ARG, "null argument");
     const shwfs_geom& g = *geom;
+    if (g.max_frames > 65535) return set_error(SHWFS_ERR_ARG, "max_frames above 65535 (frames index the launch grid's y)");
     if (g.n_lenses <= 0 || g.np_sub <= 0 || g.subap_npx <= 0 || g.max_frames <= 0)
         return set_error(SHWFS_ERR_ARG, "non-positive geometry field");
     if (g.ef_size != g.n_lenses * g.np_sub)
@@ -2272,6 +2273,7 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
                          const double* xw, const double* yw, const double* mw,
                          int max_frames, int device, shslopec_plan** out) {
     if (!out || !xw || !yw || !mw) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (max_frames > 65535) return set_error(SHWFS_ERR_ARG, "max_frames above 65535 (frames index the launch grid's y)");
     if (n_sub <= 0 || np_sub <= 0 || max_frames <= 0 || frame_h <= 0 || frame_w <= 0)
         return set_error(SHWFS_ERR_ARG, "non-positive size");
     if (!block_origin && !idx) return set_error(SHWFS_ERR_ARG, "need block_origin or idx");
@@ -2516,6 +2518,7 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
         return set_error(SHWFS_ERR_ARG, "inconsistent sizes");
     if (ef_size > 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "pupil wider than 1024 px");
     if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
+    if (n_modes > 65535) return set_error(SHWFS_ERR_ARG, "more than 65535 modes in one call");
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int mpb = 16;
     dim3 grid(n_lenses, (n_modes + mpb - 1) / mpb);
