@@ -74,3 +74,18 @@ def test_argument_errors_are_reported():
     assert rc == -1 and b'outside' in lib.shwfs_last_error()
     rc = lib.shwfs_plan_set_timing_events(None, None, None)
     assert rc == -1
+
+
+def test_grid_limits_are_rejected():
+    lib = _native.load_library()
+    h = ctypes.c_void_p()
+    g = _native.ShwfsGeom(n_lenses=4, ef_size=32, np_sub=8, fft_size=32, spec_size=16, subap_npx=8,
+                          max_frames=70000, parseval=1, norm_const=1.0, pixel_area=1.0)
+    w = np.zeros((8, 16))
+    rc = lib.shwfs_plan_create(ctypes.byref(g), w.ctypes.data_as(ctypes.c_void_p), w.ctypes.data_as(ctypes.c_void_p),
+                               0, ctypes.byref(h))
+    assert rc == -1 and b'65535' in lib.shwfs_last_error()
+    p = _native.ShccdParams(in_rows=8, in_cols=8, out_rows=8, out_cols=8, binning=1, flags=0, qe=1.0)
+    rc = lib.shccd_readout(ctypes.byref(p), 70000, None, 0, ctypes.c_void_p(16), 64, 1.0, None, 0,
+                           ctypes.c_void_p(16), 64, None)
+    assert rc == -1 and b'65535' in lib.shwfs_last_error()
