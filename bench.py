@@ -40,9 +40,23 @@ def parse():
     ap.add_argument('--fpb', type=int, default=0, help='frames per block of the SH kernel (0: plan default)')
     ap.add_argument('--copy-probe', action='store_true',
                     help='also run two 256 MiB device copies (reference traffic for rocprofv3 counters)')
-    ap.add_argument('--extras', type=int, default=1,
-                    help='1: also time the LGS sensor and the 1000-mode interaction matrix')
-    return ap.parse_args()
+    ap.add_argument('--extras', default='all',
+                    help="extra measurements: 'all' (or 1), 'none' (or 0), or a comma list of "
+                         "bf16, ccd, lgs, im, multi")
+    args = ap.parse_args()
+    names = ('bf16', 'ccd', 'lgs', 'im', 'multi')
+    e = str(args.extras).strip().lower()
+    args.extras = set(names) if e in ('all', '1') else set() if e in ('none', '0', '') else \
+        {x.strip() for x in e.split(',') if x.strip()}
+    unknown = args.extras - set(names)
+    if unknown:
+        ap.error(f'unknown extras {sorted(unknown)}')
+    return args
+
+
+def progress(msg):
+    """One line per phase on stderr (stdout carries only the JSON result)."""
+    print(f'bench: {msg}', file=sys.stderr, flush=True)
 
 
 def cpu_baseline(wl, A, frames, idxs, budget_s):
@@ -342,7 +356,9 @@ def main():
                      'fused_cog': bool(fused), 'frames_per_block': plan.fpb,
                      'bytes_per_launch': bytes_per_frame * F},
     }
-    if args.extras and fused:
+    progress(f'main step: {fps:.0f} frames/s')
+    if 'bf16' in args.extras and fused:
+        progress('bf16-split variant')
         # same step with the bf16-split MFMA binning (variant 3): x = hi + lo in bf16, all four cross
         # products on the matrix cores; ~1e-5 relative image error (FP32 path: ~4e-7)
         plan3 = SHPlan(g, max_frames=F, device=0, variant=3)
@@ -355,12 +371,20 @@ def main():
         result['bf16_split_variant'] = {'frames_per_s': round(F / dt3, 1), 'ms_per_step': round(dt3 * 1e3, 4),
                                         'dtype': 'bf16 hi+lo pieces, f32 accumulation'}
         plan.bind_slopec(sl_plan)
-    if args.extras:
+    if 'ccd' in args.extras:
+        progress('ccd readout')
         result['ccd'] = bench_ccd(torch, dev, plan, sl_plan, At, ph, ccd, slopes, flux, totals, F, g)
+    if 'lgs' in args.extras:
+        progress('lgs sensor')
         result['lgs'] = bench_lgs(torch, dev)
+    if 'im' in args.extras:
+        progress('interaction matrix')
         result['interaction_matrix'] = bench_im(torch, dev)
+    if 'multi' in args.extras:
+        progress('multi-sensor batch')
         result['multi_sensor'] = bench_multi(torch, dev)
     if args.cpu_seconds > 0:
+        progress('cpu baseline')
         result['cpu_baseline'] = cpu_baseline(wl, A, frames, idxs, args.cpu_seconds)
     print(json.dumps(result), flush=True)
 

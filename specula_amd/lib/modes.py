@@ -42,14 +42,20 @@ def fourier_modes(pupil, n_modes, device='cpu', dtype=torch.float32, seed=None):
     yy, xx = torch.meshgrid(coords, coords, indexing='ij')
     freqs = _frequencies(n_modes)
     rng = np.random.default_rng(seed) if seed is not None else None
+    offs = [0.0 if rng is None else float(rng.uniform(0, 2 * math.pi)) for _ in freqs]
     out = torch.empty((n_modes, n, n), dtype=dtype, device=device)
-    for m, (kx, ky, par) in enumerate(freqs):
-        off = 0.0 if rng is None else float(rng.uniform(0, 2 * math.pi))
+    wf = w.to(torch.float64)
+    chunk = 100                                          # modes per batch of tensor operations
+    for m0 in range(0, n_modes, chunk):
+        f = freqs[m0:m0 + chunk]
+        kx = torch.tensor([v[0] for v in f], dtype=torch.float64, device=device).view(-1, 1, 1)
+        ky = torch.tensor([v[1] for v in f], dtype=torch.float64, device=device).view(-1, 1, 1)
+        par = torch.tensor([v[2] for v in f], device=device).view(-1, 1, 1)
+        off = torch.tensor(offs[m0:m0 + chunk], dtype=torch.float64, device=device).view(-1, 1, 1)
         arg = 2 * math.pi * (kx * xx + ky * yy) + off
-        mode = torch.cos(arg) if par == 0 else torch.sin(arg)
-        mode = torch.where(w, mode, torch.zeros_like(mode))
-        mean = mode.sum() / npx
-        mode = torch.where(w, mode - mean, torch.zeros_like(mode))
-        rms = torch.sqrt((mode * mode).sum() / npx)
-        out[m] = (mode / rms).to(dtype)
+        mode = torch.where(par == 0, torch.cos(arg), torch.sin(arg)) * wf
+        mean = mode.sum(dim=(1, 2), keepdim=True) / npx
+        mode = (mode - mean) * wf
+        rms = torch.sqrt((mode * mode).sum(dim=(1, 2), keepdim=True) / npx)
+        out[m0:m0 + len(f)] = (mode / rms).to(dtype)
     return out
