@@ -1202,8 +1202,8 @@ __global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
 
 
 // Fused-path variant: the frame totals come as per-wave partial sums from the SH kernel, so every
-// block reduces them (same fixed order in every block) and handles its own slice of subapertures.
-// grid = (ceil(n_sub / blockDim), n_frames).
+// block reduces them (same fixed order in every block) and handles its own slice of subapertures
+// (grid-stride over them).  grid = (a few blocks, n_frames).
 __global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const double* __restrict__ tpart,
                                           int n_parts, int n_sub, float* __restrict__ slopes,
                                           int64_t slopes_stride, double* __restrict__ flux,
@@ -1221,16 +1221,17 @@ __global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const 
     sf = sft.x;
     st = sft.y;
     const double limit = 1.0 / ((st / n_sub) * 1e-3);
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n_sub) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_sub; s += gridDim.x * blockDim.x) {
         const double* m = mom + ((int64_t)f * n_sub + s) * 4;
-        double factor = 1.0 / m[1];
+        const double2 m01 = *reinterpret_cast<const double2*>(m);
+        const double2 m23 = *reinterpret_cast<const double2*>(m + 2);
+        double factor = 1.0 / m01.y;
         if (factor > limit) factor = 0.0;
         if (factor_out) factor_out[(int64_t)f * n_sub + s] = factor;
         float* sl = slopes + f * slopes_stride;
-        sl[s] = (float)(m[2] * factor);
-        sl[n_sub + s] = (float)(m[3] * factor);
-        flux[(int64_t)f * n_sub + s] = m[0];
+        sl[s] = (float)(m23.x * factor);
+        sl[n_sub + s] = (float)(m23.y * factor);
+        flux[(int64_t)f * n_sub + s] = m01.x;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         totals[2 * f] = sf;
@@ -2261,7 +2262,7 @@ int shwfs_forward_slopes(shwfs_plan* p, int n_frames, const float* A, int64_t a_
     int rc = sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, ccd, ccd_stride, &cc, st);
     if (rc) return rc;
     DeviceGuard dg(p->device);
-    hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 255) / 256, n_frames), dim3(256), 0, st,
+    hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 4095) / 4096, n_frames), dim3(1024), 0, st,
                        c->d_mom, p->d_tpart, p->n_parts, c->n_sub, slopes, slopes_stride, flux, totals,
                        (double*)nullptr);
     SHWFS_LAUNCH_CHECK();
