@@ -673,7 +673,9 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
                         w0 = clamp0(v.x); w1 = clamp0(v.y); w2 = clamp0(v.z); w3 = clamp0(v.w);
                     }
                     const float yv = sw[kYwWord + Y];   // y weight of row Y (kept in LDS padding)
-                    l_f += (v.x + v.y) + (v.z + v.w);
+                    // no threshold: the image is non-negative up to rounding, so the flux is the
+                    // clamped total (accumulated once, below)
+                    if constexpr (COG != 3) l_f += (v.x + v.y) + (v.z + v.w);
                     l_y = fmaf((w0 + w1) + (w2 + w3), yv, l_y);
                     cw0 += w0; cw1 += w1; cw2 += w2; cw3 += w3;
                 }
@@ -686,14 +688,17 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             double m_f = l_f, m_t = l_t, m_x = l_x, m_y = l_y;
 #pragma unroll
             for (int o = 1; o < X4; o <<= 1) {
-                m_f += __shfl_xor(m_f, o, kWave); m_t += __shfl_xor(m_t, o, kWave);
+                if constexpr (COG != 3) m_f += __shfl_xor(m_f, o, kWave);
+                m_t += __shfl_xor(m_t, o, kWave);
                 m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
             }
 #pragma unroll
             for (int o = LPR; o < 64; o <<= 1) {
-                m_f += __shfl_xor(m_f, o, kWave); m_t += __shfl_xor(m_t, o, kWave);
+                if constexpr (COG != 3) m_f += __shfl_xor(m_f, o, kWave);
+                m_t += __shfl_xor(m_t, o, kWave);
                 m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
             }
+            if constexpr (COG == 3) m_f = m_t;
             const bool writer = live && lane < LPR && x4 == 0 && wc.slot >= 0;
             if (writer) {
                 double* m = g.mom + ((int64_t)f * g.n_valid + wc.slot) * 4;
