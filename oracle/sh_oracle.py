@@ -36,6 +36,16 @@ Pinning (what checks each function; see tests/test_oracle_pins.py):
                               affine functions, no reads outside the input
   sh_forward (interpolated) . flat-wavefront slopes of scao_sn_n8_th0.5_ref.fits (fov_ovs 2),
                               reduces to the plain path for an identity resampling
+  zernike_degree / zernike_map  Noll's table (closed forms of Z1..Z11), orthonormality on a fine
+                              disk
+  compute_zern_ifunc, dm_phase, pushpull_time_history, accumulate_intmat, recmat_fits_data
+                              the reference's calibration golden scao_rec_n8_th0.5_ref.fits
+                              (test_sh_calibration.py, params_scao_sh_test{,_rec}.yml) to
+                              2.6e-5 (its bar: decimal 3), which also pins sh_forward at fov_ovs 2
+                              with cut 24 and shslopec end to end
+  ccd_readout ............... closed forms (block sums x binning^2, QE), Poisson moments, the
+                              integer frame after Poisson (readout truncation), the ADU clamp;
+                              statistical only for the noise streams (numpy generators)
 """
 import math
 
