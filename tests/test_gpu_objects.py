@@ -128,3 +128,36 @@ def test_subapdata_restore_block_mode_and_thresholds():
         sl.post_trigger()
         ref = O.shslopec(img.astype(np.float32).astype(np.float64), idxs, thr_value=thr, thr_pedestal=ped)
         check_slopes(sl.slopes.slopes.cpu().numpy(), ref['slopes'], 8)
+
+
+def test_slopec_post_trigger_recmat_and_filtering():
+    """Slopec.post_trigger (slopec.py:92-104): slopes @ recmat, and the modal filtering
+    s - (s @ filt_recmat) @ filt_intmat^T, as device GEMMs on the ShSlopec output."""
+    from specula_amd.data_objects.intmat import Intmat
+    from specula_amd.data_objects.recmat import Recmat
+    wl = workloads.scao_ngs_20x20()
+    go = oracle_geometry(wl)
+    img = O.sh_forward(wl.pupil(), wl.frames(1)[0], go)
+    idxs, dmap = detect_subaps(img, 20, 0.5)
+    sd = SubapData(idxs=idxs, display_map=dmap, nx=20, ny=20, target_device_idx=0)
+    pix = Pixels(*img.shape, target_device_idx=0)
+    pix.pixels = torch.tensor(img, dtype=torch.float32, device='cuda:0')
+    pix.generation_time = 1
+    n_sl = 2 * idxs.shape[0]
+    rng = np.random.default_rng(4)
+    R = rng.standard_normal((n_sl, 12)).astype(np.float32)
+    s_ref = O.shslopec(img.astype(np.float32).astype(np.float64), idxs)['slopes']
+    sl = ShSlopec(sd, target_device_idx=0)
+    sl.recmat = Recmat(R, target_device_idx=0)
+    sl.inputs['in_pixels'].set(pix)
+    sl.check_ready(1); sl.trigger(); sl.post_trigger()
+    np.testing.assert_allclose(sl.slopes.slopes.cpu().numpy(), s_ref @ R, rtol=0, atol=1e-3 * np.abs(s_ref @ R).max())
+    # the reference's formula m @ filt_intmat.T with m = s @ filt_recmat (n_modes) takes the
+    # filtering interaction matrix as (n_slopes, n_modes)
+    IM = rng.standard_normal((n_sl, 5)).astype(np.float32)
+    RM = rng.standard_normal((n_sl, 5)).astype(np.float32) * 1e-2
+    sl2 = ShSlopec(sd, filtmat=(Intmat(IM, target_device_idx=0), Recmat(RM, target_device_idx=0)), target_device_idx=0)
+    sl2.inputs['in_pixels'].set(pix)
+    sl2.check_ready(1); sl2.trigger(); sl2.post_trigger()
+    expect = s_ref - (s_ref @ RM) @ IM.T
+    np.testing.assert_allclose(sl2.slopes.slopes.cpu().numpy(), expect, rtol=0, atol=2e-3 * 2 / 7)
