@@ -205,7 +205,7 @@ struct NgsArgs {
     int64_t ph_stride;
     float* ccd;
     int64_t ccd_stride;
-    const float* roww;   // npx_pad x M   (P)
+    const float* roww;   // P: npx_pad x M (NGS), or [L lags][16 rows] complex, transposed (LGS)
     const float* colw;   // npx_pad x M   (Q)
     const double* amp;   // [frame][2]: sum(A), sum(A^2)     (frame index f * amp_fstep)
     int amp_fstep;
@@ -823,15 +823,19 @@ sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
 // lag grid |m| <= np-1 (S = |FFT_M(E)|^2, M >= 2np-1), the binned image is B = Re(P D Q^T) where
 // P, Q (npx x L, complex) already carry N^2/M^2, the rebin, the cut and the mask.
 // One block = TS subapertures, 256 threads:
-//   1: task (s, x)   column FFT over y of the field                         -> buf[s][ky][x]
-//   2: task (s, ky)  pruned FFT over x, S = |.|^2, forward FFT over kx of S  -> buf[s][lx][ky] (L lags)
-//   3: task (s, lx)  forward FFT over ky, D = F * K_s, V[Y] = sum_my P[Y][my] D[my]
-//   4: task (s, Y, half)  B[Y][X] = Re sum_lx V[Y][lx] Q[X][lx]
+//   1:  task (s, x)       column FFT over y of the field                      -> buf[s][ky][x]
+//   2:  task (s, ky)      lag row of |FFT_x(.)|^2 by direct autocorrelation     -> buf[s][lx][ky]
+//   3a: task (s, lx)      forward FFT over ky, D = F * K_s, in place            -> buf[s][lx][my]
+//   3b: task (s, lx, Y/8) V[Y] = sum_my P[Y][my] D[my], P from scalar loads    -> buf[s][Y][lx]
+//   4:  task (s, Y, half) B[Y][X] = Re sum_lx V[Y][lx] Q[X][lx]
+// Every LDS row is padded by one complex element, so lanes that walk different rows of one
+// array hit different banks (row lengths NP, M and LH are multiples of 32 dwords otherwise).
 template <int NP, int M, int TS>
 struct LgsShape {
     static constexpr int L = 2 * NP - 1;
     static constexpr int LH = NP;                 // lag columns m_x = 0 .. NP-1 (Hermitian half)
-    static constexpr int E1 = M * NP, E2 = LH * M, E3 = 16 * LH;
+    static constexpr int R1 = NP + 1, R2 = M + 1, R3 = LH + 1;   // padded row lengths
+    static constexpr int E1 = M * R1, E2 = LH * R2, E3 = 16 * R3;
     static constexpr int LDS_F2 = TS * (E1 > E2 ? (E1 > E3 ? E1 : E3) : (E2 > E3 ? E2 : E3));
 };
 
@@ -842,12 +846,13 @@ struct LgsShape {
 template <int NP, int M, int TS>
 __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, float2* buf, double* red) {
     constexpr int L = 2 * NP - 1;
-    constexpr int LH = LgsShape<NP, M, TS>::LH;
+    using Sh = LgsShape<NP, M, TS>;
+    constexpr int LH = Sh::LH, R1 = Sh::R1, R2 = Sh::R2;
     static_assert(L <= M, "lag grid must fit the spectrum grid");
+    static_assert(TS * LH <= 128, "phase 3b maps (s, lx) onto two chunks of 64 lanes");
     const int t = threadIdx.x;
     const int n = g.n_lenses;
     const int sub0 = tile * TS;
-    const float2* Pc = reinterpret_cast<const float2*>(g.roww);   // npx_pad x L complex
     const float2* Qc = reinterpret_cast<const float2*>(g.colw);
 
     // phase 1
@@ -873,7 +878,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         float2 G[M];
         fft_pruned<M, NP, false>(e, G);
 #pragma unroll
-        for (int ky = 0; ky < M; ++ky) buf[(s * M + ky) * NP + x] = G[ky];
+        for (int ky = 0; ky < M; ++ky) buf[(s * M + ky) * R1 + x] = G[ky];
     }
     __syncthreads();
 
@@ -887,7 +892,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             const int s = task / M, ky = task - (task / M) * M;
             float2 row[NP];
 #pragma unroll
-            for (int x = 0; x < NP; ++x) row[x] = buf[(s * M + ky) * NP + x];
+            for (int x = 0; x < NP; ++x) row[x] = buf[(s * M + ky) * R1 + x];
             // FFT_M(|FFT_M(row)|^2)[m] = M sum_x row[x] conj(row[x + m]) (linear for |m| < NP since
             // row is zero beyond NP): the lags m_x = 0 .. NP-1 directly, NP (NP + 1) / 2 products
 #pragma unroll
@@ -909,51 +914,59 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         if (task < TS * M) {
             const int s = task / M, ky = task - (task / M) * M;
 #pragma unroll
-            for (int lh = 0; lh < LH; ++lh) buf[(s * LH + lh) * M + ky] = R[h][lh];
+            for (int lh = 0; lh < LH; ++lh) buf[(s * LH + lh) * R2 + ky] = R[h][lh];
         }
     }
     __syncthreads();
 
-    // phase 3: column transforms over ky, kernel multiply, left binning operator
-    constexpr int T3 = (TS * LH + 255) / 256;
-    float2 V[T3][16];
+    // phase 3a: column transforms over ky and the kernel multiply, written back in place
+    for (int task = t; task < TS * LH; task += 256) {
+        const int s = task / LH, lh = task - (task / LH) * LH;
+        const int sub = sub0 + s;
+        if (sub >= g.n_sub) continue;
+        float2* col = buf + (s * LH + lh) * R2;
+        float2 F[M];
 #pragma unroll
-    for (int h = 0; h < T3; ++h) {
-        const int task = t + 256 * h;
+        for (int ky = 0; ky < M; ++ky) F[ky] = col[ky];
+        fft_radix2<M, false>(F);
+        const float2* kr = g.kspec + (int64_t)sub * L * L + lh + NP - 1;   // lag column m_x = lh
 #pragma unroll
-        for (int Y = 0; Y < 16; ++Y) V[h][Y] = make_float2(0.f, 0.f);
-        if (task < TS * LH) {
-            const int s = task / LH, lh = task - (task / LH) * LH;
-            const int li = lh + NP - 1;                                  // lag index of m_x = lh
-            const int sub = sub0 + s;
-            float2 F[M];
+        for (int mi = 0; mi < L; ++mi) col[mi] = cmul(F[(mi - (NP - 1) + M) % M], kr[mi * L]);
+    }
+    __syncthreads();
+
+    // phase 3b: left binning operator.  Wave w takes 64 (s, lx) pairs (chunk w % 2) and the rows
+    // Y = 8 (w / 2) .. +7, so P[Y][my] is wave-uniform: roww holds P transposed, [my][16][re, im],
+    // read through the constant address space (scalar loads, one 64-byte line per my).
+    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int pair = (wave & 1) * 64 + (t & 63);
+    const int yh = wave >> 1;
+    const bool active3 = pair < TS * LH && 8 * yh < g.npx;
+    float2 V[8];
 #pragma unroll
-            for (int ky = 0; ky < M; ++ky) F[ky] = buf[(s * LH + lh) * M + ky];
-            fft_radix2<M, false>(F);
-            if (sub < g.n_sub) {
-                const float2* kr = g.kspec + (int64_t)sub * L * L + li;
+    for (int y = 0; y < 8; ++y) V[y] = make_float2(0.f, 0.f);
+    if (active3) {
+        const float2* col = buf + pair * R2;
+        using cfx4 = const __attribute__((address_space(4))) floatx4;
+        cfx4* pt = (cfx4*)(g.roww) + yh * 4;
 #pragma unroll
-                for (int mi = 0; mi < L; ++mi) {
-                    const float2 d = cmul(F[(mi - (NP - 1) + M) % M], kr[mi * L]);
+        for (int mi = 0; mi < L; ++mi) {
+            const float2 d = col[mi];
 #pragma unroll
-                    for (int Y = 0; Y < 16; ++Y) {          // rows >= npx of P are zero
-                        const float2 pc = Pc[Y * L + mi];
-                        V[h][Y].x = fmaf(pc.x, d.x, fmaf(-pc.y, d.y, V[h][Y].x));
-                        V[h][Y].y = fmaf(pc.x, d.y, fmaf(pc.y, d.x, V[h][Y].y));
-                    }
-                }
+            for (int q = 0; q < 4; ++q) {
+                const floatx4 pc = pt[mi * 8 + q];                 // rows 2q, 2q + 1 of this group
+                V[2 * q].x = fmaf(pc.x, d.x, fmaf(-pc.y, d.y, V[2 * q].x));
+                V[2 * q].y = fmaf(pc.x, d.y, fmaf(pc.y, d.x, V[2 * q].y));
+                V[2 * q + 1].x = fmaf(pc.z, d.x, fmaf(-pc.w, d.y, V[2 * q + 1].x));
+                V[2 * q + 1].y = fmaf(pc.z, d.y, fmaf(pc.w, d.x, V[2 * q + 1].y));
             }
         }
     }
     __syncthreads();
+    if (active3) {
+        const int s = pair / LH, lh = pair - (pair / LH) * LH;
 #pragma unroll
-    for (int h = 0; h < T3; ++h) {
-        const int task = t + 256 * h;
-        if (task < TS * LH) {
-            const int s = task / LH, lh = task - (task / LH) * LH;
-#pragma unroll
-            for (int Y = 0; Y < 16; ++Y) buf[(s * 16 + Y) * LH + lh] = V[h][Y];
-        }
+        for (int y = 0; y < 8; ++y) buf[(s * 16 + 8 * yh + y) * Sh::R3 + lh] = V[y];
     }
     __syncthreads();
 
@@ -972,7 +985,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         const int sub = sub0 + s;
         if (sub >= g.n_sub) continue;
         const int i = sub / n, j = sub - (sub / n) * n;
-        const float2* vrow = buf + (s * 16 + Y) * LH;
+        const float2* vrow = buf + (s * 16 + Y) * LgsShape<NP, M, TS>::R3;
         float2 v[LH];
 #pragma unroll
         for (int lh = 0; lh < LH; ++lh) v[lh] = vrow[lh];
@@ -1914,10 +1927,11 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         p->ts = lgs_ts;
         rw.assign((size_t)16 * p->L * 2, 0.f);
         cw.assign((size_t)16 * p->L * 2, 0.f);
-        for (size_t q = 0; q < (size_t)g.subap_npx * p->L * 2; ++q) {
-            rw[q] = (float)row_w[q];
-            cw[q] = (float)col_w[q];
-        }
+        for (size_t q = 0; q < (size_t)g.subap_npx * p->L * 2; ++q) cw[q] = (float)col_w[q];
+        for (int Y = 0; Y < g.subap_npx; ++Y)             // P transposed: [my][Y (16)][re, im]
+            for (int mi = 0; mi < p->L; ++mi)
+                for (int c = 0; c < 2; ++c)
+                    rw[((size_t)mi * 16 + Y) * 2 + c] = (float)row_w[((size_t)Y * p->L + mi) * 2 + c];
     } else {
         p->rows_alloc = use_mfma ? 16 : p->npx_pad;
         p->ts = use_mfma ? 4 * (64 / g.np_sub) : ngs_tile_size(g.np_sub);
