@@ -826,8 +826,8 @@ sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
 //   1:  task (s, x)       column FFT over y of the field                      -> buf[s][ky][x]
 //   2:  task (s, ky)      lag row of |FFT_x(.)|^2 by direct autocorrelation     -> buf[s][lx][ky]
 //   3a: task (s, lx)      forward FFT over ky, D = F * K_s, in place            -> buf[s][lx][my]
-//   3b: task (s, lx, Y/8) V[Y] = sum_my P[Y][my] D[my], P from scalar loads    -> buf[s][Y][lx]
-//   4:  task (s, Y, half) B[Y][X] = Re sum_lx V[Y][lx] Q[X][lx]
+//   3b: MFMA              V[Y][lx] = sum_my P[Y][my] D[my][lx] (FP32 16x16x4)   -> buf[s][Y][lx]
+//   4:  MFMA              B[Y][X] = Re sum_lx V[Y][lx] Q[X][lx] (FP32 16x16x4)  -> CCD
 // Every LDS row is padded by one complex element, so lanes that walk different rows of one
 // array hit different banks (row lengths NP, M and LH are multiples of 32 dwords otherwise).
 template <int NP, int M, int TS>
@@ -849,7 +849,6 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     using Sh = LgsShape<NP, M, TS>;
     constexpr int LH = Sh::LH, R1 = Sh::R1, R2 = Sh::R2;
     static_assert(L <= M, "lag grid must fit the spectrum grid");
-    static_assert(TS * LH <= 128, "phase 3b maps (s, lx) onto two chunks of 64 lanes");
     const int t = threadIdx.x;
     const int n = g.n_lenses;
     const int sub0 = tile * TS;
@@ -935,76 +934,110 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     }
     __syncthreads();
 
-    // phase 3b: left binning operator.  Wave w takes 64 (s, lx) pairs (chunk w % 2) and the rows
-    // Y = 8 (w / 2) .. +7, so P[Y][my] is wave-uniform: roww holds P transposed, [my][16][re, im],
-    // read through the constant address space (scalar loads, one 64-byte line per my).
-    const int wave = __builtin_amdgcn_readfirstlane(t >> 6);
-    const int pair = (wave & 1) * 64 + (t & 63);
-    const int yh = wave >> 1;
-    const bool active3 = pair < TS * LH && 8 * yh < g.npx;
-    float2 V[8];
+    // phase 3b: left binning operator V = P~ D on the matrix cores (FP32 16x16x4, K = lags m_y).
+    // A = P~ (16 rows Y x 4 lags per step: lane l holds P~[l % 16][4 ks + l / 16], kept in
+    // registers for the tile), B = D (4 lags x 16 columns: SPN = 16 / LH subapertures side by
+    // side, lag column m_x = column % LH), V[Y][n] out in lane n + 16 (Y / 4), register Y % 4.
+    // Re V = Re P~ Re D - Im P~ Im D, Im V = Re P~ Im D + Im P~ Re D: four products per K step.
+    {
+        constexpr int SPN = 16 / LH;                 // subapertures per 16-column block
+        constexpr int KS = (L + 3) / 4;              // K steps (lags beyond L: P~ = 0)
+        static_assert(TS == 8 * SPN, "eight column blocks per tile, two per wave");
+        static_assert(4 * KS <= M, "K steps stay inside the written part of an LDS row");
+        const int lane = t & 63, wave = t >> 6;
+        const int n16 = lane & 15, k4 = lane >> 4;
+        const float2* Pt = reinterpret_cast<const float2*>(g.roww);       // [m_y][16 rows] complex
+        float pr[KS], pim[KS], npim[KS];
 #pragma unroll
-    for (int y = 0; y < 8; ++y) V[y] = make_float2(0.f, 0.f);
-    if (active3) {
-        const float2* col = buf + pair * R2;
-        using cfx4 = const __attribute__((address_space(4))) floatx4;
-        cfx4* pt = (cfx4*)(g.roww) + yh * 4;
+        for (int ks = 0; ks < KS; ++ks) {
+            const int mi = 4 * ks + k4;
+            const float2 pc = mi < L ? Pt[mi * 16 + n16] : make_float2(0.f, 0.f);
+            pr[ks] = pc.x; pim[ks] = pc.y; npim[ks] = -pc.y;
+        }
+        const int so = n16 / LH, lh = n16 - (n16 / LH) * LH;
+        const bool col_ok = so < SPN;
+        floatx4 vr[2], vi[2];
 #pragma unroll
-        for (int mi = 0; mi < L; ++mi) {
-            const float2 d = col[mi];
+        for (int bk = 0; bk < 2; ++bk) {
+            const int s = (2 * wave + bk) * SPN + so;
+            const float2* col = buf + (col_ok ? s * LH + lh : 0) * R2;
+            floatx4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const floatx4 pc = pt[mi * 8 + q];                 // rows 2q, 2q + 1 of this group
-                V[2 * q].x = fmaf(pc.x, d.x, fmaf(-pc.y, d.y, V[2 * q].x));
-                V[2 * q].y = fmaf(pc.x, d.y, fmaf(pc.y, d.x, V[2 * q].y));
-                V[2 * q + 1].x = fmaf(pc.z, d.x, fmaf(-pc.w, d.y, V[2 * q + 1].x));
-                V[2 * q + 1].y = fmaf(pc.z, d.y, fmaf(pc.w, d.x, V[2 * q + 1].y));
+            for (int ks = 0; ks < KS; ++ks) {
+                const float2 d = col[4 * ks + k4];
+                ar = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], d.x, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], d.y, ai, 0, 0, 0);
+                ar = __builtin_amdgcn_mfma_f32_16x16x4f32(npim[ks], d.y, ar, 0, 0, 0);
+                ai = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[ks], d.x, ai, 0, 0, 0);
+            }
+            vr[bk] = ar; vi[bk] = ai;
+        }
+        __syncthreads();
+        if (col_ok) {
+#pragma unroll
+            for (int bk = 0; bk < 2; ++bk) {
+                const int s = (2 * wave + bk) * SPN + so;
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    buf[(s * 16 + 4 * k4 + r) * Sh::R3 + lh] = make_float2(vr[bk][r], vi[bk][r]);
             }
         }
+        __syncthreads();
     }
-    __syncthreads();
-    if (active3) {
-        const int s = pair / LH, lh = pair - (pair / LH) * LH;
-#pragma unroll
-        for (int y = 0; y < 8; ++y) buf[(s * 16 + 8 * yh + y) * Sh::R3 + lh] = V[y];
-    }
-    __syncthreads();
 
-    // phase 4: right binning operator, real part, scale, store
+    // phase 4: right binning operator on the matrix cores, real part, scale, store.
+    // B[Y][X] = sum_lx (Re V[Y][lx] Re Q'[X][lx] - Im V[Y][lx] Im Q'[X][lx]), Q' = Q~ with the
+    // columns m_x > 0 counted twice.  A = V (lane l: row Y = l % 16, lag 4 ks + l / 16), B = Q'^T
+    // (lane l: X = l % 16, same lag), B[Y][X] out in lane X + 16 (Y / 4), register Y % 4, so each
+    // store instruction writes rows of 16 consecutive CCD pixels.
     double scale = 1.0;
     if (g.parseval) {
         const double* am = g.amp + (int64_t)f * g.amp_fstep;
         scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
     }
     const int W = n * g.npx;
-    const int halfw = (g.npx + 1) / 2;
     double bsum = 0.0;
-    for (int task = t; task < TS * g.npx * 2; task += 256) {
-        const int s = task / (2 * g.npx), rem = task - s * 2 * g.npx;
-        const int Y = rem >> 1, hx = rem & 1;
-        const int sub = sub0 + s;
-        if (sub >= g.n_sub) continue;
-        const int i = sub / n, j = sub - (sub / n) * n;
-        const float2* vrow = buf + (s * 16 + Y) * LgsShape<NP, M, TS>::R3;
-        float2 v[LH];
+    {
+        constexpr int KQ = (LH + 3) / 4;
+        constexpr int SPW = TS / 4;                   // subapertures per wave
+        static_assert(TS % 4 == 0, "whole subapertures per wave");
+        const int lane = t & 63, wave = t >> 6;
+        const int n16 = lane & 15, k4 = lane >> 4;
+        float qr[KQ], nqi[KQ];
 #pragma unroll
-        for (int lh = 0; lh < LH; ++lh) v[lh] = vrow[lh];
-        float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx;
-        const int x0 = hx * halfw, x1 = min(g.npx, x0 + halfw);
-        for (int X = x0; X < x1; ++X) {
-            float acc0 = 0.f, acc = 0.f;
-            {
-                const float2 qc = Qc[X * L + NP - 1];                     // m_x = 0: once
-                acc0 = fmaf(v[0].x, qc.x, -v[0].y * qc.y);
-            }
+        for (int ks = 0; ks < KQ; ++ks) {
+            const int lh = 4 * ks + k4;
+            const float2 qc = lh < LH ? Qc[n16 * L + NP - 1 + lh] : make_float2(0.f, 0.f);   // 16 rows
+            const float w = lh == 0 ? 1.f : 2.f;
+            qr[ks] = w * qc.x;
+            nqi[ks] = -w * qc.y;
+        }
 #pragma unroll
-            for (int lh = 1; lh < LH; ++lh) {                             // m_x > 0: twice
-                const float2 qc = Qc[X * L + NP - 1 + lh];
-                acc = fmaf(v[lh].x, qc.x, fmaf(-v[lh].y, qc.y, acc));
+        for (int u = 0; u < SPW; ++u) {
+            const int s = wave * SPW + u;
+            const int sub = sub0 + s;
+            if (sub >= g.n_sub) break;                // wave-uniform
+            const float2* vrow = buf + (s * 16 + n16) * Sh::R3;
+            floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int ks = 0; ks < KQ; ++ks) {
+                const int lh = 4 * ks + k4;
+                const float2 v = lh < LH ? vrow[lh] : make_float2(0.f, 0.f);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(v.x, qr[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(v.y, nqi[ks], acc, 0, 0, 0);
             }
-            acc = fmaf(2.0f, acc, acc0);
-            bsum += (double)acc;
-            dst[X] = (float)((double)acc * scale);
+            if (n16 < g.npx) {
+                const int i = sub / n, j = sub - (sub / n) * n;
+                float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx) * W + (int64_t)j * g.npx + n16;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int Y = 4 * k4 + r;
+                    if (Y < g.npx) {
+                        bsum += (double)acc[r];
+                        dst[(int64_t)Y * W] = (float)((double)acc[r] * scale);
+                    }
+                }
+            }
         }
     }
     if (!g.parseval) {
@@ -1014,7 +1047,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 }
 
 template <int NP, int M, int TS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 sh_lgs_forward_kernel(NgsArgs g) {
     extern __shared__ float4 lds_raw[];
     __shared__ double red[4];
@@ -1663,7 +1696,7 @@ __global__ void batch_stamp_kernel(unsigned long long* __restrict__ ts, int n_se
 // One persistent launch per body type (KIND 0: NGS MFMA np 8 tiles, KIND 1: LGS np 16 tiles), so
 // that each runs with its own register and LDS footprint; blocks stride over the item list.
 template <int KIND>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict__ items, int n_items,
                 unsigned long long* __restrict__ ts) {
     extern __shared__ float4 lds_raw[];

@@ -65,6 +65,30 @@ def test_small_lgs_parity(normalised, cut):
         check_intensity(out[k], ref, 8, 8)
 
 
+
+@pytest.mark.parametrize('ef,npx,fovmul', [(48, 6, 1.6), (96, 12, 2), (96, 8, 1.6), (128, 12, 2)])
+def test_lgs_np_variants_parity(ef, npx, fovmul):
+    """np_sub 6, 12 and 16 (8 x 8 lenslets), npx below 16 and cut or not: the tile shapes
+    (8 or 16 subapertures per block, one or two subapertures per 16-column matrix block)."""
+    np_sub = ef // 8
+    pitch = 0.05
+    turb = 589e-9 / (np_sub * pitch) * O.RAD2ASEC
+    kw = dict(wavelengthInNm=589, subap_wanted_fov=npx * turb * fovmul / 2, sensor_pxscale=turb * fovmul / 2,
+              subap_on_diameter=8, subap_npx=npx, fov_ovs_coeff=1.0)
+    g = sh_geometry(ef, pitch, fov_resolution_arcsec=turb / 4, **kw)
+    go = O.sh_geometry(ef_size=ef, pixel_pitch=pitch, fov_resolution_arcsec=turb / 4, **kw)
+    assert (g.ovs_np_sub, g.subap_npx) == (np_sub, npx)
+    print(f'lgs np={np_sub} N={g.fft_size} cut={g.cutpixels} npx={npx}', flush=True)
+    rng = np.random.default_rng(11)
+    K = O.process_kernels(rng.random((64, g.fft_size, g.fft_size)) ** 6, 8, 8)
+    A = annular_pupil(ef, 0.2)
+    scr = von_karman_opd(2 * ef, pitch, seed=5)
+    frames = np.stack([scr[:ef, k:k + ef] for k in range(2)]).astype(np.float32)
+    out, _ = run_lgs(g, A, frames, K, True)
+    for k in range(2):
+        ref = O.sh_forward(A, frames[k].astype(np.float64), go, kernels=K)
+        check_intensity(out[k], ref, 8, npx)
+
 def test_lgs_40x40_parity():
     wl = workloads.lgs_40x40()
     g = wl.geometry()
