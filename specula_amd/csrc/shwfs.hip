@@ -938,7 +938,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     // A = P~ (16 rows Y x 4 lags per step: lane l holds P~[l % 16][4 ks + l / 16], kept in
     // registers for the tile), B = D (4 lags x 16 columns: SPN = 16 / LH subapertures side by
     // side, lag column m_x = column % LH), V[Y][n] out in lane n + 16 (Y / 4), register Y % 4.
-    // Re V = Re P~ Re D - Im P~ Im D, Im V = Re P~ Im D + Im P~ Re D: four products per K step.
+    // Three real products per K step (Gauss / Karatsuba): T1 = Re P~ Re D, T2 = Im P~ Im D,
+    // T3 = (Re P~ + Im P~)(Re D + Im D); Re V = T1 - T2, Im V = T3 - T1 - T2.
     {
         constexpr int SPN = 16 / LH;                 // subapertures per 16-column block
         constexpr int KS = (L + 3) / 4;              // K steps (lags beyond L: P~ = 0)
@@ -947,12 +948,12 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         const int lane = t & 63, wave = t >> 6;
         const int n16 = lane & 15, k4 = lane >> 4;
         const float2* Pt = reinterpret_cast<const float2*>(g.roww);       // [m_y][16 rows] complex
-        float pr[KS], pim[KS], npim[KS];
+        float pr[KS], pim[KS], psum[KS];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
             const int mi = 4 * ks + k4;
             const float2 pc = mi < L ? Pt[mi * 16 + n16] : make_float2(0.f, 0.f);
-            pr[ks] = pc.x; pim[ks] = pc.y; npim[ks] = -pc.y;
+            pr[ks] = pc.x; pim[ks] = pc.y; psum[ks] = pc.x + pc.y;
         }
         const int so = n16 / LH, lh = n16 - (n16 / LH) * LH;
         const bool col_ok = so < SPN;
@@ -961,16 +962,16 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         for (int bk = 0; bk < 2; ++bk) {
             const int s = (2 * wave + bk) * SPN + so;
             const float2* col = buf + (col_ok ? s * LH + lh : 0) * R2;
-            floatx4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+            floatx4 t1 = {0.f, 0.f, 0.f, 0.f}, t2 = {0.f, 0.f, 0.f, 0.f}, t3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
                 const float2 d = col[4 * ks + k4];
-                ar = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], d.x, ar, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], d.y, ai, 0, 0, 0);
-                ar = __builtin_amdgcn_mfma_f32_16x16x4f32(npim[ks], d.y, ar, 0, 0, 0);
-                ai = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[ks], d.x, ai, 0, 0, 0);
+                t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], d.x, t1, 0, 0, 0);
+                t2 = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[ks], d.y, t2, 0, 0, 0);
+                t3 = __builtin_amdgcn_mfma_f32_16x16x4f32(psum[ks], d.x + d.y, t3, 0, 0, 0);
             }
-            vr[bk] = ar; vi[bk] = ai;
+            vr[bk] = t1 - t2;
+            vi[bk] = t3 - t1 - t2;
         }
         __syncthreads();
         if (col_ok) {
@@ -987,9 +988,9 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 
     // phase 4: right binning operator on the matrix cores, real part, scale, store.
     // B[Y][X] = sum_lx (Re V[Y][lx] Re Q'[X][lx] - Im V[Y][lx] Im Q'[X][lx]), Q' = Q~ with the
-    // columns m_x > 0 counted twice.  A = V (lane l: row Y = l % 16, lag 4 ks + l / 16), B = Q'^T
-    // (lane l: X = l % 16, same lag), B[Y][X] out in lane X + 16 (Y / 4), register Y % 4, so each
-    // store instruction writes rows of 16 consecutive CCD pixels.
+    // columns m_x > 0 counted twice, computed as B^T = Q' V^T: A = Q' (lane l: X = l % 16, lag
+    // 4 ks + l / 16), B = V^T (lane l: row Y = l % 16, same lag), B^T[X][Y] out in lane
+    // Y + 16 (X / 4), register X % 4: each lane holds 4 consecutive pixels of one CCD row.
     double scale = 1.0;
     if (g.parseval) {
         const double* am = g.amp + (int64_t)f * g.amp_fstep;
@@ -1000,6 +1001,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     {
         constexpr int KQ = (LH + 3) / 4;
         constexpr int SPW = TS / 4;                   // subapertures per wave
+        const float fscale = (float)scale;
+        const bool vec4 = g.store_vec && (g.npx & 3) == 0;   // 16-byte aligned CCD rows
         static_assert(TS % 4 == 0, "whole subapertures per wave");
         const int lane = t & 63, wave = t >> 6;
         const int n16 = lane & 15, k4 = lane >> 4;
@@ -1023,19 +1026,21 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             for (int ks = 0; ks < KQ; ++ks) {
                 const int lh = 4 * ks + k4;
                 const float2 v = lh < LH ? vrow[lh] : make_float2(0.f, 0.f);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(v.x, qr[ks], acc, 0, 0, 0);
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(v.y, nqi[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qr[ks], v.x, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(nqi[ks], v.y, acc, 0, 0, 0);
             }
-            if (n16 < g.npx) {
+            const int Y = n16, X0 = 4 * k4;
+            if (Y < g.npx && X0 < g.npx) {
                 const int i = sub / n, j = sub - (sub / n) * n;
-                float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx) * W + (int64_t)j * g.npx + n16;
+                float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X0;
+                if (!g.parseval) bsum += ((double)acc[0] + (double)acc[1]) + ((double)acc[2] + (double)acc[3]);
+                const floatx4 o = acc * fscale;
+                if (vec4) {
+                    *reinterpret_cast<floatx4*>(dst) = o;
+                } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int Y = 4 * k4 + r;
-                    if (Y < g.npx) {
-                        bsum += (double)acc[r];
-                        dst[(int64_t)Y * W] = (float)((double)acc[r] * scale);
-                    }
+                    for (int r = 0; r < 4; ++r)
+                        if (X0 + r < g.npx) dst[r] = o[r];
                 }
             }
         }
