@@ -524,6 +524,17 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
     const int lane = threadIdx.x & 63;
     const int n = g.n_lenses;
 
+    // A subaperture whose pupil samples are all zero has an all-zero image.  A wave without a lit
+    // subaperture skips the transforms and stores zeros; the binning skips dark subapertures, whose
+    // LDS chunk then keeps the (zero) autocorrelation of phase B.
+    bool lit_lane = false;
+#pragma unroll
+    for (int x = 0; x < NP; ++x) lit_lane |= (av[x] != 0.f);
+    const unsigned long long lit = __ballot(lit_lane);        // bit q * NP + y: row y of subaperture q
+    constexpr unsigned long long kSubBits = (NP == 64) ? ~0ull : ((1ull << NP) - 1);
+    const bool dark = lit == 0;
+
+    if (!dark) {
     {   // A: row transforms of the field
         const int q = lane / NP, y = lane - (lane / NP) * NP;
         const double k_rev = g.k * (1.0 / kTwoPi), t_rev = g.tilt * (1.0 / kTwoPi);
@@ -599,6 +610,7 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
         qa_lh = swap_halves(qa_hl);
     }
     auto bin_one = [&](int q) {
+        if (((lit >> (q * NP)) & kSubBits) == 0) return;       // dark subaperture: chunk stays zero
         floatx4* chunk = reinterpret_cast<floatx4*>(sw + q * QS + wc.cword);
         const floatx4 sv = *chunk;
         if constexpr (BF3) {
@@ -636,6 +648,7 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
         for (int q = 0; q < wv.nq; ++q) bin_one(q);
     }
     wave_lds_sync();
+    }   // !dark
 
     // D: stores (+ moments)
     float* ccdf = g.ccd + f * g.ccd_stride;
@@ -657,7 +670,8 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             for (int t = 0; t < NY; ++t) {
                 const int Y = h + YSTEP * t;
                 if (NPX % YSTEP != 0 && Y >= NPX) break;
-                const float4 v = *reinterpret_cast<const float4*>(sw + q * QS + (Y << 4) + ((x4 ^ lds_swz(Y)) << 2));
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (!dark) v = *reinterpret_cast<const float4*>(sw + q * QS + (Y << 4) + ((x4 ^ lds_swz(Y)) << 2));
                 const floatx4 o = {v.x, v.y, v.z, v.w};
                 __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(ccdw + (unsigned)(YSTEP * t * W)));
                 if constexpr (!PARSEVAL) bsum += (double)((v.x + v.y) + (v.z + v.w));
@@ -723,7 +737,7 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             const int q = t2 % wv.nq, Y = t2 / wv.nq;
             int i, j;
             wave_sub_rc(wv, n, q, i, j);
-            const float v = sw[q * QS + (Y << 4) + (((X >> 2) ^ lds_swz(Y)) << 2) + (X & 3)];
+            const float v = dark ? 0.f : sw[q * QS + (Y << 4) + (((X >> 2) ^ lds_swz(Y)) << 2) + (X & 3)];
             if constexpr (!PARSEVAL) bsum += (double)v;
             ccdf[(i * npx + Y) * W + j * npx + X] = v;
         }
@@ -805,17 +819,16 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
     }
 }
 
-// grid = (n_tiles, ceil(n_frames / fpb)); block = 4 independent waves.
+// grid = (n_tiles * WPB, ceil(n_frames / fpb)); block = one wave (the waves share nothing, so a
+// wave of dark subapertures that ends early frees its slot and its LDS for the next one).
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NP == 8 ? 4 : 3, 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP == 8 ? 4 : 3, 8)))
 sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
     using Sh = NgsMfmaShape<NP>;
-    __shared__ float4 lds_all[Sh::LDS_F4];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ float4 lds_wave[Sh::WAVE_F4];
     const int f0 = blockIdx.y * fpb;
     const int f1 = min(n_frames, f0 + fpb);
-    ngs_mfma_frames<NP, PARSEVAL, NPX, COG, BF3>(g, blockIdx.x * Sh::WPB + w, f0, f1,
-                                            reinterpret_cast<float*>(lds_all + w * Sh::WAVE_F4));
+    ngs_mfma_frames<NP, PARSEVAL, NPX, COG, BF3>(g, blockIdx.x, f0, f1, reinterpret_cast<float*>(lds_wave));
 }
 
 // ---------------------------------------------------------------- LGS forward (sh.py:381-394)
@@ -1819,9 +1832,8 @@ int ngs_tile_size(int np) { return 256 / np; }
 
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 int launch_mfma(const NgsArgs& a, int n_tiles, int n_frames, int fpb, hipStream_t st) {
-    dim3 grid(n_tiles, (n_frames + fpb - 1) / fpb);
-    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG, BF3>), grid, dim3(64 * NgsMfmaShape<NP>::WPB), 0,
-                       st, a, n_frames, fpb);
+    dim3 grid(n_tiles * NgsMfmaShape<NP>::WPB, (n_frames + fpb - 1) / fpb);
+    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG, BF3>), grid, dim3(64), 0, st, a, n_frames, fpb);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
