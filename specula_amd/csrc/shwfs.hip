@@ -214,6 +214,7 @@ struct NgsArgs {
     double* bpart;       // [frame][n_parts]: partial sums of unscaled B   (non-Parseval mode)
     const float2* kspec; // LGS: [n_sub][L][L] kernel spectra on the lag grid (L = 2 np - 1)
     const int32_t* slot; // fused CoG: [n_sub] slope index of each subaperture, -1 if not valid
+    const int32_t* order; // NGS MFMA: [n_sub] subaperture at each wave position (nullptr: identity)
     double* mom;         // fused CoG: [frame][n_valid][4] flux, thresholded total, x and y sums
     double* tpart;       // fused CoG: [frame][n_parts][2] per-wave sums of flux and thresholded total
     const float* cogw;   // fused CoG: x weights [npx], then y weights [npx]
@@ -441,14 +442,28 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Subapertures owned by one wave.
+// Subapertures owned by one wave: positions sub0 .. sub0 + nq - 1 of the subaperture order
+// (row-major lenslet index, or the plan's lit-first order for a registered static pupil).
 struct NgsWave {
-    int sub0;     // first subaperture (row-major lenslet index)
+    int sub0;     // first position
     int nq;       // how many of the SW exist
-    int i0, j0;   // lenslet row and column of sub0
+    int i0, j0;   // lenslet row and column of sub0 (identity order)
+    const int32_t* order;
 };
 
+// Lenslet index of the wave's subaperture q (q >= nq: the last one, for address arithmetic only).
+__device__ __forceinline__ int wave_sub(const NgsWave& wv, int q) {
+    const int pos = wv.sub0 + min(q, wv.nq - 1);
+    return wv.order ? wv.order[pos] : pos;
+}
+
 __device__ __forceinline__ void wave_sub_rc(const NgsWave& wv, int n, int q, int& i, int& j) {
+    if (wv.order) {
+        const int s = wave_sub(wv, q);
+        i = s / n;
+        j = s - i * n;
+        return;
+    }
     j = wv.j0 + q;
     i = wv.i0;
     while (j >= n) { j -= n; ++i; }
@@ -461,6 +476,7 @@ __device__ __forceinline__ NgsWave ngs_wave(const NgsArgs& g, int wave_id) {
     wv.nq = max(0, min(NgsMfmaShape<NP>::SW, g.n_sub - wv.sub0));
     wv.i0 = wv.sub0 / g.n_lenses;
     wv.j0 = wv.sub0 - wv.i0 * g.n_lenses;
+    wv.order = g.order;
     return wv;
 }
 
@@ -787,7 +803,7 @@ __device__ __forceinline__ NgsWaveConst<NP, NPX, COG> ngs_wave_const(const NgsAr
         if (lane < NPX) sw[kYwWord + lane] = g.cogw[NPX + lane];
         wave_lds_sync();
         (void)YSTEP; (void)NY; (void)h;
-        if (q < wv.nq) wc.slot = g.slot[wv.sub0 + q];
+        if (q < wv.nq) wc.slot = g.slot[wave_sub(wv, q)];
     }
     return wc;
 }
@@ -1070,6 +1086,47 @@ sh_lgs_forward_kernel(NgsArgs g) {
     extern __shared__ float4 lds_raw[];
     __shared__ double red[4];
     lgs_tile<NP, M, TS>(g, blockIdx.x, blockIdx.y, reinterpret_cast<float2*>(lds_raw), red);
+}
+
+// Subaperture order of a static pupil: subapertures with a nonzero pupil sample first, then the
+// dark ones, each group in lenslet order (stable partition; one block, chunks of 1024).  Waves of
+// the NGS kernels then hold either lit or dark subapertures only, and dark waves skip the
+// transforms.  Any permutation gives the same images: the kernels test darkness themselves.
+__global__ void __launch_bounds__(1024) subap_order_kernel(const float* __restrict__ A, int ef, int n, int np,
+                                                           int n_sub, int32_t* __restrict__ order) {
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    int n_lit = 0;
+    for (int pass = 0; pass < 2; ++pass) {            // 0: count lit subapertures, 1: place them
+        int lit_base = 0, dark_base = 0;
+        for (int c0 = 0; c0 < n_sub; c0 += 1024) {
+            const int sidx = c0 + t;
+            bool lit = false;
+            if (sidx < n_sub) {
+                const int i = sidx / n, j = sidx - (sidx / n) * n;
+                const float* a = A + (int64_t)(i * np) * ef + (int64_t)j * np;
+                for (int y = 0; y < np; ++y)
+                    for (int x = 0; x < np; ++x) lit |= a[(int64_t)y * ef + x] != 0.f;
+            }
+            const unsigned long long b = __ballot(lit);
+            const int in_wave = __popcll(b & ((1ull << lane) - 1ull));
+            if (lane == 0) wsum[wv] = __popcll(b);
+            __syncthreads();
+            int before = 0, total = 0;
+            for (int w = 0; w < 16; ++w) {
+                const int c = wsum[w];
+                before += (w < wv) ? c : 0;
+                total += c;
+            }
+            __syncthreads();
+            const int excl = before + in_wave;                         // lit ones ahead in the chunk
+            if (pass == 1 && sidx < n_sub)
+                order[lit ? lit_base + excl : n_lit + dark_base + (t - excl)] = sidx;
+            lit_base += total;
+            dark_base += min(1024, n_sub - c0) - total;
+        }
+        n_lit = lit_base;
+    }
 }
 
 // Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
@@ -1780,6 +1837,8 @@ struct shwfs_plan {
     double* d_tpart = nullptr;     // [max_frames][n_parts][2] fused CoG frame-total partials
     shslopec_plan* cog = nullptr;  // bound ShSlopec plan of the fused forward (shwfs_plan_bind_slopec)
     int32_t* d_slot = nullptr;     // [n_sub] slope index per subaperture (-1: not valid)
+    int32_t* d_order = nullptr;    // [n_sub] lit-first subaperture order of the static pupil (NGS MFMA)
+    bool order_valid = false;      // d_order describes static_A
     float* d_cogw = nullptr;       // [2 npx] CoG x weights, y weights
     hipEvent_t ev_start = nullptr; // optional timing hook around the main kernel
     hipEvent_t ev_stop = nullptr;
@@ -2030,6 +2089,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
     PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
     PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_parts * sizeof(double));
+    if (!g.lgs && use_mfma) PLAN_ALLOC(p->d_order, (size_t)p->n_sub * sizeof(int32_t));
     if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
 #undef PLAN_ALLOC
     if ((!rl.empty() &&
@@ -2055,6 +2115,7 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_bpart);
     (void)hipFree(p->d_slot);
+    (void)hipFree(p->d_order);
     (void)hipFree(p->d_tpart);
     (void)hipFree(p->d_ext);
     (void)hipFree(p->d_A_rs);
@@ -2182,6 +2243,9 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     a.vec = (g.ef_size % 4 == 0) && aligned16(A) && aligned16(phase) && (a_stride % 4 == 0) &&
             (n_frames == 1 || ph_stride % 4 == 0);
     a.store_vec = aligned16(ccd) && (ccd_side % 4 == 0) && (n_frames == 1 || ccd_stride % 4 == 0);
+    // lit-first order of the registered pupil (Parseval only: the images do not depend on how
+    // subapertures are grouped into waves, the non-Parseval partial sums would)
+    a.order = (cached && p->order_valid && g.parseval) ? p->d_order : nullptr;
     if (cog) {
         a.cog = cog->thr_mode + 1;
         a.thr = cog->thr;
@@ -2230,6 +2294,13 @@ int shwfs_plan_set_pupil(shwfs_plan* p, const float* A, int64_t a_stride, int n_
     DeviceGuard dg(p->device);
     int rc = pupil_stats(p, A, a_stride, amp_frames, S0, reinterpret_cast<hipStream_t>(stream));
     if (rc) return rc;
+    p->order_valid = false;
+    if (p->d_order && a_stride == 0 && !p->ef_in) {
+        hipLaunchKernelGGL(subap_order_kernel, dim3(1), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), A,
+                           g.ef_size, g.n_lenses, g.np_sub, p->n_sub, p->d_order);
+        SHWFS_LAUNCH_CHECK();
+        p->order_valid = true;
+    }
     p->static_A = A;
     p->static_a_stride = a_stride;
     p->static_frames = amp_frames;
