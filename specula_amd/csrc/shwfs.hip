@@ -214,7 +214,7 @@ struct NgsArgs {
     double* bpart;       // [frame][n_parts]: partial sums of unscaled B   (non-Parseval mode)
     const float2* kspec; // LGS: [n_sub][L][L] kernel spectra on the lag grid (L = 2 np - 1)
     const int32_t* slot; // fused CoG: [n_sub] slope index of each subaperture, -1 if not valid
-    const int32_t* order; // NGS MFMA: [n_sub] subaperture at each wave position (nullptr: identity)
+    const int32_t* order; // NGS MFMA, LGS: [n_sub] subaperture at each wave / tile position (nullptr: identity)
     double* mom;         // fused CoG: [frame][n_valid][4] flux, thresholded total, x and y sums
     double* tpart;       // fused CoG: [frame][n_parts][2] per-wave sums of flux and thresholded total
     const float* cogw;   // fused CoG: x weights [npx], then y weights [npx]
@@ -883,22 +883,66 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     const int sub0 = tile * TS;
     const float2* Qc = reinterpret_cast<const float2*>(g.colw);
 
-    // phase 1
-    for (int task = t; task < TS * NP; task += 256) {
-        const int s = task / NP, x = task - (task / NP) * NP;
-        const int sub = sub0 + s;
+    // tile position s -> lenslet index (the plan's lit-first order for a registered static pupil)
+    auto lenslet = [&](int s) { const int pos = sub0 + s; return g.order ? g.order[pos] : pos; };
+
+    // phase 1a: pupil samples of the task's column (one task per thread); a tile without a lit
+    // sample has a zero image and stores it without transforming
+    static_assert(TS * NP <= 256, "one phase-1 task per thread");
+    const int s1 = t / NP, x1 = t - (t / NP) * NP;
+    const bool task1 = t < TS * NP && sub0 + s1 < g.n_sub;
+    int64_t off1 = 0;
+    float av[NP];
+    bool lit = false;
+#pragma unroll
+    for (int y = 0; y < NP; ++y) av[y] = 0.f;
+    if (task1) {
+        const int sub = lenslet(s1);
+        const int i = sub / n, j = sub - (sub / n) * n;
+        off1 = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x1;
+        const float* ar = g.A + f * g.a_stride + off1;
+#pragma unroll
+        for (int y = 0; y < NP; ++y) {
+            av[y] = ar[(int64_t)y * g.ef_size];
+            lit |= av[y] != 0.f;
+        }
+    }
+    if (!__syncthreads_or(lit)) {
+        const int W = n * g.npx;
+        if (g.store_vec && (g.npx & 3) == 0) {
+            const int q4 = g.npx / 4, per = g.npx * q4;
+            for (int e = t; e < TS * per; e += 256) {
+                const int s = e / per, r = e - (e / per) * per;
+                if (sub0 + s >= g.n_sub) break;
+                const int sub = lenslet(s);
+                const int i = sub / n, j = sub - (sub / n) * n, Y = r / q4, X4 = r - (r / q4) * q4;
+                float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + 4 * X4;
+                *reinterpret_cast<floatx4*>(dst) = floatx4{0.f, 0.f, 0.f, 0.f};
+            }
+        } else {
+            const int per = g.npx * g.npx;
+            for (int e = t; e < TS * per; e += 256) {
+                const int s = e / per, r = e - (e / per) * per;
+                if (sub0 + s >= g.n_sub) break;
+                const int sub = lenslet(s);
+                const int i = sub / n, j = sub - (sub / n) * n, Y = r / g.npx, X = r - (r / g.npx) * g.npx;
+                g.ccd[f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X] = 0.f;
+            }
+        }
+        if (!g.parseval && t == 0) g.bpart[(int64_t)f * g.n_parts + tile] = 0.0;   // its share of sum(B)
+        return;
+    }
+
+    // phase 1b: field and column FFT over y
+    if (t < TS * NP) {
         float2 e[NP];
-        if (sub < g.n_sub) {
-            const int i = sub / n, j = sub - (sub / n) * n;
-            const int64_t off = (int64_t)(i * NP) * g.ef_size + (int64_t)j * NP + x;
-            const float* ar = g.A + f * g.a_stride + off;
-            const float* pr = g.phase + f * g.ph_stride + off;
+        if (task1) {
+            const float* pr = g.phase + f * g.ph_stride + off1;
             const double k_rev = g.k * (1.0 / kTwoPi), tilt_rev = g.tilt * (1.0 / kTwoPi);
-            const double ax = g.a0 + x;
+            const double ax = g.a0 + x1;
 #pragma unroll
             for (int y = 0; y < NP; ++y)
-                e[y] = field_sample_rev(ar[(int64_t)y * g.ef_size], pr[(int64_t)y * g.ef_size], k_rev, tilt_rev,
-                                        ax + g.a0 + y);
+                e[y] = field_sample_rev(av[y], pr[(int64_t)y * g.ef_size], k_rev, tilt_rev, ax + g.a0 + y);
         } else {
 #pragma unroll
             for (int y = 0; y < NP; ++y) e[y] = make_float2(0.f, 0.f);
@@ -906,7 +950,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         float2 G[M];
         fft_pruned<M, NP, false>(e, G);
 #pragma unroll
-        for (int ky = 0; ky < M; ++ky) buf[(s * M + ky) * R1 + x] = G[ky];
+        for (int ky = 0; ky < M; ++ky) buf[(s1 * M + ky) * R1 + x1] = G[ky];
     }
     __syncthreads();
 
@@ -950,8 +994,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     // phase 3a: column transforms over ky and the kernel multiply, written back in place
     for (int task = t; task < TS * LH; task += 256) {
         const int s = task / LH, lh = task - (task / LH) * LH;
-        const int sub = sub0 + s;
-        if (sub >= g.n_sub) continue;
+        if (sub0 + s >= g.n_sub) continue;
+        const int sub = lenslet(s);
         float2* col = buf + (s * LH + lh) * R2;
         float2 F[M];
 #pragma unroll
@@ -1047,8 +1091,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 #pragma unroll
         for (int u = 0; u < SPW; ++u) {
             const int s = wave * SPW + u;
-            const int sub = sub0 + s;
-            if (sub >= g.n_sub) break;                // wave-uniform
+            if (sub0 + s >= g.n_sub) break;           // wave-uniform
+            const int sub = lenslet(s);
             const float2* vrow = buf + (s * 16 + n16) * Sh::R3;
             floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -1837,7 +1881,7 @@ struct shwfs_plan {
     double* d_tpart = nullptr;     // [max_frames][n_parts][2] fused CoG frame-total partials
     shslopec_plan* cog = nullptr;  // bound ShSlopec plan of the fused forward (shwfs_plan_bind_slopec)
     int32_t* d_slot = nullptr;     // [n_sub] slope index per subaperture (-1: not valid)
-    int32_t* d_order = nullptr;    // [n_sub] lit-first subaperture order of the static pupil (NGS MFMA)
+    int32_t* d_order = nullptr;    // [n_sub] lit-first subaperture order of the static pupil (NGS MFMA, LGS)
     bool order_valid = false;      // d_order describes static_A
     float* d_cogw = nullptr;       // [2 npx] CoG x weights, y weights
     hipEvent_t ev_start = nullptr; // optional timing hook around the main kernel
@@ -2089,7 +2133,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
     PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
     PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_parts * sizeof(double));
-    if (!g.lgs && use_mfma) PLAN_ALLOC(p->d_order, (size_t)p->n_sub * sizeof(int32_t));
+    if (g.lgs || use_mfma) PLAN_ALLOC(p->d_order, (size_t)p->n_sub * sizeof(int32_t));
     if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
 #undef PLAN_ALLOC
     if ((!rl.empty() &&

@@ -102,6 +102,15 @@ def test_lgs_40x40_parity():
     ref = O.sh_forward(A, frames[0].astype(np.float32).astype(np.float64), oracle_geometry(wl),
                        kernels=kern.kernels)
     check_intensity(out[0], ref, 40, 16)
+    # a registered static pupil regroups the tiles lit-first (1248 lit of 1600 subapertures;
+    # dark tiles store zeros): bit-identical image
+    At = torch.tensor(A, dtype=torch.float32, device=DEV)
+    ph = torch.tensor(np.asarray(frames, np.float32), device=DEV)
+    plan.set_static_pupil(At, 1.0)
+    out2 = torch.empty((1, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan.forward(At, ph, out2, 1.0, n_frames=1)
+    torch.cuda.synchronize()
+    assert np.array_equal(out2.cpu().numpy(), out)
 
 
 def test_sh_object_gaussian_lgs_and_sodium_inputs():
