@@ -42,9 +42,9 @@ def parse():
                     help='also run two 256 MiB device copies (reference traffic for rocprofv3 counters)')
     ap.add_argument('--extras', default='all',
                     help="extra measurements: 'all' (or 1), 'none' (or 0), or a comma list of "
-                         "bf16, ccd, lgs, im, multi")
+                         "bf16, ccd, lgs, im, multi, hbm")
     args = ap.parse_args()
-    names = ('bf16', 'ccd', 'lgs', 'im', 'multi')
+    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm')
     e = str(args.extras).strip().lower()
     args.extras = set(names) if e in ('all', '1') else set() if e in ('none', '0', '') else \
         {x.strip() for x in e.split(',') if x.strip()}
@@ -203,6 +203,35 @@ def bench_multi(torch, dev, reps=20):
                          'candidates_ms': {f"{c['grid']}/{c['order']}": round(c['seconds'] * 1e3, 3)
                                            for c in rep['candidates']}},
             'forward_completion_us': per}
+
+
+def bench_hbm(torch, dev):
+    """Practical HBM ceilings of this GPU, for reading the roofline fraction: a 1 GiB write
+    (torch fill) and a 1 GiB device-to-device copy (read + write), median of 20 timed runs."""
+    n = 256 * 1024 * 1024
+    a = torch.empty(n, dtype=torch.float32, device=dev)
+    b = torch.empty(n, dtype=torch.float32, device=dev)
+    a.fill_(1.0)
+    b.fill_(2.0)
+
+    def med(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(20):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) * 1e-3)
+        return float(np.median(ts))
+    w = med(lambda: a.fill_(0.5))
+    c = med(lambda: b.copy_(a))
+    del a, b
+    return {'write_GBps': round(4 * n / w / 1e9, 1), 'copy_GBps': round(8 * n / c / 1e9, 1),
+            'bytes': 4 * n, 'note': 'copy counts read + write bytes'}
 
 
 def bench_ccd(torch, dev, plan, sl_plan, At, ph, ccd, slopes, flux, totals, F, g):
@@ -384,6 +413,9 @@ def main():
     if 'multi' in args.extras:
         progress('multi-sensor batch')
         result['multi_sensor'] = bench_multi(torch, dev)
+    if 'hbm' in args.extras:
+        progress('hbm ceilings')
+        result['hbm_ceiling'] = bench_hbm(torch, dev)
     if args.cpu_seconds > 0:
         progress('cpu baseline')
         result['cpu_baseline'] = cpu_baseline(wl, A, frames, idxs, args.cpu_seconds)
