@@ -482,33 +482,43 @@ __device__ __forceinline__ NgsWave ngs_wave(const NgsArgs& g, int wave_id) {
 
 // Pupil samples of lane (q, y): row y of subaperture q (A and phase).  Kept separate from the
 // transform so that a caller can issue the next frame's loads before transforming this one.
-// `off`: pupil offset of this lane's row segment, -1 for lanes without a subaperture.  `fs`: the
-// frame's flux scale (Parseval mode), loaded with the samples so that its latency is hidden too.
-template <int NP, bool PARSEVAL>
-__device__ __forceinline__ void ngs_mfma_load(const NgsArgs& g, int off, int f, float (&av)[NP], float (&pv)[NP],
-                                              float& fs) {
-    fs = PARSEVAL ? g.fscale[f * g.scale_fstep] : 1.f;
+// `off`: pupil offset of this lane's row segment, -1 for lanes without a subaperture.
+template <int NP>
+__device__ __forceinline__ void ngs_row_load(const float* base, int off, bool vec, float (&v)[NP]) {
     if (off >= 0) {
-        const float* ar = (g.A + f * g.a_stride) + (unsigned)off;
-        const float* pr = (g.phase + f * g.ph_stride) + (unsigned)off;
+        const float* r = base + (unsigned)off;
         if constexpr (NP % 4 == 0) {
-            if (g.vec) {
+            if (vec) {
 #pragma unroll
                 for (int c = 0; c < NP / 4; ++c) {
-                    const float4 a4 = reinterpret_cast<const float4*>(ar)[c];
-                    const float4 p4 = reinterpret_cast<const float4*>(pr)[c];
-                    av[4 * c] = a4.x; av[4 * c + 1] = a4.y; av[4 * c + 2] = a4.z; av[4 * c + 3] = a4.w;
-                    pv[4 * c] = p4.x; pv[4 * c + 1] = p4.y; pv[4 * c + 2] = p4.z; pv[4 * c + 3] = p4.w;
+                    const float4 q = reinterpret_cast<const float4*>(r)[c];
+                    v[4 * c] = q.x; v[4 * c + 1] = q.y; v[4 * c + 2] = q.z; v[4 * c + 3] = q.w;
                 }
                 return;
             }
         }
 #pragma unroll
-        for (int x = 0; x < NP; ++x) { av[x] = ar[x]; pv[x] = pr[x]; }
+        for (int x = 0; x < NP; ++x) v[x] = r[x];
     } else {
 #pragma unroll
-        for (int x = 0; x < NP; ++x) { av[x] = 0.f; pv[x] = 0.f; }
+        for (int x = 0; x < NP; ++x) v[x] = 0.f;
     }
+}
+
+// Phase row of frame f (only where the lane's pupil row has a nonzero sample: elsewhere the field
+// is zero whatever the phase) and the frame's flux scale (Parseval mode), loaded together so that
+// their latency is hidden together.
+template <int NP, bool PARSEVAL>
+__device__ __forceinline__ void ngs_phase_load(const NgsArgs& g, int off, int f, bool lit, float (&pv)[NP],
+                                               float& fs) {
+    fs = PARSEVAL ? g.fscale[f * g.scale_fstep] : 1.f;
+    ngs_row_load<NP>(g.phase + f * g.ph_stride, lit ? off : -1, g.vec, pv);
+}
+
+__device__ __forceinline__ bool any_nonzero_row(const float* v, int n) {
+    bool lit = false;
+    for (int x = 0; x < n; ++x) lit |= v[x] != 0.f;
+    return lit;
 }
 
 // Per-wave constants that do not change from frame to frame.
@@ -824,9 +834,16 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
     }
     const NgsWaveConst<NP, NPX, COG> wc = ngs_wave_const<NP, NPX, COG>(g, wv, sw);
     float av[NP], pv[NP], an[NP], pn[NP], fs, fn = 1.f;
-    ngs_mfma_load<NP, PARSEVAL>(g, wc.ld_off, f0, av, pv, fs);
+    ngs_row_load<NP>(g.A + f0 * g.a_stride, wc.ld_off, g.vec, av);
+    // static pupil (a_stride 0): a lane whose pupil row has no nonzero sample never loads its
+    // phase (the field there is zero whatever the phase); a per-frame pupil loads every row
+    const bool lit = g.a_stride != 0 || any_nonzero_row(av, NP);
+    ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f0, lit, pv, fs);
     for (int f = f0; f < f1; ++f) {
-        if (f + 1 < f1) ngs_mfma_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, an, pn, fn);
+        if (f + 1 < f1) {
+            ngs_row_load<NP>(g.A + (f + 1) * g.a_stride, wc.ld_off, g.vec, an);
+            ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, lit, pn, fn);
+        }
         if (f > f0) wave_lds_sync();
         ngs_mfma_frame<NP, PARSEVAL, NPX, COG, BF3>(g, wv, wave_id, f, av, pv, fs, sw, wc);
 #pragma unroll
@@ -838,7 +855,8 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
 // grid = (n_tiles * WPB, ceil(n_frames / fpb)); block = one wave (the waves share nothing, so a
 // wave of dark subapertures that ends early frees its slot and its LDS for the next one).
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NP == 8 ? 4 : 3, 8)))
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(NP == 8 && !(BF3 && (COG == 1 || COG == 2)) ? 4 : 3, 8)))
 sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
     using Sh = NgsMfmaShape<NP>;
     __shared__ float4 lds_wave[Sh::WAVE_F4];
@@ -1762,11 +1780,15 @@ im_gather_lds_kernel(const float* __restrict__ grid, const int32_t* __restrict__
         for (int e = threadIdx.x; e < n4; e += 4 * blockDim.x) {
             float4 v[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (e + u * (int)blockDim.x < n4) v[u] = g4[e + u * blockDim.x];
+            for (int u = 0; u < 4; ++u) {
+                const int k = e + u * (int)blockDim.x;
+                v[u] = k < n4 ? g4[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (e + u * (int)blockDim.x < n4) l4[e + u * blockDim.x] = v[u];
+            for (int u = 0; u < 4; ++u) {
+                const int k = e + u * (int)blockDim.x;
+                if (k < n4) l4[k] = v[u];
+            }
         }
     } else {
         for (int e = threadIdx.x; e < 2 * nn; e += blockDim.x) row_l[e] = gx[e];
