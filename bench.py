@@ -21,9 +21,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the main kernel, measured with rocprofv3 --pmc FETCH_SIZE and then
 # WRITE_SIZE (KiB per dispatch) and scaled by what the same counters read for a 256 MiB device copy
 # (fetch x2.0, write x1.0); command and raw figures: profiles/pmc_traffic_elt_ngs_80x80.md.
-# Keyed by (CoG fused, frames per launch, default frames per block).  The static pupil is fetched
-# about once per launch, so the traffic is below the algorithmic bytes (pupil counted per frame).
-MEASURED_TRAFFIC = {(True, 100, True): 837_200_000}
+# Keyed by (CoG fused, frames per launch, default frames per block).  The static pupil is served
+# from L2 after its first read and phase rows under a dark pupil row are not read, so the traffic
+# is below the algorithmic bytes (pupil and phase counted in full per frame).
+MEASURED_TRAFFIC = {(True, 100, True): 834_600_000}
 FP32_PEAK_TFLOPS = 157.3
 
 
