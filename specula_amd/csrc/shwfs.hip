@@ -2792,6 +2792,7 @@ struct shwfs_batch {
     double* d_amp_part = nullptr;
     unsigned* d_counter = nullptr;
     unsigned long long* d_ts = nullptr;   // [1 + n_sensors]
+    int32_t* d_order = nullptr;       // lit-first subaperture order of each sensor's pupil, concatenated
 };
 
 extern "C" {
@@ -2806,6 +2807,7 @@ int shwfs_batch_destroy(shwfs_batch* b) {
     (void)hipFree(b->d_amp_part);
     (void)hipFree(b->d_counter);
     (void)hipFree(b->d_ts);
+    (void)hipFree(b->d_order);
     delete b;
     return SHWFS_OK;
 }
@@ -2889,8 +2891,20 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         (e = hipMalloc((void**)&b->d_ts, (n_sensors + 1) * sizeof(unsigned long long))) != hipSuccess ||
         (e = hipMemset(b->d_counter, 0, n_sensors * sizeof(unsigned))) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
-    // the pupils are static for the batch: amplitude statistics once, here
+    size_t n_order = 0;
+    for (int s = 0; s < n_sensors; ++s) n_order += (size_t)plans[s]->n_sub;
+    if ((e = hipMalloc((void**)&b->d_order, n_order * sizeof(int32_t))) != hipSuccess)
+        return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
+    // the pupils are static for the batch: amplitude statistics and the lit-first subaperture
+    // order (Parseval sensors: their images do not depend on the grouping) once, here
+    size_t order_off = 0;
     for (int s = 0; s < n_sensors; ++s) {
+        const shwfs_geom& gs = plans[s]->g;
+        int32_t* ord = b->d_order + order_off;
+        order_off += (size_t)plans[s]->n_sub;
+        hipLaunchKernelGGL(subap_order_kernel, dim3(1), dim3(1024), 0, 0, A[s], gs.ef_size, gs.n_lenses, gs.np_sub,
+                           plans[s]->n_sub, ord);
+        args[s].order = args[s].parseval ? ord : nullptr;
         const int64_t px = (int64_t)plans[s]->g.ef_size * plans[s]->g.ef_size;
         hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, px,
                            b->d_amp_part + (size_t)s * amp_blocks * 2, b->d_counter + s, b->d_amp + 2 * s,
