@@ -1150,12 +1150,27 @@ sh_lgs_forward_kernel(NgsArgs g) {
     lgs_tile<NP, M, TS>(g, blockIdx.x, blockIdx.y, reinterpret_cast<float2*>(lds_raw), red);
 }
 
-// Subaperture order of a static pupil: subapertures with a nonzero pupil sample first, then the
-// dark ones, each group in lenslet order (stable partition; one block, chunks of 1024).  Waves of
-// the NGS kernels then hold either lit or dark subapertures only, and dark waves skip the
-// transforms.  Any permutation gives the same images: the kernels test darkness themselves.
-__global__ void __launch_bounds__(1024) subap_order_kernel(const float* __restrict__ A, int ef, int n, int np,
-                                                           int n_sub, int32_t* __restrict__ order) {
+// Lit flag of each subaperture (a nonzero pupil sample): one wave per subaperture, a ballot over
+// its np x np samples.
+__global__ void __launch_bounds__(256) subap_lit_kernel(const float* __restrict__ A, int ef, int n, int np, int n_sub,
+                                                        uint8_t* __restrict__ lit) {
+    const int sidx = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (sidx >= n_sub) return;                         // wave-uniform
+    const int i = sidx / n, j = sidx - (sidx / n) * n;
+    const float* a = A + (int64_t)(i * np) * ef + (int64_t)j * np;
+    bool any = false;
+    for (int e = lane; e < np * np; e += 64) any |= a[(int64_t)(e / np) * ef + (e % np)] != 0.f;
+    const unsigned long long b = __ballot(any);
+    if (lane == 0) lit[sidx] = b != 0 ? 1 : 0;
+}
+
+// Subaperture order of a static pupil: lit subapertures first, then the dark ones, each group in
+// lenslet order (stable partition of the flags; one block, chunks of 1024).  Waves of the NGS
+// kernels (tiles of the LGS kernel) then hold either lit or dark subapertures only, and dark ones
+// skip the transforms.  Any permutation gives the same images: the kernels test darkness
+// themselves.
+__global__ void __launch_bounds__(1024) subap_order_kernel(const uint8_t* __restrict__ lit_flags, int n_sub,
+                                                           int32_t* __restrict__ order) {
     __shared__ int wsum[16];
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
     int n_lit = 0;
@@ -1163,13 +1178,7 @@ __global__ void __launch_bounds__(1024) subap_order_kernel(const float* __restri
         int lit_base = 0, dark_base = 0;
         for (int c0 = 0; c0 < n_sub; c0 += 1024) {
             const int sidx = c0 + t;
-            bool lit = false;
-            if (sidx < n_sub) {
-                const int i = sidx / n, j = sidx - (sidx / n) * n;
-                const float* a = A + (int64_t)(i * np) * ef + (int64_t)j * np;
-                for (int y = 0; y < np; ++y)
-                    for (int x = 0; x < np; ++x) lit |= a[(int64_t)y * ef + x] != 0.f;
-            }
+            const bool lit = sidx < n_sub && lit_flags[sidx] != 0;
             const unsigned long long b = __ballot(lit);
             const int in_wave = __popcll(b & ((1ull << lane) - 1ull));
             if (lane == 0) wsum[wv] = __popcll(b);
@@ -1189,6 +1198,17 @@ __global__ void __launch_bounds__(1024) subap_order_kernel(const float* __restri
         }
         n_lit = lit_base;
     }
+}
+
+// Lit-first order of pupil A (ef x ef) for n x n subapertures of np x np samples; `lit` holds
+// n_sub bytes of scratch.
+int launch_subap_order(const float* A, int ef, int n, int np, int n_sub, uint8_t* lit, int32_t* order,
+                       hipStream_t st) {
+    hipLaunchKernelGGL(subap_lit_kernel, dim3((n_sub + 3) / 4), dim3(256), 0, st, A, ef, n, np, n_sub, lit);
+    SHWFS_LAUNCH_CHECK();
+    hipLaunchKernelGGL(subap_order_kernel, dim3(1), dim3(1024), 0, st, lit, n_sub, order);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
 }
 
 // Non-Parseval normalisation: scale = S0 * pixel_area * sum(A) / sum(B), then rescale in place.
@@ -1904,6 +1924,7 @@ struct shwfs_plan {
     shslopec_plan* cog = nullptr;  // bound ShSlopec plan of the fused forward (shwfs_plan_bind_slopec)
     int32_t* d_slot = nullptr;     // [n_sub] slope index per subaperture (-1: not valid)
     int32_t* d_order = nullptr;    // [n_sub] lit-first subaperture order of the static pupil (NGS MFMA, LGS)
+    uint8_t* d_lit = nullptr;      // [n_sub] lit flags (scratch of the order computation)
     bool order_valid = false;      // d_order describes static_A
     float* d_cogw = nullptr;       // [2 npx] CoG x weights, y weights
     hipEvent_t ev_start = nullptr; // optional timing hook around the main kernel
@@ -2155,7 +2176,10 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
     PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
     PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_parts * sizeof(double));
-    if (g.lgs || use_mfma) PLAN_ALLOC(p->d_order, (size_t)p->n_sub * sizeof(int32_t));
+    if (g.lgs || use_mfma) {
+        PLAN_ALLOC(p->d_order, (size_t)p->n_sub * sizeof(int32_t));
+        PLAN_ALLOC(p->d_lit, (size_t)p->n_sub);
+    }
     if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
 #undef PLAN_ALLOC
     if ((!rl.empty() &&
@@ -2182,6 +2206,7 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_bpart);
     (void)hipFree(p->d_slot);
     (void)hipFree(p->d_order);
+    (void)hipFree(p->d_lit);
     (void)hipFree(p->d_tpart);
     (void)hipFree(p->d_ext);
     (void)hipFree(p->d_A_rs);
@@ -2362,9 +2387,9 @@ int shwfs_plan_set_pupil(shwfs_plan* p, const float* A, int64_t a_stride, int n_
     if (rc) return rc;
     p->order_valid = false;
     if (p->d_order && a_stride == 0 && !p->ef_in) {
-        hipLaunchKernelGGL(subap_order_kernel, dim3(1), dim3(1024), 0, reinterpret_cast<hipStream_t>(stream), A,
-                           g.ef_size, g.n_lenses, g.np_sub, p->n_sub, p->d_order);
-        SHWFS_LAUNCH_CHECK();
+        rc = launch_subap_order(A, g.ef_size, g.n_lenses, g.np_sub, p->n_sub, p->d_lit, p->d_order,
+                                reinterpret_cast<hipStream_t>(stream));
+        if (rc) return rc;
         p->order_valid = true;
     }
     p->static_A = A;
@@ -2793,6 +2818,7 @@ struct shwfs_batch {
     unsigned* d_counter = nullptr;
     unsigned long long* d_ts = nullptr;   // [1 + n_sensors]
     int32_t* d_order = nullptr;       // lit-first subaperture order of each sensor's pupil, concatenated
+    uint8_t* d_lit = nullptr;         // lit flags (scratch of the order computation)
 };
 
 extern "C" {
@@ -2808,6 +2834,7 @@ int shwfs_batch_destroy(shwfs_batch* b) {
     (void)hipFree(b->d_counter);
     (void)hipFree(b->d_ts);
     (void)hipFree(b->d_order);
+    (void)hipFree(b->d_lit);
     delete b;
     return SHWFS_OK;
 }
@@ -2893,7 +2920,8 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
     size_t n_order = 0;
     for (int s = 0; s < n_sensors; ++s) n_order += (size_t)plans[s]->n_sub;
-    if ((e = hipMalloc((void**)&b->d_order, n_order * sizeof(int32_t))) != hipSuccess)
+    if ((e = hipMalloc((void**)&b->d_order, n_order * sizeof(int32_t))) != hipSuccess ||
+        (e = hipMalloc((void**)&b->d_lit, n_order)) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
     // the pupils are static for the batch: amplitude statistics and the lit-first subaperture
     // order (Parseval sensors: their images do not depend on the grouping) once, here
@@ -2901,9 +2929,10 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     for (int s = 0; s < n_sensors; ++s) {
         const shwfs_geom& gs = plans[s]->g;
         int32_t* ord = b->d_order + order_off;
+        const int rco = launch_subap_order(A[s], gs.ef_size, gs.n_lenses, gs.np_sub, plans[s]->n_sub,
+                                           b->d_lit + order_off, ord, 0);
+        if (rco) return cleanup(rco);
         order_off += (size_t)plans[s]->n_sub;
-        hipLaunchKernelGGL(subap_order_kernel, dim3(1), dim3(1024), 0, 0, A[s], gs.ef_size, gs.n_lenses, gs.np_sub,
-                           plans[s]->n_sub, ord);
         args[s].order = args[s].parseval ? ord : nullptr;
         const int64_t px = (int64_t)plans[s]->g.ef_size * plans[s]->g.ef_size;
         hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, px,
