@@ -84,9 +84,12 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
 /* Register a pupil amplitude that does not change between calls: its statistics (sum(A),
  * sum(A^2) and the Parseval flux scale for S0) are computed once here, on `stream`, and later
  * shwfs_forward / shwfs_forward_slopes / shwfs_vjp calls with the same A pointer, frame stride and
- * S0 reuse them instead of recomputing them per call.  The caller must register A again after
- * changing its contents; A = NULL forgets the registration.  No reference counterpart: SH
- * recomputes the masked area from in_ef.A on every trigger (processing_objects/sh.py:422-428). */
+ * S0 reuse them instead of recomputing them per call.  For a single-frame pupil (frame stride 0,
+ * no resampling) it also builds the lit-first subaperture order used by the Parseval NGS and LGS
+ * kernels (subapertures with a nonzero pupil sample first): a performance hint only, the images
+ * are the same with or without it.  The caller must register A again after changing its
+ * contents; A = NULL forgets the registration.  No reference counterpart: SH recomputes the masked
+ * area from in_ef.A on every trigger (processing_objects/sh.py:422-428). */
 int shwfs_plan_set_pupil(shwfs_plan* plan, const float* A, int64_t a_frame_stride, int n_frames,
                          double S0, void* stream);
 
