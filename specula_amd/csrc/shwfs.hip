@@ -559,6 +559,9 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
     const unsigned long long lit = __ballot(lit_lane);        // bit q * NP + y: row y of subaperture q
     constexpr unsigned long long kSubBits = (NP == 64) ? ~0ull : ((1ull << NP) - 1);
     const bool dark = lit == 0;
+    // what a dark wave stores: 0 times the frame's scale, so that a frame without any light gives
+    // the reference's 0/0 (NaN) normalisation exactly as the full transform would
+    const float dark_value = PARSEVAL ? 0.f * fs : 0.f;
 
     if (!dark) {
     {   // A: row transforms of the field
@@ -696,7 +699,7 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             for (int t = 0; t < NY; ++t) {
                 const int Y = h + YSTEP * t;
                 if (NPX % YSTEP != 0 && Y >= NPX) break;
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                float4 v = make_float4(dark_value, dark_value, dark_value, dark_value);
                 if (!dark) v = *reinterpret_cast<const float4*>(sw + q * QS + (Y << 4) + ((x4 ^ lds_swz(Y)) << 2));
                 const floatx4 o = {v.x, v.y, v.z, v.w};
                 __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(ccdw + (unsigned)(YSTEP * t * W)));
@@ -763,7 +766,7 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             const int q = t2 % wv.nq, Y = t2 / wv.nq;
             int i, j;
             wave_sub_rc(wv, n, q, i, j);
-            const float v = dark ? 0.f : sw[q * QS + (Y << 4) + (((X >> 2) ^ lds_swz(Y)) << 2) + (X & 3)];
+            const float v = dark ? dark_value : sw[q * QS + (Y << 4) + (((X >> 2) ^ lds_swz(Y)) << 2) + (X & 3)];
             if constexpr (!PARSEVAL) bsum += (double)v;
             ccdf[(i * npx + Y) * W + j * npx + X] = v;
         }
@@ -925,7 +928,15 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             lit |= av[y] != 0.f;
         }
     }
+    double scale = 1.0;                               // Parseval flux scale of the frame
+    if (g.parseval) {
+        const double* am = g.amp + (int64_t)f * g.amp_fstep;
+        scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
+    }
     if (!__syncthreads_or(lit)) {
+        // 0 times the scale: a frame without any light gives the reference's 0/0 (NaN) as the
+        // full transform would
+        const float z = (float)(0.0 * scale);
         const int W = n * g.npx;
         if (g.store_vec && (g.npx & 3) == 0) {
             const int q4 = g.npx / 4, per = g.npx * q4;
@@ -935,7 +946,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                 const int sub = lenslet(s);
                 const int i = sub / n, j = sub - (sub / n) * n, Y = r / q4, X4 = r - (r / q4) * q4;
                 float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + 4 * X4;
-                *reinterpret_cast<floatx4*>(dst) = floatx4{0.f, 0.f, 0.f, 0.f};
+                *reinterpret_cast<floatx4*>(dst) = floatx4{z, z, z, z};
             }
         } else {
             const int per = g.npx * g.npx;
@@ -944,7 +955,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                 if (sub0 + s >= g.n_sub) break;
                 const int sub = lenslet(s);
                 const int i = sub / n, j = sub - (sub / n) * n, Y = r / g.npx, X = r - (r / g.npx) * g.npx;
-                g.ccd[f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X] = 0.f;
+                g.ccd[f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X] = z;
             }
         }
         if (!g.parseval && t == 0) g.bpart[(int64_t)f * g.n_parts + tile] = 0.0;   // its share of sum(B)
@@ -1082,11 +1093,6 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     // columns m_x > 0 counted twice, computed as B^T = Q' V^T: A = Q' (lane l: X = l % 16, lag
     // 4 ks + l / 16), B = V^T (lane l: row Y = l % 16, same lag), B^T[X][Y] out in lane
     // Y + 16 (X / 4), register X % 4: each lane holds 4 consecutive pixels of one CCD row.
-    double scale = 1.0;
-    if (g.parseval) {
-        const double* am = g.amp + (int64_t)f * g.amp_fstep;
-        scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
-    }
     const int W = n * g.npx;
     double bsum = 0.0;
     {

@@ -202,6 +202,36 @@ def test_streaming_frames_per_block_is_exact(fpb):
     assert torch.equal(out, ref)
 
 
+
+def test_per_frame_pupils_with_dark_subapertures():
+    """One pupil per frame (a_frame_stride = ef^2): the darkness of each wave is decided per frame
+    from that frame's pupil (whole waves and single subapertures go dark in frame 1, all of them in
+    frame 2), and each frame has its own flux normalisation."""
+    wl = workloads.scao_ngs_20x20()
+    g = wl.geometry()
+    go = oracle_geometry(wl)
+    A = wl.pupil().astype(np.float32)
+    A1 = A.copy()
+    A1[: g.ef_size // 2] = 0.0                                    # top half dark
+    A1[:, 5 * 8: 6 * 8 + 3] = 0.0                                 # a column of subapertures (and a part)
+    A2 = np.zeros_like(A)
+    pupils = np.stack([A, A1, A2])
+    frames = wl.frames(3).astype(np.float32)
+    plan = SHPlan(g, max_frames=3, device=0)
+    out = torch.empty((3, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan.forward(torch.tensor(pupils, device=DEV), torch.tensor(frames, device=DEV), out, 1.0, n_frames=3)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy()
+    for k in range(2):
+        ref = O.sh_forward(pupils[k].astype(np.float64), frames[k].astype(np.float64), go)
+        check_intensity(img[k], ref, g.n_lenses, g.subap_npx)
+    lit = (pupils[1].reshape(20, 8, 20, 8) != 0).any(3).any(1)
+    tiles = img[1].reshape(20, g.subap_npx, 20, g.subap_npx)
+    assert np.all(tiles.transpose(0, 2, 1, 3)[~lit] == 0)
+    # no light at all: the reference's normalisation is 0/0, NaN everywhere (the oracle says so too)
+    ref2 = O.sh_forward(pupils[2].astype(np.float64), frames[2].astype(np.float64), go)
+    assert np.all(np.isnan(ref2)) and np.all(np.isnan(img[2]))
+
 def test_static_pupil_registration():
     """shwfs_plan_set_pupil: registered statistics give the same image as per-call statistics;
     a different amplitude (other storage or re-registered contents) is not served from it."""
