@@ -43,9 +43,9 @@ def parse():
                     help='also run two 256 MiB device copies (reference traffic for rocprofv3 counters)')
     ap.add_argument('--extras', default='all',
                     help="extra measurements: 'all' (or 1), 'none' (or 0), or a comma list of "
-                         "bf16, ccd, lgs, im, multi, hbm")
+                         "bf16, ccd, lgs, im, multi, hbm, variants")
     args = ap.parse_args()
-    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm')
+    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm', 'variants')
     e = str(args.extras).strip().lower()
     args.extras = set(names) if e in ('all', '1') else set() if e in ('none', '0', '') else \
         {x.strip() for x in e.split(',') if x.strip()}
@@ -204,6 +204,17 @@ def bench_multi(torch, dev, reps=20):
                          'candidates_ms': {f"{c['grid']}/{c['order']}": round(c['seconds'] * 1e3, 3)
                                            for c in rep['candidates']}},
             'forward_completion_us': per}
+
+
+def bench_variants(torch, dev):
+    """The plan's binning variants on elt_ngs_80x80 as the autotuner times them (100 frames, flat
+    phase, per-call pupil statistics): register-GEMV binning on the VALU after the FFTs, FP32
+    MFMA binning, bf16-split MFMA binning."""
+    from specula_amd import autotune, workloads
+    wl = workloads.elt_ngs_80x80()
+    best, times = autotune.tune_plan_variant(wl.geometry(), wl.pupil(), n_frames=100, use_cache=False)
+    names = {1: 'register_gemv_valu', 2: 'fp32_mfma', 3: 'bf16_split_mfma'}
+    return {'best': names[best], 'us_per_frame': {names[v]: round(t * 1e6, 3) for v, t in times.items()}}
 
 
 def bench_hbm(torch, dev):
@@ -414,6 +425,9 @@ def main():
     if 'multi' in args.extras:
         progress('multi-sensor batch')
         result['multi_sensor'] = bench_multi(torch, dev)
+    if 'variants' in args.extras:
+        progress('binning variants')
+        result['binning_variants'] = bench_variants(torch, dev)
     if 'hbm' in args.extras:
         progress('hbm ceilings')
         result['hbm_ceiling'] = bench_hbm(torch, dev)
