@@ -129,6 +129,24 @@ def test_counter_reproducibility_and_batch_independence():
     assert np.mean(a[0] == a[1]) < 0.2                    # frames of a batch draw different numbers
 
 
+
+def test_lean_and_full_photon_kernels_agree():
+    """The common-case photon kernel and the full one (forced here by unit pixel gains, which
+    also carries binning, CIC and EM gain) draw from the same generator keyed by (pixel, counter,
+    try): the same values, bit for bit, on a ragged frame mixing dark pixels, CDF inversion
+    (signal below 10) and PTRS."""
+    rng = np.random.default_rng(5)
+    rows, cols = 97, 211                                      # 20467 px: ragged last chunk
+    lam = rng.gamma(0.6, 60.0, size=(2, rows, cols)).astype(np.float32)
+    lam[:, :, :40] = 0.0                                      # dark columns
+    lam[:, 10:30] *= 0.05                                     # many pixels below 10
+    p = params(rows, cols, photon_noise=True, readout_noise=True, readout_level=1.5, quantum_eff=0.8)
+    lean = run(p, lam, counter=3)
+    full = run(p, lam, counter=3, gains=torch.ones((rows, cols), dtype=torch.float32, device=DEV))
+    frac = [np.mean(lam == 0), np.mean((lam > 0) & (lam * 0.8 < 10)), np.mean(lam * 0.8 >= 10)]
+    assert min(frac) > 0.05, frac
+    np.testing.assert_array_equal(lean, full)
+
 def test_ccd_object_integrates_over_dt_and_resets():
     from specula_amd.data_objects.intensity import Intensity
     from specula_amd.data_objects.simul_params import SimulParams
