@@ -138,8 +138,12 @@ def bench_lgs(torch, dev):
     plan.set_static_pupil(A, 1.0)      # as the SH object does for an unchanged telescope pupil
     dt = timed(lambda: plan.forward(A, ph, out, 1.0, n_frames=F), 10, torch.cuda.synchronize)
     kbytes = g.n_lenses ** 2 * (2 * g.ovs_np_sub - 1) * g.ovs_np_sub * 8     # Hermitian half of the lag grid read
+    frame_bytes = 4 * (2 * g.ef_size ** 2 + g.ccd_side ** 2) + kbytes          # pupil + phase + CCD + spectra
+    gbps = frame_bytes * F / dt / 1e9
     return {'workload': 'lgs_40x40', 'frames_per_s': round(F / dt, 1), 'ms_per_frame': round(dt * 1e3 / F, 5),
-            'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
+            'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'algorithmic_MB_per_frame': round(frame_bytes / 1e6, 2),
+            'algorithmic_GBps': round(gbps, 1), 'hbm_frac': round(gbps / HBM_PEAK_GBS, 4),
+            'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
 
 
 def bench_im(torch, dev, n_modes=1000):
