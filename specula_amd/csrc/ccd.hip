@@ -107,14 +107,6 @@ __constant__ double kLogFact[32] = {
     54.78472939811232, 58.00360522298052, 61.26170176100201, 64.55753862700634,
     67.88974313718153, 71.257038967168, 74.65823634883017, 78.0922235533153};
 
-// ln(k!) = lnGamma(k + 1): table below 32, Stirling's series (to x^-7) above (error < 1e-16).
-__device__ __forceinline__ double log_factorial(double k) {
-    if (k < 32.0) return kLogFact[(int)k];
-    const double x = k + 1.0, r = 1.0 / x, r2 = r * r;
-    const double series = r * (1.0 / 12 - r2 * (1.0 / 360 - r2 * (1.0 / 1260 - r2 * (1.0 / 1680))));
-    return (x - 0.5) * log(x) - x + 0.91893853320467274178 + series;
-}
-
 __device__ __forceinline__ float u01(uint32_t w) { return ((w >> 8) + 1u) * (1.0f / 16777216.0f); }   // (0, 1]
 
 // N(0,1) from two uniform words (Box-Muller; hardware log2 and cos of revolutions).

@@ -11,7 +11,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from specula_amd import RAD2ASEC
-from specula_amd.lib.grids import make_mask, make_xy
+from specula_amd.lib.grids import make_mask
 
 
 @dataclass

@@ -7,7 +7,6 @@ the phase, and interaction matrices built from them.
 Slopes follow SubapData order (column-major subaperture blocks, sh_subap_calibrator.py:56-78),
 [sx..., sy...] (slopes.py:27-32).  Phases are in nm (ElectricField.phaseInNm).
 """
-import ctypes
 
 import numpy as np
 import torch
