@@ -261,6 +261,36 @@ def test_static_pupil_registration():
     assert torch.equal(out, ref2)
 
 
+
+@pytest.mark.parametrize('np_sub', [5, 6, 7])
+def test_static_pupil_order_other_wave_widths(np_sub):
+    """MFMA path with 12, 10 and 9 subapertures per wave (np 5, 6, 7; Parseval geometry, generic
+    stores): the lit-first order of a registered pupil gives the unregistered image bit for bit,
+    and both match the oracle."""
+    ef, pitch = 20 * np_sub, 0.05
+    turb = 500e-9 / (np_sub * pitch) * O.RAD2ASEC
+    kw = dict(wavelengthInNm=500, subap_wanted_fov=np_sub * turb, sensor_pxscale=turb, subap_on_diameter=20,
+              subap_npx=np_sub, fov_ovs_coeff=1.0)
+    g = sh_geometry(ef, pitch, fov_resolution_arcsec=turb / 4, **kw)
+    go = O.sh_geometry(ef_size=ef, pixel_pitch=pitch, fov_resolution_arcsec=turb / 4, **kw)
+    A = annular_pupil(ef, 0.3)
+    scr = von_karman_opd(256, pitch, seed=9)
+    frames = np.stack([scr[:ef, k:k + ef] for k in range(3)]).astype(np.float32)
+    plan = SHPlan(g, max_frames=3, device=0)
+    assert plan.parseval
+    At = torch.tensor(A.astype(np.float32), device=DEV)
+    ph = torch.tensor(frames, device=DEV)
+    ref = torch.empty((3, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan.forward(At, ph, ref, 1.0, n_frames=3)
+    plan.set_static_pupil(At, 1.0)
+    out = torch.empty_like(ref)
+    plan.forward(At, ph, out, 1.0, n_frames=3)
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
+    for k in range(3):
+        check_intensity(out[k].cpu().numpy(), O.sh_forward(A, frames[k].astype(np.float64), go), g.n_lenses,
+                        g.subap_npx)
+
 def test_sh_object_follows_in_place_pupil_change():
     """The drop-in SH keys its pupil statistics on the tensor's storage and in-place version."""
     from specula_amd.data_objects.electric_field import ElectricField
