@@ -194,7 +194,7 @@ class BaseProcessingObj(BaseTimeObj):
     """specula/base_processing_obj.py:8-184.
 
     ``trigger()`` replays a captured HIP graph of ``trigger_code()`` when the object has one
-    (``build_stream(capture=True)``), else calls ``trigger_code()`` on ``self.stream``."""
+    (``build_stream(capture=True)``), else calls ``trigger_code()`` on the current stream."""
 
     _streams = {}
 
@@ -251,8 +251,14 @@ class BaseProcessingObj(BaseTimeObj):
         pass
 
     def post_trigger(self):
-        if self.target_device_idx >= 0 and self.stream is not None:
-            self.stream.synchronize()
+        # base_processing_obj.py:93-96 synchronises the object's stream after a graph launch so
+        # that the next object sees the outputs.  Here the caller's current stream is made to wait
+        # for it on the GPU instead: later work on that stream (the next object's trigger, a host
+        # read) is ordered after these outputs without blocking the host.  Eager triggers run on
+        # the current stream itself (base_processing_obj.py:150-153) and need nothing.
+        if self.target_device_idx >= 0 and self.stream is not None and self.cuda_graph is not None:
+            with torch.cuda.device(self.target_device_idx):
+                torch.cuda.current_stream().wait_stream(self.stream)
 
     @classmethod
     def device_stream(cls, target_device_idx):
@@ -291,16 +297,18 @@ class BaseProcessingObj(BaseTimeObj):
         return self.ready
 
     def trigger(self):
+        """base_processing_obj.py:145-154: a captured graph is launched on the object's stream
+        (ordered after the caller's current stream, which produced the inputs); otherwise
+        trigger_code() runs on the current stream (native calls take stream=None = current)."""
         if self.ready:
-            if self.target_device_idx >= 0 and self.stream is not None:
+            if self.target_device_idx >= 0 and self.stream is not None and self.cuda_graph is not None:
                 with torch.cuda.device(self.target_device_idx):
-                    # order after whatever produced the inputs on the caller's stream
                     self.stream.wait_stream(torch.cuda.current_stream())
                     with torch.cuda.stream(self.stream):
-                        if self.cuda_graph is not None:
-                            self.cuda_graph.replay()
-                        else:
-                            self.trigger_code()
+                        self.cuda_graph.replay()
+            elif self.target_device_idx >= 0:
+                with torch.cuda.device(self.target_device_idx):
+                    self.trigger_code()
             else:
                 self.trigger_code()
             self.ready = False

@@ -140,7 +140,7 @@ class CCD(BaseProcessingObj):
         fresh = in_i.generation_time == self.current_time
         readout = (self.current_time + self.loop_dt - self._dt - self._start_time) % self._dt == 0
         lib = _native.lib()
-        st = _native.stream_handle(self.stream)
+        st = _native.stream_handle(None)
         integ = self._integrated_i.i
         src = in_i.i if in_i.i.dtype == torch.float32 and in_i.i.is_contiguous() else in_i.i.float().contiguous()
         if readout:

@@ -350,7 +350,7 @@ class SH(BaseProcessingObj):
                 self._plan = self._make_plan(lgs=True, kernels_normalised=normalised)
                 self._graph_key = None
             self._kspec = torch.tensor(self._kernelobj.lag_spectra(self.geometry.ovs_np_sub), device=self.device)
-            self._plan.set_kernels(self._kspec, stream=self.stream)
+            self._plan.set_kernels(self._kspec, stream=None)
             self._kernel_version = self._kernelobj.version
 
     # ------------------------------------------------------------------ per frame
@@ -367,8 +367,8 @@ class SH(BaseProcessingObj):
         ef, A, ph = self._inputs_fp32()
         if not self.use_graph and not self._plan.static_pupil_is(A, ef.S0, 0):
             # pupil statistics once per distinct amplitude (storage and in-place version)
-            self._plan.set_static_pupil(A, ef.S0, stream=self.stream)
-        self._plan.forward(A, ph, self._out_i.i, ef.S0, n_frames=1, stream=self.stream)
+            self._plan.set_static_pupil(A, ef.S0, stream=None)
+        self._plan.forward(A, ph, self._out_i.i, ef.S0, n_frames=1, stream=None)
 
     def trigger(self):
         if self.ready and self.use_graph:

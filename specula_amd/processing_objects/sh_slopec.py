@@ -135,6 +135,7 @@ class ShSlopec(Slopec):
                                          precision=self.precision)
         self._plan = None
         self._plan_key = None
+        self._single_mask = None
         self._flux = torch.zeros(subapdata.n_subaps, dtype=torch.float64, device=self.device)
         self._totals = torch.zeros(2, dtype=torch.float64, device=self.device)
         self.set_xy_weights()
@@ -185,10 +186,12 @@ class ShSlopec(Slopec):
             thr = 0.0
         plan = self._get_plan(pixels)
         plan.forward(pixels, self.slopes.slopes, self._flux, self._totals, thr=thr,
-                     pedestal=self.thr_pedestal, stream=self.stream)
+                     pedestal=self.thr_pedestal, stream=None)
         if self.mult_factor != 0:
             self.slopes.slopes *= self.mult_factor
-        self.slopes.single_mask = self.subapdata.single_mask()
+        if self._single_mask is None or self._single_mask[0] is not self.subapdata:   # built once per SubapData
+            self._single_mask = (self.subapdata, self.subapdata.single_mask())
+        self.slopes.single_mask = self._single_mask[1]
         self.slopes.display_map = self.subapdata.display_map
         self.slopes.generation_time = self.current_time
         self.flux_per_subaperture_vector.value = self._flux
