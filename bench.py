@@ -43,9 +43,9 @@ def parse():
                     help='also run two 256 MiB device copies (reference traffic for rocprofv3 counters)')
     ap.add_argument('--extras', default='all',
                     help="extra measurements: 'all' (or 1), 'none' (or 0), or a comma list of "
-                         "bf16, ccd, lgs, im, multi, hbm, variants")
+                         "bf16, ccd, lgs, im, multi, hbm, variants, objects")
     args = ap.parse_args()
-    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm', 'variants')
+    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm', 'variants', 'objects')
     e = str(args.extras).strip().lower()
     args.extras = set(names) if e in ('all', '1') else set() if e in ('none', '0', '') else \
         {x.strip() for x in e.split(',') if x.strip()}
@@ -208,6 +208,63 @@ def bench_multi(torch, dev, reps=20):
                          'candidates_ms': {f"{c['grid']}/{c['order']}": round(c['seconds'] * 1e3, 3)
                                            for c in rep['candidates']}},
             'forward_completion_us': per}
+
+
+def bench_objects(torch, dev, steps=200):
+    """One simulation-loop step of the drop-in objects at elt_ngs_80x80: ElectricField phase
+    update, SH (check_ready / trigger / post_trigger) and ShSlopec on the SH pixels, one frame per
+    step, host overhead included; eager and with the SH graph replayed."""
+    from specula_amd import workloads
+    from specula_amd.data_objects.electric_field import ElectricField
+    from specula_amd.data_objects.pixels import Pixels
+    from specula_amd.data_objects.subap_data import SubapData
+    from specula_amd.lib.subaps import detect_subaps
+    from specula_amd.processing_objects.sh import SH
+    from specula_amd.processing_objects.sh_slopec import ShSlopec
+    wl = workloads.elt_ngs_80x80()
+    g = wl.geometry()
+    ef = ElectricField(wl.ef_size, wl.ef_size, wl.pixel_pitch, S0=1.0, target_device_idx=0)
+    ef.A[:] = torch.tensor(wl.pupil(), device=dev)
+    frames = torch.tensor(wl.frames(8).astype(np.float32), device=dev)
+    out = {}
+    for use_graph in (False, True):
+        sh = SH(**wl.sh_kwargs, target_device_idx=0)
+        if wl.fov_resolution_arcsec:
+            sh._fov_resolution_arcsec = wl.fov_resolution_arcsec
+        sh.inputs['in_ef'].set(ef)
+        sh.setup()
+        sh.use_graph = use_graph
+        ef.phaseInNm[:] = 0.0
+        ef.generation_time = 1
+        sh.check_ready(1)
+        sh.trigger()
+        sh.post_trigger()
+        idxs, dmap = detect_subaps(sh.outputs['out_i'].i.cpu().numpy(), g.n_lenses, 0.5)
+        sl = ShSlopec(SubapData(idxs=idxs, display_map=dmap, nx=g.n_lenses, ny=g.n_lenses, target_device_idx=0),
+                      target_device_idx=0)
+        pix = Pixels(*sh.outputs['out_i'].i.shape, target_device_idx=0)
+        sl.inputs['in_pixels'].set(pix)
+
+        def step(t):
+            ef.phaseInNm.copy_(frames[t % 8])
+            ef.generation_time = t
+            sh.check_ready(t)
+            sh.trigger()
+            sh.post_trigger()
+            pix.pixels = sh.outputs['out_i'].i
+            pix.generation_time = t
+            sl.check_ready(t)
+            sl.trigger()
+            sl.post_trigger()
+        for t in range(2, 12):
+            step(t)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(12, 12 + steps):
+            step(t)
+        torch.cuda.synchronize()
+        out['graph_us' if use_graph else 'eager_us'] = round((time.perf_counter() - t0) / steps * 1e6, 1)
+    return {'workload': 'elt_ngs_80x80, 1 frame per step', **out}
 
 
 def bench_variants(torch, dev):
@@ -429,6 +486,9 @@ def main():
     if 'multi' in args.extras:
         progress('multi-sensor batch')
         result['multi_sensor'] = bench_multi(torch, dev)
+    if 'objects' in args.extras:
+        progress('drop-in objects')
+        result['object_step'] = bench_objects(torch, dev)
     if 'variants' in args.extras:
         progress('binning variants')
         result['binning_variants'] = bench_variants(torch, dev)

@@ -252,13 +252,10 @@ class BaseProcessingObj(BaseTimeObj):
 
     def post_trigger(self):
         # base_processing_obj.py:93-96 synchronises the object's stream after a graph launch so
-        # that the next object sees the outputs.  Here the caller's current stream is made to wait
-        # for it on the GPU instead: later work on that stream (the next object's trigger, a host
-        # read) is ordered after these outputs without blocking the host.  Eager triggers run on
-        # the current stream itself (base_processing_obj.py:150-153) and need nothing.
-        if self.target_device_idx >= 0 and self.stream is not None and self.cuda_graph is not None:
-            with torch.cuda.device(self.target_device_idx):
-                torch.cuda.current_stream().wait_stream(self.stream)
+        # that the next object sees the outputs.  Here graphs are replayed, and eager triggers
+        # run, on the caller's current stream, so later work on it is ordered after these outputs
+        # without any synchronisation.
+        pass
 
     @classmethod
     def device_stream(cls, target_device_idx):
@@ -279,6 +276,7 @@ class BaseProcessingObj(BaseTimeObj):
 
     def capture_stream(self):
         with torch.cuda.device(self.target_device_idx):
+            self.stream.wait_stream(torch.cuda.current_stream())     # the warm-up reads the inputs
             with torch.cuda.stream(self.stream):
                 self.trigger_code()
             self.stream.synchronize()
@@ -297,18 +295,16 @@ class BaseProcessingObj(BaseTimeObj):
         return self.ready
 
     def trigger(self):
-        """base_processing_obj.py:145-154: a captured graph is launched on the object's stream
-        (ordered after the caller's current stream, which produced the inputs); otherwise
-        trigger_code() runs on the current stream (native calls take stream=None = current)."""
+        """base_processing_obj.py:145-154: a captured graph is launched, else trigger_code() runs
+        (native calls take stream=None: the current stream).  Both go on the caller's current
+        stream, which produced the inputs; the object's own stream only serves the capture."""
         if self.ready:
-            if self.target_device_idx >= 0 and self.stream is not None and self.cuda_graph is not None:
+            if self.target_device_idx >= 0:
                 with torch.cuda.device(self.target_device_idx):
-                    self.stream.wait_stream(torch.cuda.current_stream())
-                    with torch.cuda.stream(self.stream):
+                    if self.cuda_graph is not None:
                         self.cuda_graph.replay()
-            elif self.target_device_idx >= 0:
-                with torch.cuda.device(self.target_device_idx):
-                    self.trigger_code()
+                    else:
+                        self.trigger_code()
             else:
                 self.trigger_code()
             self.ready = False
