@@ -373,10 +373,13 @@ class SH(BaseProcessingObj):
     def trigger(self):
         if self.ready and self.use_graph:
             ef, A, ph = self._inputs_fp32()
+            if not self._plan.static_pupil_is(A, ef.S0, 0):
+                # the captured forward reads the pupil statistics and the subaperture order from
+                # plan buffers: refresh them here, outside the graph, when the amplitude changes
+                self._plan.set_static_pupil(A, ef.S0, stream=None)
             key = (A.data_ptr(), ph.data_ptr(), float(ef.S0), self._out_i.i.data_ptr())
             if key != self._graph_key:
                 self._keep = (A, ph)
-                self._plan.set_static_pupil(None)   # the graph recomputes them on every replay
                 self.capture_stream()
                 self._graph_key = key
         super().trigger()

@@ -93,6 +93,21 @@ def test_graph_replay_matches_eager():
         _ = (ef.phaseInNm * 2).sum().item()          # unrelated eager work between replays
         assert torch.equal(graph.outputs['out_i'].i, eager.outputs['out_i'].i)
     assert graph.cuda_graph is not None
+    # an in-place change of the pupil: both refresh the registered statistics and subaperture
+    # order (the graph outside its replay) and stay equal
+    g0 = graph.cuda_graph
+    ef.A[: wl.ef_size // 3] = 0.0
+    for t, ph in enumerate(frames[:2], start=len(frames) + 1):
+        ef.phaseInNm[:] = torch.tensor(ph, device='cuda:0')
+        ef.generation_time = t
+        for obj in (graph, eager):
+            obj.check_ready(t)
+            obj.trigger()
+            obj.post_trigger()
+        assert torch.equal(graph.outputs['out_i'].i, eager.outputs['out_i'].i)
+    assert graph.cuda_graph is g0                    # same storage: no re-capture
+    img = graph.outputs['out_i'].i.cpu().numpy()
+    assert np.all(img[: img.shape[0] // 3 - 8] == 0)        # rows of subapertures under the dark band
 
 
 def test_setup_errors():
