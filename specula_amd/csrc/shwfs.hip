@@ -2499,7 +2499,7 @@ int shwfs_forward_slopes(shwfs_plan* p, int n_frames, const float* A, int64_t a_
     int rc = sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, ccd, ccd_stride, &cc, st);
     if (rc) return rc;
     DeviceGuard dg(p->device);
-    hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 4095) / 4096, n_frames), dim3(1024), 0, st,
+    hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 1023) / 1024, n_frames), dim3(256), 0, st,
                        c->d_mom, p->d_tpart, p->n_parts, c->n_sub, slopes, slopes_stride, flux, totals,
                        (double*)nullptr);
     SHWFS_LAUNCH_CHECK();
