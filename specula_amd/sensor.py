@@ -163,17 +163,18 @@ class SHSlopes(torch.autograd.Function):
         ctx.sensor = sensor
         ctx.save_for_backward(ph)
         ctx.in_shape = phase.shape
-        return sensor.forward(ph).clone()
+        step = sensor.max_frames
+        return torch.cat([sensor.forward(ph[f0:f0 + step]).clone() for f0 in range(0, ph.shape[0], step)])
 
     @staticmethod
     def backward(ctx, grad_slopes):
         (ph,) = ctx.saved_tensors
         sensor = ctx.sensor
         F = ph.shape[0]
-        if F == 1:
-            g = sensor.vjp(ph[0], grad_slopes.reshape(1, -1))
-        else:
-            g = torch.cat([sensor.vjp(ph[f], grad_slopes[f:f + 1]) for f in range(F)])
+        gs = grad_slopes.reshape(F, -1)
+        # frame f's cotangent at frame f's phase: one batched VJP per max_frames frames
+        step = sensor.max_frames
+        g = torch.cat([sensor.vjp(ph[f0:f0 + step], gs[f0:f0 + step]) for f0 in range(0, F, step)])
         return g.reshape(ctx.in_shape), None
 
 

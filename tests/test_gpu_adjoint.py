@@ -81,6 +81,21 @@ def test_autograd_function():
     torch.testing.assert_close(s[0], sen.forward(ph.detach()[None])[0])
 
 
+def test_autograd_function_batched_frames():
+    """5 frames through a sensor of max_frames 2: chunks of 2, 2 and 1 frames, each frame's
+    gradient equal to its own single-frame VJP."""
+    wl = workloads.scao_ngs_20x20()
+    sen = make(wl, 2)
+    ph = torch.tensor(wl.frames(5).astype(np.float32), device=DEV, requires_grad=True)
+    w = torch.randn(5, 2 * sen.n_sub, device=DEV)
+    s = sh_slopes(ph, sen)
+    assert s.shape == (5, 2 * sen.n_sub)
+    (s * w).sum().backward()
+    for f in range(5):
+        ref = sen.vjp(ph.detach()[f], w[f:f + 1])[0]
+        torch.testing.assert_close(ph.grad[f], ref)
+
+
 def test_adjoint_im_matches_pushpull():
     """IM of 40 smooth modes at 50 nm RMS: Jacobian-built (scale 2) vs push-pull
     (im_rec_calibrator.py:58-63 convention).  The difference is the O(a^2) non-linearity of
