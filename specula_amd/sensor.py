@@ -180,3 +180,44 @@ class SHSlopes(torch.autograd.Function):
 
 def sh_slopes(phase, sensor):
     return SHSlopes.apply(phase, sensor)
+
+
+def fit_modes(sensor, modes, target_slopes, phase0=None, coeffs0=None, max_iter=50):
+    """Modal least-squares fit through the sensor model: the coefficients c minimising
+    ||slopes(phase0 + sum_m c_m modes[m]) - target||^2, by L-BFGS with a strong-Wolfe line search
+    on gradients from the hand-written adjoint (SHSlopes.backward).  This is the fit behind
+    non-common-path aberration estimates and phase retrieval from measured slopes: the
+    non-linear sensor response is modelled, not linearised as an interaction matrix would.
+
+    modes (K, ef, ef) float32 device tensor [nm per unit coefficient]; target_slopes (2 n_sub,)
+    in the sensor's slope order; phase0 (ef, ef) known phase [nm] (None: flat); coeffs0 (K,)
+    starting point (None: zeros).  Returns (coefficients (K,) float32, final loss)."""
+    ef = sensor.geom.ef_size
+    K = modes.shape[0]
+    if tuple(modes.shape[1:]) != (ef, ef):
+        raise ValueError(f'modes must be (K, {ef}, {ef}), got {tuple(modes.shape)}')
+    m = modes.reshape(K, ef * ef).to(torch.float32)
+    base = (torch.zeros(ef * ef, dtype=torch.float32, device=sensor.device) if phase0 is None
+            else phase0.reshape(ef * ef).to(torch.float32))
+    tgt = target_slopes.reshape(1, -1).to(torch.float32)
+    if tgt.shape[1] != 2 * sensor.n_sub:
+        raise ValueError(f'target has {tgt.shape[1]} slopes, the sensor {2 * sensor.n_sub}')
+    c = (torch.zeros(K, dtype=torch.float32, device=sensor.device) if coeffs0 is None
+         else coeffs0.detach().to(torch.float32).clone())
+    c.requires_grad_(True)
+    opt = torch.optim.LBFGS([c], lr=1.0, max_iter=max_iter, line_search_fn='strong_wolfe')
+
+    def loss_of(coeffs):
+        r = sh_slopes((base + coeffs @ m).reshape(ef, ef), sensor) - tgt
+        return (r * r).sum()
+
+    def closure():
+        opt.zero_grad()
+        loss = loss_of(c)
+        loss.backward()
+        return loss
+
+    opt.step(closure)
+    with torch.no_grad():
+        final = float(loss_of(c.detach()))
+    return c.detach(), final
