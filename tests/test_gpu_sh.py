@@ -314,3 +314,27 @@ def test_sh_object_follows_in_place_pupil_change():
         totals.append(float(sh.outputs['out_i'].i.sum()))
     masked = float(torch.tensor(wl.pupil()).sum()) * wl.pixel_pitch ** 2
     np.testing.assert_allclose(totals, [masked, 0.5 * masked, 0.5 * masked], rtol=1e-5)
+
+
+def test_frame_count_outside_plan_is_refused_before_any_launch():
+    """n_frames 0 and above max_frames: the C ABI returns SHWFS_ERR_ARG with its message (no
+    launch), and the output buffer keeps its contents; the plan still works afterwards."""
+    import ctypes
+    from specula_amd import _native
+    wl = workloads.scao_ngs_20x20()
+    g = wl.geometry()
+    plan = SHPlan(g, max_frames=2, device=0)
+    ef, ccd = g.ef_size, g.ccd_side
+    A = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
+    ph = torch.tensor(wl.frames(3).astype(np.float32), device=DEV)
+    out = torch.full((3, ccd, ccd), -7.0, device=DEV)
+    lib = _native.lib()
+    for n in (0, 3, -1):
+        rc = lib.shwfs_forward(plan._h, n, _native.ptr(A), 0, _native.ptr(ph), ef * ef, 1.0, _native.ptr(out),
+                               ccd * ccd, _native.stream_handle(None))
+        assert rc == _native.SHWFS_ERR_ARG and b'outside [1, max_frames]' in lib.shwfs_last_error()
+    torch.cuda.synchronize()
+    assert bool((out == -7.0).all())
+    plan.forward(A, ph[:2], out[:2], 1.0, n_frames=2)
+    torch.cuda.synchronize()
+    assert bool(torch.isfinite(out[:2]).all()) and bool((out[2] == -7.0).all())
