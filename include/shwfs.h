@@ -4,7 +4,8 @@
  * Plain C types only: every array argument is a device pointer on the plan's device,
  * every stream is a hipStream_t passed as void*.  All functions return 0 on success or a
  * negative error code; shwfs_last_error() returns the message of the last failure on the
- * calling thread.  No function allocates device memory except the *_create calls.
+ * calling thread.  No function allocates device memory except the *_create calls and the set-up
+ * calls shwfs_plan_set_pupil (resampled plans), shwfs_plan_set_resampling and shwfs_plan_bind_slopec.
  *
  * Each entry point names the reference interface it replaces (paths relative to the
  * SPECULA checkout).  The reference path is Python; the ctypes binding a maintainer
@@ -92,6 +93,12 @@ int shwfs_forward(shwfs_plan* plan, int n_frames,
  * area from in_ef.A on every trigger (processing_objects/sh.py:422-428). */
 int shwfs_plan_set_pupil(shwfs_plan* plan, const float* A, int64_t a_frame_stride, int n_frames,
                          double S0, void* stream);
+
+/* 1 when (A, a_frame_stride, S0) is the pupil registered with shwfs_plan_set_pupil, 0 otherwise.
+ * Calls of shwfs_forward with another amplitude or S0 keep the registration (its statistics live
+ * in buffers of their own); shwfs_plan_set_resampling and shwfs_plan_set_pupil(NULL) drop it.
+ * No reference counterpart (the registration is a performance hint). */
+int shwfs_plan_pupil_registered(const shwfs_plan* plan, const float* A, int64_t a_frame_stride, double S0);
 
 /* Field resampling for fov_ovs != 1, phase rotation or shift (SH.setup, sh.py:448-458, and the
  * interpolation block of SH.trigger_code, sh.py:350-360).  After this call shwfs_forward and
@@ -182,7 +189,10 @@ int shwfs_vjp(shwfs_plan* plan, int n_frames, const float* A, int64_t a_frame_st
  *   im[m][n_valid + slot] = scale * sum_{p in subaperture} jy[p] * modes[m][p]
  * jx, jy: ef x ef maps (d slope / d phase_nm); modes: n_modes maps of ef x ef (stride
  * mode_stride); slot: n_lenses^2 int32, slope index of each subaperture or -1; scale = 2
- * reproduces the push-pull convention IM = (s(+a) - s(-a)) / a.
+ * reproduces the push-pull convention IM = (s(+a) - s(-a)) / a.  Precondition: the slots in
+ * [0, n_valid) are each claimed by one subaperture.  Slot values outside [0, n_valid) are ignored
+ * (never used as an index); a slot in [0, n_valid) that no subaperture claims gets zero rows with
+ * `work`, and is left unwritten without it.
  * work: optional device workspace of n_modes * 2 * n_lenses^2 floats followed by n_valid int32
  * (16-byte aligned); with it the results are written in grid order and reordered into slot
  * order by a second pass (coalesced stores); NULL writes slot order directly. */

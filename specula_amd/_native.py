@@ -18,7 +18,8 @@ EXPORTS = (
     'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events', 'shwfs_plan_set_frames_per_block',
     'shwfs_set_kernels', 'shwfs_forward',
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
-    'shwfs_plan_bind_slopec', 'shwfs_forward_slopes', 'shwfs_plan_set_pupil', 'shwfs_plan_set_resampling',
+    'shwfs_plan_bind_slopec', 'shwfs_forward_slopes', 'shwfs_plan_set_pupil', 'shwfs_plan_pupil_registered',
+    'shwfs_plan_set_resampling',
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian',
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
@@ -78,6 +79,7 @@ def _declare(lib):
         'shslopec_forward': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
         'shwfs_plan_bind_slopec': (_I, [_P, _P]),
         'shwfs_plan_set_pupil': (_I, [_P, _P, _I64, _I, _D, _P]),
+        'shwfs_plan_pupil_registered': (_I, [_P, _P, _I64, _D]),
         'shwfs_plan_set_resampling': (_I, [_P, _I, _P, _P, _P]),
         'shwfs_forward_slopes': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _F, _I, _P, _I64, _P, _P, _P]),
         'shslopec_vjp': (_I, [_P, _I, _P, _I64, _F, _I, _P, _I64, _P, _P]),

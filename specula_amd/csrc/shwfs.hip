@@ -716,9 +716,8 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
                         w0 = clamp0(v.x); w1 = clamp0(v.y); w2 = clamp0(v.z); w3 = clamp0(v.w);
                     }
                     const float yv = sw[kYwWord + Y];   // y weight of row Y (kept in LDS padding)
-                    // no threshold: the image is non-negative up to rounding, so the flux is the
-                    // clamped total (accumulated once, below)
-                    if constexpr (COG != 3) l_f += (v.x + v.y) + (v.z + v.w);
+                    // flux: the unclamped pixel sum (sh_slopec.py:365), as the two-pass CoG takes it
+                    l_f += (v.x + v.y) + (v.z + v.w);
                     l_y = fmaf((w0 + w1) + (w2 + w3), yv, l_y);
                     cw0 += w0; cw1 += w1; cw2 += w2; cw3 += w3;
                 }
@@ -731,17 +730,16 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             double m_f = l_f, m_t = l_t, m_x = l_x, m_y = l_y;
 #pragma unroll
             for (int o = 1; o < X4; o <<= 1) {
-                if constexpr (COG != 3) m_f += __shfl_xor(m_f, o, kWave);
+                m_f += __shfl_xor(m_f, o, kWave);
                 m_t += __shfl_xor(m_t, o, kWave);
                 m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
             }
 #pragma unroll
             for (int o = LPR; o < 64; o <<= 1) {
-                if constexpr (COG != 3) m_f += __shfl_xor(m_f, o, kWave);
+                m_f += __shfl_xor(m_f, o, kWave);
                 m_t += __shfl_xor(m_t, o, kWave);
                 m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
             }
-            if constexpr (COG == 3) m_f = m_t;
             const bool writer = live && lane < LPR && x4 == 0 && wc.slot >= 0;
             if (writer) {
                 double* m = g.mom + ((int64_t)f * g.n_valid + wc.slot) * 4;
@@ -1665,7 +1663,7 @@ __global__ void im_from_jacobian_kernel(const float* __restrict__ jx, const floa
     const int col = threadIdx.x;
     const int j = col / NP, xin = col - (col / NP) * NP;
     const int sl = (col < ef) ? slot[i * n_lenses + j] : -1;
-    const bool valid = sl >= 0;
+    const bool valid = sl >= 0 && sl < n_valid;      // slots outside [0, n_valid) are ignored
     float jxr[NP], jyr[NP];
 #pragma unroll
     for (int r = 0; r < NP; ++r) {
@@ -1724,7 +1722,7 @@ im_from_jacobian_v4_kernel(const float* __restrict__ jx, const float* __restrict
     const int rs = t % RS, qd = t / RS;
     const int col = qd * 4;
     const int sl = (col < ef) ? slot[i * n_lenses + col / NP] : -1;
-    const bool valid = sl >= 0;
+    const bool valid = sl >= 0 && sl < n_valid;      // slots outside [0, n_valid) are ignored
     const int64_t row0 = (int64_t)(i * NP + rs * 4) * ef + col;
     float4 jxr[4], jyr[4];
 #pragma unroll
@@ -1780,12 +1778,16 @@ im_from_jacobian_v4_kernel(const float* __restrict__ jx, const float* __restrict
     }
 }
 
-// inv[slot[g]] = g for every valid subaperture g of the grid.
-__global__ void slot_inverse_kernel(const int32_t* __restrict__ slot, int nn, int32_t* __restrict__ inv) {
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < nn) {
+// inv[slot[g]] = g for every subaperture g of the grid whose slot lies in [0, n_valid); slots
+// that no subaperture claims keep -1 (their rows come out zero).  One block: the -1 fill is
+// ordered before the scatter by the block barrier.
+__global__ void __launch_bounds__(1024) slot_inverse_kernel(const int32_t* __restrict__ slot, int nn, int n_valid,
+                                                            int32_t* __restrict__ inv) {
+    for (int s = threadIdx.x; s < n_valid; s += blockDim.x) inv[s] = -1;
+    __syncthreads();
+    for (int g = threadIdx.x; g < nn; g += blockDim.x) {
         const int s = slot[g];
-        if (s >= 0) inv[s] = g;
+        if (s >= 0 && s < n_valid) inv[s] = g;
     }
 }
 
@@ -1823,8 +1825,8 @@ im_gather_lds_kernel(const float* __restrict__ grid, const int32_t* __restrict__
     float* row = im + (int64_t)m * 2 * n_valid;
     for (int s = threadIdx.x; s < n_valid; s += blockDim.x) {
         const int g = inv[s];
-        row[s] = row_l[g];
-        row[n_valid + s] = row_l[nn + g];
+        row[s] = g >= 0 ? row_l[g] : 0.f;
+        row[n_valid + s] = g >= 0 ? row_l[nn + g] : 0.f;
     }
 }
 
@@ -1836,8 +1838,8 @@ __global__ void im_gather_kernel(const float* __restrict__ grid, const int32_t* 
     float* row = im + (int64_t)m * 2 * n_valid;
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n_valid; s += gridDim.x * blockDim.x) {
         const int g = inv[s];
-        row[s] = gx[g];
-        row[n_valid + s] = gx[nn + g];
+        row[s] = g >= 0 ? gx[g] : 0.f;
+        row[n_valid + s] = g >= 0 ? gx[nn + g] : 0.f;
     }
 }
 
@@ -1912,14 +1914,18 @@ struct shwfs_plan {
     float* d_rowlag = nullptr;     // MFMA path: row operator in the lag domain, P~ = P A (16 x 16)
     float* d_colw = nullptr;
     float* d_kspec = nullptr;
-    double* d_amp = nullptr;       // [max_frames][2]
-    float* d_scale = nullptr;      // [max_frames] Parseval flux scale
+    double* d_amp = nullptr;       // [max_frames][2] statistics of the last unregistered amplitude
+    float* d_scale = nullptr;      // [max_frames] its Parseval flux scale
+    double* d_amp_st = nullptr;    // [max_frames][2] statistics of the registered (static) pupil
+    float* d_scale_st = nullptr;   // [max_frames] its Parseval flux scale
     int ef_in = 0;                 // resampling: side of the input field (0: no resampling)
     ResampleMap rmap{};
     int2* d_ext = nullptr;         // [ef_in^2] edge-extrapolation sources of the phase
     float* d_A_rs = nullptr;       // [max_frames][ef_size^2] resampled amplitude
+    float* d_A_rs_st = nullptr;    // [static_frames][ef_size^2] resampled registered pupil
+    int A_rs_st_frames = 0;        // frames allocated in d_A_rs_st
     float* d_ph_rs = nullptr;      // [max_frames][ef_size^2] resampled phase
-    const float* static_A = nullptr;   // pupil whose statistics are held in d_amp / d_scale
+    const float* static_A = nullptr;   // registered pupil, statistics in d_amp_st / d_scale_st
     int64_t static_a_stride = 0;
     int static_frames = 0;
     double static_S0 = 0.0;
@@ -2179,6 +2185,8 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     PLAN_ALLOC(p->d_colw, cw.size() * sizeof(float));
     PLAN_ALLOC(p->d_amp, (size_t)g.max_frames * 2 * sizeof(double));
     PLAN_ALLOC(p->d_scale, (size_t)g.max_frames * sizeof(float));
+    PLAN_ALLOC(p->d_amp_st, (size_t)g.max_frames * 2 * sizeof(double));
+    PLAN_ALLOC(p->d_scale_st, (size_t)g.max_frames * sizeof(float));
     PLAN_ALLOC(p->d_amp_part, (size_t)g.max_frames * p->amp_blocks * 2 * sizeof(double));
     PLAN_ALLOC(p->d_counter, (size_t)g.max_frames * sizeof(unsigned));
     PLAN_ALLOC(p->d_bpart, (size_t)g.max_frames * p->n_parts * sizeof(double));
@@ -2207,6 +2215,8 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_kspec);
     (void)hipFree(p->d_amp);
     (void)hipFree(p->d_scale);
+    (void)hipFree(p->d_amp_st);
+    (void)hipFree(p->d_scale_st);
     (void)hipFree(p->d_amp_part);
     (void)hipFree(p->d_counter);
     (void)hipFree(p->d_bpart);
@@ -2216,6 +2226,7 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_tpart);
     (void)hipFree(p->d_ext);
     (void)hipFree(p->d_A_rs);
+    (void)hipFree(p->d_A_rs_st);
     (void)hipFree(p->d_ph_rs);
     (void)hipFree(p->d_cogw);
     delete p;
@@ -2268,23 +2279,24 @@ int launch_resample(const shwfs_plan* p, const float* in, int64_t in_stride, con
     return SHWFS_OK;
 }
 
-// Statistics of the amplitude frames (masked area, square sum, Parseval scale); with resampling
-// the amplitude is first resampled (d_A_rs) and the square sum is taken there.
-int pupil_stats(shwfs_plan* p, const float* A, int64_t a_stride, int amp_frames, double S0, hipStream_t st) {
+// Statistics of the amplitude frames (masked area, square sum, Parseval scale) into amp / scale;
+// with resampling the amplitude is first resampled (d_A_rs) and the square sum is taken there.
+int pupil_stats(shwfs_plan* p, const float* A, int64_t a_stride, int amp_frames, double S0, double* amp,
+                float* scale, float* A_rs, hipStream_t st) {
     const shwfs_geom& g = p->g;
     const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
     const float* B = nullptr;
     int64_t b_stride = 0, n_in = ef_px;
     if (p->ef_in) {
         n_in = (int64_t)p->ef_in * p->ef_in;
-        int rc = launch_resample(p, A, a_stride, nullptr, p->d_A_rs, amp_frames, st);
+        int rc = launch_resample(p, A, a_stride, nullptr, A_rs, amp_frames, st);
         if (rc) return rc;
-        B = p->d_A_rs;
+        B = A_rs;
         b_stride = ef_px;
     }
     hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st, A, a_stride, n_in,
-                       p->d_amp_part, p->d_counter, p->d_amp, S0 * g.pixel_area, g.norm_const,
-                       g.parseval ? p->d_scale : nullptr, B, b_stride, ef_px);
+                       p->d_amp_part, p->d_counter, amp, S0 * g.pixel_area, g.norm_const,
+                       g.parseval ? scale : nullptr, B, b_stride, ef_px);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2306,31 +2318,33 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     DeviceGuard dg(p->device);
 
     // amplitude statistics: one pass per distinct amplitude frame, unless this is the pupil
-    // registered with shwfs_plan_set_pupil (statistics already on the device)
+    // registered with shwfs_plan_set_pupil (statistics already on the device, in buffers of their
+    // own, so that calls with another amplitude or S0 leave the registration intact)
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
     const bool cached = A == p->static_A && a_stride == p->static_a_stride && S0 == p->static_S0 &&
                         amp_frames <= p->static_frames;
+    double* amp = cached ? p->d_amp_st : p->d_amp;
+    float* fscale = cached ? p->d_scale_st : p->d_scale;
     if (!cached) {
-        p->static_A = nullptr;
-        int rc = pupil_stats(p, A, a_stride, amp_frames, S0, st);
+        int rc = pupil_stats(p, A, a_stride, amp_frames, S0, amp, fscale, p->d_A_rs, st);
         if (rc) return rc;
     }
-    if (p->ef_in) {   // resampled field: the SH kernels read d_A_rs / d_ph_rs
+    if (p->ef_in) {   // resampled field: the SH kernels read the resampled amplitude and d_ph_rs
         int rc = launch_resample(p, phase, ph_stride, p->d_ext, p->d_ph_rs, n_frames, st);
         if (rc) return rc;
-        A = p->d_A_rs;
+        A = cached ? p->d_A_rs_st : p->d_A_rs;
         a_stride = (a_stride == 0) ? 0 : ef_px;
         phase = p->d_ph_rs;
         ph_stride = ef_px;
     }
 
     NgsArgs a{};
-    a.fscale = p->d_scale; a.scale_fstep = (a_stride == 0) ? 0 : 1;
+    a.fscale = fscale; a.scale_fstep = (a_stride == 0) ? 0 : 1;
     a.A = A; a.a_stride = a_stride;
     a.phase = phase; a.ph_stride = ph_stride;
     a.ccd = ccd; a.ccd_stride = ccd_stride;
     a.roww = (p->mfma && !g.lgs) ? p->d_rowlag : p->d_roww; a.colw = p->d_colw;
-    a.amp = p->d_amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
+    a.amp = amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
     a.bpart = p->d_bpart; a.n_parts = p->n_parts;
     a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
     a.n_sub = p->n_sub; a.n_tiles = p->n_tiles; a.parseval = g.parseval;
@@ -2362,7 +2376,7 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     if (p->ev_stop) SHWFS_HIP_CHECK(hipEventRecord(p->ev_stop, st));
     if (!g.parseval) {
         hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, n_frames), dim3(256), 0, st, ccd, ccd_stride,
-                           ccd_side * ccd_side, p->d_bpart, p->n_parts, p->d_amp,
+                           ccd_side * ccd_side, p->d_bpart, p->n_parts, amp,
                            (a_stride == 0) ? 0 : 2, S0, g.pixel_area);
         SHWFS_LAUNCH_CHECK();
     }
@@ -2389,7 +2403,15 @@ int shwfs_plan_set_pupil(shwfs_plan* p, const float* A, int64_t a_stride, int n_
     if (a_stride != 0 && a_stride < in_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
     DeviceGuard dg(p->device);
-    int rc = pupil_stats(p, A, a_stride, amp_frames, S0, reinterpret_cast<hipStream_t>(stream));
+    if (p->ef_in && p->A_rs_st_frames < amp_frames) {   // set-up time allocation of the resampled copy
+        (void)hipFree(p->d_A_rs_st);
+        p->d_A_rs_st = nullptr;
+        p->A_rs_st_frames = 0;
+        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_A_rs_st, (size_t)amp_frames * g.ef_size * g.ef_size * sizeof(float)));
+        p->A_rs_st_frames = amp_frames;
+    }
+    int rc = pupil_stats(p, A, a_stride, amp_frames, S0, p->d_amp_st, p->d_scale_st, p->d_A_rs_st,
+                         reinterpret_cast<hipStream_t>(stream));
     if (rc) return rc;
     p->order_valid = false;
     if (p->d_order && a_stride == 0 && !p->ef_in) {
@@ -2403,6 +2425,11 @@ int shwfs_plan_set_pupil(shwfs_plan* p, const float* A, int64_t a_stride, int n_
     p->static_frames = amp_frames;
     p->static_S0 = S0;
     return SHWFS_OK;
+}
+
+int shwfs_plan_pupil_registered(const shwfs_plan* p, const float* A, int64_t a_stride, double S0) {
+    if (!p) return set_error(SHWFS_ERR_ARG, "null plan");
+    return (A != nullptr && A == p->static_A && a_stride == p->static_a_stride && S0 == p->static_S0) ? 1 : 0;
 }
 
 int shwfs_plan_set_resampling(shwfs_plan* p, int ef_in, const int32_t* ext_src1, const int32_t* ext_src2,
@@ -2698,17 +2725,19 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
     DeviceGuard dg(p->device);
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
-    if (!(A == p->static_A && a_stride == p->static_a_stride && amp_frames <= p->static_frames)) {
-        p->static_A = nullptr;       // d_amp now describes this A
+    // sum(A), sum(A^2) do not depend on S0: the registered pupil's statistics serve any S0
+    const bool cached = A == p->static_A && a_stride == p->static_a_stride && amp_frames <= p->static_frames;
+    double* amp = cached ? p->d_amp_st : p->d_amp;
+    if (!cached) {
         hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, p->d_amp, 0.0, 0.0, (float*)nullptr,
+                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, amp, 0.0, 0.0, (float*)nullptr,
                            (const float*)nullptr, (int64_t)0, (int64_t)0);
         SHWFS_LAUNCH_CHECK();
     }
     VjpArgs a{};
     a.A = A; a.a_stride = a_stride; a.phase = phase; a.ph_stride = ph_stride;
     a.gccd = grad_ccd; a.gccd_stride = gccd_stride; a.gphase = grad_phase; a.gph_stride = gph_stride;
-    a.roww = p->d_roww; a.colw = p->d_colw; a.amp = p->d_amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
+    a.roww = p->d_roww; a.colw = p->d_colw; a.amp = amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
     a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.n_sub = p->n_sub;
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0; a.pixel_area = g.pixel_area;
     a.norm_const = g.norm_const;
@@ -2769,7 +2798,7 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
         float* gridbuf = static_cast<float*>(work);
         int32_t* inv = work ? reinterpret_cast<int32_t*>(gridbuf + (int64_t)n_modes * 2 * nn) : nullptr;
         if (work) {
-            hipLaunchKernelGGL(slot_inverse_kernel, dim3((nn + 255) / 256), dim3(256), 0, st, slot, nn, inv);
+            hipLaunchKernelGGL(slot_inverse_kernel, dim3(1), dim3(1024), 0, st, slot, nn, n_valid, inv);
             SHWFS_LAUNCH_CHECK();
         }
 #define IMJ4(NPV, G) hipLaunchKernelGGL((im_from_jacobian_v4_kernel<NPV, G>), grid, dim3(threads4), 0, st, jx, jy,    \

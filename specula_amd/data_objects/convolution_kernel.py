@@ -19,53 +19,67 @@ from specula_amd import ASEC2RAD
 from specula_amd.base import BaseDataObj
 
 
-def _sample_rebin(a, shape):
-    """rebin2d(sample=True) (rebin.py:18-21): nearest sample at float(old)/new steps."""
-    slices = [slice(0, old, float(old) / new) for old, new in zip(a.shape, shape)]
-    idx = np.mgrid[slices].astype(np.int32)
-    return a[tuple(idx)]
+def _field_angles(ssp, overs, ps):
+    """Field angles [arcsec] of the ssp kernel pixels along one axis.  The beacon is evaluated on
+    ssp * overs points spanning -ssp ps/2 .. +ssp ps/2 end to end, then every overs-th point is kept
+    (the nearest-sample rebin of convolution_kernel.py:99, rebin.py:18-21)."""
+    n_fine = ssp * overs
+    step = ssp * ps / (n_fine - 1) if n_fine > 1 else 0.0
+    return -ssp * ps / 2 + np.arange(ssp) * overs * step
 
 
 def lgs_map_sh(nsh, diam, rl, zb, dz, profz, fwhmb, ps, ssp, overs=2, theta=(0.0, 0.0),
                rprof_type=0, dtype=np.float32):
-    """Geometric sodium-LGS image seen by every subaperture (convolution_kernel.py:14-108):
-    a Gaussian (or top-hat) beam cross-section, sampled along the line of sight at the layer
-    offsets ``dz`` with weights ``profz``, projected from the launcher at ``rl``.  Returns
-    (nsh*nsh, ssp, ssp), index [q*nsh + p] for subaperture row p, column q."""
-    theta = np.asarray(theta, dtype=np.float64)
-    ossp = ssp * overs
-    lin = np.linspace(-diam / 2, diam / 2, nsh, dtype=dtype)
-    fov = np.linspace(-ssp * ps / 2, ssp * ps / 2, ossp, dtype=dtype)
-    xsh, ysh = np.meshgrid(lin, lin)
-    xfov, yfov = np.meshgrid(fov, fov)
-    sigma = (fwhmb * ASEC2RAD * zb) / (2 * np.sqrt(2 * np.log(2)))
-    inv_s2 = 1.0 / sigma ** 2
-    rb = np.array([theta[0] * ASEC2RAD * zb, theta[1] * ASEC2RAD * zb, 0], dtype=dtype)
-    beam = zb * np.array([0, 0, 1], dtype=dtype) + rb - np.array(rl, dtype=dtype)
-    el = beam / beam[2]
-    fx = np.tile(xfov * ASEC2RAD * zb, (nsh, nsh)).reshape(ossp * nsh, ossp * nsh)
-    fy = np.tile(yfov * ASEC2RAD * zb, (nsh, nsh)).reshape(ossp * nsh, ossp * nsh)
-    ex = (fx - np.repeat(np.repeat(xsh, ossp, axis=0), ossp, axis=1)) / zb
-    ey = (fy - np.repeat(np.repeat(ysh, ossp, axis=0), ossp, axis=1)) / zb
+    """Geometric image of a sodium beacon in every subaperture of an nsh x nsh lenslet array
+    (the model of convolution_kernel.py:14-108; no diffraction).
+
+    The laser leaves the launcher at ``rl`` [m] towards the point ``theta`` [arcsec] at altitude
+    ``zb`` (the focus); its cross-section is a Gaussian of FWHM ``fwhmb`` [arcsec at zb]
+    (``rprof_type`` 1: a top-hat of radius sigma).  The sodium column is sampled at altitudes
+    zb + dz[l] with weights profz[l].  A kernel pixel of subaperture centre c (a point of
+    linspace(-diam/2, diam/2, nsh) per axis) at field angle a looks along the ray from c through
+    the point a zb of the focal plane; at altitude zb + dz that ray and the beam are
+        (1 + dz/zb) a zb - (dz/zb) c      and      rb + dz (rb - rl) / (zb - rl_z)
+    apart horizontally (rb = theta zb), the same altitude for both so no vertical term.  For the
+    Gaussian beam the weight factorises into an x and a y profile per layer, so the kernel of
+    subaperture (row p, column q) is sum_l w_l gy_l[p] (x) gx_l[q].
+
+    Returns (nsh*nsh, ssp, ssp) in ``dtype``, index [q*nsh + p] (the reference's cube order, which
+    ConvolutionKernel.process_kernels transposes), FP64 arithmetic throughout."""
+    rl = np.asarray(rl, dtype=np.float64).reshape(-1)
+    if rl.size == 1:
+        rl = np.array([rl[0], rl[0], rl[0]])
+    theta = np.asarray(theta, dtype=np.float64).reshape(2)
+    dz = np.asarray(dz, dtype=np.float64).reshape(-1)
+    w = np.asarray(profz, dtype=np.float64).reshape(-1)
+    keep = w > 0
+    dz, w = dz[keep], w[keep]
+    sigma = fwhmb * ASEC2RAD * zb / (2 * np.sqrt(2 * np.log(2)))     # beam sigma at the focus [m]
     if rprof_type == 0:
-        gnorm = 1.0 / (sigma * np.pi * np.sqrt(2.0))
+        peak = 1.0 / (sigma * np.pi * np.sqrt(2.0))
     elif rprof_type == 1:
-        gnorm = 1.0 / (np.pi / 4 * (fwhmb * ASEC2RAD * zb) ** 2)
+        peak = 1.0 / (np.pi / 4 * (fwhmb * ASEC2RAD * zb) ** 2)
     else:
         raise ValueError('Unsupported radial profile type')
-    fmap = np.zeros((nsh * ossp, nsh * ossp), dtype=dtype)
-    for z, w in zip(dz, profz):
-        if w <= 0:
-            continue
-        d2 = ((fx + z * ex - (rb[0] + z * el[0])) ** 2 +
-              (fy + z * ey - (rb[1] + z * el[1])) ** 2 +
-              (0.0 + z * 1.0 - (rb[2] + z * el[2])) ** 2)
-        if rprof_type == 0:
-            fmap += (gnorm * w) * np.exp(d2 * (-0.5 * inv_s2), dtype=dtype)
-        else:
-            fmap += (gnorm * w) * ((d2 * inv_s2) <= 1.0)
-    ccd = _sample_rebin(fmap, (ssp * nsh, ssp * nsh))
-    return ccd.reshape(nsh, ssp, nsh, ssp).transpose((2, 0, 1, 3)).reshape(nsh * nsh, ssp, ssp)
+    centres = np.linspace(-diam / 2, diam / 2, nsh)                   # subaperture centres [m]
+    fpos = _field_angles(ssp, overs, ps) * ASEC2RAD * zb               # field points at the focus [m]
+    rb = theta * ASEC2RAD * zb                                         # beacon point at the focus
+    drift = (rb - rl[:2]) / (zb - rl[2])                               # beam drift per metre of altitude
+    # horizontal separation ray - beam at each layer: sep[l, axis, subap, pixel]
+    grow = 1.0 + dz / zb
+    sep = (grow[:, None, None, None] * fpos[None, None, None, :]
+           - (dz / zb)[:, None, None, None] * centres[None, None, :, None]
+           - (rb[None, :, None, None] + dz[:, None, None, None] * drift[None, :, None, None]))
+    if rprof_type == 0:
+        prof = np.exp(-0.5 * (sep / sigma) ** 2)                       # separable Gaussian profiles
+        # kern[q, p, y, x] = sum_l w_l peak gy[l, p, y] gx[l, q, x]
+        kern = np.einsum('l,lpy,lqx->qpyx', w * peak, prof[:, 1], prof[:, 0], optimize=True)
+    else:
+        kern = np.zeros((nsh, nsh, ssp, ssp))
+        for l in range(len(w)):
+            r2 = (sep[l, 1][None, :, :, None] ** 2 + sep[l, 0][:, None, None, :] ** 2) / sigma ** 2
+            kern += (w[l] * peak) * (r2 <= 1.0)
+    return np.ascontiguousarray(kern.reshape(nsh * nsh, ssp, ssp), dtype=dtype)
 
 
 class ConvolutionKernel(BaseDataObj):

@@ -130,9 +130,15 @@ class SHPlan:
         self._static_key = (A.data_ptr(), A._version, a_frame_stride, float(S0))
 
     def static_pupil_is(self, A, S0, a_frame_stride):
-        """True when ``A`` (same storage, not modified since) is the registered static pupil."""
+        """True when ``A`` (same storage, not modified since) is the registered static pupil; the
+        library's own registration (shwfs_plan_pupil_registered) is consulted as well."""
         key = getattr(self, '_static_key', None)
-        return key is not None and key == (A.data_ptr(), A._version, a_frame_stride, float(S0))
+        if key is None or key != (A.data_ptr(), A._version, a_frame_stride, float(S0)):
+            return False
+        rc = _native.lib().shwfs_plan_pupil_registered(self._h, _native.ptr(A), a_frame_stride, float(S0))
+        if rc < 0:
+            _native.check(rc, 'shwfs_plan_pupil_registered')
+        return rc == 1
 
     def bind_slopec(self, slopec):
         """Fuse the centre of gravity of a SlopecPlan into this plan's forward

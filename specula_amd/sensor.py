@@ -78,7 +78,14 @@ class SHSensor:
         C = grad_slopes.shape[0]
         if C > self.max_frames:
             raise ValueError(f'{C} cotangents > max_frames {self.max_frames}')
+        if grad_slopes.dim() != 2 or grad_slopes.shape[1] != 2 * self.n_sub:
+            raise ValueError(f'grad_slopes must be (C, {2 * self.n_sub}), got {tuple(grad_slopes.shape)}')
         shared = phase.dim() == 2 or phase.shape[0] == 1
+        if tuple(phase.shape[-2:]) != (ef, ef):
+            raise ValueError(f'phase must be ({ef}, {ef}) or (C, {ef}, {ef}), got {tuple(phase.shape)}')
+        if not shared and phase.shape[0] != C:
+            # the VJP kernel reads one phase frame per cotangent
+            raise ValueError(f'{phase.shape[0]} phase frames for {C} cotangents (give one phase or one per cotangent)')
         ph = phase.reshape(-1, ef, ef).contiguous()
         ph_fwd = ph[:1].expand(C, ef, ef).contiguous() if shared else ph
         self.forward(ph_fwd, stream=stream)                 # pixels of the current state

@@ -89,3 +89,24 @@ def test_grid_limits_are_rejected():
     rc = lib.shccd_readout(ctypes.byref(p), 70000, None, 0, ctypes.c_void_p(16), 64, 1.0, None, 0,
                            ctypes.c_void_p(16), 64, None)
     assert rc == -1 and b'65535' in lib.shwfs_last_error()
+
+
+def test_sensor_vjp_rejects_mismatched_frames_before_any_launch():
+    """SHSensor.vjp: a per-frame phase must hold one frame per cotangent (the VJP kernel reads
+    phase frame c for cotangent c); checked on the host, so no device buffer is touched."""
+    import pytest
+    import torch
+    from types import SimpleNamespace
+    from specula_amd.sensor import SHSensor
+    s = SHSensor.__new__(SHSensor)
+    s.geom = SimpleNamespace(ef_size=16, ccd_side=8)
+    s.max_frames = 4
+    s.n_sub = 5
+    with pytest.raises(ValueError, match='2 phase frames for 3 cotangents'):
+        s.vjp(torch.zeros((2, 16, 16)), torch.zeros((3, 10)))
+    with pytest.raises(ValueError, match='grad_slopes must be'):
+        s.vjp(torch.zeros((16, 16)), torch.zeros((3, 9)))
+    with pytest.raises(ValueError, match='phase must be'):
+        s.vjp(torch.zeros((3, 16, 15)), torch.zeros((3, 10)))
+    with pytest.raises(ValueError, match='cotangents > max_frames'):
+        s.vjp(torch.zeros((16, 16)), torch.zeros((5, 10)))
