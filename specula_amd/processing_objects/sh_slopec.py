@@ -50,6 +50,7 @@ class SlopecPlan:
 
     def __init__(self, idxs, frame_shape, weights, max_frames=1, device=0):
         idxs = np.asarray(idxs, dtype=np.int64)
+        self.idxs = idxs                      # host copy of the subaperture pixel indices
         self.n_sub, npp = idxs.shape
         self.np_sub = int(round(np.sqrt(npp)))
         self.frame_shape = tuple(frame_shape)

@@ -127,10 +127,12 @@ def multi_sensor_entries(n_lgs=6, n_pushpull=2000, amp_nm=50.0, device=0, with_s
         wl = lgs_40x40()
         g = wl.geometry()
         ang = 2 * np.pi * k / max(n_lgs, 1)
-        kern = lgs_kernel(g, launcher_pos=(20.0 * np.cos(ang), 20.0 * np.sin(ang), 0.0))
+        launcher = (20.0 * np.cos(ang), 20.0 * np.sin(ang), 0.0)
+        kern = lgs_kernel(g, launcher_pos=launcher)
         plan = SHPlan(g, lgs=True, max_frames=1, device=device, kernels_normalised=kern.kernels_normalised())
         plan.set_kernels(torch.tensor(kern.lag_spectra(g.ovs_np_sub), device=dev))
         e = make_entry(f'lgs40_{k}', plan, wl.pupil(), 1, with_slopes=with_slopes, device=device)
+        e.meta.update(workload=wl.name, launcher_pos=launcher)
         wl.seed = 100 + k
         e.phase.copy_(torch.tensor(wl.frames(1).astype(np.float32), device=dev))
         entries.append(e)
@@ -139,6 +141,7 @@ def multi_sensor_entries(n_lgs=6, n_pushpull=2000, amp_nm=50.0, device=0, with_s
         g = wl.geometry()
         e = make_entry(wl.name, SHPlan(g, max_frames=1, device=device), wl.pupil(), 1, with_slopes=with_slopes,
                        device=device)
+        e.meta.update(workload=wl.name)
         wl.seed = 100 + n_lgs + j
         e.phase.copy_(torch.tensor(wl.frames(1).astype(np.float32), device=dev))
         entries.append(e)
@@ -147,6 +150,7 @@ def multi_sensor_entries(n_lgs=6, n_pushpull=2000, amp_nm=50.0, device=0, with_s
         g = wl.geometry()
         e = make_entry('elt80_pushpull', SHPlan(g, max_frames=n_pushpull, device=device), wl.pupil(), n_pushpull,
                        with_slopes=with_slopes, device=device)
+        e.meta.update(workload=wl.name, amp_nm=amp_nm)
         half = n_pushpull // 2
         modes = fourier_modes(wl.pupil(), half, device=dev)
         phi0 = entries[-1].phase[0]

@@ -65,7 +65,7 @@ def test_vjp_matches_gpu_finite_differences_50_directions():
     err = ((adj - fd).abs() / fd.pow(2).mean().sqrt()).cpu().numpy()
     print(f'50 directions: median rel err {np.median(rel):.2e}, max {rel.max():.2e}; '
           f'max err / rms(fd) {err.max():.2e}', flush=True)
-    assert np.median(rel) < 1e-2 and err.max() < 1e-2
+    assert np.median(rel) < 2e-3 and err.max() < 3e-3
 
 
 def test_autograd_function():
@@ -149,8 +149,13 @@ def test_adjoint_im_matches_pushpull():
     np.testing.assert_allclose(im_adj.double().cpu().numpy(), ref, rtol=1e-4, atol=1e-6 * np.abs(ref).max())
     im_pp = sen.pushpull_interaction_matrix(ph, modes, amp=50.0)
     rel = float((im_adj - im_pp).norm() / im_pp.norm())
+    half = sen.pushpull_interaction_matrix(ph, modes, amp=25.0)
     small = sen.pushpull_interaction_matrix(ph, modes, amp=5.0)
     rel_small = float((im_adj - small).norm() / small.norm())
-    print(f'IM rel diff vs push-pull: a=50 nm {rel:.2e}, a=5 nm {rel_small:.2e}', flush=True)
-    assert rel_small < rel or rel_small < 1e-3
-    assert rel < 5e-2 and rel_small < 1e-2
+    # Richardson extrapolation (4 IM(a/2) - IM(a)) / 3 cancels push-pull's O(a^2) term
+    rich = (4 * half.double() - im_pp.double()) / 3
+    rel_rich = float((im_adj.double() - rich).norm() / rich.norm())
+    print(f'IM rel diff vs push-pull: a=50 nm {rel:.2e}, a=5 nm {rel_small:.2e}, Richardson {rel_rich:.2e}',
+          flush=True)
+    assert rel_small < rel
+    assert rel_rich <= 1e-3 and rel_small <= 1e-3

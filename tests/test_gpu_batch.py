@@ -66,3 +66,62 @@ def test_autotuner(entries, tmp_path, monkeypatch):
     wl = workloads.elt_ngs_80x80()
     best, times = autotune.tune_plan_variant(wl.geometry(), wl.pupil(), n_frames=8)
     assert best in times and set(times) == {1, 2, 3}
+
+
+# --------------------------------------------------------------------------- the full uneven work list
+
+def test_full_multi_sensor_work_list_matches_per_sensor_and_oracle():
+    """The whole uneven batch of the benchmark: 6 LGS 40x40 (N 128, launchers on a 20 m circle),
+    NGS 10x10 and 20x20 (N 32) and 80x80 (N 64), and 2000 push-pull fields of the 80x80 sensor
+    (2009 frames) in one persistent SH launch plus the per-sensor CoG launches, replayed from a HIP
+    graph.  Images and slopes equal the per-sensor launches bit for bit; one frame of every sensor
+    type (including the 10x10 / N 32 sensor and both signs of a push-pull pair) matches the FP64
+    oracle chain (images 1e-4, slopes 1e-3 CCD px)."""
+    from helpers import check_intensity, check_slopes, oracle_geometry
+    from oracle import sh_oracle as O
+    full = workloads.multi_sensor_entries(n_lgs=6, n_pushpull=2000)
+    assert [e.n_frames for e in full] == [1] * 9 + [2000]
+    print('full work list: ' + ', '.join(f'{e.name} x{e.n_frames}' for e in full), flush=True)
+    refs, srefs = [], []
+    for e in full:
+        out = torch.empty_like(e.ccd)
+        e.plan.forward(e.pupil, e.phase, out, e.S0, n_frames=e.n_frames)
+        s = torch.empty_like(e.slopes)
+        e.slopec.forward(out, s, torch.empty_like(e.flux), torch.empty_like(e.totals), n_frames=e.n_frames)
+        refs.append(out)
+        srefs.append(s)
+    b = MultiSensorBatch(full, grid_blocks=0, order=2)
+    b.capture()
+    for rep in range(2):
+        for e in full:
+            e.ccd.fill_(-1.0)
+            e.slopes.zero_()
+        torch.cuda.synchronize()
+        b.replay()
+        torch.cuda.synchronize()
+        for e, r, s in zip(full, refs, srefs):
+            assert torch.equal(e.ccd, r) and torch.equal(e.slopes, s), (rep, e.name)
+    lat = b.latencies_us()
+    assert np.all(lat > 0)
+    checked = {}
+    for e in full:
+        wl = getattr(workloads, e.meta['workload'])()
+        key = e.name if e.name.startswith('elt80') else e.meta['workload']
+        if key in checked:
+            continue
+        go = oracle_geometry(wl)
+        A = e.pupil.cpu().numpy().astype(np.float64)
+        kern = None
+        if 'launcher_pos' in e.meta:
+            kern = workloads.lgs_kernel(e.plan.geom, launcher_pos=e.meta['launcher_pos']).kernels
+        idxs = e.slopec.idxs
+        frames = [0, 1000] if e.n_frames > 1 else [0]
+        for f in frames:
+            ph = e.phase[f].cpu().numpy().astype(np.float64)
+            ref = O.sh_forward(A, ph, go, kernels=kern)
+            st = check_intensity(e.ccd[f].cpu().numpy(), ref, go['n_lenses'], go['subap_npx'])
+            err = check_slopes(e.slopes[f].cpu().numpy(), O.shslopec(ref, idxs)['slopes'], go['subap_npx'])
+            print(f'  {e.name} frame {f}: image {st["img"]:.1e}, slopes {err / (2 / (go["subap_npx"] - 1)):.1e} px',
+                  flush=True)
+        checked[key] = True
+    assert set(checked) == {'lgs_40x40', 'scao_ngs_10x10', 'scao_ngs_20x20', 'elt_ngs_80x80', 'elt80_pushpull'}

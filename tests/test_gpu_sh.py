@@ -251,7 +251,8 @@ def test_static_pupil_registration():
     A2[: g.ef_size // 2] *= 0.5                      # other storage: recomputed per call
     ref2 = torch.empty_like(ref)
     plan.forward(A2, ph, ref2, 2.0, n_frames=2)
-    plan.forward(A, ph, out, 2.0, n_frames=2)      # back to A: recomputed, registration dropped
+    assert plan.static_pupil_is(A, 2.0, 0)          # another amplitude leaves the registration
+    plan.forward(A, ph, out, 2.0, n_frames=2)      # back to A: served from the registration
     torch.cuda.synchronize()
     assert torch.equal(out, ref) and not torch.equal(ref2, ref)
     assert abs(float(ref2.sum()) / float(ref.sum()) - float(A2.sum()) / float(A.sum())) < 1e-5
@@ -260,6 +261,41 @@ def test_static_pupil_registration():
     torch.cuda.synchronize()
     assert torch.equal(out, ref2)
 
+
+
+def test_registration_survives_other_calls_and_graph_replays():
+    """A forward with another S0 or amplitude computes its statistics in buffers of its own: the
+    registered pupil stays registered (library query), and a graph captured on the registered
+    pupil replays the same image after such calls."""
+    wl = workloads.elt_ngs_80x80()
+    g = wl.geometry()
+    A = torch.tensor(wl.pupil().astype(np.float32), device=DEV)
+    ph = torch.tensor(wl.frames(2).astype(np.float32), device=DEV)
+    plan = SHPlan(g, max_frames=2, device=0)
+    plan.set_static_pupil(A, 1.0)
+    ref = torch.empty((2, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    plan.forward(A, ph, ref, 1.0, n_frames=2)
+    other = torch.empty_like(ref)
+    plan.forward(A, ph, other, 3.0, n_frames=2)                      # same A, other S0
+    A_half = A * 0.5
+    plan.forward(A_half, ph, other, 1.0, n_frames=2)                  # other amplitude
+    assert plan.static_pupil_is(A, 1.0, 0) and not plan.static_pupil_is(A, 3.0, 0)
+    out = torch.empty_like(ref)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.forward(A, ph, out, 1.0, n_frames=2, stream=s)
+    s.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=s):
+        plan.forward(A, ph, out, 1.0, n_frames=2, stream=s)
+    for k in range(3):
+        out.zero_()
+        plan.forward(A_half, ph, other, 5.0 + k, n_frames=2)          # eager calls between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), k
+    torch.testing.assert_close(other.sum(), ref.sum() * 7.0 * 0.5, rtol=1e-5, atol=0)
 
 
 @pytest.mark.parametrize('np_sub', [5, 6, 7])
