@@ -200,6 +200,18 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
                            int n_modes, const float* modes, int64_t mode_stride, const int32_t* slot,
                            int n_valid, float scale, float* im, void* work, void* stream);
 
+/* Same interaction matrix without any allocation and without a workspace argument: no reference
+ * counterpart.  For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The grid-order results go through a
+ * library-owned 16 MiB device scratch array (one per GPU, held until the process ends), modes in
+ * chunks of (16 MiB - n_valid * 4 bytes) / (8 n_lenses^2 bytes) (327 modes per chunk at 80 x 80);
+ * results equal shwfs_im_from_jacobian with a workspace bit for bit.  Rejects with SHWFS_ERR_ARG a
+ * geometry whose single mode does not fit (n_valid * 4 + 8 n_lenses^2 > 16 MiB).  Calls that
+ * overlap on one GPU (other streams, other threads) share the scratch and race: serialise them. */
+int shwfs_im_from_jacobian_noalloc(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy,
+                                   int n_modes, const float* modes, int64_t mode_stride, const int32_t* slot,
+                                   int n_valid, float scale, float* im, void* stream);
+
 /* ---- multi-sensor batch -----------------------------------------------------------------
  * No reference counterpart (the reference triggers one SH object at a time, each optionally a
  * captured CUDA graph, base_processing_obj.py:114-155).  One persistent launch per body type

@@ -24,11 +24,16 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/shwfs.h"
 #include "fft_reg.h"
+
+// Scratch of the no-allocation interaction-matrix entry point (see scratch_of_current_device).
+constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 using namespace shwfs;
 
@@ -2700,6 +2705,116 @@ VjpLauncher pick_vjp(int np, int M, int npx, int* ts) {
     if (np == 16 && M == 32) { *ts = 16; return launch_vjp<16, 32, 16, 0>; }
     return nullptr;
 }
+// ---------------------------------------------------------------- interaction-matrix launches
+// Argument checks shared by shwfs_im_from_jacobian and its no-allocation variant.
+int im_check_args(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
+                  const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float* im) {
+    if (!jx || !jy || !modes || !slot || !im) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_lenses <= 0 || np_sub <= 0 || n_modes <= 0 || n_valid <= 0 || ef_size != n_lenses * np_sub)
+        return set_error(SHWFS_ERR_ARG, "inconsistent sizes");
+    if (ef_size > 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "pupil wider than 1024 px");
+    if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
+    if (n_modes > 65535) return set_error(SHWFS_ERR_ARG, "more than 65535 modes in one call");
+    if (np_sub != 4 && np_sub != 8 && np_sub != 16)
+        return set_error(SHWFS_ERR_UNSUPPORTED, "interaction matrix kernels instantiated for np_sub 4, 8, 16");
+    return SHWFS_OK;
+}
+
+bool im_v4_eligible(int np_sub, int ef_size, int64_t mode_stride, const float* modes, const float* jx,
+                    const float* jy) {
+    return (np_sub == 8 || np_sub == 16) && ef_size % 4 == 0 && mode_stride % 4 == 0 &&
+           ((reinterpret_cast<uintptr_t>(modes) | reinterpret_cast<uintptr_t>(jx) |
+             reinterpret_cast<uintptr_t>(jy)) & 15u) == 0;
+}
+
+constexpr int kImModesPerBlock = 16;
+
+// Scalar contraction, results written in slot order (no scratch).
+int im_launch_scalar(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
+                     const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float scale, float* im,
+                     hipStream_t st) {
+    dim3 grid(n_lenses, (n_modes + kImModesPerBlock - 1) / kImModesPerBlock);
+    const int threads = (ef_size + 63) / 64 * 64;
+#define IMJ(NPV) hipLaunchKernelGGL(im_from_jacobian_kernel<NPV>, grid, dim3(threads), 0, st, jx, jy, ef_size, \
+                                    n_lenses, modes, mode_stride, n_modes, kImModesPerBlock, slot, n_valid, scale, im)
+    switch (np_sub) {
+        case 4: IMJ(4); break;
+        case 8: IMJ(8); break;
+        default: IMJ(16); break;
+    }
+#undef IMJ
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+// float4 contraction.  gridbuf == nullptr: results written in slot order directly.  Otherwise the
+// modes go in chunks of `chunk`: each chunk's results are written in grid order into gridbuf
+// (chunk * 2 * n_lenses^2 floats, coalesced) and permuted into slot order by a gather pass through
+// inv (n_valid int32, built once).
+int im_launch_v4(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
+                 const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float scale, float* im,
+                 float* gridbuf, int32_t* inv, int chunk, hipStream_t st) {
+    const int threads4 = (ef_size / 4 * (np_sub / 4) + 63) / 64 * 64;
+    const int nn = n_lenses * n_lenses;
+    const bool grid_order = gridbuf != nullptr;
+    if (grid_order) {
+        hipLaunchKernelGGL(slot_inverse_kernel, dim3(1), dim3(1024), 0, st, slot, nn, n_valid, inv);
+        SHWFS_LAUNCH_CHECK();
+    } else {
+        chunk = n_modes;
+    }
+    for (int m0 = 0; m0 < n_modes; m0 += chunk) {
+        const int c = std::min(chunk, n_modes - m0);
+        const float* md = modes + (int64_t)m0 * mode_stride;
+        float* im_c = im + (int64_t)m0 * 2 * n_valid;
+        dim3 grid(n_lenses, (c + kImModesPerBlock - 1) / kImModesPerBlock);
+#define IMJ4(NPV, G) hipLaunchKernelGGL((im_from_jacobian_v4_kernel<NPV, G>), grid, dim3(threads4), 0, st, jx, jy,     \
+                                        ef_size, n_lenses, md, mode_stride, c, kImModesPerBlock, slot, n_valid, scale, \
+                                        G ? gridbuf : im_c)
+        if (np_sub == 8) {
+            if (grid_order) IMJ4(8, true); else IMJ4(8, false);
+        } else {
+            if (grid_order) IMJ4(16, true); else IMJ4(16, false);
+        }
+#undef IMJ4
+        SHWFS_LAUNCH_CHECK();
+        if (grid_order) {
+            const int lds = 2 * nn * (int)sizeof(float);
+            if (lds <= 64 * 1024)
+                hipLaunchKernelGGL(im_gather_lds_kernel, dim3(c), dim3(256), lds, st, gridbuf, inv, nn, n_valid, im_c);
+            else
+                hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, c), dim3(256), 0, st, gridbuf, inv,
+                                   nn, n_valid, im_c);
+            SHWFS_LAUNCH_CHECK();
+        }
+    }
+    return SHWFS_OK;
+}
+
+// ---------------------------------------------------------------- no-allocation scratch
+// Scratch of shwfs_im_from_jacobian_noalloc: a file-scope device array, one copy per GPU, held until
+// the process ends.  Host code never passes `scratch` itself to a kernel (in host code the name is
+// a stand-in, not a device address): the device address is looked up once per device with
+// hipGetSymbolAddress while that device is current, and that pointer is used everywhere.
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxScratchDevices] = {};
+
+int scratch_of_current_device(unsigned char** out) {
+    int dev = -1;
+    SHWFS_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices) return set_error(SHWFS_ERR_ARG, "device index outside the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        SHWFS_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return set_error(SHWFS_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return SHWFS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2780,61 +2895,42 @@ int shslopec_vjp(shslopec_plan* p, int n_frames, const float* pixels, int64_t pi
 int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
                            const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float scale,
                            float* im, void* work, void* stream) {
-    if (!jx || !jy || !modes || !slot || !im) return set_error(SHWFS_ERR_ARG, "null argument");
-    if (n_lenses <= 0 || np_sub <= 0 || n_modes <= 0 || n_valid <= 0 || ef_size != n_lenses * np_sub)
-        return set_error(SHWFS_ERR_ARG, "inconsistent sizes");
-    if (ef_size > 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "pupil wider than 1024 px");
-    if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
-    if (n_modes > 65535) return set_error(SHWFS_ERR_ARG, "more than 65535 modes in one call");
+    int rc = im_check_args(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, im);
+    if (rc) return rc;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int mpb = 16;
-    dim3 grid(n_lenses, (n_modes + mpb - 1) / mpb);
-    const bool v4 = (np_sub == 8 || np_sub == 16) && ef_size % 4 == 0 && mode_stride % 4 == 0 &&
-                    ((reinterpret_cast<uintptr_t>(modes) | reinterpret_cast<uintptr_t>(jx) |
-                      reinterpret_cast<uintptr_t>(jy)) & 15u) == 0;
-    if (v4) {
-        const int threads4 = (ef_size / 4 * (np_sub / 4) + 63) / 64 * 64;
-        const int nn = n_lenses * n_lenses;
-        float* gridbuf = static_cast<float*>(work);
-        int32_t* inv = work ? reinterpret_cast<int32_t*>(gridbuf + (int64_t)n_modes * 2 * nn) : nullptr;
-        if (work) {
-            hipLaunchKernelGGL(slot_inverse_kernel, dim3(1), dim3(1024), 0, st, slot, nn, n_valid, inv);
-            SHWFS_LAUNCH_CHECK();
-        }
-#define IMJ4(NPV, G) hipLaunchKernelGGL((im_from_jacobian_v4_kernel<NPV, G>), grid, dim3(threads4), 0, st, jx, jy,    \
-                                        ef_size, n_lenses, modes, mode_stride, n_modes, mpb, slot, n_valid, scale, \
-                                        G ? gridbuf : im)
-        if (np_sub == 8) {
-            if (work) IMJ4(8, true); else IMJ4(8, false);
-        } else {
-            if (work) IMJ4(16, true); else IMJ4(16, false);
-        }
-#undef IMJ4
-        SHWFS_LAUNCH_CHECK();
-        if (work) {
-            const int lds = 2 * nn * (int)sizeof(float);
-            if (lds <= 64 * 1024)
-                hipLaunchKernelGGL(im_gather_lds_kernel, dim3(n_modes), dim3(256), lds, st, gridbuf, inv, nn, n_valid,
-                                   im);
-            else
-                hipLaunchKernelGGL(im_gather_kernel, dim3((n_valid + 255) / 256, n_modes), dim3(256), 0, st, gridbuf,
-                                   inv, nn, n_valid, im);
-            SHWFS_LAUNCH_CHECK();
-        }
-        return SHWFS_OK;
-    }
-    const int threads = (ef_size + 63) / 64 * 64;
-#define IMJ(NPV) hipLaunchKernelGGL(im_from_jacobian_kernel<NPV>, grid, dim3(threads), 0, st, jx, jy, ef_size, \
-                                    n_lenses, modes, mode_stride, n_modes, mpb, slot, n_valid, scale, im)
-    switch (np_sub) {
-        case 4: IMJ(4); break;
-        case 8: IMJ(8); break;
-        case 16: IMJ(16); break;
-        default: return set_error(SHWFS_ERR_UNSUPPORTED, "interaction matrix kernel instantiated for np_sub 4, 8, 16");
-    }
-#undef IMJ
-    SHWFS_LAUNCH_CHECK();
-    return SHWFS_OK;
+    if (!im_v4_eligible(np_sub, ef_size, mode_stride, modes, jx, jy))
+        return im_launch_scalar(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale,
+                                im, st);
+    const int64_t nn = (int64_t)n_lenses * n_lenses;
+    float* gridbuf = static_cast<float*>(work);
+    int32_t* inv = work ? reinterpret_cast<int32_t*>(gridbuf + (int64_t)n_modes * 2 * nn) : nullptr;
+    return im_launch_v4(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale, im,
+                        gridbuf, inv, n_modes, st);
+}
+
+int shwfs_im_from_jacobian_noalloc(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy,
+                                   int n_modes, const float* modes, int64_t mode_stride, const int32_t* slot,
+                                   int n_valid, float scale, float* im, void* stream) {
+    int rc = im_check_args(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, im);
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!im_v4_eligible(np_sub, ef_size, mode_stride, modes, jx, jy))   // this path needs no scratch
+        return im_launch_scalar(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale,
+                                im, st);
+    const int64_t nn = (int64_t)n_lenses * n_lenses;
+    const size_t inv_bytes = ((size_t)n_valid * sizeof(int32_t) + 255) / 256 * 256;
+    const size_t mode_bytes = (size_t)2 * nn * sizeof(float);        // grid-order results of one mode
+    if (inv_bytes + mode_bytes > kScratchBytes)
+        return set_error(SHWFS_ERR_ARG, "shwfs_im_from_jacobian_noalloc: n_valid and n_lenses^2 need more than the "
+                                        "16 MiB scratch for one mode (n_valid * 4 + 8 n_lenses^2 bytes)");
+    unsigned char* base = nullptr;
+    rc = scratch_of_current_device(&base);
+    if (rc) return rc;
+    int32_t* inv = reinterpret_cast<int32_t*>(base);
+    float* gridbuf = reinterpret_cast<float*>(base + inv_bytes);
+    const int chunk = (int)std::min<size_t>((size_t)n_modes, (kScratchBytes - inv_bytes) / mode_bytes);
+    return im_launch_v4(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale, im,
+                        gridbuf, inv, chunk, st);
 }
 
 }  // extern "C"

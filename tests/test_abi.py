@@ -110,3 +110,12 @@ def test_sensor_vjp_rejects_mismatched_frames_before_any_launch():
         s.vjp(torch.zeros((3, 16, 15)), torch.zeros((3, 10)))
     with pytest.raises(ValueError, match='cotangents > max_frames'):
         s.vjp(torch.zeros((16, 16)), torch.zeros((5, 10)))
+
+
+def test_noalloc_im_rejects_sizes_beyond_its_scratch_before_any_launch():
+    """shwfs_im_from_jacobian_noalloc: n_valid * 4 + 8 n_lenses^2 bytes must fit the 16 MiB scratch;
+    the size check precedes any HIP call (aligned dummy pointers are never dereferenced)."""
+    lib = _native.load_library()
+    p = ctypes.c_void_p(1 << 20)
+    rc = lib.shwfs_im_from_jacobian_noalloc(80, 8, 640, p, p, 10, p, 640 * 640, p, 5_000_000, 2.0, p, None)
+    assert rc == -1 and b'16 MiB scratch' in lib.shwfs_last_error()
