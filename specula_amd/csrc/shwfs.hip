@@ -512,12 +512,27 @@ __device__ __forceinline__ void ngs_row_load(const float* base, int off, bool ve
 
 // Phase row of frame f (only where the lane's pupil row has a nonzero sample: elsewhere the field
 // is zero whatever the phase) and the frame's flux scale (Parseval mode), loaded together so that
-// their latency is hidden together.
+// their latency is hidden together.  Lanes that do not load keep their registers: the caller zeroes
+// them once, and a lane's `lit` does not change across the frames of one call.
 template <int NP, bool PARSEVAL>
 __device__ __forceinline__ void ngs_phase_load(const NgsArgs& g, int off, int f, bool lit, float (&pv)[NP],
                                                float& fs) {
     fs = PARSEVAL ? g.fscale[f * g.scale_fstep] : 1.f;
-    ngs_row_load<NP>(g.phase + f * g.ph_stride, lit ? off : -1, g.vec, pv);
+    if (lit && off >= 0) {
+        const float* r = g.phase + f * g.ph_stride + (unsigned)off;
+        if constexpr (NP % 4 == 0) {
+            if (g.vec) {
+#pragma unroll
+                for (int c = 0; c < NP / 4; ++c) {
+                    const float4 q = reinterpret_cast<const float4*>(r)[c];
+                    pv[4 * c] = q.x; pv[4 * c + 1] = q.y; pv[4 * c + 2] = q.z; pv[4 * c + 3] = q.w;
+                }
+                return;
+            }
+        }
+#pragma unroll
+        for (int x = 0; x < NP; ++x) pv[x] = r[x];
+    }
 }
 
 __device__ __forceinline__ bool any_nonzero_row(const float* v, int n) {
@@ -546,7 +561,7 @@ struct NgsWaveConst {
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& wv, int wave_id, int f,
                                                const float (&av)[NP], const float (&phv)[NP], float fs, float* sw,
-                                               const NgsWaveConst<NP, NPX, COG>& wc) {
+                                               const NgsWaveConst<NP, NPX, COG>& wc, float sq_sub) {
     using Sh = NgsMfmaShape<NP>;
     constexpr int M = Sh::M;
     constexpr int SW = Sh::SW;
@@ -695,53 +710,65 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
         const int x4 = lane % X4, q = (lane / X4) % SW, h = lane / LPR;
         const bool live = q < wv.nq;
         (void)x4; (void)W;
-        const int dst0 = wc.st_off;
-        // this lane's 4 x NY pixels: flux, thresholded column sums, y moment (FP32 partials)
-        float l_f = 0.f, l_y = 0.f, cw0 = 0.f, cw1 = 0.f, cw2 = 0.f, cw3 = 0.f;
-        float* ccdw = ccdf + dst0;
+        float* ccdw = ccdf + wc.st_off;
+        // this lane's 4 x NY pixels: thresholded column sums and y moment (FP32 partials)
+        float l_y = 0.f, cw0 = 0.f, cw1 = 0.f, cw2 = 0.f, cw3 = 0.f;
         if (live) {
+            if (dark) {                  // wave-uniform: no transform ran, every pixel is 0 x scale
+                const floatx4 o = {dark_value, dark_value, dark_value, dark_value};
 #pragma unroll
-            for (int t = 0; t < NY; ++t) {
-                const int Y = h + YSTEP * t;
-                if (NPX % YSTEP != 0 && Y >= NPX) break;
-                float4 v = make_float4(dark_value, dark_value, dark_value, dark_value);
-                if (!dark) v = *reinterpret_cast<const float4*>(sw + q * QS + (Y << 4) + ((x4 ^ lds_swz(Y)) << 2));
-                const floatx4 o = {v.x, v.y, v.z, v.w};
-                __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(ccdw + (unsigned)(YSTEP * t * W)));
-                if constexpr (!PARSEVAL) bsum += (double)((v.x + v.y) + (v.z + v.w));
-                if constexpr (COG != 0) {
-                    float w0, w1, w2, w3;
-                    if constexpr (COG == 1) {          // subtract the threshold, clamp at zero
-                        w0 = clamp0(v.x - g.thr); w1 = clamp0(v.y - g.thr);
-                        w2 = clamp0(v.z - g.thr); w3 = clamp0(v.w - g.thr);
-                    } else if constexpr (COG == 2) {   // pedestal
-                        w0 = v.x < g.thr ? 0.f : v.x; w1 = v.y < g.thr ? 0.f : v.y;
-                        w2 = v.z < g.thr ? 0.f : v.z; w3 = v.w < g.thr ? 0.f : v.w;
-                    } else {                           // no threshold: clamp at zero only
-                        w0 = clamp0(v.x); w1 = clamp0(v.y); w2 = clamp0(v.z); w3 = clamp0(v.w);
+                for (int t = 0; t < NY; ++t) {
+                    if (NPX % YSTEP != 0 && h + YSTEP * t >= NPX) break;
+                    __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(ccdw + (unsigned)(YSTEP * t * W)));
+                }
+                if constexpr (!PARSEVAL) bsum = 0.0;
+                if constexpr (COG != 0) cw0 = cw1 = cw2 = cw3 = l_y = dark_value;
+            } else {
+#pragma unroll
+                for (int t = 0; t < NY; ++t) {
+                    const int Y = h + YSTEP * t;
+                    if (NPX % YSTEP != 0 && Y >= NPX) break;
+                    const float4 v = *reinterpret_cast<const float4*>(sw + q * QS + (Y << 4) + ((x4 ^ lds_swz(Y)) << 2));
+                    const floatx4 o = {v.x, v.y, v.z, v.w};
+                    __builtin_nontemporal_store(o, reinterpret_cast<floatx4*>(ccdw + (unsigned)(YSTEP * t * W)));
+                    if constexpr (!PARSEVAL) bsum += (double)((v.x + v.y) + (v.z + v.w));
+                    if constexpr (COG != 0) {
+                        float w0, w1, w2, w3;
+                        if constexpr (COG == 1) {          // subtract the threshold, clamp at zero
+                            w0 = clamp0(v.x - g.thr); w1 = clamp0(v.y - g.thr);
+                            w2 = clamp0(v.z - g.thr); w3 = clamp0(v.w - g.thr);
+                        } else if constexpr (COG == 2) {   // pedestal
+                            w0 = v.x < g.thr ? 0.f : v.x; w1 = v.y < g.thr ? 0.f : v.y;
+                            w2 = v.z < g.thr ? 0.f : v.z; w3 = v.w < g.thr ? 0.f : v.w;
+                        } else {
+                            // no threshold: the reference clamps at zero (sh_slopec.py:382-386), a no-op on
+                            // the exact image, which is a sum of squared moduli; the FP32 image's negative
+                            // pixels are rounding residues far below its own error, so they are kept
+                            w0 = v.x; w1 = v.y; w2 = v.z; w3 = v.w;
+                        }
+                        const float yv = sw[kYwWord + Y];   // y weight of row Y (kept in LDS padding)
+                        l_y = fmaf((w0 + w1) + (w2 + w3), yv, l_y);
+                        cw0 += w0; cw1 += w1; cw2 += w2; cw3 += w3;
                     }
-                    const float yv = sw[kYwWord + Y];   // y weight of row Y (kept in LDS padding)
-                    // flux: the unclamped pixel sum (sh_slopec.py:365), as the two-pass CoG takes it
-                    l_f += (v.x + v.y) + (v.z + v.w);
-                    l_y = fmaf((w0 + w1) + (w2 + w3), yv, l_y);
-                    cw0 += w0; cw1 += w1; cw2 += w2; cw3 += w3;
                 }
             }
         }
         if constexpr (COG != 0) {
-            // FP32 partials of 4 x NY pixels per lane; FP64 across lanes (and subapertures)
-            const float l_t = (cw0 + cw1) + (cw2 + cw3);
+            // flux (pre-threshold subaperture sum, sh_slopec.py:365) from the Parseval identity of this
+            // geometry: the subaperture's image sums to norm_const x its sum(A^2), times the frame scale
+            const double m_f = (double)sq_sub * ((double)fs * g.norm_const);
+            // FP32 partials of 4 x NY pixels per lane; FP64 across lanes
             const float l_x = fmaf(cw3, wc.xw[3], fmaf(cw2, wc.xw[2], fmaf(cw1, wc.xw[1], cw0 * wc.xw[0])));
-            double m_f = l_f, m_t = l_t, m_x = l_x, m_y = l_y;
+            // the total comes from the same pixels as the x and y sums (sh_slopec.py:392), so that
+            // errors common to the subaperture's pixels cancel in the slopes
+            double m_x = l_x, m_y = l_y, m_t = (double)((cw0 + cw1) + (cw2 + cw3));
 #pragma unroll
             for (int o = 1; o < X4; o <<= 1) {
-                m_f += __shfl_xor(m_f, o, kWave);
                 m_t += __shfl_xor(m_t, o, kWave);
                 m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
             }
 #pragma unroll
             for (int o = LPR; o < 64; o <<= 1) {
-                m_f += __shfl_xor(m_f, o, kWave);
                 m_t += __shfl_xor(m_t, o, kWave);
                 m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
             }
@@ -824,6 +851,22 @@ __device__ __forceinline__ NgsWaveConst<NP, NPX, COG> ngs_wave_const(const NgsAr
     return wc;
 }
 
+// Sum of A^2 over the pupil samples of a subaperture, delivered to the lanes of the CCD store layout
+// (NgsStoreShape) that hold it: per pupil row in the load layout (lane (q, y)), across the NP rows of
+// subaperture q, then fetched by the store lanes of q.  FP32 (exact for a 0/1 pupil).
+template <int NP, int NPX>
+__device__ __forceinline__ float subap_sq_store_layout(const float (&av)[NP]) {
+    static_assert((NP & (NP - 1)) == 0, "rows of a subaperture are reduced with xor steps");
+    float s = 0.f;
+#pragma unroll
+    for (int x = 0; x < NP; ++x) s = fmaf(av[x], av[x], s);
+#pragma unroll
+    for (int o = 1; o < NP; o <<= 1) s += __shfl_xor(s, o, kWave);
+    using St = NgsStoreShape<NP, NPX>;
+    const int q = ((threadIdx.x & 63) / St::X4) % St::SW;
+    return __shfl(s, q * NP, kWave);
+}
+
 // Frames [f0, f1) of the wave's subapertures; the next frame's pupil loads are issued before the
 // current frame is transformed, so HBM latency overlaps the FFT / MFMA work.
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
@@ -839,21 +882,30 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
         return;
     }
     const NgsWaveConst<NP, NPX, COG> wc = ngs_wave_const<NP, NPX, COG>(g, wv, sw);
-    float av[NP], pv[NP], an[NP], pn[NP], fs, fn = 1.f;
+    float av[NP], pv[NP], pn[NP], fs, fn = 1.f;
     ngs_row_load<NP>(g.A + f0 * g.a_stride, wc.ld_off, g.vec, av);
-    // static pupil (a_stride 0): a lane whose pupil row has no nonzero sample never loads its
-    // phase (the field there is zero whatever the phase); a per-frame pupil loads every row
-    const bool lit = g.a_stride != 0 || any_nonzero_row(av, NP);
-    ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f0, lit, pv, fs);
-    for (int f = f0; f < f1; ++f) {
-        if (f + 1 < f1) {
-            ngs_row_load<NP>(g.A + (f + 1) * g.a_stride, wc.ld_off, g.vec, an);
-            ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, lit, pn, fn);
-        }
-        if (f > f0) wave_lds_sync();
-        ngs_mfma_frame<NP, PARSEVAL, NPX, COG, BF3>(g, wv, wave_id, f, av, pv, fs, sw, wc);
+    // static pupil (a_stride 0): the amplitude is loaded once, and a lane whose pupil row has no
+    // nonzero sample never loads its phase (the field there is zero whatever the phase; its phase
+    // registers stay zero); a per-frame pupil loads every row, its amplitude at the top of each
+    // frame (only the phase is prefetched a frame ahead)
+    const bool static_a = g.a_stride == 0;
+    const bool lit = !static_a || any_nonzero_row(av, NP);
 #pragma unroll
-        for (int x = 0; x < NP; ++x) { av[x] = an[x]; pv[x] = pn[x]; }
+    for (int x = 0; x < NP; ++x) { pv[x] = 0.f; pn[x] = 0.f; }
+    ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f0, lit, pv, fs);
+    // fused CoG: sum(A^2) of each store lane's subaperture (once for a static pupil, else per frame)
+    float sq = 0.f;
+    if constexpr (COG != 0 && NPX > 0) sq = subap_sq_store_layout<NP, NPX>(av);
+    for (int f = f0; f < f1; ++f) {
+        if (f + 1 < f1) ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, lit, pn, fn);
+        if (!static_a && f > f0) ngs_row_load<NP>(g.A + f * g.a_stride, wc.ld_off, g.vec, av);
+        if (f > f0) wave_lds_sync();
+        if constexpr (COG != 0 && NPX > 0) {
+            if (f > f0 && g.a_stride != 0) sq = subap_sq_store_layout<NP, NPX>(av);
+        }
+        ngs_mfma_frame<NP, PARSEVAL, NPX, COG, BF3>(g, wv, wave_id, f, av, pv, fs, sw, wc, sq);
+#pragma unroll
+        for (int x = 0; x < NP; ++x) pv[x] = pn[x];
         fs = fn;
     }
 }

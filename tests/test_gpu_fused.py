@@ -57,8 +57,12 @@ def test_fused_equals_two_pass(name, thr_rel, pedestal, variant):
     assert torch.equal(ccd1, ccd2)                       # same kernel arithmetic for the image
     # moments: FP32 partials of 4 x 8 pixels per lane (fused) vs FP64 per pixel (two-pass)
     torch.testing.assert_close(s1, s2, rtol=0, atol=2e-6)
-    torch.testing.assert_close(f1, f2, rtol=2e-6, atol=0)
-    torch.testing.assert_close(t1, t2, rtol=1e-6, atol=0)
+    # flux: the fused path takes the exact Parseval value (fs norm sum(A^2) of the subaperture), the
+    # two-pass path sums the image pixels: they differ by the image's own flux error (FP32 MFMA
+    # ~1e-7, bf16-split ~1e-5 relative); both are checked against the oracle below
+    ftol = 2e-6 if variant != 3 else 2e-5
+    torch.testing.assert_close(f1, f2, rtol=ftol, atol=0)
+    torch.testing.assert_close(t1, t2, rtol=ftol, atol=0)
     out = ccd1[0].double().cpu().numpy()
     ref = O.shslopec(out, idxs, thr_value=thr if thr > 0 else -1, thr_pedestal=pedestal)
     check_slopes(s1[0].cpu().numpy(), ref['slopes'], g.subap_npx)
