@@ -24,7 +24,15 @@ HBM_PEAK_GBS = 8000.0
 # Keyed by (CoG fused, frames per launch, default frames per block).  The static pupil is served
 # from L2 after its first read and phase rows under a dark pupil row are not read, so the traffic
 # is below the algorithmic bytes (pupil and phase counted in full per frame).
-MEASURED_TRAFFIC = {(True, 100, True): 834_600_000}
+MEASURED_TRAFFIC = {(True, 100, True): 835_400_000}
+# Executed floating-point work per frame of the main kernel, from rocprofv3 counters of the same
+# launches (profiles/pmc_flops_elt_ngs_80x80.md): SQ_INSTS_VALU_MFMA_MOPS_F32 x 512 (FP32 MFMA) +
+# (SQ_INSTS_VALU_FLOPS_FP32 + SQ_INSTS_VALU_FLOPS_FP64) x 64 lanes, divided by the 100 frames.
+MEASURED_FLOP_PER_FRAME = {(True, True): 111_692_800}
+# LGS kernel (lgs_40x40, 50-frame launches): measured HBM bytes per frame (FETCH_SIZE x2.0 +
+# WRITE_SIZE x1.0, profiles/pmc_traffic_lgs_40x40.md); the kernel spectra come from the caches
+# after the first frame.
+MEASURED_LGS_TRAFFIC_PER_FRAME = 3_308_000
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -53,6 +61,17 @@ def parse():
     if unknown:
         ap.error(f'unknown extras {sorted(unknown)}')
     return args
+
+
+def executed_flops(fused, default_fpb, frames, kern_ms):
+    """Executed FP32 + FP64 arithmetic of the main kernel (counter-measured per frame) per second,
+    against MI355X's FP32 peak."""
+    per = MEASURED_FLOP_PER_FRAME.get((bool(fused), bool(default_fpb)))
+    if per is None:
+        return None
+    tflops = per * frames / (kern_ms * 1e-3) / 1e12
+    return {'flop_per_frame': per, 'tflops': round(tflops, 2), 'fp32_peak_tflops': FP32_PEAK_TFLOPS,
+            'fp32_peak_frac': round(tflops / FP32_PEAK_TFLOPS, 4)}
 
 
 def progress(msg):
@@ -142,8 +161,11 @@ def bench_lgs(torch, dev):
     frame_bytes = 4 * (2 * g.ef_size ** 2 + g.ccd_side ** 2) + kbytes          # pupil + phase + CCD + spectra
     gbps = frame_bytes * F / dt / 1e9
     return {'workload': 'lgs_40x40', 'frames_per_s': round(F / dt, 1), 'ms_per_frame': round(dt * 1e3 / F, 5),
+            # algorithmic bytes count the kernel spectra on the Hermitian half of the lag grid
+            # (|m| <= np - 1, m_x >= 0: all the correlation uses), not the full N x N spectra
             'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'algorithmic_MB_per_frame': round(frame_bytes / 1e6, 2),
             'algorithmic_GBps': round(gbps, 1), 'hbm_frac': round(gbps / HBM_PEAK_GBS, 4),
+            'measured_traffic_MB_per_frame': round(MEASURED_LGS_TRAFFIC_PER_FRAME / 1e6, 3),
             'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
 
 
@@ -451,8 +473,10 @@ def main():
         'subap_ffts_per_s': round(fps * g.n_lenses ** 2, 1),
         'sh_kernel_ms_per_launch': round(kern_ms, 5),
         'sh_kernel_frames_per_s': round(F / (kern_ms * 1e-3), 1),
-        'fft_gflops_nominal': round(fft_flop * F / (kern_ms * 1e-3) / 1e9, 1),
-        'fft_fp32_peak_frac': round(fft_flop * F / (kern_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS, 4),
+        # the reference's FFT work (N = 64 padded transforms, 5 N^2 log2 N^2 per subaperture) per
+        # second: an equivalent rate, not executed arithmetic (the kernel never runs those FFTs)
+        'fft_equivalent_gflops': round(fft_flop * F / (kern_ms * 1e-3) / 1e9, 1),
+        'executed': executed_flops(fused, args.fpb in (0, 4), F, kern_ms),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4),
                      'traffic': MEASURED_TRAFFIC.get((bool(fused), F, args.fpb in (0, 4))),

@@ -150,7 +150,11 @@ int shslopec_forward(shslopec_plan* plan, int n_frames,
 /* Fused SH forward + ShSlopec: the centre of gravity of each valid subaperture is taken from
  * the binned CCD tile while it is still on chip, so the CCD frames are written once and never
  * read back.  Results equal shwfs_forward followed by shslopec_forward on the CCD frames
- * (same outputs and conventions as those two calls; the moment sums are FP64).
+ * (same outputs and conventions as those two calls; the moment sums are FP64), to FP32 rounding:
+ * the flux of a subaperture is its exact value from the Parseval identity of this geometry
+ * (frame scale x norm_const x the subaperture's sum(A^2)) rather than the sum of its FP32 pixels,
+ * and without threshold the moments keep the FP32 image's rounding-level negative pixels, which the
+ * reference's clamp at zero (sh_slopec.py:382-386) would remove from its exact, non-negative image.
  * shwfs_plan_bind_slopec checks that the pair can be fused: NGS MFMA plan with np_sub 8,
  * Parseval normalisation and 8 or 16 px per subaperture; ShSlopec plan made from block origins
  * lying on the lenslet grid of this CCD, with the default separable weights.  It returns
