@@ -1,7 +1,7 @@
 """Measurement-driven choice of launch shapes.
 
-* ``tune_plan_variant``: register-GEMV, FP32 MFMA or bf16-split MFMA binning for an NGS plan (the
-  matrix cores are used only when they measure faster).
+* ``tune_plan_variant``: register-GEMV, FP32 MFMA or bf16-split MFMA binning, or the any-size LDS FFT
+  path, for an NGS plan (the matrix cores are used only when they measure faster).
 * ``tune_batch``: persistent grid size and work-list order of a MultiSensorBatch.
 
 Results are cached in a JSON file keyed by device name and workload signature
@@ -57,8 +57,10 @@ def device_name(device=0):
     return torch.cuda.get_device_properties(device).name
 
 
-def tune_plan_variant(geom, pupil, n_frames=32, device=0, use_cache=True):
-    """Time the binning variants of an NGS plan; returns (best_variant, {variant: seconds/frame})."""
+def tune_plan_variant(geom, pupil, n_frames=32, device=0, use_cache=True, variants=(1, 2, 3, 4)):
+    """Time the kernel variants of an NGS plan -- register-GEMV binning (1), FP32 MFMA binning (2),
+    bf16-split MFMA binning (3), full M-point LDS FFTs of the any-size path (4) -- and return
+    (best_variant, {variant: seconds/frame}); variants the geometry does not support are skipped."""
     key = f'variant|{device_name(device)}|{geom.n_lenses}|{geom.ovs_np_sub}|{geom.fft_size}|{geom.subap_npx}|{n_frames}'
     if use_cache:
         hit = _load().get(key)
@@ -69,7 +71,7 @@ def tune_plan_variant(geom, pupil, n_frames=32, device=0, use_cache=True):
     ph = torch.zeros((n_frames, geom.ef_size, geom.ef_size), dtype=torch.float32, device=dev)
     out = torch.empty((n_frames, geom.ccd_side, geom.ccd_side), dtype=torch.float32, device=dev)
     times = {}
-    for v in (1, 2, 3):
+    for v in variants:
         try:
             plan = SHPlan(geom, max_frames=n_frames, device=device, variant=v)
         except RuntimeError:

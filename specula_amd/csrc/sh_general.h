@@ -1,0 +1,274 @@
+// sh_general.h -- SH forward for any subaperture sampling: np_sub up to 512, spectrum grid M a power of
+// two from 16 to 1024, any number of CCD pixels per subaperture.  Included by shwfs.hip (uses NgsArgs,
+// wave_sum and the phase reduction of that file).
+//
+// Same spectral formulation as the specialised kernels (B = P S Q^T for NGS, B = Re(P~ D Q~^T) for
+// LGS, S = |FFT_M(E)|^2), evaluated with full FFTs on the M grid instead of register-resident
+// pruned FFT_16/32 and lag-domain autocorrelations, so that the large subapertures of the
+// reference's own set-ups (e.g. np 20 / M 64 LGS, np 80 / M 256, np 121 / M 256, np 240 / M 512,
+// np 300 / M 1024; main/scao/params_eris.yml, params_morfeo_*.yml) run on the GPU.
+//
+// One work item = (frame, subaperture); a persistent grid of 256-thread blocks strides over the
+// items.  Each block owns a slab of device scratch (allocated with the plan) for the intermediates
+// that exceed LDS, and an LDS buffer in which CH vectors of length M are transformed at a time:
+//   1  row pass     rows y < np of E = A exp(i psi) (zero padded to M)  -> FFT_M over x  -> G[y][kx]
+//   2  column pass  columns kx of G (rows >= np are zero)              -> FFT_M over y  -> S = |.|^2
+//        NGS: U[Y][kx] = sum_ky P[Y][ky] S[ky][kx]                      (npx x M, slab)
+//        LGS: S kept (M x M real, slab)
+//   3  NGS: B[Y][X] = sum_kx U[Y][kx] Q[X][kx]
+//      LGS: FFT_M of S over kx (keep m_x = 0 .. np-1, the Hermitian half) -> T[ky][mx]
+//           FFT_M of T over ky (keep |m_y| <= np-1), times the kernel spectrum -> D[my][mx]
+//           V[Y][mx] = sum_my P~[Y][my] D[my][mx];  B[Y][X] = Re sum_mx c_mx V[Y][mx] Q~[X][mx]
+//           (c_0 = 1, c_mx>0 = 2: D and the operators are transforms of real functions)
+//   4  store the npx x npx tile scaled by the frame's Parseval scale, or its unscaled sum for the
+//      non-Parseval rescale pass.
+// FFTs: iterative radix-2 decimation in time after a bit-reversal permutation, twiddles
+// exp(-2 pi i k / M) from a table built on the host in FP64.  Accumulations of the binning products
+// in FP32 over M terms (M <= 1024); the image sum of the non-Parseval mode in FP64.
+
+struct GenArgs {
+    const float2* tw;    // [M/2] exp(-2 pi i k / M)
+    float* slab;         // [gridDim.x][slab_floats]
+    int64_t slab_floats; // floats of scratch per block
+    int M, log2M, np, ch;
+};
+
+constexpr int kGenThreads = 256;
+constexpr int kGenLdsF2 = 8192;           // 64 KiB of complex working vectors per block
+
+// Bit-reverse `vecs` vectors of length M held at buf[v * M + i] (in place, pairs swapped once).
+__device__ __forceinline__ void gen_bitrev(float2* buf, int vecs, int M, int log2M) {
+    for (int idx = threadIdx.x; idx < vecs * M; idx += blockDim.x) {
+        const int v = idx >> log2M, i = idx & (M - 1);
+        const int r = (int)(__brev((unsigned)i) >> (32 - log2M));
+        if (i < r) {
+            float2 a = buf[v * M + i];
+            buf[v * M + i] = buf[v * M + r];
+            buf[v * M + r] = a;
+        }
+    }
+}
+
+// Forward FFT (exp(-2 pi i jk/M), unnormalised) of `vecs` vectors of length M in LDS.
+__device__ void gen_fft(float2* buf, int vecs, int M, int log2M, const float2* __restrict__ tw_lds) {
+    gen_bitrev(buf, vecs, M, log2M);
+    __syncthreads();
+    const int halfM = M >> 1;
+    for (int s = 0, half = 1; half < M; ++s, half <<= 1) {
+        const int tw_step = M >> (s + 1);             // twiddle index stride at this stage
+        for (int idx = threadIdx.x; idx < vecs * halfM; idx += blockDim.x) {
+            const int v = idx / halfM, b = idx - v * halfM;
+            const int j = b & (half - 1);
+            const int k = ((b >> s) << (s + 1)) + j;
+            const float2 w = tw_lds[j * tw_step];
+            float2* p = buf + v * M + k;
+            const float2 a = p[0], c0 = p[half];
+            const float2 c = make_float2(c0.x * w.x - c0.y * w.y, c0.x * w.y + c0.y * w.x);
+            p[0] = make_float2(a.x + c.x, a.y + c.y);
+            p[half] = make_float2(a.x - c.x, a.y - c.y);
+        }
+        __syncthreads();
+    }
+}
+
+// Slab layout of one block (floats): NGS: G [np][M] complex, U [npx][M];
+// LGS: G [np][M] complex (reused for T [M][np] complex), S [M][M], D [L][np] complex.
+__device__ __forceinline__ void gen_slab(const GenArgs& q, int npx, bool lgs, float*& base, float2*& G, float*& U,
+                                         float*& S, float2*& T, float2*& D) {
+    base = q.slab + (int64_t)blockIdx.x * q.slab_floats;
+    G = reinterpret_cast<float2*>(base);
+    T = G;
+    const int64_t g_f = 2LL * q.np * q.M;
+    const int64_t t_f = 2LL * q.M * q.np;
+    const int64_t gt = g_f > t_f ? g_f : t_f;
+    U = base + gt;
+    S = base + gt;
+    D = reinterpret_cast<float2*>(base + gt + (int64_t)q.M * q.M);
+    (void)npx; (void)lgs;
+}
+
+__global__ void __launch_bounds__(kGenThreads)
+sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
+    __shared__ float2 buf[kGenLdsF2];
+    __shared__ float2 tw[512];
+    __shared__ double red[kGenThreads / 32];
+    __shared__ int lit_flag;
+    const int M = q.M, log2M = q.log2M, np = q.np, CH = q.ch;
+    const int npx = g.npx, n = g.n_lenses;
+    const bool lgs = g.kspec != nullptr;
+    const int L = 2 * np - 1;
+    for (int k = threadIdx.x; k < (M >> 1); k += blockDim.x) tw[k] = q.tw[k];
+    float* base;
+    float2 *G, *T, *D;
+    float *U, *S;
+    gen_slab(q, npx, lgs, base, G, U, S, T, D);
+    const int64_t n_items = (int64_t)n_frames * g.n_sub;
+    const double k_rev = g.k * (1.0 / kTwoPi), t_rev = g.tilt * (1.0 / kTwoPi);
+    for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
+        const int f = (int)(item / g.n_sub);
+        const int sub = (int)(item - (int64_t)f * g.n_sub);
+        const int i = sub / n, j = sub - (sub / n) * n;
+        const float* Af = g.A + (int64_t)f * g.a_stride;
+        const float* Pf = g.phase + (int64_t)f * g.ph_stride;
+        const int64_t off = (int64_t)(i * np) * g.ef_size + (int64_t)j * np;
+        const float fs = g.parseval ? g.fscale[f * g.scale_fstep] : 1.f;
+        float* ccdf = g.ccd + (int64_t)f * g.ccd_stride;
+        const int W = n * npx;
+        __syncthreads();                                   // slab and LDS of the previous item are free
+        if (threadIdx.x == 0) lit_flag = 0;
+        __syncthreads();
+        bool lit = false;
+        for (int e = threadIdx.x; e < np * np; e += blockDim.x) {
+            const int y = e / np, x = e - (e / np) * np;
+            lit |= Af[off + (int64_t)y * g.ef_size + x] != 0.f;
+        }
+        if (lit) lit_flag = 1;                             // benign race: every writer stores 1
+        __syncthreads();
+        if (!lit_flag) {                                   // zero image: 0 x scale, no transform
+            const float v = g.parseval ? 0.f * fs : 0.f;
+            for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
+                const int Y = e / npx, X = e - (e / npx) * npx;
+                ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = v;
+            }
+            if (!g.parseval && threadIdx.x == 0) g.bpart[(int64_t)f * g.n_parts + sub] = 0.0;
+            continue;
+        }
+
+        // 1: row transforms of the field (rows y < np), CH rows at a time
+        for (int y0 = 0; y0 < np; y0 += CH) {
+            const int rows = min(CH, np - y0);
+            for (int e = threadIdx.x; e < rows * M; e += blockDim.x) {
+                const int r = e >> log2M, x = e & (M - 1);
+                float2 v = make_float2(0.f, 0.f);
+                if (x < np) {
+                    const int y = y0 + r;
+                    const int64_t o = off + (int64_t)y * g.ef_size + x;
+                    const float a = Af[o];
+                    if (a != 0.f) {
+                        double rv = fma((double)Pf[o], k_rev, -t_rev * (2.0 * g.a0 + (double)(x + y)));
+                        rv -= rint(rv);
+                        const float rf = (float)rv;
+                        v = make_float2(a * __builtin_amdgcn_cosf(rf), a * __builtin_amdgcn_sinf(rf));
+                    }
+                }
+                buf[e] = v;
+            }
+            __syncthreads();
+            gen_fft(buf, rows, M, log2M, tw);
+            for (int e = threadIdx.x; e < rows * M; e += blockDim.x) G[(int64_t)y0 * M + e] = buf[e];
+            __syncthreads();
+        }
+
+        // 2: column transforms, |.|^2, and the row binning (NGS) or the power spectrum (LGS)
+        for (int c0 = 0; c0 < M; c0 += CH) {
+            const int cols = min(CH, M - c0);
+            for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
+                const int c = e >> log2M, y = e & (M - 1);
+                buf[e] = y < np ? G[(int64_t)y * M + c0 + c] : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+            gen_fft(buf, cols, M, log2M, tw);
+            for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
+                const float2 h = buf[e];
+                buf[e] = make_float2(fmaf(h.x, h.x, h.y * h.y), 0.f);
+            }
+            __syncthreads();
+            if (!lgs) {
+                // U[Y][kx] = sum_ky P[Y][ky] S[ky][kx] for the chunk's columns
+                for (int e = threadIdx.x; e < npx * cols; e += blockDim.x) {
+                    const int Y = e / cols, c = e - (e / cols) * cols;
+                    const float* pr = g.roww + (int64_t)Y * M;
+                    const float2* sc = buf + c * M;
+                    float acc = 0.f;
+                    for (int ky = 0; ky < M; ++ky) acc = fmaf(pr[ky], sc[ky].x, acc);
+                    U[(int64_t)Y * M + c0 + c] = acc;
+                }
+            } else {
+                for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
+                    const int c = e >> log2M, ky = e & (M - 1);
+                    S[(int64_t)ky * M + c0 + c] = buf[e].x;
+                }
+            }
+            __syncthreads();
+        }
+
+        double bsum = 0.0;
+        if (!lgs) {
+            // 3 (NGS): B[Y][X] = sum_kx U[Y][kx] Q[X][kx]
+            for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
+                const int Y = e / npx, X = e - (e / npx) * npx;
+                const float* ur = U + (int64_t)Y * M;
+                const float* qr = g.colw + (int64_t)X * M;
+                float acc = 0.f;
+                for (int kx = 0; kx < M; ++kx) acc = fmaf(ur[kx], qr[kx], acc);
+                bsum += (double)acc;
+                ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = g.parseval ? acc * fs : acc;
+            }
+        } else {
+            // 3a (LGS): FFT of S over kx per row ky, keep m_x = 0 .. np-1  ->  T[ky][mx]
+            for (int r0 = 0; r0 < M; r0 += CH) {
+                const int rows = min(CH, M - r0);
+                for (int e = threadIdx.x; e < rows * M; e += blockDim.x)
+                    buf[e] = make_float2(S[(int64_t)r0 * M + e], 0.f);
+                __syncthreads();
+                gen_fft(buf, rows, M, log2M, tw);
+                for (int e = threadIdx.x; e < rows * np; e += blockDim.x) {
+                    const int r = e / np, mx = e - (e / np) * np;
+                    T[(int64_t)(r0 + r) * np + mx] = buf[r * M + mx];
+                }
+                __syncthreads();
+            }
+            // 3b: FFT of T over ky per column mx, keep |m_y| <= np-1, times the kernel spectrum
+            const float2* ks = g.kspec + (int64_t)sub * L * L;
+            for (int c0 = 0; c0 < np; c0 += CH) {
+                const int cols = min(CH, np - c0);
+                for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
+                    const int c = e >> log2M, ky = e & (M - 1);
+                    buf[e] = T[(int64_t)ky * np + c0 + c];
+                }
+                __syncthreads();
+                gen_fft(buf, cols, M, log2M, tw);
+                for (int e = threadIdx.x; e < cols * L; e += blockDim.x) {
+                    const int c = e / L, ly = e - (e / L) * L;        // ly = my + np - 1
+                    const int mx = c0 + c, my = ly - (np - 1);
+                    const float2 d = buf[c * M + ((my + M) & (M - 1))];
+                    const float2 kk = ks[(int64_t)ly * L + (mx + np - 1)];
+                    D[(int64_t)ly * np + mx] = make_float2(d.x * kk.x - d.y * kk.y, d.x * kk.y + d.y * kk.x);
+                }
+                __syncthreads();
+            }
+            // 3c: V[Y][mx] = sum_my P~[Y][my] D[my][mx] (complex, in LDS if it fits, else in S)
+            float2* V = (npx * np <= kGenLdsF2) ? buf : reinterpret_cast<float2*>(S);
+            const float2* Pt = reinterpret_cast<const float2*>(g.roww);     // [L][16 or npx pad]: see plan
+            for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {
+                const int Y = e / np, mx = e - (e / np) * np;
+                float ar = 0.f, ai = 0.f;
+                for (int ly = 0; ly < L; ++ly) {
+                    const float2 p = Pt[(int64_t)Y * L + ly];
+                    const float2 d = D[(int64_t)ly * np + mx];
+                    ar = fmaf(p.x, d.x, fmaf(-p.y, d.y, ar));
+                    ai = fmaf(p.x, d.y, fmaf(p.y, d.x, ai));
+                }
+                V[e] = make_float2(ar, ai);
+            }
+            __syncthreads();
+            const float2* Qt = reinterpret_cast<const float2*>(g.colw);
+            for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
+                const int Y = e / npx, X = e - (e / npx) * npx;
+                float acc = 0.f;
+                for (int mx = 0; mx < np; ++mx) {
+                    const float2 v = V[Y * np + mx];
+                    const float2 qq = Qt[(int64_t)X * L + (mx + np - 1)];
+                    const float re = fmaf(v.x, qq.x, -v.y * qq.y);
+                    acc = fmaf(mx == 0 ? 1.f : 2.f, re, acc);
+                }
+                bsum += (double)acc;
+                ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = g.parseval ? acc * fs : acc;
+            }
+        }
+        if (!g.parseval) {
+            const double tot = block_sum(bsum, red);
+            if (threadIdx.x == 0) g.bpart[(int64_t)f * g.n_parts + sub] = tot;
+        }
+    }
+}

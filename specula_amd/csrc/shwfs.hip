@@ -923,6 +923,9 @@ sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
     ngs_mfma_frames<NP, PARSEVAL, NPX, COG, BF3>(g, blockIdx.x, f0, f1, reinterpret_cast<float*>(lds_wave));
 }
 
+// ---------------------------------------------------------------- general path (any np, M)
+#include "sh_general.h"
+
 // ---------------------------------------------------------------- LGS forward (sh.py:381-394)
 // psf = Re IFFT_N(FFT_N(|FFT_N(E)|^2) * K).  With D[m] = (N^2/M^2) FFT_M(S)[m mod M] K[m] on the
 // lag grid |m| <= np-1 (S = |FFT_M(E)|^2, M >= 2np-1), the binned image is B = Re(P D Q^T) where
@@ -1963,6 +1966,11 @@ struct shwfs_plan {
     int amp_blocks = 0;
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
+    bool general = false;          // any-size path (sh_general_kernel, variant 4)
+    float2* d_tw = nullptr;        // general path: [M/2] twiddles
+    float* d_slab = nullptr;       // general path: [gen_blocks][slab_floats] scratch
+    int gen_blocks = 0;
+    int64_t slab_floats = 0;
     bool bf3 = false;              // MFMA binning from bf16 pieces (three-term split, variant 3)
     bool kernels_set = false;      // LGS: kernel spectra uploaded
     int fpb = 4;                   // frames per block of the streaming MFMA kernel (1: no streaming)
@@ -2107,6 +2115,27 @@ int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     return SHWFS_OK;
 }
 
+int launch_general(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
+    const shwfs_geom& g = p->g;
+    GenArgs q{};
+    q.tw = p->d_tw;
+    q.slab = p->d_slab;
+    q.slab_floats = p->slab_floats;
+    q.M = g.spec_size;
+    int lg = 0;
+    while ((1 << lg) < q.M) ++lg;
+    q.log2M = lg;
+    q.np = g.np_sub;
+    q.ch = kGenLdsF2 / q.M;
+    NgsArgs b = a;
+    if (!g.lgs) b.kspec = nullptr;                 // the kernel takes the LGS branch on kspec != nullptr
+    const int64_t items = (int64_t)n_frames * p->n_sub;
+    const int blocks = (int)std::min<int64_t>(items, p->gen_blocks);
+    hipLaunchKernelGGL(sh_general_kernel, dim3(blocks), dim3(kGenThreads), 0, st, b, q, n_frames);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
 NgsLauncher pick_lgs(int np, int M, int* ts) {
     if (np == 16 && M == 32) { *ts = 8; return launch_lgs<16, 32, 8>; }
     if (np == 12 && M == 32) { *ts = 8; return launch_lgs<12, 32, 8>; }
@@ -2159,22 +2188,31 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         return set_error(SHWFS_ERR_ARG, "ef_size must equal n_lenses * np_sub");
     if (g.spec_size < 2 * g.np_sub - 1)
         return set_error(SHWFS_ERR_ARG, "spec_size must be >= 2*np_sub-1");
-    if (g.variant < 0 || g.variant > 3) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1, 2 or 3");
+    if (g.variant < 0 || g.variant > 4) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1, 2, 3 or 4");
     NgsLauncher mf = nullptr, v1 = nullptr, lg = nullptr;
     int lgs_ts = 0;
-    bool use_mfma = false;
+    bool use_mfma = false, use_general = g.variant == 4;
+    const bool general_fits = (g.spec_size & (g.spec_size - 1)) == 0 && g.spec_size >= 16 && g.spec_size <= 1024;
     if (g.lgs) {
         lg = pick_lgs(g.np_sub, g.spec_size, &lgs_ts);
-        if (!lg) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS path instantiated for (np_sub, spec_size) in {(16,32),(12,32),(8,16),(6,16)}");
-        if (g.subap_npx > 16) return set_error(SHWFS_ERR_UNSUPPORTED, "LGS path needs subap_npx <= 16");
-    } else {
+        if (!lg || g.subap_npx > 16) use_general = true;     // no specialised LGS kernel: any-size path
+        if (g.variant == 1 || g.variant == 2 || g.variant == 3)
+            return set_error(SHWFS_ERR_ARG, "variants 1-3 are NGS binning variants");
+    } else if (!use_general) {
         mf = pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx);
         v1 = pick_ngs(g.np_sub, g.spec_size);
         use_mfma = (g.variant >= 2) || (g.variant == 0 && mf != nullptr);
         if (use_mfma && !mf) return set_error(SHWFS_ERR_UNSUPPORTED, "MFMA variant needs spec_size 16, np_sub 5..8, subap_npx <= 16");
-        if (!use_mfma && !v1)
-            return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
+        if (!use_mfma && !v1) {
+            if (g.variant != 0) return set_error(SHWFS_ERR_UNSUPPORTED, "np_sub / spec_size combination not instantiated");
+            use_general = true;                               // no specialised NGS kernel: any-size path
+        }
     }
+    if (use_general && !general_fits)
+        return set_error(SHWFS_ERR_UNSUPPORTED, "spectrum grid must be a power of two in [16, 1024] (np_sub <= 512)");
+    if (use_general && g.lgs && (int64_t)g.subap_npx * g.np_sub > kGenLdsF2 &&
+        2LL * g.subap_npx * g.np_sub > (int64_t)g.spec_size * g.spec_size)
+        return set_error(SHWFS_ERR_UNSUPPORTED, "LGS any-size path: subap_npx * np_sub too large for its work buffers");
     DeviceGuard dg(device);
     auto* p = new shwfs_plan();
     p->g = g;
@@ -2182,12 +2220,21 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     p->n_sub = g.n_lenses * g.n_lenses;
     p->npx_pad = (g.subap_npx + 3) / 4 * 4;
     p->mfma = use_mfma;
+    p->general = use_general;
     p->bf3 = use_mfma && g.variant == 3;
     p->L = 2 * g.np_sub - 1;
     p->amp_blocks = 256;
     const int M = g.spec_size;
     std::vector<float> rw, cw;
-    if (g.lgs) {
+    if (use_general) {
+        // operators in their natural row-major layout: NGS [npx][M] real, LGS [npx][L] complex
+        const size_t per = g.lgs ? (size_t)p->L * 2 : (size_t)M;
+        p->rows_alloc = g.subap_npx;
+        rw.assign((size_t)g.subap_npx * per, 0.f);
+        cw.assign((size_t)g.subap_npx * per, 0.f);
+        for (size_t q = 0; q < rw.size(); ++q) { rw[q] = (float)row_w[q]; cw[q] = (float)col_w[q]; }
+        p->ts = 1;
+    } else if (g.lgs) {
         p->rows_alloc = 16;
         p->ts = lgs_ts;
         rw.assign((size_t)16 * p->L * 2, 0.f);
@@ -2210,6 +2257,15 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     }
     p->n_tiles = (p->n_sub + p->ts - 1) / p->ts;
     p->n_parts = (!g.lgs && use_mfma) ? p->n_tiles * NgsMfmaShape<8>::WPB : p->n_tiles;
+    if (use_general) {
+        // per-block scratch (sh_general.h): G / T, then U (NGS) or S and D (LGS); the number of
+        // persistent blocks keeps the total at or below 1 GiB
+        const int64_t gt = 2LL * g.np_sub * M;
+        p->slab_floats = g.lgs ? gt + (int64_t)M * M + 2LL * p->L * g.np_sub : gt + (int64_t)g.subap_npx * M;
+        p->slab_floats = (p->slab_floats + 63) / 64 * 64;          // 256-byte aligned slabs
+        const int64_t cap = ((int64_t)1 << 30) / (p->slab_floats * (int64_t)sizeof(float));
+        p->gen_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(1024, cap));
+    }
     auto cleanup = [&](int code) { shwfs_plan_destroy(p); return code; };
     hipError_t e = hipSuccess;
 #define PLAN_ALLOC(ptr, bytes)                                                                  \
@@ -2252,11 +2308,23 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         PLAN_ALLOC(p->d_lit, (size_t)p->n_sub);
     }
     if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
+    std::vector<float2> twh;
+    if (use_general) {
+        PLAN_ALLOC(p->d_tw, (size_t)(M / 2) * sizeof(float2));
+        PLAN_ALLOC(p->d_slab, (size_t)p->gen_blocks * p->slab_floats * sizeof(float));
+        twh.resize(M / 2);
+        for (int k = 0; k < M / 2; ++k) {
+            const double th = -2.0 * M_PI * (double)k / (double)M;
+            twh[k] = make_float2((float)std::cos(th), (float)std::sin(th));
+        }
+    }
 #undef PLAN_ALLOC
     if ((!rl.empty() &&
          (e = hipMemcpy(p->d_rowlag, rl.data(), rl.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) ||
         (e = hipMemcpy(p->d_roww, rw.data(), rw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(p->d_colw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (!twh.empty() &&
+         (e = hipMemcpy(p->d_tw, twh.data(), twh.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) ||
         (e = hipMemset(p->d_counter, 0, (size_t)g.max_frames * sizeof(unsigned))) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("plan upload: ") + hipGetErrorString(e)));
     *out = p;
@@ -2286,6 +2354,8 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_A_rs_st);
     (void)hipFree(p->d_ph_rs);
     (void)hipFree(p->d_cogw);
+    (void)hipFree(p->d_tw);
+    (void)hipFree(p->d_slab);
     delete p;
     return SHWFS_OK;
 }
@@ -2425,7 +2495,8 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     }
     if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan: call shwfs_set_kernels first");
     int lts = 0;
-    NgsLauncher L = g.lgs ? pick_lgs(g.np_sub, g.spec_size, &lts)
+    NgsLauncher L = p->general ? launch_general
+                  : g.lgs ? pick_lgs(g.np_sub, g.spec_size, &lts)
                           : (p->mfma ? pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx) : pick_ngs(g.np_sub, g.spec_size));
     if (p->ev_start) SHWFS_HIP_CHECK(hipEventRecord(p->ev_start, st));
     int rc = L(p, a, n_frames, st);
@@ -2877,6 +2948,7 @@ int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, con
     if (!p || !A || !phase || !grad_ccd || !grad_phase) return set_error(SHWFS_ERR_ARG, "null argument");
     const shwfs_geom& g = p->g;
     if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the NGS path");
+    if (p->general) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the specialised NGS kernels (not variant 4)");
     if (p->ef_in) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available without field resampling (fov_ovs 1)");
     if (!g.parseval)
         return set_error(SHWFS_ERR_UNSUPPORTED, "VJP needs the Parseval normalisation (no FoV cut, all-ones mask)");
@@ -3038,7 +3110,7 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         const shwfs_geom& g = p->g;
         int kind;
         if (!g.lgs && p->mfma && g.np_sub == 8 && g.spec_size == 16) kind = 0;
-        else if (g.lgs && g.np_sub == 16 && g.spec_size == 32) kind = 1;
+        else if (g.lgs && !p->general && g.np_sub == 16 && g.spec_size == 32) kind = 1;
         else return set_error(SHWFS_ERR_UNSUPPORTED, "batch supports NGS MFMA np 8 and LGS np 16 plans");
         if (!g.parseval) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs Parseval-normalised sensors");
         if (p->ef_in) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs sensors without field resampling");

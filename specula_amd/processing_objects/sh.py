@@ -28,7 +28,9 @@ class SHPlan:
     def __init__(self, geom, lgs=False, max_frames=1, device=0, variant=0, kernels_normalised=True):
         """variant: 0 picks the kernel automatically, 1 forces register-GEMV binning, 2 forces
         FP32 MFMA binning (spec_size 16, np_sub 5..8, subap_npx <= 16), 3 MFMA binning from split
-        bf16 pieces (x = hi + lo, all four cross terms; relative error ~2^-17 per operand).
+        bf16 pieces (x = hi + lo, all four cross terms; relative error ~2^-17 per operand), 4 the
+        any-size path (full M-point LDS FFTs, any np_sub up to 512, any subap_npx; NGS and LGS),
+        which variant 0 also takes when no specialised kernel fits the geometry.
         lgs: per-subaperture kernel correlation; spectra come from ``set_kernels``.  With
         ``kernels_normalised`` (unit-sum kernels) the Parseval normalisation applies."""
         row, col, M, norm = build_operators(geom, lgs=lgs)
