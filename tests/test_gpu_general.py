@@ -159,3 +159,57 @@ def test_general_path_equals_specialised_kernels_where_both_apply():
     a, _ = run_plan(g, A, frames, lgs=True, kern=kern)
     b, _ = run_plan(g, A, frames, lgs=True, kern=kern, variant=4)
     check_intensity(b[0], a[0].astype(np.float64), g.n_lenses, g.subap_npx)
+
+
+# Every distinct SH block of the reference's example and test set-ups (values restated from the
+# YAML files; pupil px, pixel pitch, LGS launcher present, SH keywords).
+REFERENCE_SH_BLOCKS = [
+    ('main/scao/params_eris.yml:102', 160, 0.05, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 5, 'sensor_pxscale': 0.85, 'subap_on_diameter': 40, 'subap_npx': 6}),
+    ('main/scao/params_eris.yml:148', 160, 0.05, False, {'wavelengthInNm': 700, 'subap_wanted_fov': 4.8, 'sensor_pxscale': 0.4, 'subap_on_diameter': 4, 'subap_npx': 12}),
+    ('main/scao/params_eris_ngs.yml:91', 160, 0.05, False, {'wavelengthInNm': 768, 'subap_wanted_fov': 2.4, 'sensor_pxscale': 0.4, 'subap_on_diameter': 40, 'subap_npx': 6}),
+    ('main/scao/params_morfeo_full.yml:171', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': 6.2}),
+    ('main/scao/params_morfeo_full.yml:185', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': 14.2}),
+    ('main/scao/params_morfeo_full.yml:186', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': -6.2}),
+    ('main/scao/params_morfeo_full.yml:259', 480, 0.0802, False, {'wavelengthInNm': 1650, 'subap_wanted_fov': 1.8, 'sensor_pxscale': 0.015, 'subap_on_diameter': 2, 'subap_npx': 120}),
+    ('main/scao/params_morfeo_full.yml:306', 480, 0.0802, False, {'wavelengthInNm': 750, 'subap_wanted_fov': 4.6, 'sensor_pxscale': 0.153, 'subap_on_diameter': 8, 'subap_npx': 30}),
+    ('main/scao/params_morfeo_ltao.yml:127', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6}),
+    ('main/scao/params_morfeo_ltao_plots.yml:120', 480, 0.0802, False, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6}),
+    ('main/scao/params_morfeo_scao_plots.yml:94', 480, 0.08125, False, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6}),
+    ('main/scao/params_scao_sh.yml:69', 160, 0.05, True, {'wavelengthInNm': 750, 'subap_wanted_fov': 1.2, 'sensor_pxscale': 0.15, 'subap_on_diameter': 20, 'subap_npx': 6}),
+    ('main/scao/params_scao_sh_plots.yml:69', 160, 0.05, True, {'wavelengthInNm': 750, 'subap_wanted_fov': 2.4, 'sensor_pxscale': 0.4, 'subap_on_diameter': 20, 'subap_npx': 6}),
+    ('test/params_scao_sh_test.yml:66', 64, 0.015625, False, {'wavelengthInNm': 600, 'subap_wanted_fov': 4.0, 'sensor_pxscale': 0.5, 'subap_on_diameter': 8, 'subap_npx': 8}),
+]
+
+
+@pytest.mark.parametrize('block', REFERENCE_SH_BLOCKS, ids=[b[0].split('/')[-1] for b in REFERENCE_SH_BLOCKS])
+def test_every_reference_sh_block_gets_a_plan(block):
+    """Full-size plans for every SH block of the reference's set-ups: none is refused.  NGS blocks
+    also run one flat-wavefront frame through the plan's kernel and check the flux known answer
+    sum(I) = S0 * sum(A) * pitch^2 (test_sh.py:18-42).  (LGS blocks stop at the plan: their full
+    kernel sets are gigabytes on the host.)"""
+    where, pp, pitch, lgs, kw = block
+    g = sh_geometry(pp, pitch, **kw)
+    print(f'{where}: np {g.ovs_np_sub} N {g.fft_size} cut {g.cutpixels} npx {g.subap_npx} n {g.n_lenses} '
+          f'fov_ovs {g.fov_ovs:.3g} rot {g.rot_deg} lgs {lgs}', flush=True)
+    if g.ef_ovs != g.n_lenses * g.ovs_np_sub:
+        # params_morfeo_scao_plots.yml: fov_ovs 122/7 gives an 8365 px resampled field for 68 lenslets
+        # of 123 px (8364): the reference's reshape of each subaperture row (sh.py:373) fails, its
+        # divisibility check (sh.py:233-235) using round(fov_ovs) does not catch it; the plan refuses
+        assert where.endswith('params_morfeo_scao_plots.yml:94')
+        with pytest.raises(RuntimeError, match='ef_size must equal n_lenses'):
+            SHPlan(g, lgs=lgs, max_frames=1, device=0)
+        return
+    plan = SHPlan(g, lgs=lgs, max_frames=1, device=0)
+    if lgs:
+        return
+    A = annular_pupil(pp, 0.15)
+    if g.needs_interpolation:
+        plan.set_resampling(A)
+    out = torch.empty((1, g.ccd_side, g.ccd_side), dtype=torch.float32, device=DEV)
+    S0 = 3.0
+    plan.forward(torch.tensor(A.astype(np.float32), device=DEV),
+                 torch.zeros((1, pp, pp), dtype=torch.float32, device=DEV), out, S0)
+    torch.cuda.synchronize()
+    tot = float(out.double().sum())
+    assert np.isfinite(tot)
+    np.testing.assert_allclose(tot, S0 * A.sum() * pitch ** 2, rtol=1e-5)
