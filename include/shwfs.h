@@ -273,6 +273,18 @@ int shccd_readout(const shccd_params* params, int n_frames, float* integrated, i
                   const float* in_i, int64_t in_stride, double factor, const float* pixel_gains,
                   uint64_t counter, float* pixels, int64_t out_stride, void* stream);
 
+/* ---- HIP graphs -----------------------------------------------------------------------------
+ * Capture of library calls into a replayable HIP graph.  Replaces the CUDA-graph capture of a
+ * processing object's trigger_code (base_processing_obj.py:114-131, launched at :145-155): begin a
+ * capture on a non-default stream, issue any library calls on that stream, end it; each replay is
+ * one launch on any stream of the device.  Thread-local capture mode.  Replays read and write the
+ * buffers that the captured calls named (the caller keeps them alive and in place). */
+typedef struct shwfs_graph shwfs_graph;
+int shwfs_graph_begin(void* stream);
+int shwfs_graph_end(void* stream, shwfs_graph** out);
+int shwfs_graph_launch(shwfs_graph* graph, void* stream);
+int shwfs_graph_destroy(shwfs_graph* graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@ EXPORTS = (
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
     'shccd_integrate', 'shccd_readout',
+    'shwfs_graph_begin', 'shwfs_graph_end', 'shwfs_graph_launch', 'shwfs_graph_destroy',
 )
 
 SHCCD_PHOTON, SHCCD_READOUT, SHCCD_EXCESS, SHCCD_DARK, SHCCD_BACKGROUND, SHCCD_CIC = 1, 2, 4, 8, 16, 32
@@ -95,6 +96,12 @@ def _declare(lib):
         'shccd_readout': (_I, [ctypes.POINTER(ShccdParams), _I, _P, _I64, _P, _I64, _D, _P, ctypes.c_uint64, _P,
                                _I64, _P]),
     }
+    sig.update({
+        'shwfs_graph_begin': (_I, [_P]),
+        'shwfs_graph_end': (_I, [_P, ctypes.POINTER(_P)]),
+        'shwfs_graph_launch': (_I, [_P, _P]),
+        'shwfs_graph_destroy': (_I, [_P]),
+    })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -126,12 +133,55 @@ def check(rc, what=''):
 
 
 def stream_handle(stream=None):
-    """hipStream_t of a torch stream (default: the current stream) as a ctypes pointer."""
+    """hipStream_t of a torch stream (default: the current stream of the current device) as a
+    ctypes pointer.  The current stream is read as a raw handle (building a torch Stream object
+    for it costs microseconds per call)."""
     import torch
-    s = torch.cuda.current_stream() if stream is None else stream
-    return ctypes.c_void_p(s.cuda_stream)
+    if stream is None:
+        return ctypes.c_void_p(torch._C._cuda_getCurrentRawStream(torch.cuda.current_device()))
+    return ctypes.c_void_p(stream.cuda_stream)
 
 
 def ptr(t):
     """Device pointer of a torch tensor."""
     return ctypes.c_void_p(t.data_ptr())
+
+
+class HipGraph:
+    """A HIP graph of library calls (shwfs_graph_*): ``capture(fn, stream)`` records the calls
+    that ``fn`` issues on ``stream`` (a non-default torch stream, made current while ``fn`` runs);
+    ``replay(stream)`` launches them on ``stream`` (default: the current stream)."""
+
+    def __init__(self):
+        self._h = None
+
+    def capture(self, fn, stream):
+        import torch
+        h = stream_handle(stream)
+        check(lib().shwfs_graph_begin(h), 'shwfs_graph_begin')
+        out = ctypes.c_void_p()
+        try:
+            with torch.cuda.stream(stream):
+                fn()
+        finally:
+            rc = lib().shwfs_graph_end(h, ctypes.byref(out))
+        check(rc, 'shwfs_graph_end')
+        self.destroy()
+        self._h = out
+        return self
+
+    def replay(self, stream=None):
+        if self._h is None:
+            raise RuntimeError('no graph captured')
+        check(lib().shwfs_graph_launch(self._h, stream_handle(stream)), 'shwfs_graph_launch')
+
+    def destroy(self):
+        if self._h is not None and self._h.value:
+            lib().shwfs_graph_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -275,15 +275,15 @@ class BaseProcessingObj(BaseTimeObj):
                 self.capture_stream()
 
     def capture_stream(self):
+        """One eager warm-up of trigger_code() on the object's stream, then its library calls
+        captured into a HIP graph by the native library (shwfs_graph_*)."""
+        from specula_amd import _native
         with torch.cuda.device(self.target_device_idx):
             self.stream.wait_stream(torch.cuda.current_stream())     # the warm-up reads the inputs
             with torch.cuda.stream(self.stream):
                 self.trigger_code()
             self.stream.synchronize()
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph, stream=self.stream):
-                self.trigger_code()
-            self.cuda_graph = graph
+            self.cuda_graph = _native.HipGraph().capture(self.trigger_code, self.stream)
 
     def check_ready(self, t):
         self.current_time = t

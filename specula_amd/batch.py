@@ -95,20 +95,18 @@ class MultiSensorBatch:
         return np.array(out[:])
 
     def capture(self, slopes=True):
-        """Capture forward() (persistent SH launch + CoG launches) in a HIP graph."""
+        """Capture forward() (persistent SH launch + CoG launches) in a HIP graph of the native
+        library (shwfs_graph_*)."""
         s = torch.cuda.Stream(device=self.device)
         with torch.cuda.device(self.device):
             with torch.cuda.stream(s):
                 self.forward(stream=s, slopes=slopes)
             s.synchronize()
-            graph = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(graph, stream=s):
-                self.forward(stream=s, slopes=slopes)
-        self.graph = graph
-        return graph
+            self.graph = _native.HipGraph().capture(lambda: self.forward(stream=s, slopes=slopes), s)
+        return self.graph
 
-    def replay(self):
-        self.graph.replay()
+    def replay(self, stream=None):
+        self.graph.replay(stream)
 
     def __del__(self):
         h = getattr(self, '_h', None)

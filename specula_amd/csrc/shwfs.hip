@@ -3249,3 +3249,57 @@ int shwfs_batch_info(shwfs_batch* b, int32_t* n_items, int32_t* grid) {
 }
 
 }  // extern "C"
+
+// ============================================================================ HIP graphs
+// Capture of library calls into a HIP graph: the caller begins a capture on its (non-default)
+// stream, issues any shwfs_* / shslopec_* / shccd_* calls on that stream, and ends it; replays
+// are one hipGraphLaunch.  Thread-local capture mode: only calls from the capturing thread are
+// recorded, and other threads may keep allocating.  The library's kernels clear their own
+// counters (no memset nodes in a graph).
+struct shwfs_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int shwfs_graph_begin(void* stream) {
+    if (!stream) return set_error(SHWFS_ERR_ARG, "graph capture needs a non-default stream");
+    SHWFS_HIP_CHECK(hipStreamBeginCapture(reinterpret_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal));
+    return SHWFS_OK;
+}
+
+int shwfs_graph_end(void* stream, shwfs_graph** out) {
+    if (!stream || !out) return set_error(SHWFS_ERR_ARG, "null argument");
+    *out = nullptr;
+    hipGraph_t graph = nullptr;
+    SHWFS_HIP_CHECK(hipStreamEndCapture(reinterpret_cast<hipStream_t>(stream), &graph));
+    if (!graph) return set_error(SHWFS_ERR_HIP, "stream capture produced no graph");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return set_error(SHWFS_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    auto* g = new shwfs_graph();
+    g->graph = graph;
+    g->exec = exec;
+    *out = g;
+    return SHWFS_OK;
+}
+
+int shwfs_graph_launch(shwfs_graph* g, void* stream) {
+    if (!g || !g->exec) return set_error(SHWFS_ERR_ARG, "null graph");
+    SHWFS_HIP_CHECK(hipGraphLaunch(g->exec, reinterpret_cast<hipStream_t>(stream)));
+    return SHWFS_OK;
+}
+
+int shwfs_graph_destroy(shwfs_graph* g) {
+    if (!g) return SHWFS_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return SHWFS_OK;
+}
+
+}  // extern "C"
