@@ -273,6 +273,16 @@ int shccd_readout(const shccd_params* params, int n_frames, float* integrated, i
                   const float* in_i, int64_t in_stride, double factor, const float* pixel_gains,
                   uint64_t counter, float* pixels, int64_t out_stride, void* stream);
 
+/* ---- FP32 matrix product on the matrix cores ----------------------------------------------
+ *   C[i][j] = alpha * sum_k A[i][k] op(B)[k][j] + beta * C[i][j] + bias[j]     (i < m, j < n)
+ * A row-major m x k (lda); op(B) = B row-major k x n (ldb) when trans_b == 0, B^T with B row-major
+ * n x k when trans_b == 1; bias: n floats or NULL; C row-major (ldc).  Device pointers.
+ * Replaces the command -> phase projection of the DM, phase[idx] = (sign * cmd) @ ifunc
+ * (processing_objects/dm.py:72-86), and the reconstruction / modal filtering products of
+ * Slopec.post_trigger (processing_objects/slopec.py:92-104). */
+int shgemm_f32(int m, int n, int k, float alpha, const float* A, int64_t lda, const float* B, int64_t ldb,
+               int trans_b, float beta, const float* bias, float* C, int64_t ldc, void* stream);
+
 /* ---- HIP graphs -----------------------------------------------------------------------------
  * Capture of library calls into a replayable HIP graph.  Replaces the CUDA-graph capture of a
  * processing object's trigger_code (base_processing_obj.py:114-131, launched at :145-155): begin a

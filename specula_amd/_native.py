@@ -25,6 +25,7 @@ EXPORTS = (
     'shwfs_batch_destroy',
     'shccd_integrate', 'shccd_readout',
     'shwfs_graph_begin', 'shwfs_graph_end', 'shwfs_graph_launch', 'shwfs_graph_destroy',
+    'shgemm_f32',
 )
 
 SHCCD_PHOTON, SHCCD_READOUT, SHCCD_EXCESS, SHCCD_DARK, SHCCD_BACKGROUND, SHCCD_CIC = 1, 2, 4, 8, 16, 32
@@ -101,6 +102,7 @@ def _declare(lib):
         'shwfs_graph_end': (_I, [_P, ctypes.POINTER(_P)]),
         'shwfs_graph_launch': (_I, [_P, _P]),
         'shwfs_graph_destroy': (_I, [_P]),
+        'shgemm_f32': (_I, [_I, _I, _I, _F, _P, _I64, _P, _I64, _I, _F, _P, _P, _I64, _P]),
     })
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -145,6 +147,28 @@ def stream_handle(stream=None):
 def ptr(t):
     """Device pointer of a torch tensor."""
     return ctypes.c_void_p(t.data_ptr())
+
+
+def gemm(A, B, out, alpha=1.0, beta=0.0, bias=None, trans_b=False, stream=None):
+    """out = alpha * A @ op(B) + beta * out + bias (shgemm_f32, FP32 matrix cores).  A (m, k), B (k, n)
+    or, with trans_b, (n, k), out (m, n): float32 device tensors with unit column stride; bias (n,)
+    or None."""
+    import torch
+    if A.dim() != 2 or B.dim() != 2 or out.dim() != 2:
+        raise ValueError('gemm takes 2-D tensors')
+    m, k = A.shape
+    n = B.shape[0] if trans_b else B.shape[1]
+    if (B.shape[1] if trans_b else B.shape[0]) != k or tuple(out.shape) != (m, n):
+        raise ValueError(f'gemm shapes do not match: A {tuple(A.shape)}, B {tuple(B.shape)}, out {tuple(out.shape)}')
+    for t in (A, B, out) + ((bias,) if bias is not None else ()):
+        if t.dtype != torch.float32 or not t.is_cuda or t.stride(-1) != 1:
+            raise ValueError('gemm operands must be float32 device tensors with unit column stride')
+    if bias is not None and (bias.dim() != 1 or bias.shape[0] != n or not bias.is_contiguous()):
+        raise ValueError(f'bias must be a contiguous ({n},) tensor')
+    check(lib().shgemm_f32(m, n, k, float(alpha), ptr(A), A.stride(0), ptr(B), B.stride(0), 1 if trans_b else 0,
+                           float(beta), ptr(bias) if bias is not None else None, ptr(out), out.stride(0),
+                           stream_handle(stream)), 'shgemm_f32')
+    return out
 
 
 class HipGraph:

@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, 'csrc')
 LIB = os.path.join(HERE, 'lib', 'libshwfs.so')
 INCLUDE = os.path.join(os.path.dirname(HERE), 'include')
 
-SOURCES = ['shwfs.hip', 'ccd.hip']
+SOURCES = ['shwfs.hip', 'ccd.hip', 'gemm.hip']
 
 # Packed-FP32 VALU ops are disabled: hipcc still vectorises float code into them at -O3
 # otherwise, and some of their forms have produced silently wrong results on this hardware.

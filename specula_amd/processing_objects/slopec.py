@@ -1,4 +1,5 @@
 """Slopec base (specula/processing_objects/slopec.py:11-104)."""
+import numpy as np
 import torch
 
 from specula_amd.base import BaseProcessingObj, BaseValue, InputValue
@@ -49,14 +50,43 @@ class Slopec(BaseProcessingObj):
     def _mat(m):
         return getattr(m, 'recmat', getattr(m, 'intmat', m))
 
+    def _device_mat(self, key, m, dtype):
+        """Matrix of a Recmat / Intmat as a contiguous device tensor of ``dtype``.  A conversion
+        (dtype change or a copy of a sliced view) is made once per source tensor; reassigning the
+        object's matrix (as Recmat.reduce_size does) triggers a new one."""
+        src = self._mat(m)
+        cache = self.__dict__.setdefault('_mat_cache', {})
+        hit = cache.get(key)
+        if hit is None or hit[0] is not src or hit[1].dtype != dtype:
+            t = torch.as_tensor(src if torch.is_tensor(src) else np.asarray(src), device=self.device, dtype=dtype)
+            hit = (src, t.contiguous())
+            cache[key] = hit
+        return hit[1]
+
+    @staticmethod
+    def _product(a, b, out, alpha=1.0, beta=0.0, trans_b=False):
+        """out = alpha a @ op(b) + beta out: the library's matrix-core GEMM for FP32 on the GPU,
+        torch for FP64 and on the CPU."""
+        if out.is_cuda and out.dtype == torch.float32:
+            from specula_amd import _native
+            return _native.gemm(a, b, out, alpha=alpha, beta=beta, trans_b=trans_b)
+        bb = b.T if trans_b else b
+        return torch.addmm(out, a, bb, beta=beta, alpha=alpha, out=out) if beta != 0 else torch.mm(a, bb, out=out)
+
     def post_trigger(self):
-        """Optional reconstruction / modal filtering (slopec.py:92-104) as library GEMMs."""
+        """Optional reconstruction / modal filtering (slopec.py:92-104) as device GEMMs."""
         super().post_trigger()
+        s = self.slopes.slopes
         if self.recmat is not None:
-            R = torch.as_tensor(self._mat(self.recmat), device=self.device, dtype=self.slopes.slopes.dtype)
-            self.slopes.slopes = self.slopes.slopes @ R
+            R = self._device_mat('recmat', self.recmat, s.dtype)
+            out = torch.empty(R.shape[1], dtype=s.dtype, device=self.device)
+            self._product(s.reshape(1, -1).contiguous(), R, out.view(1, -1))
+            self.slopes.slopes = out
         if self.filt_intmat is not None and self.filt_recmat is not None:
-            IM = torch.as_tensor(self._mat(self.filt_intmat), device=self.device, dtype=self.slopes.slopes.dtype)
-            RM = torch.as_tensor(self._mat(self.filt_recmat), device=self.device, dtype=self.slopes.slopes.dtype)
-            m = self.slopes.slopes @ RM
-            self.slopes.slopes -= m @ IM.T
+            s = self.slopes.slopes
+            IM = self._device_mat('filt_intmat', self.filt_intmat, s.dtype)
+            RM = self._device_mat('filt_recmat', self.filt_recmat, s.dtype)
+            m = torch.empty(RM.shape[1], dtype=s.dtype, device=self.device)
+            self._product(s.reshape(1, -1).contiguous(), RM, m.view(1, -1))
+            # slopes -= m @ IM^T, in place
+            self._product(m.view(1, -1), IM, s.view(1, -1), alpha=-1.0, beta=1.0, trans_b=True)

@@ -14,7 +14,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))
 def declared():
     text = open(HEADER).read()
     text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
-    return set(re.findall(r'\b((?:shwfs|shslopec|shccd)_[a-z_]+)\s*\(', text))
+    return set(re.findall(r'\b((?:shwfs|shslopec|shccd|shgemm)_[a-z0-9_]+)\s*\(', text))
 
 
 def test_library_exports_header_symbols():
