@@ -293,11 +293,11 @@ def bench_objects(torch, dev, steps=200):
 def bench_variants(torch, dev):
     """The plan's binning variants on elt_ngs_80x80 as the autotuner times them (100 frames, flat
     phase, per-call pupil statistics): register-GEMV binning on the VALU after the FFTs, FP32
-    MFMA binning, bf16-split MFMA binning."""
+    MFMA binning, bf16-split MFMA binning, and the any-size path (radix-2 LDS FFTs on the M grid)."""
     from specula_amd import autotune, workloads
     wl = workloads.elt_ngs_80x80()
     best, times = autotune.tune_plan_variant(wl.geometry(), wl.pupil(), n_frames=100, use_cache=False)
-    names = {1: 'register_gemv_valu', 2: 'fp32_mfma', 3: 'bf16_split_mfma'}
+    names = {1: 'register_gemv_valu', 2: 'fp32_mfma', 3: 'bf16_split_mfma', 4: 'general_lds_fft'}
     return {'best': names[best], 'us_per_frame': {names[v]: round(t * 1e6, 3) for v, t in times.items()}}
 
 
