@@ -248,6 +248,8 @@ int shwfs_batch_destroy(shwfs_batch* batch);
 #define SHCCD_KEEP_BIAS      0x040  /* _keep_ADU_bias                                          */
 #define SHCCD_BG_REMOVE_AVG  0x080  /* _bg_remove_average                                      */
 #define SHCCD_KEEP_DARK      0x100  /* _do_not_remove_dark                                     */
+#define SHCCD_OFFSET_APPLIED 0x200  /* the input already holds background + dark: not added
+                                       again (their subtraction after the ADU step is kept)     */
 
 typedef struct shccd_params {
     int32_t in_rows, in_cols;      /* integrated frame = size * binning (ccd.py:134-136)     */
@@ -272,6 +274,18 @@ int shccd_integrate(int n_frames, const float* in_i, int64_t in_stride, double f
 int shccd_readout(const shccd_params* params, int n_frames, float* integrated, int64_t int_stride,
                   const float* in_i, int64_t in_stride, double factor, const float* pixel_gains,
                   uint64_t counter, float* pixels, int64_t out_stride, void* stream);
+
+/* Readout with charge-transfer-efficiency matrices (cte_noise, ccd.py:223-224):
+ *   frame = cte_rows . (binned signal * QE + background + dark) . cte_cols
+ * followed by CIC, photon noise, EM gain, readout noise, pixel gains and ADU conversion exactly as
+ * shccd_readout.  cte_rows: out_rows x out_rows, cte_cols: out_cols x out_cols, row-major float32
+ * (the reference's cte_mat[:, :, 0] and cte_mat[:, :, 1]).  work: 2 * n_frames * out_rows *
+ * out_cols floats of device scratch owned by the caller.  Same random numbers as shccd_readout
+ * for the same counter. */
+int shccd_readout_cte(const shccd_params* params, int n_frames, float* integrated, int64_t int_stride,
+                      const float* in_i, int64_t in_stride, double factor, const float* cte_rows,
+                      const float* cte_cols, float* work, const float* pixel_gains, uint64_t counter,
+                      float* pixels, int64_t out_stride, void* stream);
 
 /* ---- FP32 matrix product on the matrix cores ----------------------------------------------
  *   C[i][j] = alpha * sum_k A[i][k] op(B)[k][j] + beta * C[i][j] + bias[j]     (i < m, j < n)

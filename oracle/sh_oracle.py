@@ -704,7 +704,8 @@ def ccd_readout(integrated, binning=1, quantum_eff=1.0, photon_noise=False, read
                 readout_level=0.0, darkcurrent_level=0.0, background_level=0.0, cic_level=0.0,
                 ADU_gain=8, ADU_bias=400, emccd_gain=1, excess_delta=1.0, pixelGains=None,
                 keep_ADU_bias=False, bg_remove_average=False, do_not_remove_dark=False,
-                photon_rng=None, readout_rng=None, cic_rng=None, excess_rng=None, dtype=np.float32):
+                photon_rng=None, readout_rng=None, cic_rng=None, excess_rng=None, dtype=np.float32,
+                cte_noise=False, cte_mat=None):
     """CCD.apply_binning + apply_qe + apply_noise, ccd.py:220-273, with numpy generators and the
     reference's dtype flow (float32 frame; int64 after Poisson, so an added readout noise is
     truncated by ``astype``; float64 after the ADU conversion).  ``excess_noise`` uses
@@ -720,8 +721,11 @@ def ccd_readout(integrated, binning=1, quantum_eff=1.0, photon_noise=False, read
         frame = frame * binning ** 2 * (tot / np.sum(frame))
     if quantum_eff != 1:                                               # :276-277
         frame *= dtype(quantum_eff) if dtype is np.float32 else quantum_eff
-    if background_noise or darkcurrent_noise:                          # :222-223
+    if background_noise or darkcurrent_noise:                          # :220-221
         frame += (background_level + darkcurrent_level)
+    if cte_noise:                                                      # :223-224
+        m = np.asarray(cte_mat, dtype=frame.dtype)
+        frame = np.dot(np.dot(m[:, :, 0], frame), m[:, :, 1])
     if cic_noise:                                                      # :228-229
         frame = frame + cic_rng.binomial(1, cic_level, frame.shape)
     if photon_noise:                                                   # :231-232

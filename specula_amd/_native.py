@@ -23,13 +23,13 @@ EXPORTS = (
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_im_from_jacobian', 'shwfs_im_from_jacobian_noalloc',
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
-    'shccd_integrate', 'shccd_readout',
+    'shccd_integrate', 'shccd_readout', 'shccd_readout_cte',
     'shwfs_graph_begin', 'shwfs_graph_end', 'shwfs_graph_launch', 'shwfs_graph_destroy',
     'shgemm_f32',
 )
 
 SHCCD_PHOTON, SHCCD_READOUT, SHCCD_EXCESS, SHCCD_DARK, SHCCD_BACKGROUND, SHCCD_CIC = 1, 2, 4, 8, 16, 32
-SHCCD_KEEP_BIAS, SHCCD_BG_REMOVE_AVG, SHCCD_KEEP_DARK = 64, 128, 256
+SHCCD_KEEP_BIAS, SHCCD_BG_REMOVE_AVG, SHCCD_KEEP_DARK, SHCCD_OFFSET_APPLIED = 64, 128, 256, 512
 
 
 class ShccdParams(ctypes.Structure):
@@ -96,6 +96,8 @@ def _declare(lib):
         'shccd_integrate': (_I, [_I, _P, _I64, _D, _P, _I64, _I64, _P]),
         'shccd_readout': (_I, [ctypes.POINTER(ShccdParams), _I, _P, _I64, _P, _I64, _D, _P, ctypes.c_uint64, _P,
                                _I64, _P]),
+        'shccd_readout_cte': (_I, [ctypes.POINTER(ShccdParams), _I, _P, _I64, _P, _I64, _D, _P, _P, _P, _P,
+                                   ctypes.c_uint64, _P, _I64, _P]),
     }
     sig.update({
         'shwfs_graph_begin': (_I, [_P]),

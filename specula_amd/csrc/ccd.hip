@@ -184,13 +184,20 @@ __device__ __forceinline__ float raw_pixel(const ReadoutArgs& a, int f, int q) {
     return acc;
 }
 
+// background + dark current added to every pixel (ccd.py:220-221), unless the input already
+// carries them (SHCCD_OFFSET_APPLIED: the second stage of a CTE readout)
+__device__ __forceinline__ float offset_of(const shccd_params& p) {
+    return ((p.flags & (SHCCD_BACKGROUND | SHCCD_DARK)) && !(p.flags & SHCCD_OFFSET_APPLIED))
+               ? (float)(p.background_level + p.darkcurrent_level) : 0.0f;
+}
+
 // apply_qe + background/dark (+ CIC in the FULL variants): the expected signal before the photon
 // noise, from the integrated (binned) signal.
 template <bool FULL>
 __device__ __forceinline__ float signal_of(const ReadoutArgs& a, float acc, uint64_t px, uint32_t ctr) {
     const shccd_params& p = a.p;
     if (p.qe != 1.0) acc *= (float)p.qe;
-    if (p.flags & (SHCCD_BACKGROUND | SHCCD_DARK)) acc += (float)(p.background_level + p.darkcurrent_level);
+    acc += offset_of(p);
     if (FULL && (p.flags & SHCCD_CIC)) {
         Stream s(p.cic_seed, px, ctr);
         if (p.cic_level > 0.0 && s.uniform_d() <= p.cic_level) acc += 1.0f;
@@ -286,7 +293,7 @@ __global__ void __launch_bounds__(256) ccd_readout_kernel(ReadoutArgs a, int vec
             }
             x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
             const float qe = (float)a.p.qe;
-            const float add = (a.p.flags & (SHCCD_BACKGROUND | SHCCD_DARK)) ? (float)(a.p.background_level + a.p.darkcurrent_level) : 0.0f;
+            const float add = offset_of(a.p);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 if (a.p.qe != 1.0) x[j] *= qe;
@@ -502,6 +509,51 @@ int shccd_readout(const shccd_params* p, int n_frames, float* integrated, int64_
     }
     CCD_HIP_CHECK(hipGetLastError());
     return SHWFS_OK;
+}
+
+
+// CTE readout (ccd.py:220-224): frame = cte_rows . (binned QE signal + background + dark) . cte_cols,
+// then CIC, photon, EM, readout noise, gains and ADU conversion as shccd_readout.  Three stages on
+// the stream: the readout kernel without noise writes the offset signal into work[0]; two FP32
+// matrix products (shgemm_f32) form the transferred frame in work[1] and back in work[0]; the
+// readout kernel with the noise flags and SHCCD_OFFSET_APPLIED (background and dark are not added
+// again; their subtraction after the ADU conversion is kept) reads it into `pixels`.
+int shccd_readout_cte(const shccd_params* p, int n_frames, float* integrated, int64_t int_stride, const float* in_i,
+                      int64_t in_stride, double factor, const float* cte_rows, const float* cte_cols, float* work,
+                      const float* pixel_gains, uint64_t counter, float* pixels, int64_t out_stride, void* stream) {
+    if (!p || !cte_rows || !cte_cols || !work)
+        return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout_cte: null CTE matrices or work buffer");
+    if (n_frames <= 0 || n_frames > 65535)
+        return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout_cte: n_frames outside [1, 65535]");
+    const int R = p->out_rows, C = p->out_cols;
+    const int64_t n_out = (int64_t)R * C;
+    if (R <= 0 || C <= 0) return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout_cte: empty frame");
+    float* w0 = work;                               // [n_frames][R][C]
+    float* w1 = work + (int64_t)n_frames * n_out;   // [n_frames][R][C]
+    // stage 1: binning, QE, background + dark; no noise, no gains
+    shccd_params s1 = *p;
+    s1.flags = p->flags & (SHCCD_BACKGROUND | SHCCD_DARK);
+    int rc = shccd_readout(&s1, n_frames, integrated, int_stride, in_i, in_stride, factor, nullptr, counter, w0,
+                           n_out, stream);
+    if (rc != SHWFS_OK) return rc;
+    // stage 2: w1[f] = cte_rows . w0[f] (R x R . R x C), then w0 = w1 . cte_cols over all frames at once
+    for (int f = 0; f < n_frames; ++f) {
+        rc = shgemm_f32(R, C, R, 1.0f, cte_rows, R, w0 + (int64_t)f * n_out, C, 0, 0.0f, nullptr,
+                        w1 + (int64_t)f * n_out, C, stream);
+        if (rc != SHWFS_OK) return rc;
+    }
+    if ((int64_t)n_frames * R > INT32_MAX)
+        return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_readout_cte: too many rows for one product");
+    rc = shgemm_f32(n_frames * R, C, C, 1.0f, w1, C, cte_cols, C, 0, 0.0f, nullptr, w0, C, stream);
+    if (rc != SHWFS_OK) return rc;
+    // stage 3: the noise chain on the transferred frame
+    shccd_params s3 = *p;
+    s3.in_rows = R;
+    s3.in_cols = C;
+    s3.binning = 1;
+    s3.qe = 1.0;
+    s3.flags = p->flags | SHCCD_OFFSET_APPLIED;
+    return shccd_readout(&s3, n_frames, nullptr, 0, w0, n_out, 1.0, pixel_gains, counter, pixels, out_stride, stream);
 }
 
 }  // extern "C"

@@ -6,8 +6,13 @@ integrated every step with factor ``time_step * bandw`` and read out every ``dt`
 step has integrated nothing before (dt equal to the loop step, the usual case), the integration
 and the readout are one kernel pass.
 
-Not built: ``cte_noise`` (CTE matrices), the 'auto' noise levels of calc_detector_noise, and the
-non-uniform QE matrix (no public setter in the reference); they raise ``NotImplementedError``.
+``cte_noise`` applies the charge-transfer matrices ``cte_mat[:, :, 0] . frame . cte_mat[:, :, 1]``
+between the background/dark offset and the CIC / photon noise (``shccd_readout_cte``: two FP32
+matrix-core products between two passes of the readout kernel).  'auto' read-out noise and dark
+current levels come from the detector model ``name`` at the frame rate 1/dt
+(``specula_amd.lib.detector_noise``, calc_detector_noise.py) under the reference's condition: all
+three levels given (ccd.py:67-74); an 'auto' background level, which the reference leaves
+unresolved, raises.  Not built: the non-uniform QE matrix (no public setter in the reference).
 Noise streams are counter-based (see include/shwfs.h): statistically equivalent to the
 reference's numpy generators, not the same numbers.
 """
@@ -20,6 +25,31 @@ from specula_amd import _native
 from specula_amd.base import BaseProcessingObj, InputValue
 from specula_amd.data_objects.intensity import Intensity
 from specula_amd.data_objects.pixels import Pixels
+from specula_amd.lib.detector_noise import detector_noise
+
+
+def resolve_noise_levels(readout_level, darkcurrent_level, background_level, name, dt, binning,
+                         readout_used, offset_used):
+    """Numeric (readout, dark current, background) levels as ccd.py:67-74 resolves them: when all
+    three are given and one is 'auto', the read-out noise and dark current of detector ``name`` at
+    frame rate 1/dt replace the 'auto' ones (the background is never resolved).  A level still a
+    string where apply_noise would use it raises ValueError; unused or empty levels become 0."""
+    if readout_level and darkcurrent_level and background_level and \
+            'auto' in (readout_level, darkcurrent_level, background_level):
+        ron, dark, _ = detector_noise(1.0 / dt, name, binning)
+        if readout_level == 'auto':
+            readout_level = ron
+        if darkcurrent_level == 'auto':
+            darkcurrent_level = dark
+    for key, level, used in (('readout_level', readout_level, readout_used),
+                             ('darkcurrent_level', darkcurrent_level, offset_used),
+                             ('background_level', background_level, offset_used)):
+        if isinstance(level, str) and level and used:
+            raise ValueError(f"{key}={level!r}: 'auto' levels are resolved from the detector model only when "
+                             "readout, dark-current and background levels are all given, and the background "
+                             "level never (ccd.py:67-74)")
+    return tuple(0.0 if isinstance(v, str) or v is None else float(v)
+                 for v in (readout_level, darkcurrent_level, background_level))
 
 
 class CCD(BaseProcessingObj):
@@ -41,10 +71,9 @@ class CCD(BaseProcessingObj):
         if self._dt % self.loop_dt != 0:
             raise ValueError(f'integration time dt={dt} must be a multiple of the basic simulation '
                              f'time_step={simul_params.time_step}')
-        if 'auto' in (readout_level, darkcurrent_level, background_level):
-            raise NotImplementedError("'auto' detector noise levels (calc_detector_noise) are not built")
-        if cte_noise:
-            raise NotImplementedError('CTE noise is not built')
+        readout_level, darkcurrent_level, background_level = resolve_noise_levels(
+            readout_level, darkcurrent_level, background_level, name, dt, binning, readout_noise,
+            background_noise or darkcurrent_noise)
         if excess_noise:                       # ccd.py:104-106
             emccd_gain = 400
             ADU_gain = 1 / 20
@@ -57,7 +86,19 @@ class CCD(BaseProcessingObj):
         self._background_level = float(background_level or 0.0)
         self._cic_level = float(cic_level)
         self._qe = float(quantum_eff)
+        self._cte_noise = bool(cte_noise)
         self._pixels = Pixels(size[0] // binning, size[1] // binning, target_device_idx=target_device_idx)
+        self._cte_rows = self._cte_cols = self._cte_work = None
+        if self._cte_noise:
+            r, c = self._pixels.size
+            # the reference's default cte_mat is all zeros (ccd.py:130)
+            m = np.zeros((size[0], size[1], 2), np.float32) if cte_mat is None else np.asarray(cte_mat, np.float32)
+            if m.ndim != 3 or m.shape[2] != 2 or m.shape[0] != r or m.shape[1] != r or r != c:
+                raise ValueError(f'cte_mat of shape {m.shape} does not multiply the {r} x {c} pixel frame from '
+                                 f'both sides: it must be ({r}, {r}, 2) for a square frame')
+            self._cte_rows = torch.tensor(np.ascontiguousarray(m[:, :, 0]), device=self.device)
+            self._cte_cols = torch.tensor(np.ascontiguousarray(m[:, :, 1]), device=self.device)
+            self._cte_work = torch.empty(2 * r * c, dtype=torch.float32, device=self.device)
         s = (self._pixels.size[0] * binning, self._pixels.size[1] * binning)
         self._integrated_i = Intensity(s[0], s[1], target_device_idx=target_device_idx, precision=precision)
         self._photon_seed, self._readout_seed, self._excess_seed = photon_seed, readout_seed, excess_seed
@@ -148,11 +189,18 @@ class CCD(BaseProcessingObj):
             # readout), so a readout with no pending frame and no fresh input reads a dark frame
             use_integ = self._pending or not fresh
             gains = _native.ptr(self._pixelGains) if self._pixelGains is not None else None
-            _native.check(lib.shccd_readout(
-                ctypes.byref(self._p), 1, _native.ptr(integ) if use_integ else None, integ.numel(),
-                _native.ptr(src) if fresh else None, src.numel(), self._factor(), gains,
-                ctypes.c_uint64(self._readouts), _native.ptr(self._pixels.pixels), self._pixels.pixels.numel(), st),
-                'shccd_readout')
+            if self._cte_noise:
+                _native.check(lib.shccd_readout_cte(
+                    ctypes.byref(self._p), 1, _native.ptr(integ) if use_integ else None, integ.numel(),
+                    _native.ptr(src) if fresh else None, src.numel(), self._factor(), _native.ptr(self._cte_rows),
+                    _native.ptr(self._cte_cols), _native.ptr(self._cte_work), gains, ctypes.c_uint64(self._readouts),
+                    _native.ptr(self._pixels.pixels), self._pixels.pixels.numel(), st), 'shccd_readout_cte')
+            else:
+                _native.check(lib.shccd_readout(
+                    ctypes.byref(self._p), 1, _native.ptr(integ) if use_integ else None, integ.numel(),
+                    _native.ptr(src) if fresh else None, src.numel(), self._factor(), gains,
+                    ctypes.c_uint64(self._readouts), _native.ptr(self._pixels.pixels), self._pixels.pixels.numel(),
+                    st), 'shccd_readout')
             self._pending = False
             self._readouts += 1
             self._pixels.generation_time = self.current_time
@@ -162,10 +210,11 @@ class CCD(BaseProcessingObj):
             self._pending = True
 
 
-def readout_frames(params, frames, out, factor=1.0, counter=0, pixel_gains=None, stream=None):
+def readout_frames(params, frames, out, factor=1.0, counter=0, pixel_gains=None, stream=None, cte=None):
     """Batched readout (shccd_readout) of F intensity frames (F, in_rows, in_cols) float32 into
     ``out`` (F, out_rows, out_cols): each frame integrated once with ``factor`` and read out with
-    the noise of ``params`` (an ``_native.ShccdParams``, e.g. ``CCD.params()``)."""
+    the noise of ``params`` (an ``_native.ShccdParams``, e.g. ``CCD.params()``).  ``cte``: a pair
+    (rows, cols) of device matrices applies the charge-transfer matrices (shccd_readout_cte)."""
     for t in (frames, out):
         if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda:
             raise ValueError('CCD frames must be contiguous float32 device tensors')
@@ -175,6 +224,17 @@ def readout_frames(params, frames, out, factor=1.0, counter=0, pixel_gains=None,
     if frames.numel() != F * n_in or out.numel() != F * n_out:
         raise ValueError('frame sizes do not match the detector parameters')
     gains = _native.ptr(pixel_gains) if pixel_gains is not None else None
+    if cte is not None:
+        rows, cols = cte
+        for t, n in ((rows, params.out_rows), (cols, params.out_cols)):
+            if t.dtype != torch.float32 or not t.is_contiguous() or not t.is_cuda or tuple(t.shape) != (n, n):
+                raise ValueError(f'CTE matrices must be contiguous float32 ({n}, {n}) device tensors')
+        work = torch.empty(2 * F * n_out, dtype=torch.float32, device=out.device)
+        _native.check(_native.lib().shccd_readout_cte(
+            ctypes.byref(params), F, None, 0, _native.ptr(frames), n_in, float(factor), _native.ptr(rows),
+            _native.ptr(cols), _native.ptr(work), gains, ctypes.c_uint64(counter), _native.ptr(out), n_out,
+            _native.stream_handle(stream)), 'shccd_readout_cte')
+        return out
     _native.check(_native.lib().shccd_readout(ctypes.byref(params), F, None, 0, _native.ptr(frames), n_in,
                                               float(factor), gains, ctypes.c_uint64(counter), _native.ptr(out),
                                               n_out, _native.stream_handle(stream)), 'shccd_readout')

@@ -47,6 +47,13 @@ def test_ccd_argument_errors_are_reported():
     assert rc == -1 and b'ADU gain' in lib.shwfs_last_error()
     rc = lib.shccd_integrate(1, None, 0, 1.0, None, 0, 16, None)
     assert rc == -1
+    p.flags = 0
+    rc = lib.shccd_readout_cte(ctypes.byref(p), 1, None, 0, ctypes.c_void_p(16), 64, 1.0, None, None,
+                               ctypes.c_void_p(16), None, 0, ctypes.c_void_p(16), 16, None)
+    assert rc == -1 and b'CTE matrices' in lib.shwfs_last_error()
+    rc = lib.shccd_readout_cte(ctypes.byref(p), 0, None, 0, ctypes.c_void_p(16), 64, 1.0, ctypes.c_void_p(16),
+                               ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0, ctypes.c_void_p(16), 16, None)
+    assert rc == -1 and b'n_frames' in lib.shwfs_last_error()
 
 
 def test_argument_errors_are_reported():

@@ -237,3 +237,102 @@ def test_noisy_loop_slopes_match_oracle_statistics():
           f'noise rms {ref.std(0).mean():.2e}', flush=True)
     assert np.mean(z < 3.5) > 0.99 and z.max() < 5.5
     assert 0.85 < ratio < 1.15
+
+
+def _cte_pair(n, seed):
+    """Charge-transfer matrices of an n-pixel square frame: mostly diagonal, a few per cent of the
+    charge moved to the neighbouring rows / columns, plus a small dense positive part."""
+    rng = np.random.default_rng(seed)
+    mats = []
+    for k in range(2):
+        m = np.eye(n) * 0.97 + np.eye(n, k=1) * 0.02 + np.eye(n, k=-1) * 0.005 + rng.random((n, n)) * 1e-3
+        mats.append(m.astype(np.float32))
+    return np.stack(mats, axis=2)
+
+
+def test_cte_noiseless_matches_oracle():
+    """cte_noise (ccd.py:223-224) between the background/dark offset and the noise chain: binning 2,
+    QE, background + dark, cte_mat[:, :, 0] . frame . cte_mat[:, :, 1], against the oracle in FP64;
+    the FP32 products are bounded by a few ulp x sqrt(64) of the positive terms."""
+    rng = np.random.default_rng(5)
+    img = (rng.random((128, 128)) * 500).astype(np.float32)
+    cte = _cte_pair(64, 7)
+    p = params(128, 128, binning=2, quantum_eff=0.5, background_noise=True, darkcurrent_noise=True,
+               background_level=3.0, darkcurrent_level=2.0)
+    rows = torch.tensor(np.ascontiguousarray(cte[:, :, 0]), device=DEV)
+    cols = torch.tensor(np.ascontiguousarray(cte[:, :, 1]), device=DEV)
+    x = torch.tensor(img, device=DEV)[None].contiguous()
+    out = torch.empty((1, 64, 64), dtype=torch.float32, device=DEV)
+    readout_frames(p, x, out, cte=(rows, cols))
+    torch.cuda.synchronize()
+    ref = O.ccd_readout(img, binning=2, quantum_eff=0.5, background_noise=True, darkcurrent_noise=True,
+                        background_level=3.0, darkcurrent_level=2.0, cte_noise=True, cte_mat=cte.astype(np.float64),
+                        dtype=np.float64)
+    got = out[0].cpu().numpy().astype(np.float64)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f'cte noiseless: max rel err {err:.2e}', flush=True)
+    assert err < 2e-6 * np.sqrt(64)
+    # planted mistake on the oracle side: the transposed matrices give a different frame
+    wrong = O.ccd_readout(img, binning=2, quantum_eff=0.5, background_noise=True, darkcurrent_noise=True,
+                          background_level=3.0, darkcurrent_level=2.0, cte_noise=True,
+                          cte_mat=cte.transpose(1, 0, 2).astype(np.float64), dtype=np.float64)
+    assert np.abs(got - wrong).max() / np.abs(ref).max() > 1e-3
+
+
+def test_cte_with_photon_noise_and_dark_subtraction():
+    """With photon noise the CTE output is Poisson around cte_rows . (x QE + dark) . cte_cols, ADU
+    converted, with the dark current subtracted once: mean and variance over 256 frames per pixel."""
+    rng = np.random.default_rng(9)
+    n, F = 32, 256
+    img = (rng.random((n, n)) * 200 + 20).astype(np.float32)
+    cte = _cte_pair(n, 11)
+    p = params(n, n, quantum_eff=0.8, photon_noise=True, darkcurrent_noise=True, darkcurrent_level=4.0)
+    rows = torch.tensor(np.ascontiguousarray(cte[:, :, 0]), device=DEV)
+    cols = torch.tensor(np.ascontiguousarray(cte[:, :, 1]), device=DEV)
+    x = torch.tensor(np.broadcast_to(img, (F, n, n)).copy(), device=DEV)
+    out = torch.empty((F, n, n), dtype=torch.float32, device=DEV)
+    readout_frames(p, x, out, counter=3, cte=(rows, cols))
+    torch.cuda.synchronize()
+    o = out.cpu().numpy().astype(np.float64) + 4.0          # undo the subtraction of the dark level
+    lam = cte[:, :, 0].astype(np.float64) @ (img.astype(np.float64) * np.float32(0.8) + 4.0) @ cte[:, :, 1]
+    assert np.all(np.abs(o * 8 - np.round(o * 8)) < 1e-3)    # ADU gain 8: multiples of 1/8
+    z = (o.mean(axis=0) - lam) / np.sqrt(lam / F)
+    ratio = o.var(axis=0).mean() / lam.mean()
+    print(f'cte + photon: mean z {z.mean():.3f}, rms z {np.sqrt((z ** 2).mean()):.3f}, var/mean {ratio:.4f}',
+          flush=True)
+    assert abs(z.mean()) < 5 / np.sqrt(n * n) * 1.2 and abs(np.sqrt((z ** 2).mean()) - 1) < 0.1
+    assert abs(ratio - 1) < 0.05
+
+
+def test_ccd_object_cte_and_auto_levels():
+    """The drop-in object with cte_noise reads out as shccd_readout_cte with its own parameters
+    (bit-equal), and 'auto' read-out / dark levels come from the detector model at 1/dt."""
+    from specula_amd.data_objects.intensity import Intensity
+    from specula_amd.data_objects.simul_params import SimulParams
+    from specula_amd.lib.detector_noise import detector_noise
+    n = 48
+    cte = _cte_pair(n, 13)
+    ccd = CCD(SimulParams(time_step=0.001), size=[n, n], dt=0.001, bandw=300, name='OCAM2k', photon_noise=True,
+              readout_noise=True, darkcurrent_noise=True, readout_level='auto', darkcurrent_level='auto',
+              background_level=1.0, cte_noise=True, cte_mat=cte, target_device_idx=0)
+    ron, dark, _ = detector_noise(1000.0, 'OCAM2k', 1)
+    assert ccd._readout_level == ron and ccd._darkcurrent_level == dark
+    inp = Intensity(n, n, target_device_idx=0)
+    ccd.inputs['in_i'].set(inp)
+    ccd.setup()
+    img = (np.random.default_rng(2).random((n, n)) * 2).astype(np.float32)
+    inp.i[:] = torch.tensor(img, device=DEV)
+    t = ccd.seconds_to_t(0.001)
+    inp.generation_time = t
+    ccd.check_ready(t); ccd.trigger(); ccd.post_trigger()
+    torch.cuda.synchronize()
+    got = ccd.outputs['out_pixels'].pixels.cpu().numpy()
+    out = torch.empty((1, n, n), dtype=torch.float32, device=DEV)
+    readout_frames(ccd.params(), torch.tensor(img, device=DEV)[None].contiguous(), out, factor=0.001 * 300,
+                   counter=0, cte=(torch.tensor(np.ascontiguousarray(cte[:, :, 0]), device=DEV),
+                                   torch.tensor(np.ascontiguousarray(cte[:, :, 1]), device=DEV)))
+    torch.cuda.synchronize()
+    assert np.array_equal(got, out[0].cpu().numpy())
+    with pytest.raises(ValueError, match='cte_mat'):
+        CCD(SimulParams(time_step=0.001), size=[n, n], dt=0.001, bandw=300, cte_noise=True,
+            cte_mat=np.zeros((n, n + 1, 2), np.float32), target_device_idx=0)
