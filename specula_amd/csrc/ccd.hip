@@ -337,94 +337,121 @@ __global__ void __launch_bounds__(256) ccd_readout_kernel(ReadoutArgs a, int vec
 
 // Readout with photon noise.  Rejection sampling makes the number of tries per pixel random; so
 // that a wave does not wait for its slowest lane on every pixel, each lane walks its own list of
-// pixels (q, q + stride, ...) and one loop iteration is one try of whatever pixel the lane is
-// on: a lane that accepts moves to its next pixel while the others keep trying.  Every
-// iteration makes exactly one generator call (counter = try index) so that the call is not a
-// divergent branch; try 0 also supplies the readout normal (words z, w).
+// pixels and one loop iteration is one try of whatever pixel the lane is on: a lane that accepts
+// moves to its next pixel while the others keep trying.  Every iteration makes exactly one
+// generator call (counter = try index) so that the call is not a divergent branch; try 0 also
+// supplies the readout normal (words z, w).
+//
+// Each wave owns contiguous tiles of kPhotonTile pixels (tile t of the frame, waves striding over
+// the tiles); lane l takes pixels l, l + 64, ... of the tile and parks each accepted value in the
+// wave's LDS slice, and when every lane of the wave has finished the tile the wave stores it with
+// coalesced whole-line writes (the lanes accept their pixels at different times, so storing each
+// value when it is accepted would write partial cache lines).
+constexpr int kPhotonPerLane = 16;
+constexpr int kPhotonTile = 64 * kPhotonPerLane;
+
+__device__ __forceinline__ void ccd_wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <bool FULL>
 __global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
+    __shared__ float stage[4][kPhotonTile];
     const shccd_params& p = a.p;
     const int f = blockIdx.y;
     const int n_out = p.out_rows * p.out_cols;
-    const int stride = gridDim.x * blockDim.x;
     const uint32_t ctr = (uint32_t)a.counter;
     const uint32_t k0 = key0_of(p), k1 = key1_of(p);
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    bool fresh = true;
-    uint64_t px = 0;
-    uint32_t wz = 0, ww = 0;
-    float lam = 0.0f, inv_lam = 0.0f, enlam = 0.0f, a_ = 0.0f, b_ = 0.0f, vr = 0.0f;
-    float log_invalpha = 0.0f, log_lam = 0.0f;
-    double k = 0.0;
-    uint32_t tries = 0;
-    float next_raw = 0.0f;                     // signal of the lane's next pixel, loaded one pixel ahead
-    if (!FULL && q < n_out) next_raw = raw_pixel(a, f, q);
-    while (q < n_out) {
-        if (fresh) {
-            px = (uint64_t)f * (uint64_t)n_out + (uint64_t)q;
-            if (FULL) {
-                lam = load_pixel<true>(a, f, q, px, ctr);
-            } else {
-                lam = signal_of<false>(a, next_raw, px, ctr);
-                if (q + stride < n_out) next_raw = raw_pixel(a, f, q + stride);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* st = stage[wave];
+    float* out = a.pixels + f * a.out_stride;
+    const int n_tiles = (n_out + kPhotonTile - 1) / kPhotonTile;
+    for (int tile = blockIdx.x * 4 + wave; tile < n_tiles; tile += gridDim.x * 4) {
+        const int t0 = tile * kPhotonTile;
+        const int tn = min(kPhotonTile, n_out - t0);
+        int i = lane;                          // pixel t0 + i of the tile
+        bool fresh = true;
+        uint64_t px = 0;
+        uint32_t wz = 0, ww = 0;
+        float lam = 0.0f, inv_lam = 0.0f, enlam = 0.0f, a_ = 0.0f, b_ = 0.0f, vr = 0.0f;
+        float log_invalpha = 0.0f, log_lam = 0.0f;
+        double k = 0.0;
+        uint32_t tries = 0;
+        float next_raw = 0.0f;                 // signal of the lane's next pixel, loaded one pixel ahead
+        if (!FULL && i < tn) next_raw = raw_pixel(a, f, t0 + i);
+        while (i < tn) {
+            const int q = t0 + i;
+            if (fresh) {
+                px = (uint64_t)f * (uint64_t)n_out + (uint64_t)q;
+                if (FULL) {
+                    lam = load_pixel<true>(a, f, q, px, ctr);
+                } else {
+                    lam = signal_of<false>(a, next_raw, px, ctr);
+                    if (i + 64 < tn) next_raw = raw_pixel(a, f, q + 64);
+                }
+                tries = 0;
+                k = 0.0;
+                const float slam = __builtin_amdgcn_sqrtf(lam);
+                b_ = 0.931f + 2.53f * slam;
+                a_ = -0.059f + 0.02483f * b_;
+                log_invalpha = fast_log(1.1239f + 1.1328f * fast_rcp(b_ - 3.4f));
+                vr = 0.9277f - 3.6224f * fast_rcp(b_ - 2.0f);
+                log_lam = fast_log(lam);
+                inv_lam = fast_rcp(lam);
+                enlam = __builtin_amdgcn_exp2f(-1.44269504089f * lam);
+                fresh = false;
             }
-            tries = 0;
-            k = 0.0;
-            const float slam = __builtin_amdgcn_sqrtf(lam);
-            b_ = 0.931f + 2.53f * slam;
-            a_ = -0.059f + 0.02483f * b_;
-            log_invalpha = fast_log(1.1239f + 1.1328f * fast_rcp(b_ - 3.4f));
-            vr = 0.9277f - 3.6224f * fast_rcp(b_ - 2.0f);
-            log_lam = fast_log(lam);
-            inv_lam = fast_rcp(lam);
-            enlam = __builtin_amdgcn_exp2f(-1.44269504089f * lam);
-            fresh = false;
-        }
-        const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, tries}, k0, k1);
-        if (tries == 0) {
-            wz = w.z;
-            ww = w.w;
-        }
-        const float u1 = u01(w.x), u2 = u01(w.y);
-        ++tries;
-        bool done;
-        if (!(lam > 0.0f)) {
-            k = 0.0;
-            done = true;
-        } else if (lam < 10.0f) {
-            // inversion of the CDF with one 64-bit uniform: the lane finishes its pixel in this
-            // iteration (the search ends within 100 steps for lam < 10 unless U is within 1e-40
-            // of 1)
-            const double U = ((double)w.x * 4294967296.0 + (double)w.y + 1.0) * (1.0 / 18446744073709551616.0);
-            double pk = (double)enlam, F = pk;
-            k = 0.0;
-            for (int st = 0; st < 100 && U > F; ++st) {
-                k += 1.0;
-                pk *= (double)(lam * fast_rcp((float)k));
-                F += pk;
+            const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, tries}, k0, k1);
+            if (tries == 0) {
+                wz = w.z;
+                ww = w.w;
             }
-            done = true;
-        } else {                                                   // PTRS
-            const float U = u1 - 0.5f, V = u2;
-            const float us = 0.5f - fabsf(U);
-            k = floor((double)((2.0f * a_ * fast_rcp(us) + b_) * U) + (double)lam + 0.43);
-            done = us >= 0.07f && V <= vr;
-            if (!done && !(k < 0.0 || (us < 0.013f && V > us))) {
-                const float lhs = fast_log(V) + log_invalpha - fast_log(a_ * fast_rcp(us * us) + b_);
-                const float rhs = k < 32.0 ? -lam + (float)k * log_lam - (float)kLogFact[(int)k]
-                                           : log_pmf_large(k, lam, inv_lam);
-                done = lhs <= rhs;
+            const float u1 = u01(w.x), u2 = u01(w.y);
+            ++tries;
+            bool done;
+            if (!(lam > 0.0f)) {
+                k = 0.0;
+                done = true;
+            } else if (lam < 10.0f) {
+                // inversion of the CDF with one 64-bit uniform: the lane finishes its pixel in this
+                // iteration (the search ends within 100 steps for lam < 10 unless U is within 1e-40
+                // of 1)
+                const double U = ((double)w.x * 4294967296.0 + (double)w.y + 1.0) * (1.0 / 18446744073709551616.0);
+                double pk = (double)enlam, F = pk;
+                k = 0.0;
+                for (int s = 0; s < 100 && U > F; ++s) {
+                    k += 1.0;
+                    pk *= (double)(lam * fast_rcp((float)k));
+                    F += pk;
+                }
+                done = true;
+            } else {                                                   // PTRS
+                const float U = u1 - 0.5f, V = u2;
+                const float us = 0.5f - fabsf(U);
+                k = floor((double)((2.0f * a_ * fast_rcp(us) + b_) * U) + (double)lam + 0.43);
+                done = us >= 0.07f && V <= vr;
+                if (!done && !(k < 0.0 || (us < 0.013f && V > us))) {
+                    const float lhs = fast_log(V) + log_invalpha - fast_log(a_ * fast_rcp(us * us) + b_);
+                    const float rhs = k < 32.0 ? -lam + (float)k * log_lam - (float)kLogFact[(int)k]
+                                               : log_pmf_large(k, lam, inv_lam);
+                    done = lhs <= rhs;
+                }
+            }
+            if (!done && tries >= 512u) {                              // never reached in practice
+                k = fmax(k, 0.0);
+                done = true;
+            }
+            if (done) {
+                st[i] = finish_value<FULL>(a, q, px, ctr, k, true, normal_from(wz, ww));
+                i += 64;
+                fresh = true;
             }
         }
-        if (!done && tries >= 512u) {                              // never reached in practice
-            k = fmax(k, 0.0);
-            done = true;
-        }
-        if (done) {
-            a.pixels[f * a.out_stride + q] = finish_value<FULL>(a, q, px, ctr, k, true, normal_from(wz, ww));
-            q += stride;
-            fresh = true;
-        }
+        ccd_wave_lds_sync();                   // every lane of the wave has parked its pixels
+        for (int j = lane; j < tn; j += 64) out[t0 + j] = st[j];
+        ccd_wave_lds_sync();                   // the slice is read before the next tile reuses it
     }
 }
 
@@ -434,6 +461,12 @@ __global__ void __launch_bounds__(256) ccd_integrate_kernel(const float* __restr
     const int f = blockIdx.y;
     for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_px; q += (int64_t)gridDim.x * blockDim.x)
         integrated[f * int_stride + q] += in_i[f * in_stride + q] * (float)factor;
+}
+
+// blocks of the photon kernel: one tile of kPhotonTile pixels per wave, 4 waves per block
+int photon_blocks(int64_t n) {
+    const int64_t tiles = (n + kPhotonTile - 1) / kPhotonTile;
+    return (int)std::max<int64_t>(1, std::min<int64_t>((tiles + 3) / 4, 4096));
 }
 
 // blocks for n pixels with at least `per_lane` pixels per lane (capped at 4096 blocks)
@@ -490,10 +523,10 @@ int shccd_readout(const shccd_params* p, int n_frames, float* integrated, int64_
     const bool full = p->binning != 1 || (p->flags & (SHCCD_CIC | SHCCD_EXCESS)) || pixel_gains != nullptr;
     if (p->flags & SHCCD_PHOTON) {
         if (full)
-            hipLaunchKernelGGL(ccd_photon_kernel<true>, dim3(grid_for(n_out, 32), n_frames), dim3(256), 0,
+            hipLaunchKernelGGL(ccd_photon_kernel<true>, dim3(photon_blocks(n_out), n_frames), dim3(256), 0,
                                reinterpret_cast<hipStream_t>(stream), a);
         else
-            hipLaunchKernelGGL(ccd_photon_kernel<false>, dim3(grid_for(n_out, 32), n_frames), dim3(256), 0,
+            hipLaunchKernelGGL(ccd_photon_kernel<false>, dim3(photon_blocks(n_out), n_frames), dim3(256), 0,
                                reinterpret_cast<hipStream_t>(stream), a);
     } else {
         auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
