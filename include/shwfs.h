@@ -4,8 +4,9 @@
  * Plain C types only: every array argument is a device pointer on the plan's device,
  * every stream is a hipStream_t passed as void*.  All functions return 0 on success or a
  * negative error code; shwfs_last_error() returns the message of the last failure on the
- * calling thread.  No function allocates device memory except the *_create calls and the set-up
- * calls shwfs_plan_set_pupil (resampled plans), shwfs_plan_set_resampling and shwfs_plan_bind_slopec.
+ * calling thread.  No function allocates device memory except the *_create calls, the set-up
+ * calls shwfs_plan_set_pupil (resampled plans), shwfs_plan_set_resampling and shwfs_plan_bind_slopec,
+ * and the first VJP call on a non-Parseval or resampled plan (its scratch).
  *
  * Each entry point names the reference interface it replaces (paths relative to the
  * SPECULA checkout).  The reference path is Python; the ctypes binding a maintainer
@@ -75,7 +76,8 @@ int shwfs_set_kernels(shwfs_plan* plan, const float* kspec_lags, void* stream);
 
 /* SH forward for n_frames ElectricFields: one CCD intensity per frame, normalised to
  * S0 * masked_area (sh.py:422-428).  A, phase_nm: ef_size x ef_size float32 per frame
- * (a_frame_stride may be 0 to share one amplitude); ccd: (n*subap_npx)^2 float32 per frame.
+ * (a_frame_stride, phase_frame_stride may be 0 to share one amplitude or phase); ccd:
+ * (n*subap_npx)^2 float32 per frame.
  * Replaces SH.trigger_code + SH.post_trigger (processing_objects/sh.py:348-428). */
 int shwfs_forward(shwfs_plan* plan, int n_frames,
                   const float* A, int64_t a_frame_stride,
@@ -181,12 +183,35 @@ int shslopec_vjp(shslopec_plan* plan, int n_frames, const float* pixels, int64_t
                  float thr, int thr_mode, const float* grad_slopes, int64_t grad_slopes_stride,
                  float* grad_pixels, void* stream);
 
-/* shwfs_vjp: dL/dphase_nm from dL/d(CCD intensity) for shwfs_forward (NGS path, Parseval
- * normalisation).  phase_frame_stride may be 0 to evaluate several cotangents at one phase. */
+/* shwfs_vjp: dL/dphase_nm from dL/d(CCD intensity) for shwfs_forward (specialised NGS kernels;
+ * not the LGS path or variant 4).  phase_frame_stride may be 0 to evaluate several cotangents at
+ * one phase.  Any normalisation: with the Parseval scale (no cut, all-ones mask) the scale does not
+ * depend on the phase; otherwise the image I = c(phi) B is normalised by its own sum and the VJP
+ * is taken exactly (the forward runs first into plan scratch to get c and <g, I> / sum(I)).  A
+ * resampled plan (shwfs_plan_set_resampling) takes A, phase and returns the gradient on the input
+ * grid: the sensor-grid gradient goes through the transpose of the resampling (float atomics:
+ * summation order not fixed).  The first call on a non-Parseval or resampled plan allocates its
+ * scratch (kept by the plan). */
 int shwfs_vjp(shwfs_plan* plan, int n_frames, const float* A, int64_t a_frame_stride,
               const float* phase_nm, int64_t phase_frame_stride, double S0,
               const float* grad_ccd, int64_t grad_ccd_stride, float* grad_phase, int64_t grad_phase_stride,
               void* stream);
+
+/* As shwfs_vjp, but the gradient is returned on the sensor grid (the plan's ef_size, i.e. with
+ * respect to the resampled phase; same as shwfs_vjp for a plan without resampling).  With it the
+ * slope Jacobian of a resampled sensor is block diagonal again, so shwfs_im_from_jacobian applies
+ * to modes resampled with shwfs_resample.  No reference counterpart (the reference builds
+ * interaction matrices by push-pull only, im_rec_calibrator.py:58-63). */
+int shwfs_vjp_sensor_grid(shwfs_plan* plan, int n_frames, const float* A, int64_t a_frame_stride,
+                          const float* phase_nm, int64_t phase_frame_stride, double S0,
+                          const float* grad_ccd, int64_t grad_ccd_stride, float* grad_phase_sensor,
+                          int64_t grad_phase_stride, void* stream);
+
+/* The plan's field resampling (edge extrapolation + bilinear resampling, sh.py:350-360) applied to
+ * n_frames phase-like frames on the input grid (in_frame_stride 0: one frame for all) into the
+ * sensor grid.  Errors for a plan without resampling. */
+int shwfs_resample(shwfs_plan* plan, int n_frames, const float* in, int64_t in_frame_stride, float* out,
+                   int64_t out_frame_stride, void* stream);
 
 /* Interaction matrix from the block-diagonal slope Jacobian (fov_ovs == 1):
  *   im[m][slot]           = scale * sum_{p in subaperture} jx[p] * modes[m][p]

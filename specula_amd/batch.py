@@ -3,6 +3,9 @@ frame, the per-sensor CoG launches after it, all capturable in one HIP graph.
 
 A sensor entry owns its plan, static pupil, phase frames, CCD frames and (optionally) a
 ShSlopec plan with slope outputs.  Buffers are allocated once; ``forward()`` only launches.
+Sensors with field resampling take their phase on the input grid (the batch resamples it in the
+same launch sequence); sensors whose image is normalised by its own sum (FoV cut or clipping
+mask) are rescaled after the SH launches.
 """
 import ctypes
 from dataclasses import dataclass, field
@@ -39,6 +42,8 @@ def make_entry(name, plan, pupil, n_frames, S0=1.0, with_slopes=True, energy_th=
     dev = torch.device(f'cuda:{device}')
     g = plan.geom
     A = torch.as_tensor(np.asarray(pupil, np.float32), device=dev).contiguous()
+    if g.needs_interpolation and not getattr(plan, 'resampled', False):
+        plan.set_resampling(np.asarray(pupil))      # edge-extrapolation maps from the pupil (sh.py:452-458)
     ph = torch.zeros((n_frames, g.ef_size, g.ef_size), dtype=torch.float32, device=dev)
     ccd = torch.empty((n_frames, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
     e = SensorEntry(name, plan, A, ph, ccd, S0)

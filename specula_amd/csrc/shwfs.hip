@@ -1331,6 +1331,53 @@ __global__ void resample_kernel(const float* __restrict__ in, int64_t in_stride,
     }
 }
 
+// Transpose of resample_kernel (the resampling, edge extrapolation included, is linear in the
+// phase): gin += R^T gout.  Each output pixel recomputes its taps exactly as the forward does and
+// scatters its four bilinear weights times gout into the input pixels it read (2 w to the first
+// and -w to the second source of an extrapolated pixel).  Float atomics in L2: the summation order
+// is not fixed, so results agree with a fixed-order sum to FP32 rounding.  gin must be zeroed.
+__global__ void resample_vjp_kernel(const float* __restrict__ gout, int64_t gout_stride, const int2* __restrict__ ext,
+                                    float* __restrict__ gin, int64_t gin_stride, int ef_in, int ef_out,
+                                    ResampleMap m) {
+    const int f = blockIdx.y;
+    const float* go = gout + f * gout_stride;
+    float* gi = gin + f * gin_stride;
+    const int64_t n_out = (int64_t)ef_out * ef_out;
+    const double lim = ef_in - 1;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_out; q += (int64_t)gridDim.x * blockDim.x) {
+        const float gq = go[q];
+        if (gq == 0.f) continue;
+        const int Y = (int)(q / ef_out), X = (int)(q - (int64_t)Y * ef_out);
+        const double y = fmin(fmax(m.yy * Y + m.yx * X + m.y0, 0.0), lim);
+        const double x = fmin(fmax(m.xy * Y + m.xx * X + m.x0, 0.0), lim);
+        const int i0 = min((int)floor(y), ef_in - 2), j0 = min((int)floor(x), ef_in - 2);
+        const float ty = (float)(y - i0), tx = (float)(x - j0);
+        const float w[4] = {(1.f - ty) * (1.f - tx), (1.f - ty) * tx, ty * (1.f - tx), ty * tx};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float wg = w[k] * gq;
+            if (wg == 0.f) continue;
+            const int p = (i0 + (k >> 1)) * ef_in + j0 + (k & 1);
+            const int2 e = ext ? ext[p] : make_int2(-1, -1);
+            if (e.x < 0) {
+                atomicAdd(gi + p, wg);
+            } else if (e.y < 0) {
+                atomicAdd(gi + e.x, wg);
+            } else {
+                atomicAdd(gi + e.x, 2.f * wg);
+                atomicAdd(gi + e.y, -wg);
+            }
+        }
+    }
+}
+
+// Zero n_px floats of each frame (frame stride `stride`); grid (blocks, frames).
+__global__ void zero_frames_kernel(float* __restrict__ p, int64_t stride, int64_t n_px) {
+    float* q = p + blockIdx.y * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n_px; i += (int64_t)gridDim.x * blockDim.x)
+        q[i] = 0.f;
+}
+
 // ---------------------------------------------------------------- ShSlopec CoG
 // One wavefront per subaperture: FP64 moments (flux, thresholded total, x and y numerators).
 __global__ void cog_moments_kernel(const float* __restrict__ pix, int64_t pix_stride,
@@ -1568,7 +1615,34 @@ struct VjpArgs {
     int amp_fstep;
     int n_lenses, ef_size, npx, n_sub;
     double k, tilt, a0, S0, pixel_area, norm_const;
+    const double2* cg;     // non-Parseval plans: per frame (scale c_f, cotangent offset alpha_f)
 };
+
+// Per-frame factors of the VJP of an image normalised by its own sum (non-Parseval plans): with
+// I = c B, c = S0 pixel_area sum(A) / sum(B), the image VJP is the fixed-scale VJP of the
+// cotangent g' = g - alpha, alpha = <g, I> / sum(I) (the derivative of c(phi) folds into the
+// offset).  One block per frame; `img` is the forward image I of the same inputs, `bpart` the
+// forward's partial sums of the unscaled B.
+__global__ void vjp_scale_kernel(const double* __restrict__ bpart, int n_parts, const double* __restrict__ amp,
+                                 int amp_fstep, double S0, double pixel_area, const float* __restrict__ img,
+                                 int64_t img_stride, const float* __restrict__ gccd, int64_t gccd_stride,
+                                 int64_t n_px, double2* __restrict__ cg) {
+    __shared__ double red[16];
+    const int f = blockIdx.x;
+    double t = 0.0;
+    for (int b = threadIdx.x; b < n_parts; b += blockDim.x) t += bpart[(int64_t)f * n_parts + b];
+    const float* im = img + f * img_stride;
+    const float* gc = gccd + f * gccd_stride;
+    double gi = 0.0, si = 0.0;
+    for (int64_t q = threadIdx.x; q < n_px; q += blockDim.x) {
+        const double v = im[q];
+        gi += (double)gc[q] * v;
+        si += v;
+    }
+    t = block_sum(t, red);
+    const double2 gs = block_sum2(gi, si, red);
+    if (threadIdx.x == 0) cg[f] = make_double2(S0 * pixel_area * amp[(int64_t)f * amp_fstep] / t, gs.x / gs.y);
+}
 
 // SH backward for the NGS path with Parseval normalisation (I = c B, c independent of the
 // phase): with G_B = c dL/dI on the subaperture tile,
@@ -1595,7 +1669,13 @@ sh_ngs_vjp_kernel(VjpArgs g) {
     }
     const int t = threadIdx.x, f = blockIdx.y, tile = blockIdx.x, n = g.n_lenses;
     const double* am = g.amp + (int64_t)f * g.amp_fstep;
-    const float c = (float)(g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]));
+    float c, alpha = 0.f;
+    if (g.cg) {
+        c = (float)g.cg[f].x;
+        alpha = (float)g.cg[f].y;
+    } else {
+        c = (float)(g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]));
+    }
 
     for (int e = t; e < TS * npx2; e += NT) {         // G_B tiles
         const int s = e / npx2, rem = e - s * npx2;
@@ -1604,7 +1684,7 @@ sh_ngs_vjp_kernel(VjpArgs g) {
         float v = 0.f;
         if (sub < g.n_sub) {
             const int i = sub / n, j = sub - (sub / n) * n;
-            v = g.gccd[f * g.gccd_stride + (int64_t)(i * npx + Y) * (n * npx) + j * npx + X] * c;
+            v = (g.gccd[f * g.gccd_stride + (int64_t)(i * npx + Y) * (n * npx) + j * npx + X] - alpha) * c;
         }
         gb[s * npx2 + rem] = v;
     }
@@ -1938,7 +2018,9 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
             const int wv = threadIdx.x >> 6;
             float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
             const int wave_id = w.tile * NgsMfmaShape<8>::WPB + wv;
-            if (g.npx == 16 && g.store_vec)
+            if (!g.parseval)       // image normalised by its own sum: partial sums, rescaled afterwards
+                ngs_mfma_frames<8, false, 0, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+            else if (g.npx == 16 && g.store_vec)
                 ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
             else if (g.npx == 8 && g.store_vec)
                 ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
@@ -1990,6 +2072,9 @@ struct shwfs_plan {
     float* d_A_rs_st = nullptr;    // [static_frames][ef_size^2] resampled registered pupil
     int A_rs_st_frames = 0;        // frames allocated in d_A_rs_st
     float* d_ph_rs = nullptr;      // [max_frames][ef_size^2] resampled phase
+    float* d_gph_rs = nullptr;     // VJP of a resampled plan: [max_frames][ef_size^2] sensor-grid gradient
+    float* d_vjp_ccd = nullptr;    // VJP of a non-Parseval plan: [max_frames][ccd px] forward image
+    double2* d_vjp_cg = nullptr;   // VJP of a non-Parseval plan: [max_frames] (c_f, alpha_f)
     const float* static_A = nullptr;   // registered pupil, statistics in d_amp_st / d_scale_st
     int64_t static_a_stride = 0;
     int static_frames = 0;
@@ -2353,6 +2438,9 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_A_rs);
     (void)hipFree(p->d_A_rs_st);
     (void)hipFree(p->d_ph_rs);
+    (void)hipFree(p->d_gph_rs);
+    (void)hipFree(p->d_vjp_ccd);
+    (void)hipFree(p->d_vjp_cg);
     (void)hipFree(p->d_cogw);
     (void)hipFree(p->d_tw);
     (void)hipFree(p->d_slab);
@@ -2439,7 +2527,7 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     const int64_t in_px = p->ef_in ? (int64_t)p->ef_in * p->ef_in : ef_px;   // input field size
     const int64_t ccd_side = (int64_t)g.n_lenses * g.subap_npx;
     if (a_stride != 0 && a_stride < in_px) return set_error(SHWFS_ERR_ARG, "A frame stride too small");
-    if (n_frames > 1 && ph_stride < in_px) return set_error(SHWFS_ERR_ARG, "phase frame stride too small");
+    if (n_frames > 1 && ph_stride != 0 && ph_stride < in_px) return set_error(SHWFS_ERR_ARG, "phase frame stride too small");
     if (n_frames > 1 && ccd_stride < ccd_side * ccd_side)
         return set_error(SHWFS_ERR_ARG, "ccd frame stride too small");
     DeviceGuard dg(p->device);
@@ -2938,6 +3026,100 @@ int scratch_of_current_device(unsigned char** out) {
     return SHWFS_OK;
 }
 
+// SH VJP of any specialised NGS plan.  Non-Parseval plans first run the forward into scratch (the
+// image and its unscaled sum give the per-frame scale and cotangent offset, vjp_scale_kernel);
+// resampled plans run the VJP on the sensor (resampled) grid and, unless `sensor_grid`, scatter it
+// back to the input grid through the transpose of the resampling.  First calls allocate the
+// scratch these need (plan-owned, kept).
+int vjp_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase, int64_t ph_stride,
+             double S0, const float* grad_ccd, int64_t gccd_stride, float* grad_out, int64_t gout_stride,
+             bool sensor_grid, hipStream_t st) {
+    if (!p || !A || !phase || !grad_ccd || !grad_out) return set_error(SHWFS_ERR_ARG, "null argument");
+    const shwfs_geom& g = p->g;
+    if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the NGS path");
+    if (p->general) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the specialised NGS kernels (not variant 4)");
+    if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
+    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;                      // sensor grid
+    const int64_t in_px = p->ef_in ? (int64_t)p->ef_in * p->ef_in : ef_px;     // input grid
+    const int64_t out_px = (sensor_grid || !p->ef_in) ? ef_px : in_px;
+    const int64_t ccd_px = (int64_t)g.n_lenses * g.subap_npx * g.n_lenses * g.subap_npx;
+    if (n_frames > 1 && ((a_stride != 0 && a_stride < in_px) || (ph_stride != 0 && ph_stride < in_px) ||
+                         gccd_stride < ccd_px || gout_stride < out_px))
+        return set_error(SHWFS_ERR_ARG, "frame stride too small");
+    int ts = 0;
+    VjpLauncher L = pick_vjp(g.np_sub, g.spec_size, g.subap_npx, &ts);
+    if (!L) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP instantiated for (np_sub, spec_size) in {(4,8),(6,16),(8,16),(16,32)}");
+    DeviceGuard dg(p->device);
+    const int amp_frames = (a_stride == 0) ? 1 : n_frames;
+    const double2* cg = nullptr;
+    bool cached;
+    if (!g.parseval) {
+        if (!p->d_vjp_ccd) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_vjp_ccd, (size_t)g.max_frames * ccd_px * sizeof(float)));
+        if (!p->d_vjp_cg) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_vjp_cg, (size_t)g.max_frames * sizeof(double2)));
+        // the forward's own choice of statistics buffers (registered pupil or not)
+        cached = A == p->static_A && a_stride == p->static_a_stride && S0 == p->static_S0 &&
+                 amp_frames <= p->static_frames;
+        int rc = sh_forward_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, p->d_vjp_ccd, ccd_px, nullptr, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(vjp_scale_kernel, dim3(n_frames), dim3(1024), 0, st, p->d_bpart, p->n_parts,
+                           cached ? p->d_amp_st : p->d_amp, (a_stride == 0) ? 0 : 2, S0, g.pixel_area,
+                           p->d_vjp_ccd, ccd_px, grad_ccd, gccd_stride, ccd_px, p->d_vjp_cg);
+        SHWFS_LAUNCH_CHECK();
+        cg = p->d_vjp_cg;
+    } else {
+        // sum(A), sum(A^2) do not depend on S0: the registered pupil's statistics serve any S0
+        cached = A == p->static_A && a_stride == p->static_a_stride && amp_frames <= p->static_frames;
+        if (!cached) {
+            int rc = pupil_stats(p, A, a_stride, amp_frames, S0, p->d_amp, p->d_scale, p->d_A_rs, st);
+            if (rc) return rc;
+        }
+        if (p->ef_in) {
+            int rc = launch_resample(p, phase, ph_stride, p->d_ext, p->d_ph_rs, n_frames, st);
+            if (rc) return rc;
+        }
+    }
+    const float* As = A;
+    const float* phs = phase;
+    int64_t as = a_stride, ps = ph_stride;
+    if (p->ef_in) {      // the sensor-grid field of the forward (resampled amplitude and phase)
+        As = cached ? p->d_A_rs_st : p->d_A_rs;
+        as = (a_stride == 0) ? 0 : ef_px;
+        phs = p->d_ph_rs;
+        ps = ef_px;
+    }
+    float* gs = grad_out;
+    int64_t gss = gout_stride;
+    if (p->ef_in && !sensor_grid) {
+        if (!p->d_gph_rs) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_gph_rs, (size_t)g.max_frames * ef_px * sizeof(float)));
+        gs = p->d_gph_rs;
+        gss = ef_px;
+    }
+    VjpArgs a{};
+    a.A = As; a.a_stride = as; a.phase = phs; a.ph_stride = ps;
+    a.gccd = grad_ccd; a.gccd_stride = gccd_stride; a.gphase = gs; a.gph_stride = gss;
+    a.roww = p->d_roww; a.colw = p->d_colw; a.amp = cached ? p->d_amp_st : p->d_amp;
+    a.amp_fstep = (a_stride == 0) ? 0 : 2;
+    a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.n_sub = p->n_sub;
+    a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0; a.pixel_area = g.pixel_area;
+    a.norm_const = g.norm_const;
+    a.cg = cg;
+    const int n_tiles = (p->n_sub + ts - 1) / ts;
+    // the subaperture tiles cover the sensor grid (plans have ef_size = n_lenses * np_sub)
+    int rc = L(a, n_tiles, n_frames, g.subap_npx, st);
+    if (rc) return rc;
+    if (p->ef_in && !sensor_grid) {
+        const int64_t zb = std::min<int64_t>((in_px + 255) / 256, 2048);
+        hipLaunchKernelGGL(zero_frames_kernel, dim3((unsigned)zb, n_frames), dim3(256), 0, st, grad_out, gout_stride,
+                           in_px);
+        SHWFS_LAUNCH_CHECK();
+        const int64_t rb = std::min<int64_t>((ef_px + 255) / 256, 2048);
+        hipLaunchKernelGGL(resample_vjp_kernel, dim3((unsigned)rb, n_frames), dim3(256), 0, st, gs, gss, p->d_ext,
+                           grad_out, gout_stride, p->ef_in, g.ef_size, p->rmap);
+        SHWFS_LAUNCH_CHECK();
+    }
+    return SHWFS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2945,43 +3127,32 @@ extern "C" {
 int shwfs_vjp(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase,
               int64_t ph_stride, double S0, const float* grad_ccd, int64_t gccd_stride, float* grad_phase,
               int64_t gph_stride, void* stream) {
-    if (!p || !A || !phase || !grad_ccd || !grad_phase) return set_error(SHWFS_ERR_ARG, "null argument");
-    const shwfs_geom& g = p->g;
-    if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the NGS path");
-    if (p->general) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the specialised NGS kernels (not variant 4)");
-    if (p->ef_in) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available without field resampling (fov_ovs 1)");
-    if (!g.parseval)
-        return set_error(SHWFS_ERR_UNSUPPORTED, "VJP needs the Parseval normalisation (no FoV cut, all-ones mask)");
-    if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
-    const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;
-    const int64_t ccd_px = (int64_t)g.n_lenses * g.subap_npx * g.n_lenses * g.subap_npx;
-    if (n_frames > 1 && ((a_stride != 0 && a_stride < ef_px) || (ph_stride != 0 && ph_stride < ef_px) ||
-                         gccd_stride < ccd_px || gph_stride < ef_px))
+    return vjp_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, grad_ccd, gccd_stride, grad_phase, gph_stride,
+                    false, reinterpret_cast<hipStream_t>(stream));
+}
+
+int shwfs_vjp_sensor_grid(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase,
+                          int64_t ph_stride, double S0, const float* grad_ccd, int64_t gccd_stride,
+                          float* grad_phase_sensor, int64_t gph_stride, void* stream) {
+    return vjp_impl(p, n_frames, A, a_stride, phase, ph_stride, S0, grad_ccd, gccd_stride, grad_phase_sensor,
+                    gph_stride, true, reinterpret_cast<hipStream_t>(stream));
+}
+
+int shwfs_resample(shwfs_plan* p, int n_frames, const float* in, int64_t in_stride, float* out, int64_t out_stride,
+                   void* stream) {
+    if (!p || !in || !out) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (!p->ef_in) return set_error(SHWFS_ERR_ARG, "plan has no field resampling (shwfs_plan_set_resampling)");
+    if (n_frames <= 0 || n_frames > 65535) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, 65535]");
+    const int64_t in_px = (int64_t)p->ef_in * p->ef_in, out_px = (int64_t)p->g.ef_size * p->g.ef_size;
+    if (n_frames > 1 && ((in_stride != 0 && in_stride < in_px) || out_stride < out_px))
         return set_error(SHWFS_ERR_ARG, "frame stride too small");
-    int ts = 0;
-    VjpLauncher L = pick_vjp(g.np_sub, g.spec_size, g.subap_npx, &ts);
-    if (!L) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP instantiated for (np_sub, spec_size) in {(4,8),(6,16),(8,16),(16,32)}");
     DeviceGuard dg(p->device);
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    const int amp_frames = (a_stride == 0) ? 1 : n_frames;
-    // sum(A), sum(A^2) do not depend on S0: the registered pupil's statistics serve any S0
-    const bool cached = A == p->static_A && a_stride == p->static_a_stride && amp_frames <= p->static_frames;
-    double* amp = cached ? p->d_amp_st : p->d_amp;
-    if (!cached) {
-        hipLaunchKernelGGL(amp_stats_kernel, dim3(p->amp_blocks, amp_frames), dim3(256), 0, st,
-                           A, a_stride, ef_px, p->d_amp_part, p->d_counter, amp, 0.0, 0.0, (float*)nullptr,
-                           (const float*)nullptr, (int64_t)0, (int64_t)0);
-        SHWFS_LAUNCH_CHECK();
-    }
-    VjpArgs a{};
-    a.A = A; a.a_stride = a_stride; a.phase = phase; a.ph_stride = ph_stride;
-    a.gccd = grad_ccd; a.gccd_stride = gccd_stride; a.gphase = grad_phase; a.gph_stride = gph_stride;
-    a.roww = p->d_roww; a.colw = p->d_colw; a.amp = amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
-    a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.n_sub = p->n_sub;
-    a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0; a.pixel_area = g.pixel_area;
-    a.norm_const = g.norm_const;
-    const int n_tiles = (p->n_sub + ts - 1) / ts;
-    return L(a, n_tiles, n_frames, g.subap_npx, st);
+    const int64_t blocks = std::min<int64_t>((out_px + 255) / 256, 2048);
+    hipLaunchKernelGGL(resample_kernel, dim3((unsigned)blocks, n_frames), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), in, in_stride, p->d_ext, out, out_stride, p->ef_in,
+                       p->g.ef_size, p->rmap);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
 }
 
 int shslopec_vjp(shslopec_plan* p, int n_frames, const float* pixels, int64_t pix_stride, float thr, int thr_mode,
@@ -3074,6 +3245,24 @@ struct shwfs_batch {
     unsigned long long* d_ts = nullptr;   // [1 + n_sensors]
     int32_t* d_order = nullptr;       // lit-first subaperture order of each sensor's pupil, concatenated
     uint8_t* d_lit = nullptr;         // lit flags (scratch of the order computation)
+    struct Resample {                 // resampled sensor: its phase frames resampled before the SH launches
+        const shwfs_plan* plan;
+        const float* phase;           // input-grid frames, ef_in^2 apart
+        int n_frames;
+        float* ph_rs;                 // sensor-grid frames, ef_size^2 apart
+    };
+    struct Rescale {                  // non-Parseval sensor: its images rescaled after the SH launches
+        float* ccd;
+        int64_t ccd_px;
+        double* bpart;
+        int n_parts;
+        const double* amp;
+        double S0, pixel_area;
+        int n_frames;
+    };
+    std::vector<Resample> resample;
+    std::vector<Rescale> rescale;
+    std::vector<void*> bufs;          // device buffers of the above (resampled fields, partial sums)
 };
 
 extern "C" {
@@ -3090,6 +3279,7 @@ int shwfs_batch_destroy(shwfs_batch* b) {
     (void)hipFree(b->d_ts);
     (void)hipFree(b->d_order);
     (void)hipFree(b->d_lit);
+    for (void* q : b->bufs) (void)hipFree(q);
     delete b;
     return SHWFS_OK;
 }
@@ -3102,7 +3292,7 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     if (order < 0 || order > 2) return set_error(SHWFS_ERR_ARG, "order must be 0, 1 or 2");
     std::vector<NgsArgs> args(n_sensors);
     std::vector<BatchItem> items;
-    std::vector<int> cost(n_sensors), kinds(n_sensors);
+    std::vector<int> cost(n_sensors), kinds(n_sensors), tiles_of(n_sensors);
     for (int s = 0; s < n_sensors; ++s) {
         const shwfs_plan* p = plans[s];
         if (!p || !A[s] || !phase[s] || !ccd[s]) return set_error(SHWFS_ERR_ARG, "null sensor entry");
@@ -3112,12 +3302,11 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         if (!g.lgs && p->mfma && g.np_sub == 8 && g.spec_size == 16) kind = 0;
         else if (g.lgs && !p->general && g.np_sub == 16 && g.spec_size == 32) kind = 1;
         else return set_error(SHWFS_ERR_UNSUPPORTED, "batch supports NGS MFMA np 8 and LGS np 16 plans");
-        if (!g.parseval) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs Parseval-normalised sensors");
-        if (p->ef_in) return set_error(SHWFS_ERR_UNSUPPORTED, "batch needs sensors without field resampling");
         if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan without kernels");
-        if (n_frames[s] <= 0) return set_error(SHWFS_ERR_ARG, "non-positive frame count");
+        if (n_frames[s] <= 0 || n_frames[s] > 65535) return set_error(SHWFS_ERR_ARG, "frame count outside [1, 65535]");
         const int ts = (kind == 0) ? 4 * NgsMfmaShape<8>::SW : 8;
         const int tiles = (p->n_sub + ts - 1) / ts;
+        tiles_of[s] = tiles;
         cost[s] = kind == 0 ? 1 : 4;
         NgsArgs& a = args[s];
         a = NgsArgs{};
@@ -3127,8 +3316,8 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         a.roww = (kind == 0) ? p->d_rowlag : p->d_roww; a.colw = p->d_colw; a.amp_fstep = 0; a.bpart = nullptr;
         a.kspec = reinterpret_cast<const float2*>(p->d_kspec);
         a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
-        a.n_sub = p->n_sub; a.n_tiles = tiles; a.parseval = 1;
-        a.vec = (g.ef_size % 4 == 0) && aligned16(A[s]) && aligned16(phase[s]);
+        a.n_sub = p->n_sub; a.n_tiles = tiles; a.parseval = g.parseval ? 1 : 0;
+        a.n_parts = (kind == 0) ? tiles * NgsMfmaShape<8>::WPB : tiles;
         a.store_vec = aligned16(ccd[s]) && ((int64_t)g.n_lenses * g.subap_npx) % 4 == 0;
         a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0[s];
         a.pixel_area = g.pixel_area; a.norm_const = g.norm_const;
@@ -3178,25 +3367,59 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     if ((e = hipMalloc((void**)&b->d_order, n_order * sizeof(int32_t))) != hipSuccess ||
         (e = hipMalloc((void**)&b->d_lit, n_order)) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
-    // the pupils are static for the batch: amplitude statistics and the lit-first subaperture
-    // order (Parseval sensors: their images do not depend on the grouping) once, here
+    auto alloc = [&](size_t bytes, void** q) {
+        *q = nullptr;
+        e = hipMalloc(q, bytes);
+        if (e == hipSuccess) b->bufs.push_back(*q);
+        return e == hipSuccess;
+    };
+    // the pupils are static for the batch: amplitude statistics (resampled sensors: on the
+    // resampled amplitude, as pupil_stats) and the lit-first subaperture order (Parseval sensors:
+    // their images do not depend on the grouping) once, here
     size_t order_off = 0;
     for (int s = 0; s < n_sensors; ++s) {
-        const shwfs_geom& gs = plans[s]->g;
+        const shwfs_plan* p = plans[s];
+        const shwfs_geom& gs = p->g;
+        const int64_t px = (int64_t)gs.ef_size * gs.ef_size;
+        const float* A_grid = A[s];                   // amplitude on the sensor grid
+        const float* B = nullptr;
+        int64_t n_in = px;
+        if (p->ef_in) {
+            void *a_rs = nullptr, *ph_rs = nullptr;
+            if (!alloc((size_t)px * sizeof(float), &a_rs) ||
+                !alloc((size_t)n_frames[s] * px * sizeof(float), &ph_rs))
+                return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
+            const int rc = launch_resample(p, A[s], 0, nullptr, static_cast<float*>(a_rs), 1, 0);
+            if (rc) return cleanup(rc);
+            A_grid = static_cast<const float*>(a_rs);
+            B = A_grid;
+            n_in = (int64_t)p->ef_in * p->ef_in;
+            args[s].A = A_grid;
+            args[s].phase = static_cast<const float*>(ph_rs);
+            b->resample.push_back({p, phase[s], n_frames[s], static_cast<float*>(ph_rs)});
+        }
+        args[s].vec = (gs.ef_size % 4 == 0) && aligned16(args[s].A) && aligned16(args[s].phase);
         int32_t* ord = b->d_order + order_off;
-        const int rco = launch_subap_order(A[s], gs.ef_size, gs.n_lenses, gs.np_sub, plans[s]->n_sub,
+        const int rco = launch_subap_order(A_grid, gs.ef_size, gs.n_lenses, gs.np_sub, p->n_sub,
                                            b->d_lit + order_off, ord, 0);
         if (rco) return cleanup(rco);
-        order_off += (size_t)plans[s]->n_sub;
+        order_off += (size_t)p->n_sub;
         args[s].order = args[s].parseval ? ord : nullptr;
-        const int64_t px = (int64_t)plans[s]->g.ef_size * plans[s]->g.ef_size;
-        hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, px,
+        hipLaunchKernelGGL(amp_stats_kernel, dim3(amp_blocks, 1), dim3(256), 0, 0, A[s], (int64_t)0, n_in,
                            b->d_amp_part + (size_t)s * amp_blocks * 2, b->d_counter + s, b->d_amp + 2 * s,
-                           S0[s] * plans[s]->g.pixel_area, plans[s]->g.norm_const, b->d_scale + s,
-                           (const float*)nullptr, (int64_t)0, (int64_t)0);
+                           S0[s] * gs.pixel_area, gs.norm_const, gs.parseval ? b->d_scale + s : nullptr, B,
+                           (int64_t)0, B ? px : (int64_t)0);
         args[s].amp = b->d_amp + 2 * s;
         args[s].fscale = b->d_scale + s;
         args[s].scale_fstep = 0;
+        if (!gs.parseval) {
+            void* bp = nullptr;
+            if (!alloc((size_t)n_frames[s] * args[s].n_parts * sizeof(double), &bp))
+                return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e)));
+            args[s].bpart = static_cast<double*>(bp);
+            b->rescale.push_back({ccd[s], args[s].ccd_stride, static_cast<double*>(bp), args[s].n_parts,
+                                  b->d_amp + 2 * s, S0[s], gs.pixel_area, n_frames[s]});
+        }
     }
     if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
         (e = hipMemcpy(b->d_args, args.data(), n_sensors * sizeof(NgsArgs), hipMemcpyHostToDevice)) != hipSuccess ||
@@ -3212,6 +3435,11 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(batch_stamp_kernel, dim3(1), dim3(64), 0, st, b->d_ts, b->n_sensors);
     SHWFS_LAUNCH_CHECK();
+    for (const auto& r : b->resample) {     // the resampled sensors' phases onto their sensor grids
+        const int rc = launch_resample(r.plan, r.phase, (int64_t)r.plan->ef_in * r.plan->ef_in, r.plan->d_ext, r.ph_rs,
+                                       r.n_frames, st);
+        if (rc) return rc;
+    }
     if (b->n_lgs > 0) {
         hipLaunchKernelGGL(sh_batch_kernel<1>, dim3(std::min(b->grid, b->n_lgs)), dim3(256), kBatchLgsLds, st,
                            b->d_args, b->d_items, b->n_lgs, b->d_ts);
@@ -3220,6 +3448,11 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
     if (b->n_ngs > 0) {
         hipLaunchKernelGGL(sh_batch_kernel<0>, dim3(std::min(b->grid, b->n_ngs)), dim3(256), kBatchNgsLds, st,
                            b->d_args, b->d_items + b->n_lgs, b->n_ngs, b->d_ts);
+        SHWFS_LAUNCH_CHECK();
+    }
+    for (const auto& c : b->rescale) {      // non-Parseval sensors: images normalised by their sums
+        hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, c.n_frames), dim3(256), 0, st, c.ccd, c.ccd_px, c.ccd_px,
+                           c.bpart, c.n_parts, c.amp, 0, c.S0, c.pixel_area);
         SHWFS_LAUNCH_CHECK();
     }
     return SHWFS_OK;

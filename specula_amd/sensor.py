@@ -72,9 +72,12 @@ class SHSensor:
         return self.slopes_buf[:F]
 
     # ------------------------------------------------------------------ adjoint
-    def vjp(self, phase, grad_slopes, stream=None):
-        """dL/dphase for cotangents grad_slopes (C, 2 n_sub) at ``phase`` (ef, ef) or (C, ef, ef)."""
+    def vjp(self, phase, grad_slopes, stream=None, sensor_grid=False):
+        """dL/dphase for cotangents grad_slopes (C, 2 n_sub) at ``phase`` (ef, ef) or (C, ef, ef).
+        With field resampling the gradient is on the input grid, or with ``sensor_grid`` with
+        respect to the resampled phase (ef_ovs, ef_ovs) that the subapertures sample."""
         ef, ccd = self.geom.ef_size, self.geom.ccd_side
+        efs = self.geom.ef_ovs if (sensor_grid and self.geom.needs_interpolation) else ef
         C = grad_slopes.shape[0]
         if C > self.max_frames:
             raise ValueError(f'{C} cotangents > max_frames {self.max_frames}')
@@ -92,25 +95,42 @@ class SHSensor:
         if self._gccd is None or self._gccd.shape[0] < C:
             self._gccd = torch.empty((self.max_frames, ccd, ccd), dtype=torch.float32, device=self.device)
         gs = grad_slopes.to(torch.float32).contiguous()
-        gph = torch.empty((C, ef, ef), dtype=torch.float32, device=self.device)
+        gph = torch.empty((C, efs, efs), dtype=torch.float32, device=self.device)
         lib = _native.lib()
         st = _native.stream_handle(stream)
         _native.check(lib.shslopec_vjp(self.slopec._h, C, _native.ptr(self.ccd), ccd * ccd, self.thr, 0,
                                        _native.ptr(gs), 2 * self.n_sub, _native.ptr(self._gccd), st),
                       'shslopec_vjp')
-        _native.check(lib.shwfs_vjp(self.plan._h, C, _native.ptr(self.A), 0, _native.ptr(ph),
-                                    0 if shared else ef * ef, 1.0, _native.ptr(self._gccd), ccd * ccd,
-                                    _native.ptr(gph), ef * ef, st), 'shwfs_vjp')
+        fn = lib.shwfs_vjp_sensor_grid if sensor_grid else lib.shwfs_vjp
+        _native.check(fn(self.plan._h, C, _native.ptr(self.A), 0, _native.ptr(ph), 0 if shared else ef * ef, 1.0,
+                         _native.ptr(self._gccd), ccd * ccd, _native.ptr(gph), efs * efs, st), 'shwfs_vjp')
         return gph
 
     def jacobian_maps(self, phase, stream=None):
-        """(Jx, Jy): ef x ef maps of d slope_x / d phase and d slope_y / d phase of each pixel's own
-        subaperture (block-diagonal Jacobian, two VJPs)."""
+        """(Jx, Jy): sensor-grid maps of d slope_x / d phase and d slope_y / d phase of each pixel's
+        own subaperture (block-diagonal Jacobian, two VJPs).  The sensor grid is the input grid, or
+        for a resampled sensor the resampled one (ef_ovs x ef_ovs)."""
         g = torch.zeros((2, 2 * self.n_sub), dtype=torch.float32, device=self.device)
         g[0, :self.n_sub] = 1.0
         g[1, self.n_sub:] = 1.0
-        gph = self.vjp(phase, g, stream=stream)
+        gph = self.vjp(phase, g, stream=stream, sensor_grid=True)
         return gph[0], gph[1]
+
+    def resample(self, frames, stream=None):
+        """Phase-like frames (K, ef, ef) on the input grid -> (K, ef_ovs, ef_ovs) through the
+        sensor's field resampling (edge extrapolation + bilinear, shwfs_resample); the frames
+        themselves when the sensor does not resample."""
+        if not self.geom.needs_interpolation:
+            return frames
+        ef, eo = self.geom.ef_size, self.geom.ef_ovs
+        frames = frames.reshape(-1, ef, ef).to(torch.float32).contiguous()
+        out = torch.empty((frames.shape[0], eo, eo), dtype=torch.float32, device=self.device)
+        for k0 in range(0, frames.shape[0], 65535):
+            k1 = min(frames.shape[0], k0 + 65535)
+            _native.check(_native.lib().shwfs_resample(self.plan._h, k1 - k0, _native.ptr(frames[k0:k1]), ef * ef,
+                                                       _native.ptr(out[k0:k1]), eo * eo,
+                                                       _native.stream_handle(stream)), 'shwfs_resample')
+        return out
 
     def slot_map(self):
         if self._slot is None:
@@ -123,10 +143,13 @@ class SHSensor:
         return self._slot
 
     def interaction_matrix(self, phase, modes, scale=2.0, jac=None, stream=None):
-        """IM (K, 2 n_sub) = scale * J . modes, push-pull convention for scale = 2."""
+        """IM (K, 2 n_sub) = scale * J . modes, push-pull convention for scale = 2.  ``modes``
+        (K, ef, ef) on the input grid; a resampled sensor contracts its sensor-grid Jacobian with
+        the resampled modes (the resampling is linear, so J R m = (J) (R m))."""
         jx, jy = jac if jac is not None else self.jacobian_maps(phase, stream=stream)
+        modes = self.resample(modes, stream=stream)
         K = modes.shape[0]
-        ef = self.geom.ef_size
+        ef = modes.shape[-1]
         im = torch.empty((K, 2 * self.n_sub), dtype=torch.float32, device=self.device)
         modes = modes.contiguous()
         # workspace of the grid-order pass: K * 2 * n^2 floats + n_sub int32 (kept between calls)

@@ -141,16 +141,26 @@ def test_sh_object_default_fov_ovs_matches_oracle():
         check_intensity(sh.outputs['out_i'].i.cpu().numpy(), ref, 20, 8)
 
 
-def test_resampled_plan_rejects_vjp():
+def test_resampled_plan_vjp_through_the_abi():
+    """shwfs_vjp on a resampled, non-Parseval plan (the adjoint itself is checked against finite
+    differences in test_gpu_adjoint_general.py): a zero cotangent gives an exactly zero gradient
+    over the whole input grid (the scatter target is cleared, including pixels no tap reaches),
+    and shwfs_resample refuses a plan without resampling."""
     from specula_amd import _native
     g, go, A, frames, out, plan = run(*CASES[0], n_frames=1)
     At = torch.tensor(A.astype(np.float32), device=DEV)
-    ph = torch.zeros((1, 64, 64), dtype=torch.float32, device=DEV)
+    ph = torch.tensor(frames[:1], device=DEV)
     gccd = torch.zeros((1, 64, 64), dtype=torch.float32, device=DEV)
-    gph = torch.empty_like(ph)
+    gph = torch.full_like(ph, float('nan'))
     rc = _native.lib().shwfs_vjp(plan._h, 1, _native.ptr(At), 0, _native.ptr(ph), 64 * 64, 1.0, _native.ptr(gccd),
                                  64 * 64, _native.ptr(gph), 64 * 64, None)
-    assert rc == _native.SHWFS_ERR_UNSUPPORTED
+    torch.cuda.synchronize()
+    assert rc == 0 and bool((gph == 0).all())
+    plain = SHPlan(sh_geometry(160, 0.05, **dict(CASES[2][3], rotAnglePhInDeg=0.0, xShiftPhInPixel=0.0,
+                                                   yShiftPhInPixel=0.0)), max_frames=1, device=0)
+    x = torch.zeros((1, 160, 160), dtype=torch.float32, device=DEV)
+    rc = _native.lib().shwfs_resample(plain._h, 1, _native.ptr(x), 160 * 160, _native.ptr(x), 160 * 160, None)
+    assert rc == _native.SHWFS_ERR_ARG and b'no field resampling' in _native.lib().shwfs_last_error()
 
 
 def _sh_object(**kw):
