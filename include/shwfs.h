@@ -64,6 +64,16 @@ const char* shwfs_last_error(void);
  * Replaces the geometry/state that SH.setup() builds (sh.py:439-467). */
 int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double* col_w,
                       int device, shwfs_plan** out);
+
+/* As shwfs_plan_create for a FoV mask that is not separable (make_mask with square=False
+ * clipping the cut window, sh.py:282): the mask is given as n_terms separable terms, row_w and
+ * col_w holding n_terms operators of the shape above one after the other, and
+ * B = sum_k row_k . S . col_k^T (NGS) or Re sum_k row_k . D . col_k^T (LGS).  Such a plan runs on
+ * the any-size path (variant 0 or 4) and normalises the image by its own sum (parseval = 0);
+ * n_terms = 1 is shwfs_plan_create.  Replaces the same set-up as shwfs_plan_create with
+ * squaremask=False (sh.py:50, 282). */
+int shwfs_plan_create_masked(const shwfs_geom* geom, int n_terms, const double* row_w, const double* col_w,
+                             int device, shwfs_plan** out);
 int shwfs_plan_destroy(shwfs_plan* plan);
 
 /* LGS only: per-subaperture kernel spectra sampled on the lag grid, device complex64

@@ -15,7 +15,7 @@ SHWFS_OK, SHWFS_ERR_ARG, SHWFS_ERR_HIP, SHWFS_ERR_UNSUPPORTED = 0, -1, -2, -3   
 
 EXPORTS = (
     'shwfs_version', 'shwfs_last_error',
-    'shwfs_plan_create', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events', 'shwfs_plan_set_frames_per_block',
+    'shwfs_plan_create', 'shwfs_plan_create_masked', 'shwfs_plan_destroy', 'shwfs_plan_set_timing_events', 'shwfs_plan_set_frames_per_block',
     'shwfs_set_kernels', 'shwfs_forward',
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
     'shwfs_plan_bind_slopec', 'shwfs_forward_slopes', 'shwfs_plan_set_pupil', 'shwfs_plan_pupil_registered',
@@ -71,6 +71,7 @@ def _declare(lib):
         'shwfs_version': (ctypes.c_char_p, []),
         'shwfs_last_error': (ctypes.c_char_p, []),
         'shwfs_plan_create': (_I, [ctypes.POINTER(ShwfsGeom), _P, _P, _I, ctypes.POINTER(_P)]),
+        'shwfs_plan_create_masked': (_I, [ctypes.POINTER(ShwfsGeom), _I, _P, _P, _I, ctypes.POINTER(_P)]),
         'shwfs_plan_destroy': (_I, [_P]),
         'shwfs_plan_set_timing_events': (_I, [_P, _P, _P]),
         'shwfs_plan_set_frames_per_block': (_I, [_P, _I]),

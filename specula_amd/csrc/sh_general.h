@@ -22,6 +22,10 @@
 //           (c_0 = 1, c_mx>0 = 2: D and the operators are transforms of real functions)
 //   4  store the npx x npx tile scaled by the frame's Parseval scale, or its unscaled sum for the
 //      non-Parseval rescale pass.
+// A FoV mask that is not separable (round FoV mask clipping the cut window, sh.py:282) comes as K
+// separable terms, B = sum_k P_k S Q_k^T (NGS) or Re sum_k P~_k D Q~_k^T (LGS): the NGS power
+// spectrum S is then kept whole in the slab like the LGS one, and the K binnings accumulate into
+// an npx x npx slab tile before the store.
 // FFTs: iterative radix-2 decimation in time after a bit-reversal permutation, twiddles
 // exp(-2 pi i k / M) from a table built on the host in FP64.  Accumulations of the binning products
 // in FP32 over M terms (M <= 1024); the image sum of the non-Parseval mode in FP64.
@@ -31,6 +35,8 @@ struct GenArgs {
     float* slab;         // [gridDim.x][slab_floats]
     int64_t slab_floats; // floats of scratch per block
     int M, log2M, np, ch;
+    int terms;           // separable terms of the FoV mask (1: separable mask)
+    int64_t term_stride; // floats per term in the row / column operators
 };
 
 constexpr int kGenThreads = 256;
@@ -73,18 +79,26 @@ __device__ void gen_fft(float2* buf, int vecs, int M, int log2M, const float2* _
 
 // Slab layout of one block (floats): NGS: G [np][M] complex, U [npx][M];
 // LGS: G [np][M] complex (reused for T [M][np] complex), S [M][M], D [L][np] complex.
+// With K > 1 mask terms: NGS G, S [M][M], U [npx][M], Bacc [npx][npx]; LGS as above + Bacc.
 __device__ __forceinline__ void gen_slab(const GenArgs& q, int npx, bool lgs, float*& base, float2*& G, float*& U,
-                                         float*& S, float2*& T, float2*& D) {
+                                         float*& S, float2*& T, float2*& D, float*& Bacc) {
     base = q.slab + (int64_t)blockIdx.x * q.slab_floats;
     G = reinterpret_cast<float2*>(base);
     T = G;
     const int64_t g_f = 2LL * q.np * q.M;
     const int64_t t_f = 2LL * q.M * q.np;
     const int64_t gt = g_f > t_f ? g_f : t_f;
-    U = base + gt;
+    const int64_t mm = (int64_t)q.M * q.M;
     S = base + gt;
-    D = reinterpret_cast<float2*>(base + gt + (int64_t)q.M * q.M);
-    (void)npx; (void)lgs;
+    if (lgs) {
+        U = base + gt;                                 // unused
+        D = reinterpret_cast<float2*>(base + gt + mm);
+        Bacc = base + gt + mm + 2LL * (2 * q.np - 1) * q.np;
+    } else {
+        U = q.terms > 1 ? base + gt + mm : base + gt;
+        D = nullptr;
+        Bacc = U + (int64_t)npx * q.M;
+    }
 }
 
 __global__ void __launch_bounds__(kGenThreads)
@@ -100,8 +114,9 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
     for (int k = threadIdx.x; k < (M >> 1); k += blockDim.x) tw[k] = q.tw[k];
     float* base;
     float2 *G, *T, *D;
-    float *U, *S;
-    gen_slab(q, npx, lgs, base, G, U, S, T, D);
+    float *U, *S, *Bacc;
+    gen_slab(q, npx, lgs, base, G, U, S, T, D, Bacc);
+    const int K = q.terms;
     const int64_t n_items = (int64_t)n_frames * g.n_sub;
     const double k_rev = g.k * (1.0 / kTwoPi), t_rev = g.tilt * (1.0 / kTwoPi);
     for (int64_t item = blockIdx.x; item < n_items; item += gridDim.x) {
@@ -173,7 +188,7 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
                 buf[e] = make_float2(fmaf(h.x, h.x, h.y * h.y), 0.f);
             }
             __syncthreads();
-            if (!lgs) {
+            if (!lgs && K == 1) {
                 // U[Y][kx] = sum_ky P[Y][ky] S[ky][kx] for the chunk's columns
                 for (int e = threadIdx.x; e < npx * cols; e += blockDim.x) {
                     const int Y = e / cols, c = e - (e / cols) * cols;
@@ -183,7 +198,7 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
                     for (int ky = 0; ky < M; ++ky) acc = fmaf(pr[ky], sc[ky].x, acc);
                     U[(int64_t)Y * M + c0 + c] = acc;
                 }
-            } else {
+            } else {                                   // LGS, or NGS with K mask terms: keep S
                 for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
                     const int c = e >> log2M, ky = e & (M - 1);
                     S[(int64_t)ky * M + c0 + c] = buf[e].x;
@@ -193,7 +208,36 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
         }
 
         double bsum = 0.0;
-        if (!lgs) {
+        if (!lgs && K > 1) {
+            // 3 (NGS, K mask terms): Bacc = sum_k (P_k S) Q_k^T
+            for (int k = 0; k < K; ++k) {
+                const float* Pk = g.roww + (int64_t)k * q.term_stride;
+                const float* Qk = g.colw + (int64_t)k * q.term_stride;
+                for (int e = threadIdx.x; e < npx * M; e += blockDim.x) {
+                    const int Y = e >> log2M, kx = e & (M - 1);
+                    const float* pr = Pk + (int64_t)Y * M;
+                    float acc = 0.f;
+                    for (int ky = 0; ky < M; ++ky) acc = fmaf(pr[ky], S[(int64_t)ky * M + kx], acc);
+                    U[e] = acc;
+                }
+                __syncthreads();
+                for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
+                    const int Y = e / npx, X = e - (e / npx) * npx;
+                    const float* ur = U + (int64_t)Y * M;
+                    const float* qr = Qk + (int64_t)X * M;
+                    float acc = k == 0 ? 0.f : Bacc[e];
+                    for (int kx = 0; kx < M; ++kx) acc = fmaf(ur[kx], qr[kx], acc);
+                    Bacc[e] = acc;
+                }
+                __syncthreads();
+            }
+            for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
+                const int Y = e / npx, X = e - (e / npx) * npx;
+                const float acc = Bacc[e];
+                bsum += (double)acc;
+                ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = g.parseval ? acc * fs : acc;
+            }
+        } else if (!lgs) {
             // 3 (NGS): B[Y][X] = sum_kx U[Y][kx] Q[X][kx]
             for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
                 const int Y = e / npx, X = e - (e / npx) * npx;
@@ -237,33 +281,41 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
                 }
                 __syncthreads();
             }
-            // 3c: V[Y][mx] = sum_my P~[Y][my] D[my][mx] (complex, in LDS if it fits, else in S)
+            // 3c: V[Y][mx] = sum_my P~_k[Y][my] D[my][mx] (complex, in LDS if it fits, else in S)
+            // 3d: B += Re sum_mx c_mx V[Y][mx] Q~_k[X][mx], over the K mask terms
             float2* V = (npx * np <= kGenLdsF2) ? buf : reinterpret_cast<float2*>(S);
-            const float2* Pt = reinterpret_cast<const float2*>(g.roww);     // [L][16 or npx pad]: see plan
-            for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {
-                const int Y = e / np, mx = e - (e / np) * np;
-                float ar = 0.f, ai = 0.f;
-                for (int ly = 0; ly < L; ++ly) {
-                    const float2 p = Pt[(int64_t)Y * L + ly];
-                    const float2 d = D[(int64_t)ly * np + mx];
-                    ar = fmaf(p.x, d.x, fmaf(-p.y, d.y, ar));
-                    ai = fmaf(p.x, d.y, fmaf(p.y, d.x, ai));
+            for (int k = 0; k < K; ++k) {
+                const float2* Pt = reinterpret_cast<const float2*>(g.roww + (int64_t)k * q.term_stride);
+                for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {
+                    const int Y = e / np, mx = e - (e / np) * np;
+                    float ar = 0.f, ai = 0.f;
+                    for (int ly = 0; ly < L; ++ly) {
+                        const float2 p = Pt[(int64_t)Y * L + ly];
+                        const float2 d = D[(int64_t)ly * np + mx];
+                        ar = fmaf(p.x, d.x, fmaf(-p.y, d.y, ar));
+                        ai = fmaf(p.x, d.y, fmaf(p.y, d.x, ai));
+                    }
+                    V[e] = make_float2(ar, ai);
                 }
-                V[e] = make_float2(ar, ai);
-            }
-            __syncthreads();
-            const float2* Qt = reinterpret_cast<const float2*>(g.colw);
-            for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
-                const int Y = e / npx, X = e - (e / npx) * npx;
-                float acc = 0.f;
-                for (int mx = 0; mx < np; ++mx) {
-                    const float2 v = V[Y * np + mx];
-                    const float2 qq = Qt[(int64_t)X * L + (mx + np - 1)];
-                    const float re = fmaf(v.x, qq.x, -v.y * qq.y);
-                    acc = fmaf(mx == 0 ? 1.f : 2.f, re, acc);
+                __syncthreads();
+                const float2* Qt = reinterpret_cast<const float2*>(g.colw + (int64_t)k * q.term_stride);
+                for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
+                    const int Y = e / npx, X = e - (e / npx) * npx;
+                    float acc = (K == 1 || k == 0) ? 0.f : Bacc[e];
+                    for (int mx = 0; mx < np; ++mx) {
+                        const float2 v = V[Y * np + mx];
+                        const float2 qq = Qt[(int64_t)X * L + (mx + np - 1)];
+                        const float re = fmaf(v.x, qq.x, -v.y * qq.y);
+                        acc = fmaf(mx == 0 ? 1.f : 2.f, re, acc);
+                    }
+                    if (k + 1 < K) {
+                        Bacc[e] = acc;
+                    } else {
+                        bsum += (double)acc;
+                        ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = g.parseval ? acc * fs : acc;
+                    }
                 }
-                bsum += (double)acc;
-                ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = g.parseval ? acc * fs : acc;
+                __syncthreads();
             }
         }
         if (!g.parseval) {

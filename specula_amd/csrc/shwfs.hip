@@ -2049,6 +2049,7 @@ struct shwfs_plan {
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
     bool general = false;          // any-size path (sh_general_kernel, variant 4)
+    int terms = 1;                 // separable terms of the FoV mask (> 1: general path only)
     float2* d_tw = nullptr;        // general path: [M/2] twiddles
     float* d_slab = nullptr;       // general path: [gen_blocks][slab_floats] scratch
     int gen_blocks = 0;
@@ -2212,6 +2213,8 @@ int launch_general(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st
     q.log2M = lg;
     q.np = g.np_sub;
     q.ch = kGenLdsF2 / q.M;
+    q.terms = p->terms;
+    q.term_stride = (int64_t)g.subap_npx * (g.lgs ? 2LL * p->L : (int64_t)g.spec_size);
     NgsArgs b = a;
     if (!g.lgs) b.kspec = nullptr;                 // the kernel takes the LGS branch on kspec != nullptr
     const int64_t items = (int64_t)n_frames * p->n_sub;
@@ -2264,8 +2267,18 @@ const char* shwfs_last_error(void) { return g_last_error.c_str(); }
 
 int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double* col_w,
                       int device, shwfs_plan** out) {
+    return shwfs_plan_create_masked(geom, 1, row_w, col_w, device, out);
+}
+
+int shwfs_plan_create_masked(const shwfs_geom* geom, int n_terms, const double* row_w, const double* col_w,
+                             int device, shwfs_plan** out) {
     if (!geom || !row_w || !col_w || !out) return set_error(SHWFS_ERR_ARG, "null argument");
     const shwfs_geom& g = *geom;
+    if (n_terms < 1 || n_terms > 4096) return set_error(SHWFS_ERR_ARG, "n_terms outside [1, 4096]");
+    if (n_terms > 1 && g.variant != 0 && g.variant != 4)
+        return set_error(SHWFS_ERR_ARG, "a non-separable FoV mask runs on the any-size path (variant 0 or 4)");
+    if (n_terms > 1 && g.parseval)
+        return set_error(SHWFS_ERR_ARG, "a non-separable FoV mask has no Parseval normalisation");
     if (g.max_frames > 65535) return set_error(SHWFS_ERR_ARG, "max_frames above 65535 (frames index the launch grid's y)");
     if (g.n_lenses <= 0 || g.np_sub <= 0 || g.subap_npx <= 0 || g.max_frames <= 0)
         return set_error(SHWFS_ERR_ARG, "non-positive geometry field");
@@ -2276,7 +2289,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     if (g.variant < 0 || g.variant > 4) return set_error(SHWFS_ERR_ARG, "variant must be 0, 1, 2, 3 or 4");
     NgsLauncher mf = nullptr, v1 = nullptr, lg = nullptr;
     int lgs_ts = 0;
-    bool use_mfma = false, use_general = g.variant == 4;
+    bool use_mfma = false, use_general = g.variant == 4 || n_terms > 1;
     const bool general_fits = (g.spec_size & (g.spec_size - 1)) == 0 && g.spec_size >= 16 && g.spec_size <= 1024;
     if (g.lgs) {
         lg = pick_lgs(g.np_sub, g.spec_size, &lgs_ts);
@@ -2306,6 +2319,7 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
     p->npx_pad = (g.subap_npx + 3) / 4 * 4;
     p->mfma = use_mfma;
     p->general = use_general;
+    p->terms = n_terms;
     p->bf3 = use_mfma && g.variant == 3;
     p->L = 2 * g.np_sub - 1;
     p->amp_blocks = 256;
@@ -2315,8 +2329,8 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         // operators in their natural row-major layout: NGS [npx][M] real, LGS [npx][L] complex
         const size_t per = g.lgs ? (size_t)p->L * 2 : (size_t)M;
         p->rows_alloc = g.subap_npx;
-        rw.assign((size_t)g.subap_npx * per, 0.f);
-        cw.assign((size_t)g.subap_npx * per, 0.f);
+        rw.assign((size_t)n_terms * g.subap_npx * per, 0.f);
+        cw.assign((size_t)n_terms * g.subap_npx * per, 0.f);
         for (size_t q = 0; q < rw.size(); ++q) { rw[q] = (float)row_w[q]; cw[q] = (float)col_w[q]; }
         p->ts = 1;
     } else if (g.lgs) {
@@ -2346,7 +2360,9 @@ int shwfs_plan_create(const shwfs_geom* geom, const double* row_w, const double*
         // per-block scratch (sh_general.h): G / T, then U (NGS) or S and D (LGS); the number of
         // persistent blocks keeps the total at or below 1 GiB
         const int64_t gt = 2LL * g.np_sub * M;
-        p->slab_floats = g.lgs ? gt + (int64_t)M * M + 2LL * p->L * g.np_sub : gt + (int64_t)g.subap_npx * M;
+        const int64_t bacc = n_terms > 1 ? (int64_t)g.subap_npx * g.subap_npx : 0;   // K-term accumulator
+        p->slab_floats = g.lgs ? gt + (int64_t)M * M + 2LL * p->L * g.np_sub + bacc
+                               : gt + (n_terms > 1 ? (int64_t)M * M : 0) + (int64_t)g.subap_npx * M + bacc;
         p->slab_floats = (p->slab_floats + 63) / 64 * 64;          // 256-byte aligned slabs
         const int64_t cap = ((int64_t)1 << 30) / (p->slab_floats * (int64_t)sizeof(float));
         p->gen_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(1024, cap));

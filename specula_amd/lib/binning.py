@@ -54,23 +54,25 @@ def _rebin_map_1d(n_in, n_out):
     return w
 
 
-def axis_weights(geom, lgs, axis):
+def axis_weights(geom, lgs, axis, m1=None):
     """w[Y, u] (subap_npx x N) for one axis of one subaperture.
 
-    axis 0: rows (y), axis 1: columns (x).  Raises if the FoV mask is not separable or the
-    global rebin is not periodic per subaperture."""
+    axis 0: rows (y), axis 1: columns (x).  ``m1``: the mask factor of this axis over the
+    shifted psf index (length N); None takes it from a separable FoV mask and raises if the mask
+    is not separable.  Raises if the global rebin is not periodic per subaperture."""
     N = geom.fft_size
     n = geom.n_lenses
     cs = geom.cutsize
     npx = geom.subap_npx
     if cs <= 0 or geom.cutpixels < 0:
         raise ValueError(f'unsupported FoV cut: cutpixels={geom.cutpixels}, cutsize={cs}')
-    mask = geom.fp_mask.astype(np.float64)
-    m_rows = mask.max(axis=1)
-    m_cols = mask.max(axis=0)
-    if not np.array_equal(mask, np.outer(m_rows, m_cols)):
-        raise NotImplementedError('FoV mask is not separable (squaremask=False with a clipping FoV)')
-    m1 = m_rows if axis == 0 else m_cols
+    if m1 is None:
+        mask = geom.fp_mask.astype(np.float64)
+        m_rows = mask.max(axis=1)
+        m_cols = mask.max(axis=0)
+        if not np.array_equal(mask, np.outer(m_rows, m_cols)):
+            raise NotImplementedError('FoV mask is not separable (squaremask=False with a clipping FoV)')
+        m1 = m_rows if axis == 0 else m_cols
     # global rebin of the reassembled image, then check per-subaperture periodicity
     wg = _rebin_map_1d(cs * n, npx * n)
     rho = wg[:npx, :cs]
@@ -85,6 +87,36 @@ def axis_weights(geom, lgs, axis):
         u = p if lgs else (p - N // 2) % N            # index in FFT order (fftshift for NGS)
         w[:, u] += rho[:, t] * m1[p]
     return w
+
+
+def mask_is_separable(geom):
+    mask = geom.fp_mask.astype(np.float64)
+    return bool(np.array_equal(mask, np.outer(mask.max(axis=1), mask.max(axis=0))))
+
+
+def mask_terms(geom):
+    """Exact separable decomposition of the FoV mask (sh.py:282, make_mask with square=False) on
+    the window the FoV cut keeps: W = sum_k a_k b_k^T, one term per distinct non-zero row pattern
+    of the window (a_k: indicator of the rows with that pattern, b_k: the pattern).  A disc of
+    diameter D gives about D / 2 terms.  Vectors over the shifted psf index (length N)."""
+    N = geom.fft_size
+    lo, cs = geom.cutpixels // 2, geom.cutsize
+    mask = geom.fp_mask.astype(np.float64)
+    win = mask[lo:lo + cs, lo:lo + cs]
+    groups = {}
+    for t in range(cs):
+        row = win[t]
+        if not row.any():
+            continue
+        groups.setdefault(row.tobytes(), (row, []))[1].append(lo + t)
+    terms = []
+    for row, rows in groups.values():
+        a = np.zeros(N)
+        a[rows] = 1.0
+        b = np.zeros(N)
+        b[lo:lo + cs] = row
+        terms.append((a, b))
+    return terms
 
 
 def ngs_operator(w, N, np_sub, M):
@@ -130,3 +162,27 @@ def build_operators(geom, lgs=False):
     if lgs:
         return lgs_operator(wr, N, np_sub), lgs_operator(wc, N, np_sub), M, norm
     return ngs_operator(wr, N, np_sub, M), ngs_operator(wc, N, np_sub, M), M, norm
+
+
+def build_operator_terms(geom, lgs=False):
+    """Operators of a possibly non-separable FoV mask: (row_ops, col_ops, M, norm_or_None) with
+    row_ops, col_ops stacked over the K separable terms of ``mask_terms`` (K = 1 and the
+    operators of build_operators when the mask is separable).  B = sum_k P_k S Q_k^T (NGS) or
+    Re sum_k Pt_k D Qt_k^T (LGS); with K > 1 the image is normalised by its own sum."""
+    if mask_is_separable(geom):
+        r, c, M, norm = build_operators(geom, lgs=lgs)
+        return r[None], c[None], M, norm
+    np_sub = geom.ovs_np_sub
+    M = spec_size(np_sub)
+    N = geom.fft_size
+    rows, cols = [], []
+    for a, b in mask_terms(geom):
+        wr = axis_weights(geom, lgs, 0, m1=a)
+        wc = axis_weights(geom, lgs, 1, m1=b)
+        if lgs:
+            rows.append(lgs_operator(wr, N, np_sub))
+            cols.append(lgs_operator(wc, N, np_sub))
+        else:
+            rows.append(ngs_operator(wr, N, np_sub, M))
+            cols.append(ngs_operator(wc, N, np_sub, M))
+    return np.stack(rows), np.stack(cols), M, None

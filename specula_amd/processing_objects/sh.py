@@ -17,7 +17,7 @@ from specula_amd.data_objects.electric_field import ElectricField
 from specula_amd.data_objects.intensity import Intensity
 from specula_amd.data_objects.lenslet import Lenslet
 from specula_amd.data_objects.laser_launch_telescope import LaserLaunchTelescope
-from specula_amd.lib.binning import build_operators
+from specula_amd.lib.binning import build_operator_terms
 from specula_amd.lib.resample import edge_extrapolation_maps, resampling_map
 from specula_amd.lib.geometry import sh_geometry
 
@@ -33,12 +33,14 @@ class SHPlan:
         which variant 0 also takes when no specialised kernel fits the geometry.
         lgs: per-subaperture kernel correlation; spectra come from ``set_kernels``.  With
         ``kernels_normalised`` (unit-sum kernels) the Parseval normalisation applies."""
-        row, col, M, norm = build_operators(geom, lgs=lgs)
+        rows, cols, M, norm = build_operator_terms(geom, lgs=lgs)
+        self.mask_terms = rows.shape[0]                 # > 1: round FoV mask clipping the cut window
         if lgs:
             N = geom.fft_size
-            row = row * (N * N / float(M * M))          # D = (N^2/M^2) FFT_M(S) K
+            rows = rows * (N * N / float(M * M))        # D = (N^2/M^2) FFT_M(S) K
             if not kernels_normalised:
                 norm = None
+        row, col = rows[0], cols[0]
         self.geom = geom
         self.lgs = lgs
         self.M = M
@@ -69,16 +71,17 @@ class SHPlan:
         self.fpb = 4                                    # frames per block (library default)
         self._slopec = None
         if lgs:     # complex operators as interleaved (re, im)
-            rw = np.ascontiguousarray(np.stack([row.real, row.imag], axis=-1), dtype=np.float64)
-            cw = np.ascontiguousarray(np.stack([col.real, col.imag], axis=-1), dtype=np.float64)
+            rw = np.ascontiguousarray(np.stack([rows.real, rows.imag], axis=-1), dtype=np.float64)
+            cw = np.ascontiguousarray(np.stack([cols.real, cols.imag], axis=-1), dtype=np.float64)
         else:
-            rw = np.ascontiguousarray(row, dtype=np.float64)
-            cw = np.ascontiguousarray(col, dtype=np.float64)
+            rw = np.ascontiguousarray(rows, dtype=np.float64)
+            cw = np.ascontiguousarray(cols, dtype=np.float64)
         h = ctypes.c_void_p()
         lib = _native.lib()
-        _native.check(lib.shwfs_plan_create(ctypes.byref(g), rw.ctypes.data_as(ctypes.c_void_p),
-                                            cw.ctypes.data_as(ctypes.c_void_p), device, ctypes.byref(h)),
-                      'shwfs_plan_create')
+        _native.check(lib.shwfs_plan_create_masked(ctypes.byref(g), self.mask_terms,
+                                                   rw.ctypes.data_as(ctypes.c_void_p),
+                                                   cw.ctypes.data_as(ctypes.c_void_p), device, ctypes.byref(h)),
+                      'shwfs_plan_create_masked')
         self._h = h
 
     def set_resampling(self, pupil):

@@ -213,3 +213,52 @@ def test_every_reference_sh_block_gets_a_plan(block):
     tot = float(out.double().sum())
     assert np.isfinite(tot)
     np.testing.assert_allclose(tot, S0 * A.sum() * pitch ** 2, rtol=1e-5)
+
+
+ROUND_MASK = [
+    # (name, ef, pitch, obstruction, LGS, SH keywords with squaremask=False: a round FoV mask that
+    # clips the cut window, sh.py:282, so the mask is not separable)
+    ('20x20_round', 160, 0.05, 0.14, False,
+     dict(wavelengthInNm=750, subap_wanted_fov=2.4, sensor_pxscale=0.3, subap_on_diameter=20, subap_npx=8,
+          fov_ovs_coeff=1.0, squaremask=False)),
+    ('scao_test_round_fov_ovs2', 64, 0.015625, 0.0, False,
+     dict(wavelengthInNm=600, subap_wanted_fov=4.0, sensor_pxscale=0.5, subap_on_diameter=8, subap_npx=8,
+          squaremask=False)),
+    ('eris_sh_lgs_round', 16, 0.05, 0.0, True,
+     dict(subap_wanted_fov=5, sensor_pxscale=0.85, subap_npx=6, wavelengthInNm=589, subap_on_diameter=4,
+          squaremask=False)),
+]
+
+
+@pytest.mark.parametrize('case', ROUND_MASK, ids=[c[0] for c in ROUND_MASK])
+def test_round_fov_mask_matches_oracle(case):
+    """squaremask=False: the plan takes the mask as K separable terms on the any-size path and
+    normalises by the image sum; GPU images and slopes against the FP64 oracle."""
+    from specula_amd.lib.binning import mask_is_separable
+    name, ef, pitch, obs, lgs, kw = case
+    g = sh_geometry(ef, pitch, **kw)
+    go = O.sh_geometry(ef_size=ef, pixel_pitch=pitch, **kw)
+    assert not mask_is_separable(g)
+    A = annular_pupil(ef, obs) if obs else O.make_mask(ef, 0.0, 1.0).astype(np.float64)
+    scr = von_karman_opd(256, pitch, seed=17)
+    frames = np.stack([scr[:ef, k:k + ef] * 0.3 for k in range(2)]).astype(np.float32)
+    kern = lgs_kernels(g) if lgs else None
+    out, plan = run_plan(g, A, frames, lgs, kern)
+    print(f'{name}: np {g.ovs_np_sub} N {g.fft_size} cut {g.cutpixels} npx {g.subap_npx} fov_ovs {g.fov_ovs}: '
+          f'{plan.mask_terms} mask terms, parseval {plan.parseval}', flush=True)
+    assert plan.mask_terms > 1 and not plan.parseval
+    ref0 = None
+    for k in range(2):
+        ref = O.sh_forward(A, frames[k].astype(np.float64), go, kernels=None if kern is None else kern.kernels)
+        st = check_intensity(out[k], ref, g.n_lenses, g.subap_npx)
+        print(f'  frame {k}: flux {st["total"]:.1e}, normalised image {st["img"]:.1e}', flush=True)
+        ref0 = ref if ref0 is None else ref0
+    idxs, _ = detect_subaps(O.sh_forward(A, np.zeros_like(A), go, kernels=None if kern is None else kern.kernels),
+                            g.n_lenses, 0.5)
+    sp = SlopecPlan(idxs, (g.ccd_side, g.ccd_side), compute_xy_weights(g.subap_npx), max_frames=1, device=0)
+    s = torch.empty(2 * idxs.shape[0], dtype=torch.float32, device=DEV)
+    fl = torch.empty(idxs.shape[0], dtype=torch.float64, device=DEV)
+    tt = torch.empty(2, dtype=torch.float64, device=DEV)
+    sp.forward(torch.tensor(out[0], device=DEV), s, fl, tt)
+    err = check_slopes(s.cpu().numpy(), O.shslopec(ref0, idxs)['slopes'], g.subap_npx)
+    print(f'  slopes: {err / (2 / (g.subap_npx - 1)):.1e} CCD px', flush=True)
