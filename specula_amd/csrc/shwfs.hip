@@ -17,6 +17,7 @@
 //
 // Build: hipcc --offload-arch=gfx950 -O3 -shared -fPIC (see specula_amd/build.py).
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -2124,6 +2125,12 @@ struct DeviceGuard {
     }
 };
 
+// Optional timing hook of a plan (shwfs_plan_set_timing_events): the events are attached to the
+// main kernel's dispatch (hipExtLaunchKernelGGL records them at the kernel's own start and end, so
+// the interval is the kernel's duration, not that of separate stream markers around it).  Set by
+// sh_forward_impl around the main launch only; null for every other launch.
+thread_local hipEvent_t t_ev_start = nullptr, t_ev_stop = nullptr;
+
 template <int NP, int M>
 int launch_ngs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     constexpr int TS = 256 / NP;
@@ -2132,7 +2139,7 @@ int launch_ngs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     const int lds = (g1_bytes > u_bytes ? g1_bytes : u_bytes);
     if (lds > 150 * 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "LDS budget exceeded for this geometry");
     dim3 grid(p->n_tiles, n_frames);
-    hipLaunchKernelGGL((sh_ngs_forward_kernel<NP, M, TS>), grid, dim3(TS * NP), lds, st, a);
+    hipExtLaunchKernelGGL((sh_ngs_forward_kernel<NP, M, TS>), grid, dim3(TS * NP), lds, st, t_ev_start, t_ev_stop, 0, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2142,7 +2149,8 @@ int ngs_tile_size(int np) { return 256 / np; }
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 int launch_mfma(const NgsArgs& a, int n_tiles, int n_frames, int fpb, hipStream_t st) {
     dim3 grid(n_tiles * NgsMfmaShape<NP>::WPB, (n_frames + fpb - 1) / fpb);
-    hipLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG, BF3>), grid, dim3(64), 0, st, a, n_frames, fpb);
+    hipExtLaunchKernelGGL((sh_ngs_mfma_kernel<NP, PARSEVAL, NPX, COG, BF3>), grid, dim3(64), 0, st, t_ev_start, t_ev_stop,
+                          0, a, n_frames, fpb);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2196,7 +2204,7 @@ int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     constexpr int lds = LgsShape<NP, M, TS>::LDS_F2 * (int)sizeof(float2);
     static_assert(lds <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
     dim3 grid(p->n_tiles, n_frames);
-    hipLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3(256), lds, st, a);
+    hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3(256), lds, st, t_ev_start, t_ev_stop, 0, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2219,7 +2227,8 @@ int launch_general(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st
     if (!g.lgs) b.kspec = nullptr;                 // the kernel takes the LGS branch on kspec != nullptr
     const int64_t items = (int64_t)n_frames * p->n_sub;
     const int blocks = (int)std::min<int64_t>(items, p->gen_blocks);
-    hipLaunchKernelGGL(sh_general_kernel, dim3(blocks), dim3(kGenThreads), 0, st, b, q, n_frames);
+    hipExtLaunchKernelGGL(sh_general_kernel, dim3(blocks), dim3(kGenThreads), 0, st, t_ev_start, t_ev_stop, 0, b, q,
+                          n_frames);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2602,10 +2611,11 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     NgsLauncher L = p->general ? launch_general
                   : g.lgs ? pick_lgs(g.np_sub, g.spec_size, &lts)
                           : (p->mfma ? pick_ngs_mfma(g.np_sub, g.spec_size, g.subap_npx) : pick_ngs(g.np_sub, g.spec_size));
-    if (p->ev_start) SHWFS_HIP_CHECK(hipEventRecord(p->ev_start, st));
+    t_ev_start = p->ev_start;
+    t_ev_stop = p->ev_stop;
     int rc = L(p, a, n_frames, st);
+    t_ev_start = t_ev_stop = nullptr;
     if (rc) return rc;
-    if (p->ev_stop) SHWFS_HIP_CHECK(hipEventRecord(p->ev_stop, st));
     if (!g.parseval) {
         hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, n_frames), dim3(256), 0, st, ccd, ccd_stride,
                            ccd_side * ccd_side, p->d_bpart, p->n_parts, amp,
