@@ -322,6 +322,19 @@ int shccd_readout_cte(const shccd_params* params, int n_frames, float* integrate
                       const float* cte_cols, float* work, const float* pixel_gains, uint64_t counter,
                       float* pixels, int64_t out_stride, void* stream);
 
+/* ---- push-pull calibration ------------------------------------------------------------------
+ * Fields of a modal push-pull around phi0 (n_px floats): out[k] = phi0 + amp * modes[k],
+ * out[n_modes + k] = phi0 - amp * modes[k] (2 n_modes frames, out_stride apart).  Replaces the
+ * command history of modal_pushpull_signal.py:35-65 projected through the DM (one push and one
+ * pull per mode). */
+int shwfs_pushpull_fields(int n_modes, const float* phi0, int64_t n_px, const float* modes, int64_t mode_stride,
+                          float amp, float* out, int64_t out_stride, void* stream);
+/* Interaction-matrix rows from the slopes of those fields, as ImRecCalibrator accumulates them
+ * (im_rec_calibrator.py:58-63, im[mode] += slopes / command):
+ * im[k] = slopes[k] / amp + slopes[n_modes + k] / (-amp). */
+int shwfs_pushpull_rows(int n_modes, const float* slopes, int64_t slope_stride, int n_slopes, float amp, float* im,
+                        int64_t im_stride, void* stream);
+
 /* ---- FP32 matrix product on the matrix cores ----------------------------------------------
  *   C[i][j] = alpha * sum_k A[i][k] op(B)[k][j] + beta * C[i][j] + bias[j]     (i < m, j < n)
  * A row-major m x k (lda); op(B) = B row-major k x n (ldb) when trans_b == 0, B^T with B row-major

@@ -20,7 +20,7 @@ EXPORTS = (
     'shslopec_plan_create', 'shslopec_plan_destroy', 'shslopec_forward',
     'shwfs_plan_bind_slopec', 'shwfs_forward_slopes', 'shwfs_plan_set_pupil', 'shwfs_plan_pupil_registered',
     'shwfs_plan_set_resampling',
-    'shslopec_vjp', 'shwfs_vjp', 'shwfs_vjp_sensor_grid', 'shwfs_resample', 'shwfs_im_from_jacobian', 'shwfs_im_from_jacobian_noalloc',
+    'shslopec_vjp', 'shwfs_vjp', 'shwfs_vjp_sensor_grid', 'shwfs_resample', 'shwfs_pushpull_fields', 'shwfs_pushpull_rows', 'shwfs_im_from_jacobian', 'shwfs_im_from_jacobian_noalloc',
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
     'shccd_integrate', 'shccd_readout', 'shccd_readout_cte',
@@ -89,6 +89,8 @@ def _declare(lib):
         'shwfs_vjp': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P, _I64, _P]),
         'shwfs_vjp_sensor_grid': (_I, [_P, _I, _P, _I64, _P, _I64, _D, _P, _I64, _P, _I64, _P]),
         'shwfs_resample': (_I, [_P, _I, _P, _I64, _P, _I64, _P]),
+        'shwfs_pushpull_fields': (_I, [_I, _P, _I64, _P, _I64, _F, _P, _I64, _P]),
+        'shwfs_pushpull_rows': (_I, [_I, _P, _I64, _I, _F, _P, _I64, _P]),
         'shwfs_im_from_jacobian': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P, _P]),
         'shwfs_im_from_jacobian_noalloc': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P]),
         'shwfs_batch_create': (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_P)]),

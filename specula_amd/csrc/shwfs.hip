@@ -2037,6 +2037,36 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
 
 }  // namespace
 
+// ---------------------------------------------------------------- push-pull calibration fields
+// Fields of a modal push-pull (im_rec_calibrator.py:58-63 driven by modal_pushpull_signal.py):
+// out[k] = phi0 + a m_k (push), out[K + k] = phi0 - a m_k (pull); grid (blocks, K).
+__global__ void pushpull_fields_kernel(const float* __restrict__ phi0, const float* __restrict__ modes,
+                                       int64_t mode_stride, int64_t n_px, float amp, float* __restrict__ out,
+                                       int64_t out_stride, int K) {
+    const int k = blockIdx.y;
+    const float* m = modes + k * mode_stride;
+    float* op = out + k * out_stride;
+    float* om = out + (k + (int64_t)K) * out_stride;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_px; q += (int64_t)gridDim.x * blockDim.x) {
+        const float p = phi0[q], d = amp * m[q];
+        op[q] = p + d;
+        om[q] = p - d;
+    }
+}
+
+// Interaction-matrix rows from the push-pull slopes as ImRecCalibrator accumulates them
+// (im_rec_calibrator.py:58-63: im[mode] += slopes / command, for +a then -a):
+// im[k] = s[k] / a + s[K + k] / (-a).
+__global__ void pushpull_rows_kernel(const float* __restrict__ slopes, int64_t s_stride, int n, float amp,
+                                     float* __restrict__ im, int64_t im_stride, int K) {
+    const int k = blockIdx.y;
+    const float* sp = slopes + k * s_stride;
+    const float* sm = slopes + (k + (int64_t)K) * s_stride;
+    float* r = im + k * im_stride;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+        r[q] = sp[q] / amp + sm[q] / (-amp);
+}
+
 // ============================================================================ plans
 struct shwfs_plan {
     shwfs_geom g{};
@@ -3558,6 +3588,38 @@ int shwfs_graph_destroy(shwfs_graph* g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
+    return SHWFS_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int shwfs_pushpull_fields(int n_modes, const float* phi0, int64_t n_px, const float* modes, int64_t mode_stride,
+                          float amp, float* out, int64_t out_stride, void* stream) {
+    if (!phi0 || !modes || !out) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_modes <= 0 || n_modes > 65535 || n_px <= 0) return set_error(SHWFS_ERR_ARG, "n_modes outside [1, 65535] or empty field");
+    if ((n_modes > 1 && mode_stride < n_px) || out_stride < n_px)
+        return set_error(SHWFS_ERR_ARG, "frame stride too small");
+    const int64_t blocks = std::min<int64_t>((n_px + 255) / 256, 1024);
+    hipLaunchKernelGGL(pushpull_fields_kernel, dim3((unsigned)blocks, n_modes), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), phi0, modes, mode_stride, n_px, amp, out, out_stride,
+                       n_modes);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+int shwfs_pushpull_rows(int n_modes, const float* slopes, int64_t slope_stride, int n_slopes, float amp, float* im,
+                        int64_t im_stride, void* stream) {
+    if (!slopes || !im) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (n_modes <= 0 || n_modes > 65535 || n_slopes <= 0) return set_error(SHWFS_ERR_ARG, "n_modes outside [1, 65535] or no slopes");
+    if (slope_stride < n_slopes || (n_modes > 1 && im_stride < n_slopes))
+        return set_error(SHWFS_ERR_ARG, "row stride too small");
+    if (!(amp != 0.f)) return set_error(SHWFS_ERR_ARG, "push-pull amplitude must be non-zero");
+    const int blocks = std::min((n_slopes + 255) / 256, 64);
+    hipLaunchKernelGGL(pushpull_rows_kernel, dim3(blocks, n_modes), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       slopes, slope_stride, n_slopes, amp, im, im_stride, n_modes);
+    SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
 

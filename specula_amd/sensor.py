@@ -163,22 +163,25 @@ class SHSensor:
         return im
 
     def pushpull_interaction_matrix(self, phase, modes, amp, chunk=None, stream=None):
-        """Reference convention (im_rec_calibrator.py:58-63): IM[m] = (s(phi + a m) - s(phi - a m)) / a,
-        evaluated in batches of 2*chunk sensor frames."""
+        """Reference convention (im_rec_calibrator.py:58-63): IM[m] = s(phi + a m) / a + s(phi - a m) / (-a),
+        evaluated in batches of 2*chunk sensor frames (fields and rows by shwfs_pushpull_*)."""
         K = modes.shape[0]
         chunk = chunk or max(1, self.max_frames // 2)
         ef = self.geom.ef_size
-        im = torch.empty((K, 2 * self.n_sub), dtype=torch.float32, device=self.device)
+        n = 2 * self.n_sub
+        im = torch.empty((K, n), dtype=torch.float32, device=self.device)
         buf = torch.empty((2 * chunk, ef, ef), dtype=torch.float32, device=self.device)
-        ph = phase.reshape(1, ef, ef)
+        ph = phase.reshape(ef, ef).to(torch.float32).contiguous()
+        modes = modes.reshape(K, ef, ef).to(torch.float32).contiguous()
+        lib, st = _native.lib(), _native.stream_handle(stream)
         for m0 in range(0, K, chunk):
             m1 = min(K, m0 + chunk)
             c = m1 - m0
-            torch.add(ph, modes[m0:m1], alpha=amp, out=buf[:c])
-            torch.add(ph, modes[m0:m1], alpha=-amp, out=buf[c:2 * c])
+            _native.check(lib.shwfs_pushpull_fields(c, _native.ptr(ph), ef * ef, _native.ptr(modes[m0:m1]), ef * ef,
+                                                    float(amp), _native.ptr(buf), ef * ef, st), 'shwfs_pushpull_fields')
             s = self.forward(buf[:2 * c], stream=stream)
-            torch.sub(s[:c], s[c:2 * c], out=im[m0:m1])
-            im[m0:m1] /= amp
+            _native.check(lib.shwfs_pushpull_rows(c, _native.ptr(s), n, n, float(amp), _native.ptr(im[m0:m1]), n, st),
+                          'shwfs_pushpull_rows')
         return im
 
 

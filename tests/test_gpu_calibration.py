@@ -59,6 +59,12 @@ def test_batched_pushpull_reproduces_scao_rec():
     print(f'  rec max |gpu - fixture| {err:.2e} (decimal 3: 1.5e-3); IM vs oracle {im_err:.2e} of max', flush=True)
     np.testing.assert_array_almost_equal(rec, ref, decimal=3)
     assert im_err < 1e-4
+    # the same IM from the library's push-pull fields and rows (shwfs_pushpull_fields / _rows)
+    modes = dm.project(torch.eye(40, device=DEV))
+    im2 = sen.pushpull_interaction_matrix(torch.zeros((64, 64), device=DEV), modes, 50.0)
+    d = float((im2.double() - im.double()).norm() / im.double().norm())
+    print(f'  shwfs_pushpull IM vs calibrator accumulation: rel {d:.2e}', flush=True)
+    assert d < 1e-5
 
 
 def test_drop_in_objects_calibration_loop(tmp_path):
