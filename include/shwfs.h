@@ -255,8 +255,12 @@ int shwfs_im_from_jacobian_noalloc(int n_lenses, int np_sub, int ef_size, const 
  * No reference counterpart (the reference triggers one SH object at a time, each optionally a
  * captured CUDA graph, base_processing_obj.py:114-155).  One persistent launch per body type
  * (LGS np 16 / N 128 tiles, then NGS MFMA np 8 tiles of up to 4 streamed frames) runs every
- * (sensor, frame, subaperture tile) of several Parseval-normalised sensors of uneven shapes.  All buffers are fixed at creation (graph
- * friendly): per sensor A (ef^2, static), phase (n_frames x ef^2), ccd (n_frames x ccd^2), S0.
+ * (sensor, frame, subaperture tile) of several sensors of uneven shapes; sensors with field
+ * resampling have their phases resampled before it and sensors normalised by their own image sum
+ * are rescaled after it.  Any other plan (other np, the any-size path, round FoV masks) runs its
+ * own launch in the same sequence (n_frames <= its max_frames).  All buffers are fixed at creation
+ * (graph friendly): per sensor A (input grid, static), phase (n_frames x input grid), ccd
+ * (n_frames x ccd^2), S0.
  * grid_blocks: persistent blocks (0: one per work item); order: 0 heaviest tiles first,
  * 1 sensor by sensor, 2 round robin over sensors. */
 typedef struct shwfs_batch shwfs_batch;

@@ -2003,6 +2003,11 @@ __global__ void batch_stamp_kernel(unsigned long long* __restrict__ ts, int n_se
     for (int s = threadIdx.x; s < n_sensors; s += blockDim.x) ts[1 + s] = 0ull;
 }
 
+// Completion stamp of a sensor that the batch runs with its plan's own launch.
+__global__ void batch_sensor_stamp_kernel(unsigned long long* __restrict__ ts, int sensor) {
+    if (threadIdx.x == 0) ts[1 + sensor] = (unsigned long long)wall_clock64();
+}
+
 // One persistent launch per body type (KIND 0: NGS MFMA np 8 tiles, KIND 1: LGS np 16 tiles), so
 // that each runs with its own register and LDS footprint; blocks stride over the item list.
 template <int KIND>
@@ -3316,8 +3321,18 @@ struct shwfs_batch {
         double S0, pixel_area;
         int n_frames;
     };
+    struct Direct {                   // sensor outside the persistent bodies: its plan's own launch
+        shwfs_plan* plan;
+        int sensor;
+        const float* A;
+        const float* phase;
+        float* ccd;
+        double S0;
+        int n_frames;
+    };
     std::vector<Resample> resample;
     std::vector<Rescale> rescale;
+    std::vector<Direct> direct;
     std::vector<void*> bufs;          // device buffers of the above (resampled fields, partial sums)
 };
 
@@ -3357,9 +3372,18 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         int kind;
         if (!g.lgs && p->mfma && g.np_sub == 8 && g.spec_size == 16) kind = 0;
         else if (g.lgs && !p->general && g.np_sub == 16 && g.spec_size == 32) kind = 1;
-        else return set_error(SHWFS_ERR_UNSUPPORTED, "batch supports NGS MFMA np 8 and LGS np 16 plans");
+        else kind = 2;       // any other plan: its own launch in the batch's sequence
         if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan without kernels");
         if (n_frames[s] <= 0 || n_frames[s] > 65535) return set_error(SHWFS_ERR_ARG, "frame count outside [1, 65535]");
+        if (kind == 2) {
+            if (n_frames[s] > g.max_frames)
+                return set_error(SHWFS_ERR_ARG, "frame count above the plan's max_frames (sensor run with its own launch)");
+            kinds[s] = 2;
+            cost[s] = 0;
+            tiles_of[s] = 0;
+            args[s] = NgsArgs{};
+            continue;
+        }
         const int ts = (kind == 0) ? 4 * NgsMfmaShape<8>::SW : 8;
         const int tiles = (p->n_sub + ts - 1) / ts;
         tiles_of[s] = tiles;
@@ -3435,6 +3459,11 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
     size_t order_off = 0;
     for (int s = 0; s < n_sensors; ++s) {
         const shwfs_plan* p = plans[s];
+        if (kinds[s] == 2) {
+            b->direct.push_back({plans[s], s, A[s], phase[s], ccd[s], S0[s], n_frames[s]});
+            order_off += (size_t)p->n_sub;
+            continue;
+        }
         const shwfs_geom& gs = p->g;
         const int64_t px = (int64_t)gs.ef_size * gs.ef_size;
         const float* A_grid = A[s];                   // amplitude on the sensor grid
@@ -3509,6 +3538,15 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
     for (const auto& c : b->rescale) {      // non-Parseval sensors: images normalised by their sums
         hipLaunchKernelGGL(sh_rescale_kernel, dim3(64, c.n_frames), dim3(256), 0, st, c.ccd, c.ccd_px, c.ccd_px,
                            c.bpart, c.n_parts, c.amp, 0, c.S0, c.pixel_area);
+        SHWFS_LAUNCH_CHECK();
+    }
+    for (const auto& d : b->direct) {       // sensors outside the persistent bodies
+        const shwfs_geom& g = d.plan->g;
+        const int64_t in_px = d.plan->ef_in ? (int64_t)d.plan->ef_in * d.plan->ef_in : (int64_t)g.ef_size * g.ef_size;
+        const int64_t ccd_px = (int64_t)g.n_lenses * g.subap_npx * g.n_lenses * g.subap_npx;
+        const int rc = sh_forward_impl(d.plan, d.n_frames, d.A, 0, d.phase, in_px, d.S0, d.ccd, ccd_px, nullptr, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(batch_sensor_stamp_kernel, dim3(1), dim3(64), 0, st, b->d_ts, d.sensor);
         SHWFS_LAUNCH_CHECK();
     }
     return SHWFS_OK;

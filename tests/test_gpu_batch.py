@@ -130,7 +130,8 @@ def test_full_multi_sensor_work_list_matches_per_sensor_and_oracle():
 def test_batch_with_resampled_and_cut_sensors_matches_per_sensor_and_oracle():
     """Sensors the batch formerly refused, next to a Parseval one: a rotated / shifted field
     (field resampling, FoV cut 16: image normalised by its own sum), the same sensor without
-    resampling (cut only), and scao_ngs_20x20 (Parseval), 3 frames each.  Against per-sensor
+    resampling (cut only), a round FoV mask (any-size path) and a fov_ovs 2 / np 16 sensor (both
+    run with their plans' own launches), and scao_ngs_20x20 (Parseval), 3 frames each.  Against per-sensor
     launches of the same plans (the Parseval sensor bit-equal; the self-normalised ones to FP32
     rounding of the differently grouped image sums) and one frame of each against the oracle."""
     from helpers import check_intensity
@@ -142,7 +143,10 @@ def test_batch_with_resampled_and_cut_sensors_matches_per_sensor_and_oracle():
     K20 = dict(wavelengthInNm=750, subap_wanted_fov=2.4, sensor_pxscale=0.3, subap_on_diameter=20, subap_npx=8,
                fov_ovs_coeff=1.0)
     specs = [('rot_shift_cut16', dict(K20, rotAnglePhInDeg=10.0, xShiftPhInPixel=1.0, yShiftPhInPixel=-2.0)),
-             ('cut16', K20)]
+             ('cut16', K20),
+             # outside the persistent bodies, run with their plans' own launches in the sequence:
+             ('round_mask', dict(K20, squaremask=False)),                 # any-size path, 17 mask terms
+             ('fov_ovs2_np16', dict(K20, fov_ovs_coeff=0.0))]            # np 16 register-GEMV kernel
     A = annular_pupil(160, 0.14)
     dev = torch.device('cuda:0')
     ents, oracles = [], []
@@ -173,7 +177,10 @@ def test_batch_with_resampled_and_cut_sensors_matches_per_sensor_and_oracle():
             err = ((e.ccd - r).abs().max() / r.abs().max()).item()
             print(f'{e.name}: batch vs per-sensor max rel diff {err:.2e}', flush=True)
             assert err < 1e-6
-    for e, go in zip(ents[:2], oracles):
+    lat = b.latencies_us()
+    print('completion us:', dict(zip([e.name for e in ents], np.round(lat, 1))), flush=True)
+    assert np.all(np.isfinite(lat)) and np.all(lat > 0)
+    for e, go in zip(ents[:4], oracles):
         ref = O.sh_forward(A, e.phase[0].double().cpu().numpy(), go)
         st = check_intensity(e.ccd[0].cpu().numpy(), ref, e.plan.geom.n_lenses, e.plan.geom.subap_npx)
         print(f'{e.name}: oracle total {st["total"]:.2e}, image {st["img"]:.2e}', flush=True)
