@@ -61,11 +61,29 @@ def test_autotuner(entries, tmp_path, monkeypatch):
     monkeypatch.setenv('SPECULA_AMD_TUNE_CACHE', str(tmp_path / 'tune.json'))
     b, rep = autotune.tune_batch(entries, grids=[0, 256], orders=(0, 2), reps=2)
     assert rep['grid'] in (0, 256) and rep['order'] in (0, 2) and len(rep['candidates']) == 4
+    # the pick is the fastest measured candidate
+    assert rep['seconds'] == min(c['seconds'] for c in rep['candidates'])
     b2, rep2 = autotune.tune_batch(entries, grids=[0, 256], orders=(0, 2), reps=2)
     assert rep2['grid'] == rep['grid']                             # served from the cache
     wl = workloads.elt_ngs_80x80()
-    best, times = autotune.tune_plan_variant(wl.geometry(), wl.pupil(), n_frames=8)
+    best, times = autotune.tune_plan_variant(wl.geometry(), wl.pupil(), n_frames=8, use_cache=False)
     assert best in times and set(times) == {1, 2, 3, 4}
+    assert times[best] == min(times.values())
+    # an independent re-timing confirms the pick against the slowest variant (the register-GEMV
+    # binning and the any-size path are several times slower than the matrix-core binning here)
+    from specula_amd.processing_objects.sh import SHPlan
+    worst = max(times, key=times.get)
+    dev = torch.device('cuda:0')
+    A = torch.tensor(wl.pupil().astype(np.float32), device=dev)
+    ph = torch.zeros((8, wl.ef_size, wl.ef_size), dtype=torch.float32, device=dev)
+    out = torch.empty((8, wl.geometry().ccd_side, wl.geometry().ccd_side), dtype=torch.float32, device=dev)
+    again = {}
+    for v in (best, worst):
+        plan = SHPlan(wl.geometry(), max_frames=8, device=0, variant=v)
+        again[v] = autotune._time(lambda: plan.forward(A, ph, out, 1.0, n_frames=8))
+    print(f'autotuner: pick {best} ({again[best] * 1e6:.1f} us), slowest {worst} ({again[worst] * 1e6:.1f} us)',
+          flush=True)
+    assert again[best] < again[worst]
 
 
 # --------------------------------------------------------------------------- the full uneven work list
