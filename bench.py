@@ -53,7 +53,7 @@ def parse():
                     help="extra measurements: 'all' (or 1), 'none' (or 0), or a comma list of "
                          "bf16, ccd, lgs, im, multi, hbm, variants, objects")
     args = ap.parse_args()
-    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm', 'variants', 'objects')
+    names = ('bf16', 'ccd', 'lgs', 'im', 'multi', 'hbm', 'variants', 'objects', 'setups')
     e = str(args.extras).strip().lower()
     args.extras = set(names) if e in ('all', '1') else set() if e in ('none', '0', '') else \
         {x.strip() for x in e.split(',') if x.strip()}
@@ -300,6 +300,70 @@ def bench_objects(torch, dev, steps=200):
     return {'workload': 'elt_ngs_80x80, 1 frame per step', **out}
 
 
+def bench_setups(torch, dev, frames=8):
+    """Every SH block of the reference's own set-ups (ERIS, MORFEO, SCAO; workloads.REFERENCE_SH_SETUPS)
+    at full size on this GPU: frames/s of the plan's forward over ``frames`` seeded screens per call
+    (host launch, resampling and the rescale pass included).  LGS blocks use a Gaussian kernel of
+    2 FFT pixels FWHM in every subaperture (their sodium-profile kernel sets are gigabytes; the
+    kernel values do not change the work)."""
+    from specula_amd import workloads
+    from specula_amd.lib.geometry import sh_geometry
+    from specula_amd.lib.phasescreen import annular_pupil, von_karman_opd
+    from specula_amd.processing_objects.sh import SHPlan
+    out = {}
+    for where, pp, pitch, lgs, kw in workloads.REFERENCE_SH_SETUPS:
+        g = sh_geometry(pp, pitch, **kw)
+        key = where.split('/')[-1]
+        rec = {'lgs': lgs, 'subaps': g.n_lenses, 'np_sub': g.ovs_np_sub, 'fft_size': g.fft_size, 'cut': g.cutpixels,
+               'ccd_px_per_subap': g.subap_npx, 'fov_ovs': round(float(g.fov_ovs), 4)}
+        try:
+            plan = SHPlan(g, lgs=lgs, max_frames=frames, device=0)
+        except RuntimeError as e:
+            rec['refused'] = str(e).split(':')[-1].strip()
+            out[key] = rec
+            continue
+        A = annular_pupil(pp, 0.15)
+        if g.needs_interpolation:
+            plan.set_resampling(A)
+        if lgs:
+            plan.set_kernels(torch.tensor(workloads.gaussian_lag_spectra(g), device=dev))
+        scr = von_karman_opd(1 << int(np.ceil(np.log2(pp + frames + 8))), pitch, seed=5)
+        ph = torch.tensor(np.stack([scr[:pp, k:k + pp] for k in range(frames)]).astype(np.float32), device=dev)
+        At = torch.tensor(A.astype(np.float32), device=dev)
+        ccd = torch.empty((frames, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
+        plan.set_static_pupil(At, 1.0)
+        dt = timed(lambda: plan.forward(At, ph, ccd, 1.0, n_frames=frames), 5, torch.cuda.synchronize)
+        rec.update(frames_per_s=round(frames / dt, 1), ms_per_frame=round(dt * 1e3 / frames, 4),
+                   parseval=bool(plan.parseval), mask_terms=int(plan.mask_terms))
+        out[key] = rec
+        del plan, ph, ccd
+    return out
+
+
+def cpu_baseline_setups(setups, seconds):
+    """The FP64 oracle (one core) on one flat-plus-screen frame of the NGS set-ups of
+    ``bench_setups`` whose full frame it finishes quickly, within a total budget of ``seconds``."""
+    from oracle import sh_oracle as O
+    from specula_amd import workloads
+    from specula_amd.lib.phasescreen import annular_pupil, von_karman_opd
+    t_all = time.perf_counter()
+    for where, pp, pitch, lgs, kw in workloads.REFERENCE_SH_SETUPS:
+        key = where.split('/')[-1]
+        rec = setups.get(key)
+        if lgs or rec is None or 'refused' in rec or time.perf_counter() - t_all > seconds:
+            continue
+        go = O.sh_geometry(ef_size=pp, pixel_pitch=pitch, **kw)
+        if go['ef_ovs'] > 1024:          # resampled fields of 8k px: minutes per frame in FP64
+            continue
+        A = annular_pupil(pp, 0.15)
+        ph = von_karman_opd(1 << int(np.ceil(np.log2(pp + 8))), pitch, seed=5)[:pp, :pp]
+        t0 = time.perf_counter()
+        O.sh_forward(A, ph, go)
+        dt = time.perf_counter() - t0
+        rec['cpu_oracle_frames_per_s'] = round(1.0 / dt, 3)
+        rec['gpu_over_cpu'] = round(rec['frames_per_s'] * dt, 1)
+
+
 def bench_variants(torch, dev):
     """The plan's binning variants on elt_ngs_80x80 as the autotuner times them (100 frames, flat
     phase, per-call pupil statistics): register-GEMV binning on the VALU after the FFTs, FP32
@@ -530,9 +594,14 @@ def main():
     if 'hbm' in args.extras:
         progress('hbm ceilings')
         result['hbm_ceiling'] = bench_hbm(torch, dev)
+    if 'setups' in args.extras:
+        progress('reference set-ups')
+        result['reference_setups'] = bench_setups(torch, dev)
     if args.cpu_seconds > 0:
         progress('cpu baseline')
         result['cpu_baseline'] = cpu_baseline(wl, A, frames, idxs, args.cpu_seconds)
+        if 'reference_setups' in result:
+            cpu_baseline_setups(result['reference_setups'], args.cpu_seconds)
     print(json.dumps(result), flush=True)
 
 

@@ -26,54 +26,94 @@
 // separable terms, B = sum_k P_k S Q_k^T (NGS) or Re sum_k P~_k D Q~_k^T (LGS): the NGS power
 // spectrum S is then kept whole in the slab like the LGS one, and the K binnings accumulate into
 // an npx x npx slab tile before the store.
-// FFTs: iterative radix-2 decimation in time after a bit-reversal permutation, twiddles
-// exp(-2 pi i k / M) from a table built on the host in FP64.  Accumulations of the binning products
+// FFTs: four-step transforms (register FFT_R1 and FFT_R2 with M = R1 R2, one padded transpose
+// through LDS), twiddles exp(-2 pi i k / M) from a table built on the host in FP64.  Accumulations of the binning products
 // in FP32 over M terms (M <= 1024); the image sum of the non-Parseval mode in FP64.
 
 struct GenArgs {
     const float2* tw;    // [M/2] exp(-2 pi i k / M)
     float* slab;         // [gridDim.x][slab_floats]
     int64_t slab_floats; // floats of scratch per block
-    int M, log2M, np, ch;
+    int M, log2M, np, ch, log2ch;   // ch = 1 << log2ch vectors per LDS chunk
     int terms;           // separable terms of the FoV mask (1: separable mask)
     int64_t term_stride; // floats per term in the row / column operators
 };
 
 constexpr int kGenThreads = 256;
-constexpr int kGenLdsF2 = 8192;           // 64 KiB of complex working vectors per block
-
-// Bit-reverse `vecs` vectors of length M held at buf[v * M + i] (in place, pairs swapped once).
-__device__ __forceinline__ void gen_bitrev(float2* buf, int vecs, int M, int log2M) {
-    for (int idx = threadIdx.x; idx < vecs * M; idx += blockDim.x) {
-        const int v = idx >> log2M, i = idx & (M - 1);
-        const int r = (int)(__brev((unsigned)i) >> (32 - log2M));
-        if (i < r) {
-            float2 a = buf[v * M + i];
-            buf[v * M + i] = buf[v * M + r];
-            buf[v * M + r] = a;
-        }
-    }
+// Complex working vectors per block: 32 KiB for M <= 256 (two blocks per CU with the transpose
+// scratch), 64 KiB for M >= 512 (one block per CU; such plans have few, large subapertures, so
+// the longer chunks matter more than the occupancy).  The smaller one bounds the LGS V tile.
+constexpr int kGenLdsF2 = 4096 + 64;       // 16 rows of 256 + 1 (the LDS rows are padded to M + 1)
+constexpr int kGenLdsBigF2 = 8192 + 128;   // 16 rows of 512 + 1, 8 of 1024 + 1
+constexpr int gen_cap(int M) { return M >= 512 ? kGenLdsBigF2 : kGenLdsF2; }
+// vectors per chunk: the largest power of two whose padded rows fit
+inline int gen_chunk_log2(int M) {
+    int l = 0;
+    while ((2 << l) * (M + 1) <= gen_cap(M)) ++l;
+    return l;
 }
 
-// Forward FFT (exp(-2 pi i jk/M), unnormalised) of `vecs` vectors of length M in LDS.
-__device__ void gen_fft(float2* buf, int vecs, int M, int log2M, const float2* __restrict__ tw_lds) {
-    gen_bitrev(buf, vecs, M, log2M);
-    __syncthreads();
-    const int halfM = M >> 1;
-    for (int s = 0, half = 1; half < M; ++s, half <<= 1) {
-        const int tw_step = M >> (s + 1);             // twiddle index stride at this stage
-        for (int idx = threadIdx.x; idx < vecs * halfM; idx += blockDim.x) {
-            const int v = idx / halfM, b = idx - v * halfM;
-            const int j = b & (half - 1);
-            const int k = ((b >> s) << (s + 1)) + j;
-            const float2 w = tw_lds[j * tw_step];
-            float2* p = buf + v * M + k;
-            const float2 a = p[0], c0 = p[half];
-            const float2 c = make_float2(c0.x * w.x - c0.y * w.y, c0.x * w.y + c0.y * w.x);
-            p[0] = make_float2(a.x + c.x, a.y + c.y);
-            p[half] = make_float2(a.x - c.x, a.y - c.y);
+// Forward FFT (exp(-2 pi i jk/M), unnormalised) of `vecs` vectors of length M = R1 R2 held at
+// buf[v * vs + i] (vs >= M: the callers pad rows to M + 1 so that their column-wise LDS accesses
+// are odd-strided), in four steps (Cooley-Tukey with the transpose through LDS):
+//   1  for each column n2 of the R1 x R2 view x[R2 n1 + n2]: FFT_R1 over n1 in registers, times
+//      the twiddle W_M^(n2 k1), written to the scratch row n2 (rows padded to R1 + 1 complex)
+//   2  for each k1: FFT_R2 over n2 of scratch column k1 in registers -> X[k1 + R1 k2], written
+//      back to buf in natural order
+// Every LDS access of a wave is to consecutive or odd-strided words (no bank conflicts), and a
+// transform costs two barriers instead of the log2(M) + 1 of a radix-2 pass per stage.  The
+// register FFTs are the compile-time ones of fft_reg.h.  Precondition: buf written and the
+// block synchronised; the result is visible to the block on return.
+template <int R1, int R2>
+__device__ void gen_fft_rr(float2* buf, float2* scr, int vecs, int vs, const float2* __restrict__ tw) {
+    constexpr int M = R1 * R2, RP = R1 + 1;
+    for (int task = threadIdx.x; task < vecs * R2; task += blockDim.x) {
+        const int v = task / R2, n2 = task - v * R2;
+        const float2* p = buf + v * vs + n2;
+        float2 x[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) x[n1] = p[n1 * R2];
+        fft_radix2<R1, false>(x);
+        float2* o = scr + (v * R2 + n2) * RP;
+        o[0] = x[0];
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) {
+            const int e = n2 * k1;                       // < M; the table holds W^e for e < M/2
+            float2 w = tw[e & (M / 2 - 1)];
+            if (e & (M / 2)) w = make_float2(-w.x, -w.y);
+            o[k1] = cmul(x[k1], w);
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    for (int task = threadIdx.x; task < vecs * R1; task += blockDim.x) {
+        const int v = task / R1, k1 = task - v * R1;
+        const float2* sc = scr + v * R2 * RP + k1;
+        float2 x[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) x[n2] = sc[n2 * RP];
+        fft_radix2<R2, false>(x);
+        float2* o = buf + v * vs + k1;
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) o[k2 * R1] = x[k2];
+    }
+    __syncthreads();
+}
+
+// The factorisations of one kernel instantiation: M <= 256 (register transforms up to FFT_16, so
+// the kernel keeps two waves per SIMD) or M = 512, 1024 (up to FFT_32).
+template <int CAP>
+__device__ void gen_fft(float2* buf, float2* scr, int vecs, int vs, int log2M, const float2* __restrict__ tw) {
+    if constexpr (CAP == kGenLdsF2) {
+        switch (log2M) {
+            case 4: gen_fft_rr<4, 4>(buf, scr, vecs, vs, tw); break;
+            case 5: gen_fft_rr<4, 8>(buf, scr, vecs, vs, tw); break;
+            case 6: gen_fft_rr<8, 8>(buf, scr, vecs, vs, tw); break;
+            case 7: gen_fft_rr<8, 16>(buf, scr, vecs, vs, tw); break;
+            default: gen_fft_rr<16, 16>(buf, scr, vecs, vs, tw); break;  // log2M = 8
+        }
+    } else {
+        if (log2M == 9) gen_fft_rr<16, 32>(buf, scr, vecs, vs, tw);
+        else gen_fft_rr<32, 32>(buf, scr, vecs, vs, tw);                 // log2M = 10
     }
 }
 
@@ -101,13 +141,17 @@ __device__ __forceinline__ void gen_slab(const GenArgs& q, int npx, bool lgs, fl
     }
 }
 
+template <int CAP>
 __global__ void __launch_bounds__(kGenThreads)
 sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
-    __shared__ float2 buf[kGenLdsF2];
+    __shared__ float2 buf[CAP];
+    __shared__ float2 fscr[CAP + CAP / 4];           // padded transpose scratch of the FFTs (R1 >= 4)
     __shared__ float2 tw[512];
     __shared__ double red[kGenThreads / 32];
     __shared__ int lit_flag;
     const int M = q.M, log2M = q.log2M, np = q.np, CH = q.ch;
+    const int VS = M + 1;                            // LDS row stride of the working vectors (odd)
+    const int lch = q.log2ch;
     const int npx = g.npx, n = g.n_lenses;
     const bool lgs = g.kspec != nullptr;
     const int L = 2 * np - 1;
@@ -166,26 +210,29 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
                         v = make_float2(a * __builtin_amdgcn_cosf(rf), a * __builtin_amdgcn_sinf(rf));
                     }
                 }
-                buf[e] = v;
+                buf[r * VS + x] = v;
             }
             __syncthreads();
-            gen_fft(buf, rows, M, log2M, tw);
-            for (int e = threadIdx.x; e < rows * M; e += blockDim.x) G[(int64_t)y0 * M + e] = buf[e];
+            gen_fft<CAP>(buf, fscr, rows, VS, log2M, tw);
+            for (int e = threadIdx.x; e < rows * M; e += blockDim.x)
+                G[(int64_t)y0 * M + e] = buf[(e >> log2M) * VS + (e & (M - 1))];
             __syncthreads();
         }
 
         // 2: column transforms, |.|^2, and the row binning (NGS) or the power spectrum (LGS)
+        const int lc2 = lch < log2M ? lch : log2M;       // stage 2 chunks: 1 << lc2 columns
         for (int c0 = 0; c0 < M; c0 += CH) {
             const int cols = min(CH, M - c0);
             for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
-                const int c = e >> log2M, y = e & (M - 1);
-                buf[e] = y < np ? G[(int64_t)y * M + c0 + c] : make_float2(0.f, 0.f);
+                const int y = e >> lc2, c = e & (cols - 1);              // consecutive threads: consecutive G words
+                buf[c * VS + y] = y < np ? G[(int64_t)y * M + c0 + c] : make_float2(0.f, 0.f);
             }
             __syncthreads();
-            gen_fft(buf, cols, M, log2M, tw);
+            gen_fft<CAP>(buf, fscr, cols, VS, log2M, tw);
             for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
-                const float2 h = buf[e];
-                buf[e] = make_float2(fmaf(h.x, h.x, h.y * h.y), 0.f);
+                float2* hp = buf + (e >> log2M) * VS + (e & (M - 1));
+                const float2 h = *hp;
+                *hp = make_float2(fmaf(h.x, h.x, h.y * h.y), 0.f);
             }
             __syncthreads();
             if (!lgs && K == 1) {
@@ -193,15 +240,15 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
                 for (int e = threadIdx.x; e < npx * cols; e += blockDim.x) {
                     const int Y = e / cols, c = e - (e / cols) * cols;
                     const float* pr = g.roww + (int64_t)Y * M;
-                    const float2* sc = buf + c * M;
+                    const float2* sc = buf + c * VS;
                     float acc = 0.f;
                     for (int ky = 0; ky < M; ++ky) acc = fmaf(pr[ky], sc[ky].x, acc);
                     U[(int64_t)Y * M + c0 + c] = acc;
                 }
             } else {                                   // LGS, or NGS with K mask terms: keep S
                 for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
-                    const int c = e >> log2M, ky = e & (M - 1);
-                    S[(int64_t)ky * M + c0 + c] = buf[e].x;
+                    const int ky = e >> lc2, c = e & (cols - 1);
+                    S[(int64_t)ky * M + c0 + c] = buf[c * VS + ky].x;
                 }
             }
             __syncthreads();
@@ -253,12 +300,12 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
             for (int r0 = 0; r0 < M; r0 += CH) {
                 const int rows = min(CH, M - r0);
                 for (int e = threadIdx.x; e < rows * M; e += blockDim.x)
-                    buf[e] = make_float2(S[(int64_t)r0 * M + e], 0.f);
+                    buf[(e >> log2M) * VS + (e & (M - 1))] = make_float2(S[(int64_t)r0 * M + e], 0.f);
                 __syncthreads();
-                gen_fft(buf, rows, M, log2M, tw);
+                gen_fft<CAP>(buf, fscr, rows, VS, log2M, tw);
                 for (int e = threadIdx.x; e < rows * np; e += blockDim.x) {
                     const int r = e / np, mx = e - (e / np) * np;
-                    T[(int64_t)(r0 + r) * np + mx] = buf[r * M + mx];
+                    T[(int64_t)(r0 + r) * np + mx] = buf[r * VS + mx];
                 }
                 __syncthreads();
             }
@@ -266,16 +313,17 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
             const float2* ks = g.kspec + (int64_t)sub * L * L;
             for (int c0 = 0; c0 < np; c0 += CH) {
                 const int cols = min(CH, np - c0);
-                for (int e = threadIdx.x; e < cols * M; e += blockDim.x) {
-                    const int c = e >> log2M, ky = e & (M - 1);
-                    buf[e] = T[(int64_t)ky * np + c0 + c];
+                for (int e = threadIdx.x; e < (M << lch); e += blockDim.x) {
+                    const int ky = e >> lch, c = e & (CH - 1);
+                    if (c < cols) buf[c * VS + ky] = T[(int64_t)ky * np + c0 + c];
                 }
                 __syncthreads();
-                gen_fft(buf, cols, M, log2M, tw);
-                for (int e = threadIdx.x; e < cols * L; e += blockDim.x) {
-                    const int c = e / L, ly = e - (e / L) * L;        // ly = my + np - 1
+                gen_fft<CAP>(buf, fscr, cols, VS, log2M, tw);
+                for (int e = threadIdx.x; e < (L << lch); e += blockDim.x) {
+                    const int ly = e >> lch, c = e & (CH - 1);            // ly = my + np - 1
+                    if (c >= cols) continue;
                     const int mx = c0 + c, my = ly - (np - 1);
-                    const float2 d = buf[c * M + ((my + M) & (M - 1))];
+                    const float2 d = buf[c * VS + ((my + M) & (M - 1))];
                     const float2 kk = ks[(int64_t)ly * L + (mx + np - 1)];
                     D[(int64_t)ly * np + mx] = make_float2(d.x * kk.x - d.y * kk.y, d.x * kk.y + d.y * kk.x);
                 }
@@ -283,7 +331,7 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
             }
             // 3c: V[Y][mx] = sum_my P~_k[Y][my] D[my][mx] (complex, in LDS if it fits, else in S)
             // 3d: B += Re sum_mx c_mx V[Y][mx] Q~_k[X][mx], over the K mask terms
-            float2* V = (npx * np <= kGenLdsF2) ? buf : reinterpret_cast<float2*>(S);
+            float2* V = (npx * np <= CAP) ? buf : reinterpret_cast<float2*>(S);
             for (int k = 0; k < K; ++k) {
                 const float2* Pt = reinterpret_cast<const float2*>(g.roww + (int64_t)k * q.term_stride);
                 for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {

@@ -2255,15 +2255,20 @@ int launch_general(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st
     while ((1 << lg) < q.M) ++lg;
     q.log2M = lg;
     q.np = g.np_sub;
-    q.ch = kGenLdsF2 / q.M;
+    q.log2ch = gen_chunk_log2(q.M);             // vectors per chunk: rows padded to M + 1 in LDS
+    q.ch = 1 << q.log2ch;
     q.terms = p->terms;
     q.term_stride = (int64_t)g.subap_npx * (g.lgs ? 2LL * p->L : (int64_t)g.spec_size);
     NgsArgs b = a;
     if (!g.lgs) b.kspec = nullptr;                 // the kernel takes the LGS branch on kspec != nullptr
     const int64_t items = (int64_t)n_frames * p->n_sub;
     const int blocks = (int)std::min<int64_t>(items, p->gen_blocks);
-    hipExtLaunchKernelGGL(sh_general_kernel, dim3(blocks), dim3(kGenThreads), 0, st, t_ev_start, t_ev_stop, 0, b, q,
-                          n_frames);
+    if (gen_cap(q.M) == kGenLdsBigF2)
+        hipExtLaunchKernelGGL(sh_general_kernel<kGenLdsBigF2>, dim3(blocks), dim3(kGenThreads), 0, st, t_ev_start,
+                              t_ev_stop, 0, b, q, n_frames);
+    else
+        hipExtLaunchKernelGGL(sh_general_kernel<kGenLdsF2>, dim3(blocks), dim3(kGenThreads), 0, st, t_ev_start,
+                              t_ev_stop, 0, b, q, n_frames);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }

@@ -159,3 +159,56 @@ def multi_sensor_entries(n_lgs=6, n_pushpull=2000, amp_nm=50.0, device=0, with_s
         del modes
         entries.append(e)
     return entries
+
+
+# The SH blocks of the reference's own set-ups (YAML file:line of the `sh` block; pupil pixels,
+# pixel pitch [m], LGS, SH keywords), restated from main/scao/*.yml and test/params_scao_sh_test.yml.
+REFERENCE_SH_SETUPS = [
+    ('main/scao/params_eris.yml:102', 160, 0.05, True,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 5, 'sensor_pxscale': 0.85, 'subap_on_diameter': 40, 'subap_npx': 6}),
+    ('main/scao/params_eris.yml:148', 160, 0.05, False,
+     {'wavelengthInNm': 700, 'subap_wanted_fov': 4.8, 'sensor_pxscale': 0.4, 'subap_on_diameter': 4, 'subap_npx': 12}),
+    ('main/scao/params_eris_ngs.yml:91', 160, 0.05, False,
+     {'wavelengthInNm': 768, 'subap_wanted_fov': 2.4, 'sensor_pxscale': 0.4, 'subap_on_diameter': 40, 'subap_npx': 6}),
+    ('main/scao/params_morfeo_full.yml:171', 480, 0.0802, True,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14,
+      'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': 6.2}),
+    ('main/scao/params_morfeo_full.yml:185', 480, 0.0802, True,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14,
+      'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': 14.2}),
+    ('main/scao/params_morfeo_full.yml:186', 480, 0.0802, True,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14,
+      'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': -6.2}),
+    ('main/scao/params_morfeo_full.yml:259', 480, 0.0802, False,
+     {'wavelengthInNm': 1650, 'subap_wanted_fov': 1.8, 'sensor_pxscale': 0.015, 'subap_on_diameter': 2, 'subap_npx': 120}),
+    ('main/scao/params_morfeo_full.yml:306', 480, 0.0802, False,
+     {'wavelengthInNm': 750, 'subap_wanted_fov': 4.6, 'sensor_pxscale': 0.153, 'subap_on_diameter': 8, 'subap_npx': 30}),
+    ('main/scao/params_morfeo_ltao.yml:127', 480, 0.0802, True,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14,
+      'fov_ovs_coeff': 1.6}),
+    ('main/scao/params_morfeo_ltao_plots.yml:120', 480, 0.0802, False,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14,
+      'fov_ovs_coeff': 1.6}),
+    ('main/scao/params_morfeo_scao_plots.yml:94', 480, 0.08125, False,
+     {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14,
+      'fov_ovs_coeff': 1.6}),
+    ('main/scao/params_scao_sh.yml:69', 160, 0.05, True,
+     {'wavelengthInNm': 750, 'subap_wanted_fov': 1.2, 'sensor_pxscale': 0.15, 'subap_on_diameter': 20, 'subap_npx': 6}),
+    ('main/scao/params_scao_sh_plots.yml:69', 160, 0.05, True,
+     {'wavelengthInNm': 750, 'subap_wanted_fov': 2.4, 'sensor_pxscale': 0.4, 'subap_on_diameter': 20, 'subap_npx': 6}),
+    ('test/params_scao_sh_test.yml:66', 64, 0.015625, False,
+     {'wavelengthInNm': 600, 'subap_wanted_fov': 4.0, 'sensor_pxscale': 0.5, 'subap_on_diameter': 8, 'subap_npx': 8}),
+]
+
+
+def gaussian_lag_spectra(geom, fwhm_px=2.0):
+    """Lag-grid spectra (n_sub, 2np-1, 2np-1) complex64 of a unit-sum circular Gaussian kernel of
+    ``fwhm_px`` FFT pixels, the same for every subaperture: a light stand-in for the sodium-profile
+    kernels of a large LGS sensor when only its timing matters (the spectrum of a real, symmetric
+    kernel on the lags |m| <= np-1 of the N-point grid)."""
+    N, p = geom.fft_size, geom.ovs_np_sub
+    m = np.arange(-(p - 1), p)
+    sig = fwhm_px / (2 * np.sqrt(2 * np.log(2)))
+    k1 = np.exp(-2 * (np.pi * sig * m / N) ** 2)            # spectrum of the sampled Gaussian
+    spec = np.outer(k1, k1).astype(np.complex64)
+    return np.broadcast_to(spec, (geom.n_lenses ** 2,) + spec.shape).copy()

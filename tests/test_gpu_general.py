@@ -163,22 +163,7 @@ def test_general_path_equals_specialised_kernels_where_both_apply():
 
 # Every distinct SH block of the reference's example and test set-ups (values restated from the
 # YAML files; pupil px, pixel pitch, LGS launcher present, SH keywords).
-REFERENCE_SH_BLOCKS = [
-    ('main/scao/params_eris.yml:102', 160, 0.05, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 5, 'sensor_pxscale': 0.85, 'subap_on_diameter': 40, 'subap_npx': 6}),
-    ('main/scao/params_eris.yml:148', 160, 0.05, False, {'wavelengthInNm': 700, 'subap_wanted_fov': 4.8, 'sensor_pxscale': 0.4, 'subap_on_diameter': 4, 'subap_npx': 12}),
-    ('main/scao/params_eris_ngs.yml:91', 160, 0.05, False, {'wavelengthInNm': 768, 'subap_wanted_fov': 2.4, 'sensor_pxscale': 0.4, 'subap_on_diameter': 40, 'subap_npx': 6}),
-    ('main/scao/params_morfeo_full.yml:171', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': 6.2}),
-    ('main/scao/params_morfeo_full.yml:185', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': 14.2}),
-    ('main/scao/params_morfeo_full.yml:186', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6, 'rotAnglePhInDeg': -6.2}),
-    ('main/scao/params_morfeo_full.yml:259', 480, 0.0802, False, {'wavelengthInNm': 1650, 'subap_wanted_fov': 1.8, 'sensor_pxscale': 0.015, 'subap_on_diameter': 2, 'subap_npx': 120}),
-    ('main/scao/params_morfeo_full.yml:306', 480, 0.0802, False, {'wavelengthInNm': 750, 'subap_wanted_fov': 4.6, 'sensor_pxscale': 0.153, 'subap_on_diameter': 8, 'subap_npx': 30}),
-    ('main/scao/params_morfeo_ltao.yml:127', 480, 0.0802, True, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6}),
-    ('main/scao/params_morfeo_ltao_plots.yml:120', 480, 0.0802, False, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6}),
-    ('main/scao/params_morfeo_scao_plots.yml:94', 480, 0.08125, False, {'wavelengthInNm': 589, 'subap_wanted_fov': 16.1, 'sensor_pxscale': 1.15, 'subap_on_diameter': 68, 'subap_npx': 14, 'fov_ovs_coeff': 1.6}),
-    ('main/scao/params_scao_sh.yml:69', 160, 0.05, True, {'wavelengthInNm': 750, 'subap_wanted_fov': 1.2, 'sensor_pxscale': 0.15, 'subap_on_diameter': 20, 'subap_npx': 6}),
-    ('main/scao/params_scao_sh_plots.yml:69', 160, 0.05, True, {'wavelengthInNm': 750, 'subap_wanted_fov': 2.4, 'sensor_pxscale': 0.4, 'subap_on_diameter': 20, 'subap_npx': 6}),
-    ('test/params_scao_sh_test.yml:66', 64, 0.015625, False, {'wavelengthInNm': 600, 'subap_wanted_fov': 4.0, 'sensor_pxscale': 0.5, 'subap_on_diameter': 8, 'subap_npx': 8}),
-]
+REFERENCE_SH_BLOCKS = workloads.REFERENCE_SH_SETUPS
 
 
 @pytest.mark.parametrize('block', REFERENCE_SH_BLOCKS, ids=[b[0].split('/')[-1] for b in REFERENCE_SH_BLOCKS])
