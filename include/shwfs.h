@@ -193,9 +193,12 @@ int shslopec_vjp(shslopec_plan* plan, int n_frames, const float* pixels, int64_t
                  float thr, int thr_mode, const float* grad_slopes, int64_t grad_slopes_stride,
                  float* grad_pixels, void* stream);
 
-/* shwfs_vjp: dL/dphase_nm from dL/d(CCD intensity) for shwfs_forward (specialised NGS kernels;
- * not the LGS path or variant 4).  phase_frame_stride may be 0 to evaluate several cotangents at
- * one phase.  Any normalisation: with the Parseval scale (no cut, all-ones mask) the scale does not
+/* shwfs_vjp: dL/dphase_nm from dL/d(CCD intensity) for shwfs_forward, for every plan: NGS and LGS
+ * (kernel spectra set), specialised kernels and the any-size path (variant 4, round FoV masks).
+ * NGS plans with (np_sub, spec_size) in {(4,8),(6,16),(8,16),(16,32)} use register-resident
+ * kernels; the others (LGS, any np_sub, spec_size a power of two in [16, 1024]) run full M-point
+ * FFTs per subaperture with per-block scratch (at most 2 GiB, allocated by the first call and kept
+ * by the plan).  phase_frame_stride may be 0 to evaluate several cotangents at one phase.  Any normalisation: with the Parseval scale (no cut, all-ones mask) the scale does not
  * depend on the phase; otherwise the image I = c(phi) B is normalised by its own sum and the VJP
  * is taken exactly (the forward runs first into plan scratch to get c and <g, I> / sum(I)).  A
  * resampled plan (shwfs_plan_set_resampling) takes A, phase and returns the gradient on the input
@@ -223,7 +226,8 @@ int shwfs_vjp_sensor_grid(shwfs_plan* plan, int n_frames, const float* A, int64_
 int shwfs_resample(shwfs_plan* plan, int n_frames, const float* in, int64_t in_frame_stride, float* out,
                    int64_t out_frame_stride, void* stream);
 
-/* Interaction matrix from the block-diagonal slope Jacobian (fov_ovs == 1):
+/* Interaction matrix from the block-diagonal slope Jacobian (any np_sub and pupil width; with
+ * field resampling, the sensor-grid Jacobian and resampled modes):
  *   im[m][slot]           = scale * sum_{p in subaperture} jx[p] * modes[m][p]
  *   im[m][n_valid + slot] = scale * sum_{p in subaperture} jy[p] * modes[m][p]
  * jx, jy: ef x ef maps (d slope / d phase_nm); modes: n_modes maps of ef x ef (stride

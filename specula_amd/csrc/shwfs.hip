@@ -1645,6 +1645,8 @@ __global__ void vjp_scale_kernel(const double* __restrict__ bpart, int n_parts, 
     if (threadIdx.x == 0) cg[f] = make_double2(S0 * pixel_area * amp[(int64_t)f * amp_fstep] / t, gs.x / gs.y);
 }
 
+#include "sh_general_vjp.h"
+
 // SH backward for the NGS path with Parseval normalisation (I = c B, c independent of the
 // phase): with G_B = c dL/dI on the subaperture tile,
 //   G_S = P^T G_B Q,   X = G_S conj(F),   H = FFT_M(X) on the np x np support,
@@ -1971,6 +1973,62 @@ im_gather_lds_kernel(const float* __restrict__ grid, const int32_t* __restrict__
     }
 }
 
+// Same contraction for any np_sub and pupil width (the any-size path's sensors): one block per
+// (subaperture, chunk of modes); its threads stride over the subaperture's np x np pixels, 8 modes
+// at a time, and the block reduces the 16 partial sums.  Subapertures without a valid slot exit.
+__global__ void __launch_bounds__(256)
+im_from_jacobian_any_kernel(const float* __restrict__ jx, const float* __restrict__ jy, int ef, int n_lenses, int np,
+                            const float* __restrict__ modes, int64_t mode_stride, int n_modes, int modes_per_block,
+                            const int32_t* __restrict__ slot, int n_valid, float scale, float* __restrict__ im) {
+    constexpr int MU = 8, NW = 256 / kWave;
+    __shared__ float red[NW][2 * MU];
+    const int sub = blockIdx.x;
+    const int sl = slot[sub];
+    if (sl < 0 || sl >= n_valid) return;                 // the whole block
+    const int i = sub / n_lenses, j = sub - (sub / n_lenses) * n_lenses;
+    const int64_t off = (int64_t)(i * np) * ef + (int64_t)j * np;
+    const int npp = np * np, w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int m0 = blockIdx.y * modes_per_block;
+    const int m1 = min(n_modes, m0 + modes_per_block);
+    for (int mb = m0; mb < m1; mb += MU) {
+        float px[MU], py[MU];
+#pragma unroll
+        for (int u = 0; u < MU; ++u) px[u] = py[u] = 0.f;
+        for (int e = threadIdx.x; e < npp; e += blockDim.x) {
+            const int y = e / np, x = e - (e / np) * np;
+            const int64_t o = off + (int64_t)y * ef + x;
+            const float a = jx[o], b = jy[o];
+#pragma unroll
+            for (int u = 0; u < MU; ++u) {
+                const float v = mb + u < m1 ? modes[(int64_t)(mb + u) * mode_stride + o] : 0.f;
+                px[u] = fmaf(a, v, px[u]);
+                py[u] = fmaf(b, v, py[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < MU; ++u)
+            for (int d = kWave / 2; d > 0; d >>= 1) {
+                px[u] += __shfl_xor(px[u], d, kWave);
+                py[u] += __shfl_xor(py[u], d, kWave);
+            }
+        if (lane == 0) {
+#pragma unroll
+            for (int u = 0; u < MU; ++u) {
+                red[w][u] = px[u];
+                red[w][MU + u] = py[u];
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * MU) {
+            const int u = threadIdx.x % MU, c = threadIdx.x / MU;
+            float t = 0.f;
+            for (int q = 0; q < NW; ++q) t += red[q][threadIdx.x];
+            if (mb + u < m1) im[(int64_t)(mb + u) * 2 * n_valid + c * n_valid + sl] = scale * t;
+        }
+        __syncthreads();
+    }
+}
+
 // Same permutation without LDS (grids too large for it): one thread per slot and mode.
 __global__ void im_gather_kernel(const float* __restrict__ grid, const int32_t* __restrict__ inv, int nn, int n_valid,
                                  float* __restrict__ im) {
@@ -2112,6 +2170,10 @@ struct shwfs_plan {
     float* d_gph_rs = nullptr;     // VJP of a resampled plan: [max_frames][ef_size^2] sensor-grid gradient
     float* d_vjp_ccd = nullptr;    // VJP of a non-Parseval plan: [max_frames][ccd px] forward image
     double2* d_vjp_cg = nullptr;   // VJP of a non-Parseval plan: [max_frames] (c_f, alpha_f)
+    float* d_vjp_slab = nullptr;   // any-size VJP: [vjp_blocks][vjp_slab_floats] scratch
+    int vjp_blocks = 0;
+    int64_t vjp_slab_floats = 0;
+    float* d_roww_nat = nullptr;   // specialised LGS plans: row operator in its natural [npx][L] layout
     const float* static_A = nullptr;   // registered pupil, statistics in d_amp_st / d_scale_st
     int64_t static_a_stride = 0;
     int static_frames = 0;
@@ -2242,6 +2304,16 @@ int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3(256), lds, st, t_ev_start, t_ev_stop, 0, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
+}
+
+// [M/2] twiddles exp(-2 pi i k / M) of the any-size path's FFTs, evaluated in FP64
+std::vector<float2> twiddle_table(int M) {
+    std::vector<float2> t(M / 2);
+    for (int k = 0; k < M / 2; ++k) {
+        const double th = -2.0 * M_PI * (double)k / (double)M;
+        t[k] = make_float2((float)std::cos(th), (float)std::sin(th));
+    }
+    return t;
 }
 
 int launch_general(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
@@ -2458,20 +2530,24 @@ int shwfs_plan_create_masked(const shwfs_geom* geom, int n_terms, const double* 
         PLAN_ALLOC(p->d_lit, (size_t)p->n_sub);
     }
     if (g.lgs) PLAN_ALLOC(p->d_kspec, (size_t)p->n_sub * p->L * p->L * 2 * sizeof(float));
+    std::vector<float> rnat;
+    if (g.lgs && !use_general) {        // the any-size VJP reads P~ in its natural layout
+        rnat.resize((size_t)g.subap_npx * p->L * 2);
+        for (size_t q = 0; q < rnat.size(); ++q) rnat[q] = (float)row_w[q];
+        PLAN_ALLOC(p->d_roww_nat, rnat.size() * sizeof(float));
+    }
     std::vector<float2> twh;
     if (use_general) {
         PLAN_ALLOC(p->d_tw, (size_t)(M / 2) * sizeof(float2));
         PLAN_ALLOC(p->d_slab, (size_t)p->gen_blocks * p->slab_floats * sizeof(float));
-        twh.resize(M / 2);
-        for (int k = 0; k < M / 2; ++k) {
-            const double th = -2.0 * M_PI * (double)k / (double)M;
-            twh[k] = make_float2((float)std::cos(th), (float)std::sin(th));
-        }
+        twh = twiddle_table(M);
     }
 #undef PLAN_ALLOC
     if ((!rl.empty() &&
          (e = hipMemcpy(p->d_rowlag, rl.data(), rl.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) ||
         (e = hipMemcpy(p->d_roww, rw.data(), rw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
+        (!rnat.empty() &&
+         (e = hipMemcpy(p->d_roww_nat, rnat.data(), rnat.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess) ||
         (e = hipMemcpy(p->d_colw, cw.data(), cw.size() * sizeof(float), hipMemcpyHostToDevice)) != hipSuccess ||
         (!twh.empty() &&
          (e = hipMemcpy(p->d_tw, twh.data(), twh.size() * sizeof(float2), hipMemcpyHostToDevice)) != hipSuccess) ||
@@ -2506,6 +2582,8 @@ int shwfs_plan_destroy(shwfs_plan* p) {
     (void)hipFree(p->d_gph_rs);
     (void)hipFree(p->d_vjp_ccd);
     (void)hipFree(p->d_vjp_cg);
+    (void)hipFree(p->d_vjp_slab);
+    (void)hipFree(p->d_roww_nat);
     (void)hipFree(p->d_cogw);
     (void)hipFree(p->d_tw);
     (void)hipFree(p->d_slab);
@@ -2984,16 +3062,30 @@ VjpLauncher pick_vjp(int np, int M, int npx, int* ts) {
 }
 // ---------------------------------------------------------------- interaction-matrix launches
 // Argument checks shared by shwfs_im_from_jacobian and its no-allocation variant.
+constexpr int kImModesPerBlock = 16;
+
 int im_check_args(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
                   const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float* im) {
     if (!jx || !jy || !modes || !slot || !im) return set_error(SHWFS_ERR_ARG, "null argument");
     if (n_lenses <= 0 || np_sub <= 0 || n_modes <= 0 || n_valid <= 0 || ef_size != n_lenses * np_sub)
         return set_error(SHWFS_ERR_ARG, "inconsistent sizes");
-    if (ef_size > 1024) return set_error(SHWFS_ERR_UNSUPPORTED, "pupil wider than 1024 px");
     if (mode_stride < (int64_t)ef_size * ef_size) return set_error(SHWFS_ERR_ARG, "mode stride too small");
     if (n_modes > 65535) return set_error(SHWFS_ERR_ARG, "more than 65535 modes in one call");
-    if (np_sub != 4 && np_sub != 8 && np_sub != 16)
-        return set_error(SHWFS_ERR_UNSUPPORTED, "interaction matrix kernels instantiated for np_sub 4, 8, 16");
+    return SHWFS_OK;
+}
+
+// np_sub 4, 8, 16 on pupils up to 1024 px wide: the row-strip kernels; anything else the any-size one
+bool im_strip_kernels(int np_sub, int ef_size) {
+    return (np_sub == 4 || np_sub == 8 || np_sub == 16) && ef_size <= 1024;
+}
+
+int im_launch_any(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
+                  const float* modes, int64_t mode_stride, const int32_t* slot, int n_valid, float scale, float* im,
+                  hipStream_t st) {
+    dim3 grid(n_lenses * n_lenses, (n_modes + kImModesPerBlock - 1) / kImModesPerBlock);
+    hipLaunchKernelGGL(im_from_jacobian_any_kernel, grid, dim3(256), 0, st, jx, jy, ef_size, n_lenses, np_sub, modes,
+                       mode_stride, n_modes, kImModesPerBlock, slot, n_valid, scale, im);
+    SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
 
@@ -3003,8 +3095,6 @@ bool im_v4_eligible(int np_sub, int ef_size, int64_t mode_stride, const float* m
            ((reinterpret_cast<uintptr_t>(modes) | reinterpret_cast<uintptr_t>(jx) |
              reinterpret_cast<uintptr_t>(jy)) & 15u) == 0;
 }
-
-constexpr int kImModesPerBlock = 16;
 
 // Scalar contraction, results written in slot order (no scratch).
 int im_launch_scalar(int n_lenses, int np_sub, int ef_size, const float* jx, const float* jy, int n_modes,
@@ -3092,18 +3182,63 @@ int scratch_of_current_device(unsigned char** out) {
     return SHWFS_OK;
 }
 
-// SH VJP of any specialised NGS plan.  Non-Parseval plans first run the forward into scratch (the
-// image and its unscaled sum give the per-frame scale and cotangent offset, vjp_scale_kernel);
-// resampled plans run the VJP on the sensor (resampled) grid and, unless `sensor_grid`, scatter it
-// back to the input grid through the transpose of the resampling.  First calls allocate the
-// scratch these need (plan-owned, kept).
+// Launch of the any-size VJP (sh_general_vjp.h) for a plan: the any-size path's own plans, LGS
+// plans, and NGS plans whose (np_sub, spec_size) has no specialised VJP kernel.  The first call
+// allocates the per-block scratch (at most 2 GiB in total) and, for plans without the any-size
+// forward, the twiddle table.
+int launch_general_vjp(shwfs_plan* p, const VjpArgs& a, int n_frames, hipStream_t st) {
+    const shwfs_geom& g = p->g;
+    const int M = g.spec_size;      // a power of two in [16, 1024] (checked by the caller)
+    if (!p->d_tw) {
+        std::vector<float2> t = twiddle_table(M);
+        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_tw, t.size() * sizeof(float2)));
+        SHWFS_HIP_CHECK(hipMemcpy(p->d_tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    if (!p->d_vjp_slab) {
+        const int64_t sf = gen_vjp_slab_floats(g.np_sub, M, g.subap_npx, g.lgs != 0, p->terms);
+        const int64_t cap = ((int64_t)2 << 30) / (sf * (int64_t)sizeof(float));
+        const int64_t need = (int64_t)g.max_frames * p->n_sub;        // items of the largest call
+        const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(1024, cap), need));
+        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_vjp_slab, (size_t)blocks * sf * sizeof(float)));
+        p->vjp_slab_floats = sf;
+        p->vjp_blocks = blocks;
+    }
+    GenVjpArgs q{};
+    q.tw = p->d_tw;
+    q.slab = p->d_vjp_slab;
+    q.slab_floats = p->vjp_slab_floats;
+    q.M = M;
+    int lg = 0;
+    while ((1 << lg) < M) ++lg;
+    q.log2M = lg;
+    q.np = g.np_sub;
+    q.log2ch = gen_chunk_log2(M);
+    q.ch = 1 << q.log2ch;
+    q.terms = p->terms;
+    q.term_stride = (int64_t)g.subap_npx * (g.lgs ? 2LL * p->L : (int64_t)M);
+    q.kspec = g.lgs ? reinterpret_cast<const float2*>(p->d_kspec) : nullptr;
+    VjpArgs b = a;
+    if (g.lgs && !p->general) b.roww = p->d_roww_nat;
+    const int64_t items = (int64_t)n_frames * p->n_sub;
+    const int blocks = (int)std::min<int64_t>(items, p->vjp_blocks);
+    if (gen_cap(M) == kGenLdsBigF2)
+        hipLaunchKernelGGL(sh_general_vjp_kernel<kGenLdsBigF2>, dim3(blocks), dim3(kGenThreads), 0, st, b, q, n_frames);
+    else
+        hipLaunchKernelGGL(sh_general_vjp_kernel<kGenLdsF2>, dim3(blocks), dim3(kGenThreads), 0, st, b, q, n_frames);
+    SHWFS_LAUNCH_CHECK();
+    return SHWFS_OK;
+}
+
+// SH VJP of any plan (NGS and LGS, every kernel path).  Non-Parseval plans first run the forward
+// into scratch (the image and its unscaled sum give the per-frame scale and cotangent offset,
+// vjp_scale_kernel); resampled plans run the VJP on the sensor (resampled) grid and, unless
+// `sensor_grid`, scatter it back to the input grid through the transpose of the resampling.  First
+// calls allocate the scratch these need (plan-owned, kept).
 int vjp_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, const float* phase, int64_t ph_stride,
              double S0, const float* grad_ccd, int64_t gccd_stride, float* grad_out, int64_t gout_stride,
              bool sensor_grid, hipStream_t st) {
     if (!p || !A || !phase || !grad_ccd || !grad_out) return set_error(SHWFS_ERR_ARG, "null argument");
     const shwfs_geom& g = p->g;
-    if (g.lgs) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the NGS path");
-    if (p->general) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP is available for the specialised NGS kernels (not variant 4)");
     if (n_frames <= 0 || n_frames > g.max_frames) return set_error(SHWFS_ERR_ARG, "n_frames outside [1, max_frames]");
     const int64_t ef_px = (int64_t)g.ef_size * g.ef_size;                      // sensor grid
     const int64_t in_px = p->ef_in ? (int64_t)p->ef_in * p->ef_in : ef_px;     // input grid
@@ -3113,8 +3248,13 @@ int vjp_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, cons
                          gccd_stride < ccd_px || gout_stride < out_px))
         return set_error(SHWFS_ERR_ARG, "frame stride too small");
     int ts = 0;
-    VjpLauncher L = pick_vjp(g.np_sub, g.spec_size, g.subap_npx, &ts);
-    if (!L) return set_error(SHWFS_ERR_UNSUPPORTED, "VJP instantiated for (np_sub, spec_size) in {(4,8),(6,16),(8,16),(16,32)}");
+    VjpLauncher L = (g.lgs || p->general) ? nullptr : pick_vjp(g.np_sub, g.spec_size, g.subap_npx, &ts);
+    if (!L) {
+        const int M = g.spec_size;
+        if ((M & (M - 1)) != 0 || M < 16 || M > 1024)
+            return set_error(SHWFS_ERR_UNSUPPORTED, "VJP: spectrum grid must be a power of two in [16, 1024]");
+        if (g.lgs && !p->kernels_set) return set_error(SHWFS_ERR_ARG, "LGS plan: kernel spectra not set (shwfs_set_kernels)");
+    }
     DeviceGuard dg(p->device);
     const int amp_frames = (a_stride == 0) ? 1 : n_frames;
     const double2* cg = nullptr;
@@ -3169,9 +3309,8 @@ int vjp_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, cons
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0; a.S0 = S0; a.pixel_area = g.pixel_area;
     a.norm_const = g.norm_const;
     a.cg = cg;
-    const int n_tiles = (p->n_sub + ts - 1) / ts;
     // the subaperture tiles cover the sensor grid (plans have ef_size = n_lenses * np_sub)
-    int rc = L(a, n_tiles, n_frames, g.subap_npx, st);
+    int rc = L ? L(a, (p->n_sub + ts - 1) / ts, n_frames, g.subap_npx, st) : launch_general_vjp(p, a, n_frames, st);
     if (rc) return rc;
     if (p->ef_in && !sensor_grid) {
         const int64_t zb = std::min<int64_t>((in_px + 255) / 256, 2048);
@@ -3259,6 +3398,9 @@ int shwfs_im_from_jacobian(int n_lenses, int np_sub, int ef_size, const float* j
     int rc = im_check_args(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, im);
     if (rc) return rc;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    if (!im_strip_kernels(np_sub, ef_size))
+        return im_launch_any(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale, im,
+                             st);
     if (!im_v4_eligible(np_sub, ef_size, mode_stride, modes, jx, jy))
         return im_launch_scalar(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale,
                                 im, st);
@@ -3275,7 +3417,10 @@ int shwfs_im_from_jacobian_noalloc(int n_lenses, int np_sub, int ef_size, const 
     int rc = im_check_args(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, im);
     if (rc) return rc;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    if (!im_v4_eligible(np_sub, ef_size, mode_stride, modes, jx, jy))   // this path needs no scratch
+    if (!im_strip_kernels(np_sub, ef_size))                              // these paths need no scratch
+        return im_launch_any(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale, im,
+                             st);
+    if (!im_v4_eligible(np_sub, ef_size, mode_stride, modes, jx, jy))
         return im_launch_scalar(n_lenses, np_sub, ef_size, jx, jy, n_modes, modes, mode_stride, slot, n_valid, scale,
                                 im, st);
     const int64_t nn = (int64_t)n_lenses * n_lenses;

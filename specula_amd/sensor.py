@@ -19,14 +19,21 @@ from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weig
 
 class SHSensor:
     def __init__(self, geom, pupil, idxs=None, energy_th=0.5, max_frames=1, device=0, thr=0.0, variant=0,
-                 extrapolation_mask=None):
+                 extrapolation_mask=None, kernel=None):
         """extrapolation_mask: with field resampling, the amplitude the edge-extrapolation maps are
         built from (SH.setup reads in_ef.A, sh.py:453; inside a simulation that is the not yet
-        propagated all-ones field, so no pixel is extrapolated).  None = ``pupil``."""
+        propagated all-ones field, so no pixel is extrapolated).  None = ``pupil``.
+        kernel: an LGS sensor's convolution kernel (a ConvolutionKernel prepared for this SH,
+        sh.py:362-391): its lag-grid spectra are uploaded once; None = NGS."""
         self.geom = geom
         self.device = torch.device(f'cuda:{device}')
         self.max_frames = max_frames
-        self.plan = SHPlan(geom, max_frames=max_frames, device=device, variant=variant)
+        lgs = kernel is not None
+        self.plan = SHPlan(geom, lgs=lgs, max_frames=max_frames, device=device, variant=variant,
+                           kernels_normalised=kernel.kernels_normalised() if lgs else True)
+        if lgs:
+            self.plan.set_kernels(torch.as_tensor(kernel.lag_spectra(geom.ovs_np_sub), device=self.device)
+                                  .to(torch.complex64).contiguous())
         ef, ccd = geom.ef_size, geom.ccd_side
         self.A = torch.as_tensor(np.asarray(pupil, dtype=np.float32), device=self.device).contiguous()
         if geom.needs_interpolation:
