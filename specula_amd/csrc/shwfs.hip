@@ -1054,16 +1054,23 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 #pragma unroll
             for (int x = 0; x < NP; ++x) row[x] = buf[(s * M + ky) * R1 + x];
             // FFT_M(|FFT_M(row)|^2)[m] = M sum_x row[x] conj(row[x + m]) (linear for |m| < NP since
-            // row is zero beyond NP): the lags m_x = 0 .. NP-1 directly, NP (NP + 1) / 2 products
+            // row is zero beyond NP): the lags m_x = 0 .. NP-1 directly, NP (NP + 1) / 2 products;
+            // the factor M is carried by the row operator, and lag 0 is real
+            {
+                float re = 0.f;
 #pragma unroll
-            for (int lh = 0; lh < LH; ++lh) {
+                for (int x = 0; x < NP; ++x) re = fmaf(row[x].x, row[x].x, fmaf(row[x].y, row[x].y, re));
+                R[h][0] = make_float2(re, 0.f);
+            }
+#pragma unroll
+            for (int lh = 1; lh < LH; ++lh) {
                 float re = 0.f, im = 0.f;
 #pragma unroll
                 for (int x = 0; x + lh < NP; ++x) {
                     re = fmaf(row[x].x, row[x + lh].x, fmaf(row[x].y, row[x + lh].y, re));
                     im = fmaf(row[x].y, row[x + lh].x, fmaf(-row[x].x, row[x + lh].y, im));
                 }
-                R[h][lh] = make_float2(re * (float)M, im * (float)M);      // m_x = lh
+                R[h][lh] = make_float2(re, im);                             // m_x = lh
             }
         }
     }
@@ -2460,10 +2467,12 @@ int shwfs_plan_create_masked(const shwfs_geom* geom, int n_terms, const double* 
         rw.assign((size_t)16 * p->L * 2, 0.f);
         cw.assign((size_t)16 * p->L * 2, 0.f);
         for (size_t q = 0; q < (size_t)g.subap_npx * p->L * 2; ++q) cw[q] = (float)col_w[q];
-        for (int Y = 0; Y < g.subap_npx; ++Y)             // P transposed: [my][Y (16)][re, im]
+        // P transposed, [my][Y (16)][re, im], times M: the kernel's lag rows are the x
+        // autocorrelations without the factor M of FFT_M(|FFT_M(row)|^2)
+        for (int Y = 0; Y < g.subap_npx; ++Y)
             for (int mi = 0; mi < p->L; ++mi)
                 for (int c = 0; c < 2; ++c)
-                    rw[((size_t)mi * 16 + Y) * 2 + c] = (float)row_w[((size_t)Y * p->L + mi) * 2 + c];
+                    rw[((size_t)mi * 16 + Y) * 2 + c] = (float)(row_w[((size_t)Y * p->L + mi) * 2 + c] * M);
     } else {
         p->rows_alloc = use_mfma ? 16 : p->npx_pad;
         p->ts = use_mfma ? 4 * (64 / g.np_sub) : ngs_tile_size(g.np_sub);
