@@ -303,7 +303,8 @@ def bench_objects(torch, dev, steps=200):
 def bench_setups(torch, dev, frames=8):
     """Every SH block of the reference's own set-ups (ERIS, MORFEO, SCAO; workloads.REFERENCE_SH_SETUPS)
     at full size on this GPU: frames/s of the plan's forward over ``frames`` seeded screens per call
-    (host launch, resampling and the rescale pass included).  LGS blocks use a Gaussian kernel of
+    (host launch, resampling and the rescale pass included), and the time per frame of its
+    vector-Jacobian product (one cotangent image per frame).  LGS blocks use a Gaussian kernel of
     2 FFT pixels FWHM in every subaperture (their sodium-profile kernel sets are gigabytes; the
     kernel values do not change the work)."""
     from specula_amd import workloads
@@ -335,8 +336,13 @@ def bench_setups(torch, dev, frames=8):
         dt = timed(lambda: plan.forward(At, ph, ccd, 1.0, n_frames=frames), 5, torch.cuda.synchronize)
         rec.update(frames_per_s=round(frames / dt, 1), ms_per_frame=round(dt * 1e3 / frames, 4),
                    parseval=bool(plan.parseval), mask_terms=int(plan.mask_terms))
+        # adjoint: one cotangent image per frame (non-Parseval plans rerun the forward inside)
+        gccd = torch.randn((frames, g.ccd_side, g.ccd_side), dtype=torch.float32, device=dev)
+        gph = torch.empty_like(ph)
+        dv = timed(lambda: plan.vjp(At, ph, gccd, gph, 1.0, n_frames=frames), 3, torch.cuda.synchronize)
+        rec['vjp_ms_per_frame'] = round(dv * 1e3 / frames, 4)
         out[key] = rec
-        del plan, ph, ccd
+        del plan, ph, ccd, gccd, gph
     return out
 
 
@@ -367,7 +373,7 @@ def cpu_baseline_setups(setups, seconds):
 def bench_variants(torch, dev):
     """The plan's binning variants on elt_ngs_80x80 as the autotuner times them (100 frames, flat
     phase, per-call pupil statistics): register-GEMV binning on the VALU after the FFTs, FP32
-    MFMA binning, bf16-split MFMA binning, and the any-size path (radix-2 LDS FFTs on the M grid)."""
+    MFMA binning, bf16-split MFMA binning, and the any-size path (four-step FFTs on the M grid)."""
     from specula_amd import autotune, workloads
     wl = workloads.elt_ngs_80x80()
     best, times = autotune.tune_plan_variant(wl.geometry(), wl.pupil(), n_frames=100, use_cache=False)

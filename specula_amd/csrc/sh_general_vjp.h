@@ -15,10 +15,14 @@
 // One work item = (frame, subaperture); a persistent grid of 256-thread blocks strides over the
 // items; each block owns a slab of device scratch (allocated with the first VJP of a plan).
 //
-// Slab layout of one block (floats):
+// Slab layout of one block (floats; each part starts on a 16-byte boundary):
 //   Es [np][np] complex | F [M][M] complex | Gb [npx][npx] | R
-//   R: NGS  Z_k [K][npx][M]                     (Z_k = G_B Q_k^T)
+//   R: NGS  Z_k [K][npx][M] (Z_k = G_B Q_k^T), then GS [M][M] when G_S is accumulated in parts
 //      LGS  W1 [npx][np] complex, Ar [L][M] complex   (W1 = G_B Q~_k, Ar rows m_y of A')
+// The NGS product G_S = sum_k P_k^T Z_k runs from LDS: chunks of gen_vjp_rows(M) rows of P_k and
+// Z_k, each thread accumulating 4 x 4 blocks of G_S in registers.  One chunk and one term (every
+// separable mask with npx <= gen_vjp_rows(M)): X is formed from the registers; otherwise the
+// partial sums go through GS.  The LGS products stage P~_k and W1 in LDS when they fit.
 
 struct GenVjpArgs {
     const float2* tw;    // [M/2] exp(-2 pi i k / M)
@@ -30,18 +34,35 @@ struct GenVjpArgs {
     const float2* kspec; // LGS: [n_sub][L][L] kernel spectra on the lag grid, else null
 };
 
-inline int64_t gen_vjp_slab_floats(int np, int M, int npx, bool lgs, int terms) {
-    const int L = 2 * np - 1;
-    const int64_t fixed = 2LL * np * np + 2LL * M * M + (int64_t)npx * npx;
-    const int64_t r = lgs ? 2LL * npx * np + 2LL * L * M : (int64_t)terms * npx * M;
-    return (fixed + r + 63) / 64 * 64;
+// rows of P_k / Z_k per LDS chunk of the NGS product (the LDS vectors hold 2 CAP floats)
+constexpr int gen_vjp_rows(int M) { return 2 * gen_cap(M) / M; }
+constexpr bool gen_vjp_direct(int M, int npx, int terms) { return terms == 1 && npx <= gen_vjp_rows(M); }
+
+struct GenVjpSlab {
+    int64_t f, gb, r, gs, total;   // float offsets of F, Gb, R and GS; floats per block
+};
+__host__ __device__ inline GenVjpSlab gen_vjp_slab(int np, int M, int npx, bool lgs, int terms) {
+    auto a4 = [](int64_t v) { return (v + 3) & ~(int64_t)3; };
+    GenVjpSlab s;
+    s.f = a4(2LL * np * np);
+    s.gb = s.f + 2LL * M * M;
+    s.r = a4(s.gb + (int64_t)npx * npx);
+    if (lgs) {
+        s.gs = s.r;
+        s.total = s.r + 2LL * npx * np + 2LL * (2 * np - 1) * M;
+    } else {
+        s.gs = a4(s.r + (int64_t)terms * npx * M);
+        s.total = s.gs + (gen_vjp_direct(M, npx, terms) ? 0 : (int64_t)M * M);
+    }
+    s.total = (s.total + 63) / 64 * 64;           // 256-byte aligned slabs
+    return s;
 }
 
 template <int CAP>
 __global__ void __launch_bounds__(kGenThreads)
 sh_general_vjp_kernel(VjpArgs g, GenVjpArgs q, int n_frames) {
-    __shared__ float2 buf[CAP];
-    __shared__ float2 fscr[CAP + CAP / 4];
+    __shared__ __align__(16) float2 buf[CAP];
+    __shared__ __align__(16) float2 fscr[CAP + CAP / 4];
     __shared__ float2 tw[512];
     __shared__ int lit_flag;
     const int M = q.M, log2M = q.log2M, np = q.np, CH = q.ch, lch = q.log2ch;
@@ -50,12 +71,14 @@ sh_general_vjp_kernel(VjpArgs g, GenVjpArgs q, int n_frames) {
     const bool lgs = q.kspec != nullptr;
     const int L = 2 * np - 1, K = q.terms;
     for (int k = threadIdx.x; k < (M >> 1); k += blockDim.x) tw[k] = q.tw[k];
+    const GenVjpSlab so = gen_vjp_slab(np, M, npx, lgs, K);
     float* base = q.slab + (int64_t)blockIdx.x * q.slab_floats;
     float2* Es = reinterpret_cast<float2*>(base);
-    float2* F = Es + (int64_t)np * np;
-    float* Gb = reinterpret_cast<float*>(F + (int64_t)M * M);
-    float* R = Gb + (int64_t)npx * npx;
+    float2* F = reinterpret_cast<float2*>(base + so.f);
+    float* Gb = base + so.gb;
+    float* R = base + so.r;
     float* Z = R;                                             // NGS
+    float* GS = base + so.gs;                                 // NGS, parts of G_S
     float2* W1 = reinterpret_cast<float2*>(R);                // LGS
     float2* Ar = W1 + (int64_t)npx * np;                      // LGS
     const int64_t n_items = (int64_t)n_frames * g.n_sub;
@@ -151,18 +174,61 @@ sh_general_vjp_kernel(VjpArgs g, GenVjpArgs q, int n_frames) {
                 }
             }
             __syncthreads();
-            for (int e = threadIdx.x; e < M * M; e += blockDim.x) {
-                const int ky = e >> log2M, kx = e & (M - 1);
-                float gs = 0.f;
-                for (int k = 0; k < K; ++k) {
-                    const float* Pk = g.roww + (int64_t)k * q.term_stride;
-                    const float* Zk = Z + (int64_t)k * npx * M;
-                    for (int Y = 0; Y < npx; ++Y) gs = fmaf(Pk[(int64_t)Y * M + ky], Zk[(int64_t)Y * M + kx], gs);
+            // G_S[ky][kx] = sum_k sum_Y P_k[Y][ky] Z_k[Y][kx] from LDS chunks of rows of P_k and Z_k,
+            // a 4 x 4 block of G_S per thread and step
+            float* Pl = reinterpret_cast<float*>(buf);
+            float* Zl = reinterpret_cast<float*>(fscr);
+            const int yc = gen_vjp_rows(M);
+            const int lm4 = log2M - 2, nb = 1 << (2 * lm4);
+            for (int k = 0; k < K; ++k) {
+                const float* Pk = g.roww + (int64_t)k * q.term_stride;
+                const float* Zk = Z + (int64_t)k * npx * M;
+                for (int y0 = 0; y0 < npx; y0 += yc) {
+                    const int rows = min(yc, npx - y0);
+                    for (int e = threadIdx.x; e < rows * M; e += blockDim.x) {
+                        Pl[e] = Pk[(int64_t)y0 * M + e];
+                        Zl[e] = Zk[(int64_t)y0 * M + e];
+                    }
+                    __syncthreads();
+                    const bool first = k == 0 && y0 == 0, last = k == K - 1 && y0 + rows >= npx;
+                    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+                        const int ky0 = (b >> lm4) << 2, kx0 = (b & ((1 << lm4) - 1)) << 2;
+                        float acc[4][4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                            if (!first) v = *reinterpret_cast<const float4*>(GS + (int64_t)(ky0 + r) * M + kx0);
+                            acc[r][0] = v.x; acc[r][1] = v.y; acc[r][2] = v.z; acc[r][3] = v.w;
+                        }
+                        for (int Y = 0; Y < rows; ++Y) {
+                            const float4 p4 = *reinterpret_cast<const float4*>(Pl + Y * M + ky0);
+                            const float4 z4 = *reinterpret_cast<const float4*>(Zl + Y * M + kx0);
+                            const float pv[4] = {p4.x, p4.y, p4.z, p4.w}, zv[4] = {z4.x, z4.y, z4.z, z4.w};
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                                for (int c2 = 0; c2 < 4; ++c2) acc[r][c2] = fmaf(pv[r], zv[c2], acc[r][c2]);
+                        }
+                        if (last) {                                   // X = G_S conj(F) in place of F
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                float2* fp = F + (int64_t)(ky0 + r) * M + kx0;
+#pragma unroll
+                                for (int c2 = 0; c2 < 4; ++c2) {
+                                    const float2 fv = fp[c2];
+                                    fp[c2] = make_float2(acc[r][c2] * fv.x, -acc[r][c2] * fv.y);
+                                }
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                *reinterpret_cast<float4*>(GS + (int64_t)(ky0 + r) * M + kx0) =
+                                    make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+                        }
+                    }
+                    __syncthreads();
                 }
-                const float2 fv = F[e];
-                F[e] = make_float2(gs * fv.x, -gs * fv.y);
             }
-            __syncthreads();
         } else {
             // 3 (LGS): A' on the lag grid (rows m_y, columns m_x < np, zero padded to M), summed over
             // the mask terms, times the kernel spectrum; Ar = its FFT over m_x; then the FFT over m_y
@@ -170,9 +236,17 @@ sh_general_vjp_kernel(VjpArgs g, GenVjpArgs q, int n_frames) {
             const float2* ks = q.kspec + (int64_t)sub * L * L;
             for (int e = threadIdx.x; e < L * M; e += blockDim.x) Ar[e] = make_float2(0.f, 0.f);
             __syncthreads();
+            // P~_k and W1 in LDS when they fit (buf and fscr are free until the transforms below)
+            const bool p_lds = npx * L <= CAP, w_lds = npx * np <= CAP + CAP / 4;
+            float2* Wv = w_lds ? fscr : W1;
             for (int k = 0; k < K; ++k) {
-                const float2* Pt = reinterpret_cast<const float2*>(g.roww + (int64_t)k * q.term_stride);
+                const float2* Pg = reinterpret_cast<const float2*>(g.roww + (int64_t)k * q.term_stride);
                 const float2* Qt = reinterpret_cast<const float2*>(g.colw + (int64_t)k * q.term_stride);
+                const float2* Pt = Pg;
+                if (p_lds) {
+                    for (int e = threadIdx.x; e < npx * L; e += blockDim.x) buf[e] = Pg[e];
+                    Pt = buf;
+                }
                 for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {     // W1 = G_B Q~ (lag columns m_x)
                     const int Y = e / np, mx = e - (e / np) * np;
                     float ar = 0.f, ai = 0.f;
@@ -182,15 +256,15 @@ sh_general_vjp_kernel(VjpArgs g, GenVjpArgs q, int n_frames) {
                         ar = fmaf(gb, qq.x, ar);
                         ai = fmaf(gb, qq.y, ai);
                     }
-                    W1[e] = make_float2(ar, ai);
+                    Wv[e] = make_float2(ar, ai);
                 }
                 __syncthreads();
                 for (int e = threadIdx.x; e < L * np; e += blockDim.x) {       // A' += c_mx P~^T W1
                     const int ly = e / np, mx = e - (e / np) * np;
                     float ar = 0.f, ai = 0.f;
                     for (int Y = 0; Y < npx; ++Y) {
-                        const float2 p = Pt[(int64_t)Y * L + ly];
-                        const float2 w = W1[Y * np + mx];
+                        const float2 p = Pt[Y * L + ly];
+                        const float2 w = Wv[Y * np + mx];
                         ar = fmaf(p.x, w.x, fmaf(-p.y, w.y, ar));
                         ai = fmaf(p.x, w.y, fmaf(p.y, w.x, ai));
                     }

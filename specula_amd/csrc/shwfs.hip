@@ -3204,7 +3204,7 @@ int launch_general_vjp(shwfs_plan* p, const VjpArgs& a, int n_frames, hipStream_
         SHWFS_HIP_CHECK(hipMemcpy(p->d_tw, t.data(), t.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
     if (!p->d_vjp_slab) {
-        const int64_t sf = gen_vjp_slab_floats(g.np_sub, M, g.subap_npx, g.lgs != 0, p->terms);
+        const int64_t sf = gen_vjp_slab(g.np_sub, M, g.subap_npx, g.lgs != 0, p->terms).total;
         const int64_t cap = ((int64_t)2 << 30) / (sf * (int64_t)sizeof(float));
         const int64_t need = (int64_t)g.max_frames * p->n_sub;        // items of the largest call
         const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(1024, cap), need));

@@ -200,6 +200,19 @@ class SHPlan:
             self._h, n_frames, _native.ptr(A), a_frame_stride, _native.ptr(phase), ef * ef,
             float(S0), _native.ptr(out), ccd * ccd, _native.stream_handle(stream)), 'shwfs_forward')
 
+    def vjp(self, A, phase, grad_ccd, grad_phase, S0, n_frames=1, a_frame_stride=None, stream=None):
+        """dL/dphase (n_frames, ef, ef) from dL/d(CCD intensity) (n_frames, ccd, ccd) at the
+        forward's inputs (shwfs_vjp; every plan kind, resampled plans on the input grid)."""
+        a_frame_stride = self._check(A, phase, grad_ccd, n_frames, a_frame_stride)
+        ef, ccd = self.geom.ef_size, self.geom.ccd_side
+        if (grad_phase.dtype != torch.float32 or not grad_phase.is_contiguous() or not grad_phase.is_cuda
+                or grad_phase.numel() != n_frames * ef * ef):
+            raise ValueError('grad_phase must be a contiguous float32 device tensor of n_frames x ef x ef')
+        _native.check(_native.lib().shwfs_vjp(
+            self._h, n_frames, _native.ptr(A), a_frame_stride, _native.ptr(phase), ef * ef, float(S0),
+            _native.ptr(grad_ccd), ccd * ccd, _native.ptr(grad_phase), ef * ef, _native.stream_handle(stream)),
+            'shwfs_vjp')
+
     def __del__(self):
         h = getattr(self, '_h', None)
         if h is not None and h.value:
