@@ -1342,40 +1342,80 @@ __global__ void resample_kernel(const float* __restrict__ in, int64_t in_stride,
 // Transpose of resample_kernel (the resampling, edge extrapolation included, is linear in the
 // phase): gin += R^T gout.  Each output pixel recomputes its taps exactly as the forward does and
 // scatters its four bilinear weights times gout into the input pixels it read (2 w to the first
-// and -w to the second source of an extrapolated pixel).  Float atomics in L2: the summation order
-// is not fixed, so results agree with a fixed-order sum to FP32 rounding.  gin must be zeroed.
-__global__ void resample_vjp_kernel(const float* __restrict__ gout, int64_t gout_stride, const int2* __restrict__ ext,
-                                    float* __restrict__ gin, int64_t gin_stride, int ef_in, int ef_out,
-                                    ResampleMap m) {
+// and -w to the second source of an extrapolated pixel, at most 3 pixels away).  A block takes a
+// tile of kRvjTY x kRvjTX output pixels; when the resampling magnifies (fov_ovs >= 1) their taps
+// fall in a small input rectangle (the affine map takes its extremes at the tile corners, floor
+// and clamp are monotonic), so the contributions accumulate in LDS and each touched input pixel
+// receives one global atomic per tile instead of one per contribution.  A rectangle larger than
+// the LDS budget, and any contribution outside it, goes to global atomics directly.  Float
+// atomics: the summation order is not fixed, so results agree with a fixed-order sum to FP32
+// rounding.  gin must be zeroed.  Grid (tiles, frames), 256 threads.
+constexpr int kRvjTY = 16, kRvjTX = 128, kRvjLds = 8192;
+__global__ void __launch_bounds__(256)
+resample_vjp_kernel(const float* __restrict__ gout, int64_t gout_stride, const int2* __restrict__ ext,
+                    float* __restrict__ gin, int64_t gin_stride, int ef_in, int ef_out, ResampleMap m, int tiles_x) {
+    __shared__ float acc[kRvjLds];
     const int f = blockIdx.y;
     const float* go = gout + f * gout_stride;
     float* gi = gin + f * gin_stride;
-    const int64_t n_out = (int64_t)ef_out * ef_out;
+    const int ty = blockIdx.x / tiles_x, tx = blockIdx.x - (blockIdx.x / tiles_x) * tiles_x;
+    const int Y0 = ty * kRvjTY, X0 = tx * kRvjTX;
+    const int Y1 = min(Y0 + kRvjTY, ef_out) - 1, X1 = min(X0 + kRvjTX, ef_out) - 1;
     const double lim = ef_in - 1;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n_out; q += (int64_t)gridDim.x * blockDim.x) {
-        const float gq = go[q];
+    double ylo = lim, yhi = 0.0, xlo = lim, xhi = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int Yc = (k & 1) ? Y1 : Y0, Xc = (k & 2) ? X1 : X0;
+        const double y = fmin(fmax(m.yy * Yc + m.yx * Xc + m.y0, 0.0), lim);
+        const double x = fmin(fmax(m.xy * Yc + m.xx * Xc + m.x0, 0.0), lim);
+        ylo = fmin(ylo, y); yhi = fmax(yhi, y);
+        xlo = fmin(xlo, x); xhi = fmax(xhi, x);
+    }
+    const int r0 = max(min((int)floor(ylo), ef_in - 2) - 3, 0);
+    const int r1 = min(min((int)floor(yhi), ef_in - 2) + 4, ef_in - 1);
+    const int c0 = max(min((int)floor(xlo), ef_in - 2) - 3, 0);
+    const int c1 = min(min((int)floor(xhi), ef_in - 2) + 4, ef_in - 1);
+    const int fw = c1 - c0 + 1;
+    const int fn = (r1 - r0 + 1) * fw;
+    const bool local = fn <= kRvjLds;                                  // block-uniform
+    if (local)
+        for (int e = threadIdx.x; e < fn; e += blockDim.x) acc[e] = 0.f;
+    __syncthreads();
+    auto add = [&](int r, int c, float v) {
+        if (local && r >= r0 && r <= r1 && c >= c0 && c <= c1) atomicAdd(&acc[(r - r0) * fw + (c - c0)], v);
+        else atomicAdd(gi + (int64_t)r * ef_in + c, v);
+    };
+    for (int e = threadIdx.x; e < kRvjTY * kRvjTX; e += blockDim.x) {
+        const int Y = Y0 + e / kRvjTX, X = X0 + (e - (e / kRvjTX) * kRvjTX);
+        if (Y > Y1 || X > X1) continue;
+        const float gq = go[(int64_t)Y * ef_out + X];
         if (gq == 0.f) continue;
-        const int Y = (int)(q / ef_out), X = (int)(q - (int64_t)Y * ef_out);
         const double y = fmin(fmax(m.yy * Y + m.yx * X + m.y0, 0.0), lim);
         const double x = fmin(fmax(m.xy * Y + m.xx * X + m.x0, 0.0), lim);
         const int i0 = min((int)floor(y), ef_in - 2), j0 = min((int)floor(x), ef_in - 2);
-        const float ty = (float)(y - i0), tx = (float)(x - j0);
-        const float w[4] = {(1.f - ty) * (1.f - tx), (1.f - ty) * tx, ty * (1.f - tx), ty * tx};
+        const float ty_ = (float)(y - i0), tx_ = (float)(x - j0);
+        const float w[4] = {(1.f - ty_) * (1.f - tx_), (1.f - ty_) * tx_, ty_ * (1.f - tx_), ty_ * tx_};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const float wg = w[k] * gq;
             if (wg == 0.f) continue;
-            const int p = (i0 + (k >> 1)) * ef_in + j0 + (k & 1);
-            const int2 e = ext ? ext[p] : make_int2(-1, -1);
-            if (e.x < 0) {
-                atomicAdd(gi + p, wg);
-            } else if (e.y < 0) {
-                atomicAdd(gi + e.x, wg);
+            const int r = i0 + (k >> 1), c = j0 + (k & 1);
+            const int2 src = ext ? ext[r * ef_in + c] : make_int2(-1, -1);
+            if (src.x < 0) {
+                add(r, c, wg);
+            } else if (src.y < 0) {
+                add(src.x / ef_in, src.x - (src.x / ef_in) * ef_in, wg);
             } else {
-                atomicAdd(gi + e.x, 2.f * wg);
-                atomicAdd(gi + e.y, -wg);
+                add(src.x / ef_in, src.x - (src.x / ef_in) * ef_in, 2.f * wg);
+                add(src.y / ef_in, src.y - (src.y / ef_in) * ef_in, -wg);
             }
         }
+    }
+    if (!local) return;                                                 // block-uniform
+    __syncthreads();
+    for (int e = threadIdx.x; e < fn; e += blockDim.x) {
+        const float v = acc[e];
+        if (v != 0.f) atomicAdd(gi + (int64_t)(r0 + e / fw) * ef_in + c0 + (e - (e / fw) * fw), v);
     }
 }
 
@@ -3326,9 +3366,9 @@ int vjp_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_stride, cons
         hipLaunchKernelGGL(zero_frames_kernel, dim3((unsigned)zb, n_frames), dim3(256), 0, st, grad_out, gout_stride,
                            in_px);
         SHWFS_LAUNCH_CHECK();
-        const int64_t rb = std::min<int64_t>((ef_px + 255) / 256, 2048);
-        hipLaunchKernelGGL(resample_vjp_kernel, dim3((unsigned)rb, n_frames), dim3(256), 0, st, gs, gss, p->d_ext,
-                           grad_out, gout_stride, p->ef_in, g.ef_size, p->rmap);
+        const int tiles_x = (g.ef_size + kRvjTX - 1) / kRvjTX, tiles_y = (g.ef_size + kRvjTY - 1) / kRvjTY;
+        hipLaunchKernelGGL(resample_vjp_kernel, dim3((unsigned)(tiles_x * tiles_y), n_frames), dim3(256), 0, st, gs,
+                           gss, p->d_ext, grad_out, gout_stride, p->ef_in, g.ef_size, p->rmap, tiles_x);
         SHWFS_LAUNCH_CHECK();
     }
     return SHWFS_OK;

@@ -154,3 +154,27 @@ def test_lgs_adjoint_im_matches_richardson_pushpull():
     print(f'LGS np 16 calibration IM (20 Zernike): adjoint vs push-pull 10 nm {r10:.2e}, Richardson {rr:.2e}',
           flush=True)
     assert rr < 1e-3
+
+
+@pytest.mark.parametrize('case', [CASES[0], CASES[1], CASES[3]], ids=[CASES[i][0] for i in (0, 1, 3)])
+def test_per_frame_pupils_equal_single_frame_calls(case):
+    """Two frames with their own pupil (one with a dark quadrant) and phase in one plan-level VJP
+    equal two one-frame calls (per-frame flux scale and cotangent offset)."""
+    g, go, A, kern, sen, ph = _setup(case, max_frames=4)
+    ef, ccd = g.ef_size, g.ccd_side
+    rng = np.random.default_rng(13)
+    A2 = np.stack([A, A]).astype(np.float32)
+    A2[1, :ef // 2, :ef // 2] = 0.0
+    ph2 = np.stack([ph, np.roll(ph, 3, axis=1) * 0.7]).astype(np.float32)
+    At = torch.tensor(A2, device=DEV)
+    pt = torch.tensor(ph2, device=DEV)
+    gc = torch.tensor(rng.standard_normal((2, ccd, ccd)), dtype=torch.float32, device=DEV)
+    both = torch.empty((2, ef, ef), dtype=torch.float32, device=DEV)
+    sen.plan.vjp(At, pt, gc, both, 1.0, n_frames=2, a_frame_stride=ef * ef)
+    for f in range(2):
+        one = torch.empty((1, ef, ef), dtype=torch.float32, device=DEV)
+        sen.plan.vjp(At[f].contiguous(), pt[f:f + 1].contiguous(), gc[f:f + 1].contiguous(), one, 1.0, n_frames=1)
+        err = ((both[f] - one[0]).abs().max() / one.abs().max()).item()
+        print(f'{case[0]}: frame {f}: two-frame call vs one-frame call {err:.2e}', flush=True)
+        assert err < 1e-5
+    assert float(both[1][:ef // 2, :ef // 2].abs().max()) == 0.0 or g.needs_interpolation
