@@ -246,7 +246,9 @@ def bench_multi(torch, dev, reps=20):
 def bench_objects(torch, dev, steps=200):
     """One simulation-loop step of the drop-in objects at elt_ngs_80x80: ElectricField phase
     update, SH (check_ready / trigger / post_trigger) and ShSlopec on the SH pixels, one frame per
-    step, host overhead included; eager and with the SH graph replayed."""
+    step, host overhead included: eager, with ``use_graph`` (the object keeps a graph only when it
+    holds at least ``graph_min_nodes`` launches; the SH trigger is one launch, so it runs direct)
+    and with the one-launch graph kept and replayed (``graph_min_nodes`` 1)."""
     from specula_amd import workloads
     from specula_amd.data_objects.electric_field import ElectricField
     from specula_amd.data_objects.pixels import Pixels
@@ -260,8 +262,11 @@ def bench_objects(torch, dev, steps=200):
     ef.A[:] = torch.tensor(wl.pupil(), device=dev)
     frames = torch.tensor(wl.frames(8).astype(np.float32), device=dev)
     out = {}
-    for use_graph in (False, True):
+    for mode in ('eager', 'use_graph', 'graph_forced'):
+        use_graph = mode != 'eager'
         sh = SH(**wl.sh_kwargs, target_device_idx=0)
+        if mode == 'graph_forced':
+            sh.graph_min_nodes = 1
         if wl.fov_resolution_arcsec:
             sh._fov_resolution_arcsec = wl.fov_resolution_arcsec
         sh.inputs['in_ef'].set(ef)
@@ -296,7 +301,10 @@ def bench_objects(torch, dev, steps=200):
         for t in range(12, 12 + steps):
             step(t)
         torch.cuda.synchronize()
-        out['graph_us' if use_graph else 'eager_us'] = round((time.perf_counter() - t0) / steps * 1e6, 1)
+        out[mode + '_us'] = round((time.perf_counter() - t0) / steps * 1e6, 1)
+        if use_graph:
+            out[mode + '_graph_kept'] = sh.cuda_graph is not None
+            out['sh_graph_nodes'] = sh.graph_nodes
     return {'workload': 'elt_ngs_80x80, 1 frame per step', **out}
 
 

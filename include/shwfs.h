@@ -364,6 +364,9 @@ int shwfs_graph_begin(void* stream);
 int shwfs_graph_end(void* stream, shwfs_graph** out);
 int shwfs_graph_launch(shwfs_graph* graph, void* stream);
 int shwfs_graph_destroy(shwfs_graph* graph);
+/* Number of nodes (kernel launches) of a captured graph: a replay pays off only when it replaces
+ * enough direct launches (the processing objects capture only above a node-count threshold). */
+int shwfs_graph_node_count(const shwfs_graph* graph, int64_t* n_nodes);
 
 #ifdef __cplusplus
 }

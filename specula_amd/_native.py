@@ -25,6 +25,7 @@ EXPORTS = (
     'shwfs_batch_destroy',
     'shccd_integrate', 'shccd_readout', 'shccd_readout_cte',
     'shwfs_graph_begin', 'shwfs_graph_end', 'shwfs_graph_launch', 'shwfs_graph_destroy',
+    'shwfs_graph_node_count',
     'shgemm_f32',
 )
 
@@ -109,6 +110,7 @@ def _declare(lib):
         'shwfs_graph_end': (_I, [_P, ctypes.POINTER(_P)]),
         'shwfs_graph_launch': (_I, [_P, _P]),
         'shwfs_graph_destroy': (_I, [_P]),
+        'shwfs_graph_node_count': (_I, [_P, ctypes.POINTER(_I64)]),
         'shgemm_f32': (_I, [_I, _I, _I, _F, _P, _I64, _P, _I64, _I, _F, _P, _P, _I64, _P]),
     })
     for name, (res, args) in sig.items():
@@ -205,6 +207,14 @@ class HipGraph:
         if self._h is None:
             raise RuntimeError('no graph captured')
         check(lib().shwfs_graph_launch(self._h, stream_handle(stream)), 'shwfs_graph_launch')
+
+    def node_count(self):
+        """Kernel launches (graph nodes) that one replay issues."""
+        if self._h is None:
+            raise RuntimeError('no graph captured')
+        n = ctypes.c_int64()
+        check(lib().shwfs_graph_node_count(self._h, ctypes.byref(n)), 'shwfs_graph_node_count')
+        return n.value
 
     def destroy(self):
         if self._h is not None and self._h.value:

@@ -197,6 +197,8 @@ class BaseProcessingObj(BaseTimeObj):
     (``build_stream(capture=True)``), else calls ``trigger_code()`` on the current stream."""
 
     _streams = {}
+    # smallest captured graph that replaces its launches (see capture_stream)
+    graph_min_nodes = 16
 
     def __init__(self, target_device_idx=None, precision=None):
         super().__init__(target_device_idx=target_device_idx, precision=precision)
@@ -206,6 +208,7 @@ class BaseProcessingObj(BaseTimeObj):
         self.stream = None
         self.ready = False
         self.cuda_graph = None
+        self.graph_nodes = 0
         self.inputs = {}
         self.local_inputs = {}
         self.last_seen = {}
@@ -276,14 +279,24 @@ class BaseProcessingObj(BaseTimeObj):
 
     def capture_stream(self):
         """One eager warm-up of trigger_code() on the object's stream, then its library calls
-        captured into a HIP graph by the native library (shwfs_graph_*)."""
+        captured into a HIP graph by the native library (shwfs_graph_*).  The graph is kept only
+        when it holds at least ``graph_min_nodes`` launches: on ROCm a graph launch costs the host
+        ~15 us and slows the next direct launch on the stream by ~12 us, against ~2 us per direct
+        kernel launch, so a graph of a few launches replays slower than the launches themselves.
+        A declined capture leaves ``cuda_graph`` None and ``trigger()`` runs trigger_code()."""
         from specula_amd import _native
         with torch.cuda.device(self.target_device_idx):
             self.stream.wait_stream(torch.cuda.current_stream())     # the warm-up reads the inputs
             with torch.cuda.stream(self.stream):
                 self.trigger_code()
             self.stream.synchronize()
-            self.cuda_graph = _native.HipGraph().capture(self.trigger_code, self.stream)
+            graph = _native.HipGraph().capture(self.trigger_code, self.stream)
+            self.graph_nodes = graph.node_count()
+            if self.graph_nodes < self.graph_min_nodes:
+                graph.destroy()
+                self.cuda_graph = None
+            else:
+                self.cuda_graph = graph
 
     def check_ready(self, t):
         self.current_time = t

@@ -3820,6 +3820,14 @@ int shwfs_graph_launch(shwfs_graph* g, void* stream) {
     return SHWFS_OK;
 }
 
+int shwfs_graph_node_count(const shwfs_graph* g, int64_t* n_nodes) {
+    if (!g || !g->graph || !n_nodes) return set_error(SHWFS_ERR_ARG, "null argument");
+    size_t n = 0;
+    SHWFS_HIP_CHECK(hipGraphGetNodes(g->graph, nullptr, &n));
+    *n_nodes = (int64_t)n;
+    return SHWFS_OK;
+}
+
 int shwfs_graph_destroy(shwfs_graph* g) {
     if (!g) return SHWFS_OK;
     if (g->exec) (void)hipGraphExecDestroy(g->exec);

@@ -82,6 +82,7 @@ def test_graph_replay_matches_eager():
     eager.setup()
     graph.setup()
     graph.use_graph = True
+    graph.graph_min_nodes = 1                       # keep the graph whatever its size
     for t, ph in enumerate(frames, start=1):
         ef.phaseInNm[:] = torch.tensor(ph, device='cuda:0')
         ef.generation_time = t
@@ -108,6 +109,29 @@ def test_graph_replay_matches_eager():
     assert graph.cuda_graph is g0                    # same storage: no re-capture
     img = graph.outputs['out_i'].i.cpu().numpy()
     assert np.all(img[: img.shape[0] // 3 - 8] == 0)        # rows of subapertures under the dark band
+
+
+def test_small_graph_declined():
+    """use_graph on a one-launch trigger: the capture holds one node, fewer than graph_min_nodes,
+    so the object keeps no graph and runs its trigger directly, with the same image."""
+    wl = workloads.scao_ngs_20x20()
+    ef = ElectricField(wl.ef_size, wl.ef_size, wl.pixel_pitch, S0=1.0, target_device_idx=0)
+    ef.A[:] = torch.tensor(wl.pupil(), device='cuda:0')
+    eager, graph = make_sh(wl), make_sh(wl)
+    for obj in (eager, graph):
+        obj.inputs['in_ef'].set(ef)
+        obj.setup()
+    graph.use_graph = True
+    for t, ph in enumerate(wl.frames(2), start=1):
+        ef.phaseInNm[:] = torch.tensor(ph, device='cuda:0')
+        ef.generation_time = t
+        for obj in (graph, eager):
+            obj.check_ready(t)
+            obj.trigger()
+            obj.post_trigger()
+        assert torch.equal(graph.outputs['out_i'].i, eager.outputs['out_i'].i)
+    assert graph.graph_nodes == 1 and graph.graph_min_nodes > 1
+    assert graph.cuda_graph is None
 
 
 def test_setup_errors():
