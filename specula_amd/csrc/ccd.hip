@@ -335,18 +335,62 @@ __global__ void __launch_bounds__(256) ccd_readout_kernel(ReadoutArgs a, int vec
     }
 }
 
-// Readout with photon noise.  Rejection sampling makes the number of tries per pixel random; so
-// that a wave does not wait for its slowest lane on every pixel, each lane walks its own list of
-// pixels and one loop iteration is one try of whatever pixel the lane is on: a lane that accepts
-// moves to its next pixel while the others keep trying.  Every iteration makes exactly one
-// generator call (counter = try index) so that the call is not a divergent branch; try 0 also
-// supplies the readout normal (words z, w).
+// Samplers of the Poisson photon count, shared by the photon kernels.  PTRS constants of a pixel
+// (lam >= 10), in Hormann's notation.
+struct PtrsConst {
+    float b, a, log_invalpha, vr, log_lam, inv_lam;
+};
+
+__device__ __forceinline__ PtrsConst ptrs_const(float lam) {
+    PtrsConst c;
+    const float slam = __builtin_amdgcn_sqrtf(lam);
+    c.b = 0.931f + 2.53f * slam;
+    c.a = -0.059f + 0.02483f * c.b;
+    c.log_invalpha = fast_log(1.1239f + 1.1328f * fast_rcp(c.b - 3.4f));
+    c.vr = 0.9277f - 3.6224f * fast_rcp(c.b - 2.0f);
+    c.log_lam = fast_log(lam);
+    c.inv_lam = fast_rcp(lam);
+    return c;
+}
+
+// One PTRS try with the uniforms u1, u2 in (0, 1]: the candidate count k, true when accepted.
+__device__ __forceinline__ bool ptrs_try(const PtrsConst& c, float lam, float u1, float u2, double& k) {
+    const float U = u1 - 0.5f, V = u2;
+    const float us = 0.5f - fabsf(U);
+    k = floor((double)((2.0f * c.a * fast_rcp(us) + c.b) * U) + (double)lam + 0.43);
+    bool done = us >= 0.07f && V <= c.vr;
+    if (!done && !(k < 0.0 || (us < 0.013f && V > us))) {
+        const float lhs = fast_log(V) + c.log_invalpha - fast_log(c.a * fast_rcp(us * us) + c.b);
+        const float rhs = k < 32.0 ? -lam + (float)k * c.log_lam - (float)kLogFact[(int)k]
+                                   : log_pmf_large(k, lam, c.inv_lam);
+        done = lhs <= rhs;
+    }
+    return done;
+}
+
+// Inversion of the CDF for 0 < lam < 10 with one 64-bit uniform made of the words w0, w1 (the
+// search ends within 100 steps unless U is within 1e-40 of 1).
+__device__ __forceinline__ double poisson_inversion(float lam, uint32_t w0, uint32_t w1) {
+    const float enlam = __builtin_amdgcn_exp2f(-1.44269504089f * lam);
+    const double U = ((double)w0 * 4294967296.0 + (double)w1 + 1.0) * (1.0 / 18446744073709551616.0);
+    double pk = (double)enlam, F = pk, k = 0.0;
+    for (int s = 0; s < 100 && U > F; ++s) {
+        k += 1.0;
+        pk *= (double)(lam * fast_rcp((float)k));
+        F += pk;
+    }
+    return k;
+}
+
+// Readout with photon noise.  Every pixel draws from the generator keyed by (pixel, readout counter,
+// try): try 0 gives the inversion uniform or the first PTRS try (words x, y) and the readout normal
+// (words z, w), further PTRS tries use calls 1, 2, ...  So a pixel's value does not depend on which
+// lane samples it or in which order, and the two kernels below give the same values bit for bit.
 //
 // Each wave owns contiguous tiles of kPhotonTile pixels (tile t of the frame, waves striding over
-// the tiles); lane l takes pixels l, l + 64, ... of the tile and parks each accepted value in the
-// wave's LDS slice, and when every lane of the wave has finished the tile the wave stores it with
-// coalesced whole-line writes (the lanes accept their pixels at different times, so storing each
-// value when it is accepted would write partial cache lines).
+// the tiles), parks each pixel's value in its LDS slice when the pixel is done, and stores the tile
+// with coalesced whole-line writes once every lane is through (the lanes finish their pixels at
+// different times, so storing each value when it is done would write partial cache lines).
 constexpr int kPhotonPerLane = 16;
 constexpr int kPhotonTile = 64 * kPhotonPerLane;
 
@@ -356,8 +400,15 @@ __device__ __forceinline__ void ccd_wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <bool FULL>
-__global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
+// rank of this lane among the lanes set in the wave mask m
+__device__ __forceinline__ int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Full variant (binning, CIC, EM gain, pixel gains): lane l walks pixels l, l + 64, ... of the
+// tile, one generator call per loop iteration, and a lane that accepts moves to its next pixel while
+// the others keep trying.
+__global__ void __launch_bounds__(256) ccd_photon_full_kernel(ReadoutArgs a) {
     __shared__ float stage[4][kPhotonTile];
     const shccd_params& p = a.p;
     const int f = blockIdx.y;
@@ -375,32 +426,18 @@ __global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
         bool fresh = true;
         uint64_t px = 0;
         uint32_t wz = 0, ww = 0;
-        float lam = 0.0f, inv_lam = 0.0f, enlam = 0.0f, a_ = 0.0f, b_ = 0.0f, vr = 0.0f;
-        float log_invalpha = 0.0f, log_lam = 0.0f;
+        float lam = 0.0f;
+        PtrsConst pc{};
         double k = 0.0;
         uint32_t tries = 0;
-        float next_raw = 0.0f;                 // signal of the lane's next pixel, loaded one pixel ahead
-        if (!FULL && i < tn) next_raw = raw_pixel(a, f, t0 + i);
         while (i < tn) {
             const int q = t0 + i;
             if (fresh) {
                 px = (uint64_t)f * (uint64_t)n_out + (uint64_t)q;
-                if (FULL) {
-                    lam = load_pixel<true>(a, f, q, px, ctr);
-                } else {
-                    lam = signal_of<false>(a, next_raw, px, ctr);
-                    if (i + 64 < tn) next_raw = raw_pixel(a, f, q + 64);
-                }
+                lam = load_pixel<true>(a, f, q, px, ctr);
                 tries = 0;
                 k = 0.0;
-                const float slam = __builtin_amdgcn_sqrtf(lam);
-                b_ = 0.931f + 2.53f * slam;
-                a_ = -0.059f + 0.02483f * b_;
-                log_invalpha = fast_log(1.1239f + 1.1328f * fast_rcp(b_ - 3.4f));
-                vr = 0.9277f - 3.6224f * fast_rcp(b_ - 2.0f);
-                log_lam = fast_log(lam);
-                inv_lam = fast_rcp(lam);
-                enlam = __builtin_amdgcn_exp2f(-1.44269504089f * lam);
+                pc = ptrs_const(lam);
                 fresh = false;
             }
             const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, tries}, k0, k1);
@@ -408,49 +445,163 @@ __global__ void __launch_bounds__(256) ccd_photon_kernel(ReadoutArgs a) {
                 wz = w.z;
                 ww = w.w;
             }
-            const float u1 = u01(w.x), u2 = u01(w.y);
             ++tries;
             bool done;
             if (!(lam > 0.0f)) {
                 k = 0.0;
                 done = true;
             } else if (lam < 10.0f) {
-                // inversion of the CDF with one 64-bit uniform: the lane finishes its pixel in this
-                // iteration (the search ends within 100 steps for lam < 10 unless U is within 1e-40
-                // of 1)
-                const double U = ((double)w.x * 4294967296.0 + (double)w.y + 1.0) * (1.0 / 18446744073709551616.0);
-                double pk = (double)enlam, F = pk;
-                k = 0.0;
-                for (int s = 0; s < 100 && U > F; ++s) {
-                    k += 1.0;
-                    pk *= (double)(lam * fast_rcp((float)k));
-                    F += pk;
-                }
+                k = poisson_inversion(lam, w.x, w.y);
                 done = true;
-            } else {                                                   // PTRS
-                const float U = u1 - 0.5f, V = u2;
-                const float us = 0.5f - fabsf(U);
-                k = floor((double)((2.0f * a_ * fast_rcp(us) + b_) * U) + (double)lam + 0.43);
-                done = us >= 0.07f && V <= vr;
-                if (!done && !(k < 0.0 || (us < 0.013f && V > us))) {
-                    const float lhs = fast_log(V) + log_invalpha - fast_log(a_ * fast_rcp(us * us) + b_);
-                    const float rhs = k < 32.0 ? -lam + (float)k * log_lam - (float)kLogFact[(int)k]
-                                               : log_pmf_large(k, lam, inv_lam);
-                    done = lhs <= rhs;
-                }
+            } else {
+                done = ptrs_try(pc, lam, u01(w.x), u01(w.y), k);
             }
             if (!done && tries >= 512u) {                              // never reached in practice
                 k = fmax(k, 0.0);
                 done = true;
             }
             if (done) {
-                st[i] = finish_value<FULL>(a, q, px, ctr, k, true, normal_from(wz, ww));
+                st[i] = finish_value<true>(a, q, px, ctr, k, true, normal_from(wz, ww));
                 i += 64;
                 fresh = true;
             }
         }
         ccd_wave_lds_sync();                   // every lane of the wave has parked its pixels
         for (int j = lane; j < tn; j += 64) out[t0 + j] = st[j];
+        ccd_wave_lds_sync();                   // the slice is read before the next tile reuses it
+    }
+}
+
+// Lean variant (the common case: no binning, CIC, EM gain or pixel gains).  A wave that mixes
+// pixels without signal (k = 0), pixels sampled by inversion (0 < lam < 10) and PTRS pixels pays for
+// all three code paths on every try, so the tile's pixels are first listed by class (wave ballots
+// into an LDS list) and each class then runs with every lane on it.  PTRS lanes take the next list
+// entry as soon as they accept, so a lane whose pixel is rejected does not hold the others back.
+// `vec`: 16-byte aligned frames, the tile is stored with float4 writes.
+__global__ void __launch_bounds__(256) ccd_photon_lean_kernel(ReadoutArgs a, int vec) {
+    __shared__ __align__(16) float val[4][kPhotonTile];   // signal of each tile pixel, then its value
+    __shared__ uint16_t order[4][kPhotonTile];     // tile pixels by class: zero | inversion | PTRS
+    const shccd_params& p = a.p;
+    const int f = blockIdx.y;
+    const int n_out = p.out_rows * p.out_cols;
+    const uint32_t ctr = (uint32_t)a.counter;
+    const uint32_t k0 = key0_of(p), k1 = key1_of(p);
+    const bool readout = (p.flags & SHCCD_READOUT) != 0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    float* vw = val[wave];
+    uint16_t* ow = order[wave];
+    float* out = a.pixels + f * a.out_stride;
+    const uint64_t fbase = (uint64_t)f * (uint64_t)n_out;
+    const int n_tiles = (n_out + kPhotonTile - 1) / kPhotonTile;
+    for (int tile = blockIdx.x * 4 + wave; tile < n_tiles; tile += gridDim.x * 4) {
+        const int t0 = tile * kPhotonTile;
+        const int tn = min(kPhotonTile, n_out - t0);
+        // signals, and the class of each of the lane's pixels lane + 64 j (2 bits each; 3: none)
+        float raw[kPhotonPerLane];
+#pragma unroll
+        for (int j = 0; j < kPhotonPerLane; ++j) raw[j] = lane + 64 * j < tn ? raw_pixel(a, f, t0 + lane + 64 * j) : 0.0f;
+        uint32_t cls = 0;
+        int n0 = 0, n1 = 0;
+#pragma unroll
+        for (int j = 0; j < kPhotonPerLane; ++j) {
+            const int i = lane + 64 * j;
+            uint32_t c = 3u;
+            if (i < tn) {
+                const float lam = signal_of<false>(a, raw[j], fbase + (uint64_t)(t0 + i), ctr);
+                vw[i] = lam;
+                c = !(lam > 0.0f) ? 0u : lam < 10.0f ? 1u : 2u;
+            }
+            cls |= c << (2 * j);
+            n0 += __popcll(__ballot(c == 0u));
+            n1 += __popcll(__ballot(c == 1u));
+        }
+        int e0 = 0, e1 = n0, e2 = n0 + n1;          // next list entry of each class
+#pragma unroll
+        for (int j = 0; j < kPhotonPerLane; ++j) {
+            const uint32_t c = (cls >> (2 * j)) & 3u;
+            const uint64_t m0 = __ballot(c == 0u), m1 = __ballot(c == 1u), m2 = __ballot(c == 2u);
+            const uint16_t i = (uint16_t)(lane + 64 * j);
+            if (c == 0u) ow[e0 + lane_rank(m0)] = i;
+            else if (c == 1u) ow[e1 + lane_rank(m1)] = i;
+            else if (c == 2u) ow[e2 + lane_rank(m2)] = i;
+            e0 += __popcll(m0);
+            e1 += __popcll(m1);
+            e2 += __popcll(m2);
+        }
+        const int n_all = e2;
+        ccd_wave_lds_sync();
+        // no signal: k = 0 (the generator only for the readout normal)
+        for (int e = lane; e < n0; e += 64) {
+            const int i = ow[e], q = t0 + i;
+            const uint64_t px = fbase + (uint64_t)q;
+            float nrm = 0.0f;
+            if (readout) {
+                const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, 0u}, k0, k1);
+                nrm = normal_from(w.z, w.w);
+            }
+            vw[i] = finish_value<false>(a, q, px, ctr, 0.0, true, nrm);
+        }
+        // 0 < lam < 10: inversion of the CDF, one generator call
+        for (int e = n0 + lane; e < n0 + n1; e += 64) {
+            const int i = ow[e], q = t0 + i;
+            const uint64_t px = fbase + (uint64_t)q;
+            const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, 0u}, k0, k1);
+            const double k = poisson_inversion(vw[i], w.x, w.y);
+            vw[i] = finish_value<false>(a, q, px, ctr, k, true, normal_from(w.z, w.w));
+        }
+        // lam >= 10: PTRS, one try per iteration; a lane that accepts takes the next entry
+        {
+            int e = n0 + n1 + lane;                    // this lane's entry (>= n_all: none left)
+            int next = n0 + n1 + 64;                   // first entry nobody has taken
+            bool fresh = true;
+            int i = 0;
+            uint64_t px = 0;
+            float lam = 0.0f;
+            PtrsConst pc{};
+            uint32_t tries = 0, wz = 0, ww = 0;
+            while (__any(e < n_all)) {
+                bool done = false;
+                if (e < n_all) {
+                    if (fresh) {
+                        i = ow[e];
+                        px = fbase + (uint64_t)(t0 + i);
+                        lam = vw[i];
+                        pc = ptrs_const(lam);
+                        tries = 0;
+                        fresh = false;
+                    }
+                    const U4 w = philox(U4{(uint32_t)px, (uint32_t)(px >> 32), ctr, tries}, k0, k1);
+                    if (tries == 0) {
+                        wz = w.z;
+                        ww = w.w;
+                    }
+                    ++tries;
+                    double k;
+                    done = ptrs_try(pc, lam, u01(w.x), u01(w.y), k);
+                    if (!done && tries >= 512u) {                  // never reached in practice
+                        k = fmax(k, 0.0);
+                        done = true;
+                    }
+                    if (done) {
+                        vw[i] = finish_value<false>(a, t0 + i, px, ctr, k, true, normal_from(wz, ww));
+                        fresh = true;
+                    }
+                }
+                const uint64_t dm = __ballot(done);
+                if (done) e = next + lane_rank(dm);
+                next += __popcll(dm);
+            }
+        }
+        ccd_wave_lds_sync();                   // every pixel of the tile has its value
+        if (vec && tn == kPhotonTile) {
+#pragma unroll
+            for (int j = 0; j < kPhotonTile / 256; ++j) {
+                const int o = 4 * (lane + 64 * j);
+                *reinterpret_cast<float4*>(out + t0 + o) = *reinterpret_cast<const float4*>(vw + o);
+            }
+        } else {
+            for (int j = lane; j < tn; j += 64) out[t0 + j] = vw[j];
+        }
         ccd_wave_lds_sync();                   // the slice is read before the next tile reuses it
     }
 }
@@ -522,12 +673,15 @@ int shccd_readout(const shccd_params* p, int n_frames, float* integrated, int64_
     // FULL variants carry binning, CIC, EM gain and pixel gains; the lean ones the common rest
     const bool full = p->binning != 1 || (p->flags & (SHCCD_CIC | SHCCD_EXCESS)) || pixel_gains != nullptr;
     if (p->flags & SHCCD_PHOTON) {
-        if (full)
-            hipLaunchKernelGGL(ccd_photon_kernel<true>, dim3(photon_blocks(n_out), n_frames), dim3(256), 0,
+        if (full) {
+            hipLaunchKernelGGL(ccd_photon_full_kernel, dim3(photon_blocks(n_out), n_frames), dim3(256), 0,
                                reinterpret_cast<hipStream_t>(stream), a);
-        else
-            hipLaunchKernelGGL(ccd_photon_kernel<false>, dim3(photon_blocks(n_out), n_frames), dim3(256), 0,
-                               reinterpret_cast<hipStream_t>(stream), a);
+        } else {
+            auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
+            const int vec = al(pixels) && (n_frames == 1 || out_stride % 4 == 0);
+            hipLaunchKernelGGL(ccd_photon_lean_kernel, dim3(photon_blocks(n_out), n_frames), dim3(256), 0,
+                               reinterpret_cast<hipStream_t>(stream), a, vec);
+        }
     } else {
         auto al = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15u) == 0; };
         const int vec = p->binning == 1 && n_out % 4 == 0 && al(pixels) && (!integrated || al(integrated)) &&
