@@ -330,6 +330,12 @@ int shccd_readout_cte(const shccd_params* params, int n_frames, float* integrate
                       const float* cte_cols, float* work, const float* pixel_gains, uint64_t counter,
                       float* pixels, int64_t out_stride, void* stream);
 
+/* The detector noise generator: Philox4x32-10 (Salmon, Moraes, Dror and Shaw, SC'11) of n
+ * counter/key sets counters_keys[6 i .. 6 i + 5] = (c0, c1, c2, c3, k0, k1) into out[4 i .. 4 i + 3]
+ * (device memory).  No counterpart in the reference (its streams are numpy's default_rng): exposes
+ * the generator that every shccd_readout* noise draw comes from, for its known-answer test. */
+int shccd_philox(int n, const uint32_t* counters_keys, uint32_t* out, void* stream);
+
 /* ---- push-pull calibration ------------------------------------------------------------------
  * Fields of a modal push-pull around phi0 (n_px floats): out[k] = phi0 + amp * modes[k],
  * out[n_modes + k] = phi0 - amp * modes[k] (2 n_modes frames, out_stride apart).  Replaces the

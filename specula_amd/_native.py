@@ -23,7 +23,7 @@ EXPORTS = (
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_vjp_sensor_grid', 'shwfs_resample', 'shwfs_pushpull_fields', 'shwfs_pushpull_rows', 'shwfs_im_from_jacobian', 'shwfs_im_from_jacobian_noalloc',
     'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
-    'shccd_integrate', 'shccd_readout', 'shccd_readout_cte',
+    'shccd_integrate', 'shccd_readout', 'shccd_readout_cte', 'shccd_philox',
     'shwfs_graph_begin', 'shwfs_graph_end', 'shwfs_graph_launch', 'shwfs_graph_destroy',
     'shwfs_graph_node_count',
     'shgemm_f32',
@@ -100,6 +100,7 @@ def _declare(lib):
         'shwfs_batch_info': (_I, [_P, _P, _P]),
         'shwfs_batch_destroy': (_I, [_P]),
         'shccd_integrate': (_I, [_I, _P, _I64, _D, _P, _I64, _I64, _P]),
+        'shccd_philox': (_I, [_I, _P, _P, _P]),
         'shccd_readout': (_I, [ctypes.POINTER(ShccdParams), _I, _P, _I64, _P, _I64, _D, _P, ctypes.c_uint64, _P,
                                _I64, _P]),
         'shccd_readout_cte': (_I, [ctypes.POINTER(ShccdParams), _I, _P, _I64, _P, _I64, _D, _P, _P, _P, _P,

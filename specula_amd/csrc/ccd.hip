@@ -48,17 +48,34 @@ struct U4 {
     uint32_t x, y, z, w;
 };
 
+// a ^ b ^ c in one instruction (v_bitop3_b32 with the truth table of a three-input xor)
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
+// one round: two 32x32 -> 64 multiplies (one v_mad_u64_u32 each) and two three-input xors
 __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
     constexpr uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
     constexpr uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;   // one 32x32->64 multiply each
-        c = U4{(uint32_t)(p1 >> 32) ^ c.y ^ k0, (uint32_t)p1, (uint32_t)(p0 >> 32) ^ c.w ^ k1, (uint32_t)p0};
+        const uint64_t p0 = (uint64_t)M0 * c.x, p1 = (uint64_t)M1 * c.z;
+        c = U4{xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1, xor3((uint32_t)(p0 >> 32), c.w, k1), (uint32_t)p0};
         k0 += W0;
         k1 += W1;
     }
     return c;
+}
+
+__global__ void philox_kat_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* c = in + 6 * i;
+    const U4 r = philox(U4{c[0], c[1], c[2], c[3]}, c[4], c[5]);
+    out[4 * i] = r.x;
+    out[4 * i + 1] = r.y;
+    out[4 * i + 2] = r.z;
+    out[4 * i + 3] = r.w;
 }
 
 // Stream of uniforms of one pixel for one noise source: draw d -> 4 words per Philox call.
@@ -496,39 +513,67 @@ __global__ void __launch_bounds__(256) ccd_photon_lean_kernel(ReadoutArgs a, int
     for (int tile = blockIdx.x * 4 + wave; tile < n_tiles; tile += gridDim.x * 4) {
         const int t0 = tile * kPhotonTile;
         const int tn = min(kPhotonTile, n_out - t0);
-        // signals, and the class of each of the lane's pixels lane + 64 j (2 bits each; 3: none)
+        // signals of the lane's pixels i = lane + 64 j, every load in flight before the first use
+        // (slots past the end of a ragged last tile read its last pixel and are marked invalid)
+        const bool whole = tn == kPhotonTile;
         float raw[kPhotonPerLane];
 #pragma unroll
-        for (int j = 0; j < kPhotonPerLane; ++j) raw[j] = lane + 64 * j < tn ? raw_pixel(a, f, t0 + lane + 64 * j) : 0.0f;
+        for (int j = 0; j < kPhotonPerLane; ++j) raw[j] = 0.0f;
+        if (a.integrated) {
+            float* ip = a.integrated + f * a.int_stride + t0;
+#pragma unroll
+            for (int j = 0; j < kPhotonPerLane; ++j) raw[j] = ip[min(lane + 64 * j, tn - 1)];
+#pragma unroll
+            for (int j = 0; j < kPhotonPerLane; ++j)
+                if (whole || lane + 64 * j < tn) ip[lane + 64 * j] = 0.0f;       // integrated_i.i *= 0 (ccd.py:217)
+        }
+        if (a.in_i) {
+            const float* src = a.in_i + f * a.in_stride + t0;
+            const float fc = (float)a.factor;
+            float w[kPhotonPerLane];
+#pragma unroll
+            for (int j = 0; j < kPhotonPerLane; ++j) w[j] = src[min(lane + 64 * j, tn - 1)];
+#pragma unroll
+            for (int j = 0; j < kPhotonPerLane; ++j) raw[j] += w[j] * fc;
+        }
+        // class of each pixel (2 bits per slot j: 0 no signal, 1 inversion, 2 PTRS, 3 past the end)
         uint32_t cls = 0;
-        int n0 = 0, n1 = 0;
+        int c0 = 0, c1 = 0;
 #pragma unroll
         for (int j = 0; j < kPhotonPerLane; ++j) {
             const int i = lane + 64 * j;
-            uint32_t c = 3u;
-            if (i < tn) {
-                const float lam = signal_of<false>(a, raw[j], fbase + (uint64_t)(t0 + i), ctr);
-                vw[i] = lam;
-                c = !(lam > 0.0f) ? 0u : lam < 10.0f ? 1u : 2u;
-            }
+            const float lam = signal_of<false>(a, raw[j], fbase + (uint64_t)(t0 + i), ctr);
+            vw[i] = lam;
+            const uint32_t c = !(whole || i < tn) ? 3u : !(lam > 0.0f) ? 0u : lam < 10.0f ? 1u : 2u;
             cls |= c << (2 * j);
-            n0 += __popcll(__ballot(c == 0u));
-            n1 += __popcll(__ballot(c == 1u));
+            c0 += c == 0u;
+            c1 += c == 1u;
         }
-        int e0 = 0, e1 = n0, e2 = n0 + n1;          // next list entry of each class
+        // list positions: the lane's pixels of each class follow those of the lower lanes (inclusive
+        // scan over the wave of the packed counts c0 + 2^16 c1; totals up to 1024 fit in 16 bits)
+        const uint32_t own = (uint32_t)c0 | ((uint32_t)c1 << 16);
+        uint32_t scan = own;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(scan, o, 64);
+            if (lane >= o) scan += v;
+        }
+        const uint32_t tot = __shfl(scan, 63, 64);
+        const int n0 = (int)(tot & 0xffffu), n1 = (int)(tot >> 16);
+        const uint32_t ex = scan - own;
+        const int valid_before = whole ? 16 * lane : lane * (tn / 64) + min(lane, tn % 64);
+        int e0 = (int)(ex & 0xffffu), e1 = n0 + (int)(ex >> 16);
+        int e2 = n0 + n1 + valid_before - (int)(ex & 0xffffu) - (int)(ex >> 16);
 #pragma unroll
         for (int j = 0; j < kPhotonPerLane; ++j) {
             const uint32_t c = (cls >> (2 * j)) & 3u;
-            const uint64_t m0 = __ballot(c == 0u), m1 = __ballot(c == 1u), m2 = __ballot(c == 2u);
-            const uint16_t i = (uint16_t)(lane + 64 * j);
-            if (c == 0u) ow[e0 + lane_rank(m0)] = i;
-            else if (c == 1u) ow[e1 + lane_rank(m1)] = i;
-            else if (c == 2u) ow[e2 + lane_rank(m2)] = i;
-            e0 += __popcll(m0);
-            e1 += __popcll(m1);
-            e2 += __popcll(m2);
+            const int pos = c == 0u ? e0 : c == 1u ? e1 : e2;
+            if (c != 3u) ow[pos] = (uint16_t)(lane + 64 * j);
+            e0 += c == 0u;
+            e1 += c == 1u;
+            e2 += c == 2u;
         }
-        const int n_all = e2;
+        const int n_all = whole ? kPhotonTile : tn;
         ccd_wave_lds_sync();
         // no signal: k = 0 (the generator only for the readout normal)
         for (int e = lane; e < n0; e += 64) {
@@ -628,6 +673,16 @@ int grid_for(int64_t n, int per_lane) {
 }  // namespace
 
 extern "C" {
+
+int shccd_philox(int n, const uint32_t* counters_keys, uint32_t* out, void* stream) {
+    if (n < 0 || (n > 0 && (!counters_keys || !out)))
+        return shwfs_internal_set_error(SHWFS_ERR_ARG, "shccd_philox: bad argument");
+    if (n == 0) return SHWFS_OK;
+    hipLaunchKernelGGL(philox_kat_kernel, dim3((n + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       counters_keys, out, n);
+    CCD_HIP_CHECK(hipGetLastError());
+    return SHWFS_OK;
+}
 
 int shccd_integrate(int n_frames, const float* in_i, int64_t in_stride, double factor, float* integrated,
                     int64_t int_stride, int64_t n_px, void* stream) {

@@ -54,6 +54,9 @@ def test_ccd_argument_errors_are_reported():
     rc = lib.shccd_readout_cte(ctypes.byref(p), 0, None, 0, ctypes.c_void_p(16), 64, 1.0, ctypes.c_void_p(16),
                                ctypes.c_void_p(16), ctypes.c_void_p(16), None, 0, ctypes.c_void_p(16), 16, None)
     assert rc == -1 and b'n_frames' in lib.shwfs_last_error()
+    assert lib.shccd_philox(-1, ctypes.c_void_p(16), ctypes.c_void_p(16), None) == -1
+    assert lib.shccd_philox(4, None, ctypes.c_void_p(16), None) == -1
+    assert lib.shccd_philox(0, None, None, None) == 0
 
 
 def test_argument_errors_are_reported():

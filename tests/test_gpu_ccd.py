@@ -43,6 +43,32 @@ def chi2_two_sample(a, b, min_count=20):
     return stats.chi2_contingency(tab)[1]
 
 
+# Known-answer vectors of Philox4x32-10 published with the Random123 library (kat_vectors):
+# counter (4 words), key (2 words) -> output (4 words).
+PHILOX_KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000, 0x00000000, 0x00000000),
+     (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff),
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344, 0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+def test_noise_generator_known_answers():
+    """Every noise draw of the readout kernels comes from this generator (shccd_philox runs the
+    same device function): the Random123 known answers pin it."""
+    inp = np.array([c for c, _ in PHILOX_KAT], dtype=np.uint32)
+    exp = np.array([o for _, o in PHILOX_KAT], dtype=np.uint32)
+    src = torch.as_tensor(inp.view(np.int32), device=DEV)
+    out = torch.zeros((len(PHILOX_KAT), 4), dtype=torch.int32, device=DEV)
+    _native.check(_native.lib().shccd_philox(len(PHILOX_KAT), _native.ptr(src), _native.ptr(out), None),
+                  'shccd_philox')
+    torch.cuda.synchronize()
+    got = out.cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(got, exp)
+
+
 @pytest.mark.parametrize('binning', [1, 2])
 def test_noiseless_readout_matches_oracle(binning):
     rng = np.random.default_rng(0)
