@@ -1086,60 +1086,35 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     }
     __syncthreads();
 
-    // phase 3a: column transforms over ky, written back in place (natural order).  Each lane also
-    // requests the kernel spectrum values it multiplies in phase 3b (layout of that phase): lanes
-    // without a transform at once, the others after theirs (while their registers hold the
-    // transform, the requests would not fit), so that the latency hides behind the transforms and
-    // the barrier.
-    constexpr int SPN = 16 / LH;                     // subapertures per 16-column block
-    constexpr int KS = (L + 3) / 4;                  // K steps (lags beyond L: P~ = 0)
-    static_assert(TS == 8 * SPN, "eight column blocks per tile, two per wave");
-    static_assert(4 * KS <= M, "K steps stay inside the written part of an LDS row");
-    const int lane3 = t & 63, wave3 = t >> 6;
-    const int n16 = lane3 & 15, k4 = lane3 >> 4;
-    const int so = n16 / LH, lhc = n16 - (n16 / LH) * LH;
-    const bool col_ok = so < SPN;
-    float2 kv[2][KS];                                // K[sub][m_y = 4 ks + k4 - (NP-1)][m_x = lhc]
-    auto kspec_request = [&]() {
+    // phase 3a: column transforms over ky and the kernel multiply, written back in place
+    for (int task = t; task < TS * LH; task += 256) {
+        const int s = task / LH, lh = task - (task / LH) * LH;
+        if (sub0 + s >= g.n_sub) continue;
+        const int sub = lenslet(s);
+        float2* col = buf + (s * LH + lh) * R2;
+        float2 F[M];
 #pragma unroll
-        for (int bk = 0; bk < 2; ++bk) {
-            const int s = (2 * wave3 + bk) * SPN + so;
-            const bool ok = col_ok && sub0 + s < g.n_sub;
-            const float2* kr = ok ? g.kspec + (int64_t)lenslet(s) * L * L + lhc + NP - 1 : g.kspec;
+        for (int ky = 0; ky < M; ++ky) F[ky] = col[ky];
+        fft_radix2<M, false>(F);
+        const float2* kr = g.kspec + (int64_t)sub * L * L + lh + NP - 1;   // lag column m_x = lh
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks) {
-                const int mi = 4 * ks + k4;
-                kv[bk][ks] = (ok && mi < L) ? kr[mi * L] : make_float2(0.f, 0.f);
-            }
-        }
-    };
-    static_assert(TS * LH <= 256, "one column transform per thread");
-    if (t < TS * LH) {
-        const int s = t / LH, lh = t - (t / LH) * LH;
-        if (sub0 + s < g.n_sub) {
-            float2* col = buf + (s * LH + lh) * R2;
-            float2 F[M];
-#pragma unroll
-            for (int ky = 0; ky < M; ++ky) F[ky] = col[ky];
-            fft_radix2<M, false>(F);
-#pragma unroll
-            for (int ky = 0; ky < M; ++ky) col[ky] = F[ky];
-        }
-        kspec_request();
-    } else {
-        kspec_request();
+        for (int mi = 0; mi < L; ++mi) col[mi] = cmul(F[(mi - (NP - 1) + M) % M], kr[mi * L]);
     }
     __syncthreads();
 
-    // phase 3b: left binning operator V = P~ D on the matrix cores (FP32 16x16x4, K = lags m_y),
-    // with D[m_y][m_x] = F[m_y mod M][m_x] K[m_y][m_x] formed on the way in.
+    // phase 3b: left binning operator V = P~ D on the matrix cores (FP32 16x16x4, K = lags m_y).
     // A = P~ (16 rows Y x 4 lags per step: lane l holds P~[l % 16][4 ks + l / 16], kept in
     // registers for the tile), B = D (4 lags x 16 columns: SPN = 16 / LH subapertures side by
     // side, lag column m_x = column % LH), V[Y][n] out in lane n + 16 (Y / 4), register Y % 4.
     // Three real products per K step (Gauss / Karatsuba): T1 = Re P~ Re D, T2 = Im P~ Im D,
     // T3 = (Re P~ + Im P~)(Re D + Im D); Re V = T1 - T2, Im V = T3 - T1 - T2.
     {
-        const int lane = lane3, wave = wave3;
+        constexpr int SPN = 16 / LH;                 // subapertures per 16-column block
+        constexpr int KS = (L + 3) / 4;              // K steps (lags beyond L: P~ = 0)
+        static_assert(TS == 8 * SPN, "eight column blocks per tile, two per wave");
+        static_assert(4 * KS <= M, "K steps stay inside the written part of an LDS row");
+        const int lane = t & 63, wave = t >> 6;
+        const int n16 = lane & 15, k4 = lane >> 4;
         const float2* Pt = reinterpret_cast<const float2*>(g.roww);       // [m_y][16 rows] complex
         float pr[KS], pim[KS], psum[KS];
 #pragma unroll
@@ -1148,7 +1123,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             const float2 pc = mi < L ? Pt[mi * 16 + n16] : make_float2(0.f, 0.f);
             pr[ks] = pc.x; pim[ks] = pc.y; psum[ks] = pc.x + pc.y;
         }
-        const int lh = lhc;
+        const int so = n16 / LH, lh = n16 - (n16 / LH) * LH;
+        const bool col_ok = so < SPN;
         floatx4 vr[2], vi[2];
 #pragma unroll
         for (int bk = 0; bk < 2; ++bk) {
@@ -1157,14 +1133,10 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             floatx4 t1 = {0.f, 0.f, 0.f, 0.f}, t2 = {0.f, 0.f, 0.f, 0.f}, t3 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks) {
-                // lag m_y = 4 ks + k4 - (NP-1) of the transform sits in row m_y mod M; past the lag
-                // grid (and for columns without a subaperture) K = 0
-                const float2 fv = col[(4 * ks + k4 + M - (NP - 1)) % M];
-                const float2 kk = kv[bk][ks];
-                const float dx = fv.x * kk.x - fv.y * kk.y, dy = fv.x * kk.y + fv.y * kk.x;
-                t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], dx, t1, 0, 0, 0);
-                t2 = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[ks], dy, t2, 0, 0, 0);
-                t3 = __builtin_amdgcn_mfma_f32_16x16x4f32(psum[ks], dx + dy, t3, 0, 0, 0);
+                const float2 d = col[4 * ks + k4];
+                t1 = __builtin_amdgcn_mfma_f32_16x16x4f32(pr[ks], d.x, t1, 0, 0, 0);
+                t2 = __builtin_amdgcn_mfma_f32_16x16x4f32(pim[ks], d.y, t2, 0, 0, 0);
+                t3 = __builtin_amdgcn_mfma_f32_16x16x4f32(psum[ks], d.x + d.y, t3, 0, 0, 0);
             }
             vr[bk] = t1 - t2;
             vi[bk] = t3 - t1 - t2;
