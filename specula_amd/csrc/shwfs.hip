@@ -559,10 +559,20 @@ struct NgsWaveConst {
 // Transform one frame of the wave's subapertures and store it (plus CoG moments when COG != 0:
 // 1 = subtract-and-clamp threshold, 2 = pedestal threshold, 3 = no threshold (clamp at zero),
 // sh_slopec.py:380-387).
+// (phv, fs_io): this frame's phase row and flux scale, replaced by (phn, fn) -- the next frame's,
+// prefetched by the caller -- once the transform no longer needs them.
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& wv, int wave_id, int f,
-                                               const float (&av)[NP], const float (&phv)[NP], float fs, float* sw,
-                                               const NgsWaveConst<NP, NPX, COG>& wc, float sq_sub) {
+                                               const float (&av)[NP], float (&phv)[NP], float& fs_io,
+                                               const float (&phn)[NP], const float& fn, float* sw,
+                                               const NgsWaveConst<NP, NPX, COG>& wc, float sq_sub,
+                                               unsigned long long lit) {
+    const float fs = fs_io;
+    auto rotate = [&]() {
+#pragma unroll
+        for (int x = 0; x < NP; ++x) phv[x] = phn[x];
+        fs_io = fn;
+    };
     using Sh = NgsMfmaShape<NP>;
     constexpr int M = Sh::M;
     constexpr int SW = Sh::SW;
@@ -573,11 +583,8 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
 
     // A subaperture whose pupil samples are all zero has an all-zero image.  A wave without a lit
     // subaperture skips the transforms and stores zeros; the binning skips dark subapertures, whose
-    // LDS chunk then keeps the (zero) autocorrelation of phase B.
-    bool lit_lane = false;
-#pragma unroll
-    for (int x = 0; x < NP; ++x) lit_lane |= (av[x] != 0.f);
-    const unsigned long long lit = __ballot(lit_lane);        // bit q * NP + y: row y of subaperture q
+    // LDS chunk then keeps the (zero) autocorrelation of phase B.  `lit`: bit q * NP + y set when
+    // row y of subaperture q has a nonzero pupil sample (ngs_lit_rows).
     constexpr unsigned long long kSubBits = (NP == 64) ? ~0ull : ((1ull << NP) - 1);
     const bool dark = lit == 0;
     // what a dark wave stores: 0 times the frame's scale, so that a frame without any light gives
@@ -697,7 +704,10 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
     } else {
         for (int q = 0; q < wv.nq; ++q) bin_one(q);
     }
+    rotate();
     wave_lds_sync();
+    } else {
+        rotate();
     }   // !dark
 
     // D: stores (+ moments)
@@ -868,8 +878,26 @@ __device__ __forceinline__ float subap_sq_store_layout(const float (&av)[NP]) {
     return __shfl(s, q * NP, kWave);
 }
 
-// Frames [f0, f1) of the wave's subapertures; the next frame's pupil loads are issued before the
-// current frame is transformed, so HBM latency overlaps the FFT / MFMA work.
+// s_waitcnt immediate that waits for every vector memory operation (vmcnt 0) and leaves the export
+// and LDS / scalar counters at their maxima (no wait on them).
+constexpr int kWaitVmcnt0 = (0x7 << 4) | (0xF << 8);
+
+// Rows of the wave's subapertures with a nonzero pupil sample (bit q * NP + y: row y of q).
+template <int NP>
+__device__ __forceinline__ unsigned long long ngs_lit_rows(const float (&av)[NP]) {
+    bool lit_lane = false;
+#pragma unroll
+    for (int x = 0; x < NP; ++x) lit_lane |= (av[x] != 0.f);
+    return __ballot(lit_lane);
+}
+
+// Frames [f0, f1) of the wave's subapertures; the next frame's phase loads are issued before the
+// current frame is transformed, so HBM latency overlaps the FFT / MFMA work.  Vector memory
+// operations are counted in order (loads and stores alike), so a wait for a register loaded before
+// other operations also waits for them: the loop is entered with every load complete, and the
+// prefetched phase is moved into the current-frame registers after the binning, before the frame's
+// stores are issued (a move after the stores would wait for them, one at the top of the next frame
+// for the loads of the frame after it).
 template <int NP, bool PARSEVAL, int NPX, int COG, bool BF3>
 __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, int f0, int f1, float* sw) {
     const NgsWave wv = ngs_wave<NP>(g, wave_id);
@@ -897,17 +925,18 @@ __device__ __forceinline__ void ngs_mfma_frames(const NgsArgs& g, int wave_id, i
     // fused CoG: sum(A^2) of each store lane's subaperture (once for a static pupil, else per frame)
     float sq = 0.f;
     if constexpr (COG != 0 && NPX > 0) sq = subap_sq_store_layout<NP, NPX>(av);
+    unsigned long long litm = ngs_lit_rows<NP>(av);
+    __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
     for (int f = f0; f < f1; ++f) {
-        if (f + 1 < f1) ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, lit, pn, fn);
-        if (!static_a && f > f0) ngs_row_load<NP>(g.A + f * g.a_stride, wc.ld_off, g.vec, av);
-        if (f > f0) wave_lds_sync();
-        if constexpr (COG != 0 && NPX > 0) {
-            if (f > f0 && g.a_stride != 0) sq = subap_sq_store_layout<NP, NPX>(av);
+        if (!static_a && f > f0) {
+            ngs_row_load<NP>(g.A + f * g.a_stride, wc.ld_off, g.vec, av);
+            litm = ngs_lit_rows<NP>(av);
+            if constexpr (COG != 0 && NPX > 0) sq = subap_sq_store_layout<NP, NPX>(av);
         }
-        ngs_mfma_frame<NP, PARSEVAL, NPX, COG, BF3>(g, wv, wave_id, f, av, pv, fs, sw, wc, sq);
-#pragma unroll
-        for (int x = 0; x < NP; ++x) pv[x] = pn[x];
-        fs = fn;
+        if (f + 1 < f1) ngs_phase_load<NP, PARSEVAL>(g, wc.ld_off, f + 1, lit, pn, fn);
+        if (f > f0) wave_lds_sync();
+        // transforms frame f from (pv, fs) and moves (pn, fn) into them before its stores
+        ngs_mfma_frame<NP, PARSEVAL, NPX, COG, BF3>(g, wv, wave_id, f, av, pv, fs, pn, fn, sw, wc, sq, litm);
     }
 }
 
