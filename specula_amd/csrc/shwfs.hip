@@ -960,7 +960,8 @@ sh_ngs_mfma_kernel(NgsArgs g, int n_frames, int fpb) {
 // psf = Re IFFT_N(FFT_N(|FFT_N(E)|^2) * K).  With D[m] = (N^2/M^2) FFT_M(S)[m mod M] K[m] on the
 // lag grid |m| <= np-1 (S = |FFT_M(E)|^2, M >= 2np-1), the binned image is B = Re(P D Q^T) where
 // P, Q (npx x L, complex) already carry N^2/M^2, the rebin, the cut and the mask.
-// One block = TS subapertures, 256 threads:
+// One block = TS subapertures, 64 x WAVES threads (WAVES = TS / (2 SPN): two 16-column blocks per wave
+// in the matrix-core phases):
 //   1:  task (s, x)       column FFT over y of the field                      -> buf[s][ky][x]
 //   2:  task (s, ky)      lag row of |FFT_x(.)|^2 by direct autocorrelation     -> buf[s][lx][ky]
 //   3a: task (s, lx)      forward FFT over ky, D = F * K_s, in place            -> buf[s][lx][my]
@@ -975,6 +976,9 @@ struct LgsShape {
     static constexpr int R1 = NP + 1, R2 = M + 1, R3 = LH + 1;   // padded row lengths
     static constexpr int E1 = M * R1, E2 = LH * R2, E3 = 16 * R3;
     static constexpr int LDS_F2 = TS * (E1 > E2 ? (E1 > E3 ? E1 : E3) : (E2 > E3 ? E2 : E3));
+    static constexpr int SPN = 16 / LH;           // subapertures per 16-column block of the MFMA phases
+    static constexpr int WAVES = TS / (2 * SPN);  // two column blocks per wave
+    static constexpr int BT = 64 * WAVES;         // threads per block
 };
 
 // The lag-domain image D = C K is Hermitian on the (m_y, m_x) grid (C is an autocorrelation, K
@@ -985,7 +989,7 @@ template <int NP, int M, int TS>
 __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, float2* buf, double* red) {
     constexpr int L = 2 * NP - 1;
     using Sh = LgsShape<NP, M, TS>;
-    constexpr int LH = Sh::LH, R1 = Sh::R1, R2 = Sh::R2;
+    constexpr int LH = Sh::LH, R1 = Sh::R1, R2 = Sh::R2, BT = Sh::BT, WAVES = Sh::WAVES;
     static_assert(L <= M, "lag grid must fit the spectrum grid");
     const int t = threadIdx.x;
     const int n = g.n_lenses;
@@ -997,7 +1001,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 
     // phase 1a: pupil samples of the task's column (one task per thread); a tile without a lit
     // sample has a zero image and stores it without transforming
-    static_assert(TS * NP <= 256, "one phase-1 task per thread");
+    static_assert(TS * NP <= BT, "one phase-1 task per thread");
     const int s1 = t / NP, x1 = t - (t / NP) * NP;
     const bool task1 = t < TS * NP && sub0 + s1 < g.n_sub;
     int64_t off1 = 0;
@@ -1028,7 +1032,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         const int W = n * g.npx;
         if (g.store_vec && (g.npx & 3) == 0) {
             const int q4 = g.npx / 4, per = g.npx * q4;
-            for (int e = t; e < TS * per; e += 256) {
+            for (int e = t; e < TS * per; e += BT) {
                 const int s = e / per, r = e - (e / per) * per;
                 if (sub0 + s >= g.n_sub) break;
                 const int sub = lenslet(s);
@@ -1038,7 +1042,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             }
         } else {
             const int per = g.npx * g.npx;
-            for (int e = t; e < TS * per; e += 256) {
+            for (int e = t; e < TS * per; e += BT) {
                 const int s = e / per, r = e - (e / per) * per;
                 if (sub0 + s >= g.n_sub) break;
                 const int sub = lenslet(s);
@@ -1072,11 +1076,11 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     __syncthreads();
 
     // phase 2 (read rows into registers, then overwrite the buffer with lag rows m_x >= 0)
-    constexpr int T2 = (TS * M + 255) / 256;
+    constexpr int T2 = (TS * M + BT - 1) / BT;
     float2 R[T2][LH];
 #pragma unroll
     for (int h = 0; h < T2; ++h) {
-        const int task = t + 256 * h;
+        const int task = t + BT * h;
         if (task < TS * M) {
             const int s = task / M, ky = task - (task / M) * M;
             float2 row[NP];
@@ -1106,7 +1110,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     __syncthreads();
 #pragma unroll
     for (int h = 0; h < T2; ++h) {
-        const int task = t + 256 * h;
+        const int task = t + BT * h;
         if (task < TS * M) {
             const int s = task / M, ky = task - (task / M) * M;
 #pragma unroll
@@ -1116,7 +1120,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     __syncthreads();
 
     // phase 3a: column transforms over ky and the kernel multiply, written back in place
-    for (int task = t; task < TS * LH; task += 256) {
+    for (int task = t; task < TS * LH; task += BT) {
         const int s = task / LH, lh = task - (task / LH) * LH;
         if (sub0 + s >= g.n_sub) continue;
         const int sub = lenslet(s);
@@ -1138,9 +1142,9 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     // Three real products per K step (Gauss / Karatsuba): T1 = Re P~ Re D, T2 = Im P~ Im D,
     // T3 = (Re P~ + Im P~)(Re D + Im D); Re V = T1 - T2, Im V = T3 - T1 - T2.
     {
-        constexpr int SPN = 16 / LH;                 // subapertures per 16-column block
+        constexpr int SPN = Sh::SPN;                 // subapertures per 16-column block
         constexpr int KS = (L + 3) / 4;              // K steps (lags beyond L: P~ = 0)
-        static_assert(TS == 8 * SPN, "eight column blocks per tile, two per wave");
+        static_assert(TS == 2 * WAVES * SPN, "two column blocks per wave");
         static_assert(4 * KS <= M, "K steps stay inside the written part of an LDS row");
         const int lane = t & 63, wave = t >> 6;
         const int n16 = lane & 15, k4 = lane >> 4;
@@ -1192,10 +1196,10 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     double bsum = 0.0;
     {
         constexpr int KQ = (LH + 3) / 4;
-        constexpr int SPW = TS / 4;                   // subapertures per wave
+        constexpr int SPW = TS / WAVES;               // subapertures per wave
         const float fscale = (float)scale;
         const bool vec4 = g.store_vec && (g.npx & 3) == 0;   // 16-byte aligned CCD rows
-        static_assert(TS % 4 == 0, "whole subapertures per wave");
+        static_assert(TS % WAVES == 0, "whole subapertures per wave");
         const int lane = t & 63, wave = t >> 6;
         const int n16 = lane & 15, k4 = lane >> 4;
         float qr[KQ], nqi[KQ];
@@ -1244,7 +1248,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
 }
 
 template <int NP, int M, int TS>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__((LgsShape<NP, M, TS>::BT)) __attribute__((amdgpu_waves_per_eu(4, 8)))
 sh_lgs_forward_kernel(NgsArgs g) {
     extern __shared__ float4 lds_raw[];
     __shared__ double red[4];
@@ -2377,7 +2381,8 @@ int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     constexpr int lds = LgsShape<NP, M, TS>::LDS_F2 * (int)sizeof(float2);
     static_assert(lds <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
     dim3 grid(p->n_tiles, n_frames);
-    hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3(256), lds, st, t_ev_start, t_ev_stop, 0, a);
+    hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3((LgsShape<NP, M, TS>::BT)), lds, st, t_ev_start,
+                          t_ev_stop, 0, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
