@@ -40,6 +40,7 @@ struct GenArgs {
 };
 
 constexpr int kGenThreads = 256;
+constexpr int kGenRowsPerTask = 8;         // rows Y of V per task of the LGS product V = P~ D
 // Complex working vectors per block: 32 KiB for M <= 256 (two blocks per CU with the transpose
 // scratch), 64 KiB for M >= 512 (one block per CU; such plans have few, large subapertures, so
 // the longer chunks matter more than the occupancy).  The smaller one bounds the LGS V tile.
@@ -331,28 +332,69 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
             }
             // 3c: V[Y][mx] = sum_my P~_k[Y][my] D[my][mx] (complex, in LDS if it fits, else in S)
             // 3d: B += Re sum_mx c_mx V[Y][mx] Q~_k[X][mx], over the K mask terms
+            // When P~_k fits the transpose scratch and a block has fewer lag columns than threads, P~_k
+            // is staged there and each task (mx, yg) keeps up to kGenRowsPerTask rows Y = yg, yg + YG,
+            // ... in registers, so D is read once per task instead of once per (Y, mx); the lag
+            // columns m_x >= 0 of Q~_k follow in the same scratch for 3d.  Same sums in the same
+            // order either way.
             float2* V = (npx * np <= CAP) ? buf : reinterpret_cast<float2*>(S);
+            const int YG = max((npx + kGenRowsPerTask - 1) / kGenRowsPerTask, min(npx, kGenThreads / np));
+            // staging pays when a task keeps more than one row (small subapertures: one row each)
+            const bool staged = YG < npx && npx * L <= CAP + CAP / 4 && npx * np <= CAP;
             for (int k = 0; k < K; ++k) {
                 const float2* Pt = reinterpret_cast<const float2*>(g.roww + (int64_t)k * q.term_stride);
-                for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {
-                    const int Y = e / np, mx = e - (e / np) * np;
-                    float ar = 0.f, ai = 0.f;
-                    for (int ly = 0; ly < L; ++ly) {
-                        const float2 p = Pt[(int64_t)Y * L + ly];
-                        const float2 d = D[(int64_t)ly * np + mx];
-                        ar = fmaf(p.x, d.x, fmaf(-p.y, d.y, ar));
-                        ai = fmaf(p.x, d.y, fmaf(p.y, d.x, ai));
+                const float2* Qt = reinterpret_cast<const float2*>(g.colw + (int64_t)k * q.term_stride);
+                if (staged) {
+                    for (int e = threadIdx.x; e < npx * L; e += blockDim.x) fscr[e] = Pt[e];
+                    __syncthreads();
+                    for (int task = threadIdx.x; task < np * YG; task += blockDim.x) {
+                        const int yg = task / np, mx = task - (task / np) * np;
+                        float ar[kGenRowsPerTask], ai[kGenRowsPerTask];
+#pragma unroll
+                        for (int r = 0; r < kGenRowsPerTask; ++r) ar[r] = ai[r] = 0.f;
+                        for (int ly = 0; ly < L; ++ly) {
+                            const float2 d = D[(int64_t)ly * np + mx];
+#pragma unroll
+                            for (int r = 0; r < kGenRowsPerTask; ++r) {
+                                const int Y = yg + r * YG;
+                                if (Y < npx) {
+                                    const float2 p = fscr[Y * L + ly];
+                                    ar[r] = fmaf(p.x, d.x, fmaf(-p.y, d.y, ar[r]));
+                                    ai[r] = fmaf(p.x, d.y, fmaf(p.y, d.x, ai[r]));
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int r = 0; r < kGenRowsPerTask; ++r) {
+                            const int Y = yg + r * YG;
+                            if (Y < npx) V[Y * np + mx] = make_float2(ar[r], ai[r]);
+                        }
                     }
-                    V[e] = make_float2(ar, ai);
+                    __syncthreads();
+                    for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {
+                        const int X = e / np, mx = e - (e / np) * np;
+                        fscr[e] = Qt[(int64_t)X * L + (mx + np - 1)];
+                    }
+                } else {
+                    for (int e = threadIdx.x; e < npx * np; e += blockDim.x) {
+                        const int Y = e / np, mx = e - (e / np) * np;
+                        float ar = 0.f, ai = 0.f;
+                        for (int ly = 0; ly < L; ++ly) {
+                            const float2 p = Pt[(int64_t)Y * L + ly];
+                            const float2 d = D[(int64_t)ly * np + mx];
+                            ar = fmaf(p.x, d.x, fmaf(-p.y, d.y, ar));
+                            ai = fmaf(p.x, d.y, fmaf(p.y, d.x, ai));
+                        }
+                        V[e] = make_float2(ar, ai);
+                    }
                 }
                 __syncthreads();
-                const float2* Qt = reinterpret_cast<const float2*>(g.colw + (int64_t)k * q.term_stride);
                 for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
                     const int Y = e / npx, X = e - (e / npx) * npx;
                     float acc = (K == 1 || k == 0) ? 0.f : Bacc[e];
                     for (int mx = 0; mx < np; ++mx) {
                         const float2 v = V[Y * np + mx];
-                        const float2 qq = Qt[(int64_t)X * L + (mx + np - 1)];
+                        const float2 qq = staged ? fscr[X * np + mx] : Qt[(int64_t)X * L + (mx + np - 1)];
                         const float re = fmaf(v.x, qq.x, -v.y * qq.y);
                         acc = fmaf(mx == 0 ? 1.f : 2.f, re, acc);
                     }
