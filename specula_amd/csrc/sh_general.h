@@ -237,10 +237,18 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
             }
             __syncthreads();
             if (!lgs && K == 1) {
-                // U[Y][kx] = sum_ky P[Y][ky] S[ky][kx] for the chunk's columns
+                // U[Y][kx] = sum_ky P[Y][ky] S[ky][kx] for the chunk's columns; P is staged in the
+                // transpose scratch (free between transforms) when it fits, so that the tasks of a
+                // chunk read it from LDS rather than each from L2
+                const bool pstage = npx * M <= 2 * (CAP + CAP / 4);
+                float* ps = reinterpret_cast<float*>(fscr);
+                if (pstage) {
+                    for (int e = threadIdx.x; e < npx * M; e += blockDim.x) ps[e] = g.roww[e];
+                    __syncthreads();
+                }
                 for (int e = threadIdx.x; e < npx * cols; e += blockDim.x) {
                     const int Y = e / cols, c = e - (e / cols) * cols;
-                    const float* pr = g.roww + (int64_t)Y * M;
+                    const float* pr = pstage ? ps + Y * M : g.roww + (int64_t)Y * M;
                     const float2* sc = buf + c * VS;
                     float acc = 0.f;
                     for (int ky = 0; ky < M; ++ky) acc = fmaf(pr[ky], sc[ky].x, acc);
@@ -286,11 +294,21 @@ sh_general_kernel(NgsArgs g, GenArgs q, int n_frames) {
                 ccdf[(int64_t)(i * npx + Y) * W + j * npx + X] = g.parseval ? acc * fs : acc;
             }
         } else if (!lgs) {
-            // 3 (NGS): B[Y][X] = sum_kx U[Y][kx] Q[X][kx]
+            // 3 (NGS): B[Y][X] = sum_kx U[Y][kx] Q[X][kx], with U and Q staged in LDS when they fit
+            const bool qstage = npx * M <= 2 * CAP;
+            float* us = reinterpret_cast<float*>(buf);
+            float* qs = reinterpret_cast<float*>(fscr);
+            if (qstage) {
+                for (int e = threadIdx.x; e < npx * M; e += blockDim.x) {
+                    us[e] = U[e];
+                    qs[e] = g.colw[e];
+                }
+                __syncthreads();
+            }
             for (int e = threadIdx.x; e < npx * npx; e += blockDim.x) {
                 const int Y = e / npx, X = e - (e / npx) * npx;
-                const float* ur = U + (int64_t)Y * M;
-                const float* qr = g.colw + (int64_t)X * M;
+                const float* ur = qstage ? us + Y * M : U + (int64_t)Y * M;
+                const float* qr = qstage ? qs + X * M : g.colw + (int64_t)X * M;
                 float acc = 0.f;
                 for (int kx = 0; kx < M; ++kx) acc = fmaf(ur[kx], qr[kx], acc);
                 bsum += (double)acc;
