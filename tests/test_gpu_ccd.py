@@ -173,6 +173,50 @@ def test_lean_and_full_photon_kernels_agree():
     assert min(frac) > 0.05, frac
     np.testing.assert_array_equal(lean, full)
 
+def _readout_raw(p, integrated, in_i, factor, counter, gains=None):
+    """shccd_readout with both sources: the integration buffer (read, then zeroed) and the fused
+    last step in_i * factor."""
+    import ctypes
+    F = in_i.shape[0]
+    n_in, n_out = p.in_rows * p.in_cols, p.out_rows * p.out_cols
+    out = torch.empty((F, p.out_rows, p.out_cols), dtype=torch.float32, device=DEV)
+    _native.check(_native.lib().shccd_readout(
+        ctypes.byref(p), F, _native.ptr(integrated), n_in, _native.ptr(in_i), n_in, float(factor),
+        _native.ptr(gains) if gains is not None else None, ctypes.c_uint64(counter), _native.ptr(out), n_out,
+        None), 'shccd_readout')
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize('readout', [False, True])
+def test_lean_and_full_photon_kernels_agree_with_integration_buffer(readout):
+    """Both signal sources at once (integration buffer + fused last step), a ragged frame of
+    several tiles, with and without readout noise (without it, pixels without signal draw
+    nothing): the lean kernel equals the full one bit for bit, and the buffer is zeroed."""
+    rng = np.random.default_rng(8)
+    F, rows, cols = 3, 61, 173                                  # 10553 px per frame: 11 tiles, ragged
+    integ = rng.gamma(0.5, 30.0, size=(F, rows, cols)).astype(np.float32)
+    integ[:, :, :25] = 0.0
+    cur = rng.gamma(0.5, 30.0, size=(F, rows, cols)).astype(np.float32)
+    cur[:, :, :25] = 0.0
+    kw = dict(photon_noise=True, quantum_eff=0.7)
+    if readout:
+        kw.update(readout_noise=True, readout_level=2.0)
+    p = params(rows, cols, **kw)
+    gains = torch.ones((rows, cols), dtype=torch.float32, device=DEV)
+    cur_t = torch.tensor(cur, device=DEV)
+    outs = []
+    for g in (None, gains):
+        buf = torch.tensor(integ, device=DEV)
+        outs.append(_readout_raw(p, buf, cur_t, 0.5, counter=11, gains=g))
+        assert float(buf.abs().max()) == 0.0
+    np.testing.assert_array_equal(outs[0], outs[1])
+    lam = (integ + cur * np.float32(0.5)) * np.float32(0.7)
+    assert np.mean(lam == 0) > 0.1 and np.mean((lam > 0) & (lam < 10)) > 0.05 and np.mean(lam >= 10) > 0.05
+    if not readout:
+        assert np.all(outs[0][lam == 0] == 0.0)
+
+
 def test_ccd_object_integrates_over_dt_and_resets():
     from specula_amd.data_objects.intensity import Intensity
     from specula_amd.data_objects.simul_params import SimulParams
