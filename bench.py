@@ -34,6 +34,31 @@ MEASURED_FLOP_PER_FRAME = {(True, True): 111_692_800}
 # after the first frame.
 MEASURED_LGS_TRAFFIC_PER_FRAME = 3_308_000
 FP32_PEAK_TFLOPS = 157.3
+# Instruction issue per frame, from rocprofv3 counters of the same launches (SQ_INSTS_VALU and
+# SQ_INSTS_MFMA per dispatch / frames per launch): profiles/pmc_issue_elt_ngs_80x80.md,
+# pmc_issue_lgs.md and pmc_issue_ccd.md.  A wave64 VALU instruction holds its SIMD 4 cycles, an
+# FP32 16x16x4 MFMA 32 cycles (FP32 MFMA does not co-issue with VALU on this part: the counters
+# show zero co-issue cycles), so (4 VALU + 32 MFMA) SIMD cycles per frame against 1024 SIMDs at
+# the 2.4 GHz peak engine clock is the issue roofline that bounds these kernels.
+MEASURED_ISSUE_PER_FRAME = {
+    'sh_ngs_mfma_kernel<8, true, 16, 3, false>': (464_520, 38_400),     # (VALU wave instr, MFMA)
+    'sh_lgs_forward_kernel<16, 32, 8>': (974_000, 40_000),
+    'ccd_photon_lean_kernel': (6_960_000, 0),
+}
+N_SIMDS = 1024
+PEAK_CLOCK_HZ = 2.4e9
+
+
+def issue_roofline(kernel, seconds_per_frame):
+    """Fraction of the SIMDs' peak issue rate that the kernel's counter-measured instructions
+    per frame take at the measured time per frame."""
+    per = MEASURED_ISSUE_PER_FRAME.get(kernel)
+    if per is None:
+        return None
+    cycles = 4 * per[0] + 32 * per[1]
+    frac = cycles / (N_SIMDS * PEAK_CLOCK_HZ * seconds_per_frame)
+    return {'bound': 'issue', 'valu_wave_instr_per_frame': per[0], 'mfma_per_frame': per[1],
+            'simd_cycles_per_frame': cycles, 'peak_clock_ghz': PEAK_CLOCK_HZ / 1e9, 'frac': round(frac, 4)}
 
 
 def parse():
@@ -176,6 +201,7 @@ def bench_lgs(torch, dev):
             'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'algorithmic_MB_per_frame': round(frame_bytes / 1e6, 2),
             'algorithmic_GBps': round(gbps, 1), 'hbm_frac': round(gbps / HBM_PEAK_GBS, 4),
             'measured_traffic_MB_per_frame': round(MEASURED_LGS_TRAFFIC_PER_FRAME / 1e6, 3),
+            'issue_roofline': issue_roofline('sh_lgs_forward_kernel<16, 32, 8>', kt / F),
             'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
 
 
@@ -452,6 +478,8 @@ def bench_ccd(torch, dev, plan, sl_plan, At, ph, ccd, slopes, flux, totals, F, g
     return {'frame': f'{g.ccd_side}x{g.ccd_side}', 'noise': 'photon + readout 1 e-, QE 0.3, ADU 8',
             'mean_photons_per_px': 300.0, 'readout_frames_per_s': round(F / t_ro, 1),
             'readout_GBps': round(nbytes / t_ro / 1e9, 1), 'readout_hbm_frac': round(nbytes / t_ro / 1e9 / HBM_PEAK_GBS, 4),
+            # the photon sampler is bound by instruction issue, not by its 13 MB per frame
+            'issue_roofline': issue_roofline('ccd_photon_lean_kernel', t_ro / F),
             'noisy_loop_frames_per_s': round(F / t_loop, 1), 'noisy_loop_ms_per_frame': round(t_loop * 1e3 / F, 5)}
 
 
@@ -571,6 +599,9 @@ def main():
         # second: an equivalent rate, not executed arithmetic (the kernel never runs those FFTs)
         'fft_equivalent_gflops': round(fft_flop * F / (kern_ms * 1e-3) / 1e9, 1),
         'executed': executed_flops(fused, args.fpb in (0, 4), F, kern_ms),
+        # the bound the counters show for this kernel: instruction issue (HBM is the contract's roofline)
+        'issue_roofline': issue_roofline(f'sh_ngs_mfma_kernel<8, true, 16, {3 if fused else 0}, false>'
+                                         if args.fpb in (0, 4) else None, kern_ms * 1e-3 / F),
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': round(achieved / HBM_PEAK_GBS, 4),
                      'traffic': MEASURED_TRAFFIC.get((bool(fused), F, args.fpb in (0, 4))),
