@@ -272,6 +272,14 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
                        const float* const* A, const float* const* phase, float* const* ccd,
                        const double* S0, int grid_blocks, int order, int device, shwfs_batch** out);
 int shwfs_batch_forward(shwfs_batch* batch, void* stream);
+/* Runs the ShSlopec of batch sensor `sensor` in the SH launch's epilogue, as shwfs_forward_slopes
+ * does for one plan (ShSlopec.calc_slopes_nofor, sh_slopec.py:308-438): the sensor's plan must have
+ * its ShSlopec plan bound (shwfs_plan_bind_slopec) and be an NGS MFMA np 8 plan with Parseval
+ * normalisation and 8 or 16 px per subaperture; every later shwfs_batch_forward then also writes
+ * its slopes (frames `slopes_stride` apart), flux and totals.  thr_mode 0 subtracts and clamps the
+ * threshold, 1 is the pedestal mode. */
+int shwfs_batch_set_slopes(shwfs_batch* batch, int sensor, float thr, int thr_mode, float* slopes,
+                           int64_t slopes_stride, double* flux, double* totals);
 /* Per-sensor completion time [us] of the last shwfs_batch_forward, from its start (synchronises
  * the stream). */
 int shwfs_batch_latencies(shwfs_batch* batch, double* us, void* stream);

@@ -21,7 +21,7 @@ EXPORTS = (
     'shwfs_plan_bind_slopec', 'shwfs_forward_slopes', 'shwfs_plan_set_pupil', 'shwfs_plan_pupil_registered',
     'shwfs_plan_set_resampling',
     'shslopec_vjp', 'shwfs_vjp', 'shwfs_vjp_sensor_grid', 'shwfs_resample', 'shwfs_pushpull_fields', 'shwfs_pushpull_rows', 'shwfs_im_from_jacobian', 'shwfs_im_from_jacobian_noalloc',
-    'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_latencies', 'shwfs_batch_info',
+    'shwfs_batch_create', 'shwfs_batch_forward', 'shwfs_batch_set_slopes', 'shwfs_batch_latencies', 'shwfs_batch_info',
     'shwfs_batch_destroy',
     'shccd_integrate', 'shccd_readout', 'shccd_readout_cte', 'shccd_philox',
     'shwfs_graph_begin', 'shwfs_graph_end', 'shwfs_graph_launch', 'shwfs_graph_destroy',
@@ -96,6 +96,7 @@ def _declare(lib):
         'shwfs_im_from_jacobian_noalloc': (_I, [_I, _I, _I, _P, _P, _I, _P, _I64, _P, _I, _F, _P, _P]),
         'shwfs_batch_create': (_I, [_I, _P, _P, _P, _P, _P, _P, _I, _I, _I, ctypes.POINTER(_P)]),
         'shwfs_batch_forward': (_I, [_P, _P]),
+        'shwfs_batch_set_slopes': (_I, [_P, _I, _F, _I, _P, _I64, _P, _P]),
         'shwfs_batch_latencies': (_I, [_P, _P, _P]),
         'shwfs_batch_info': (_I, [_P, _P, _P]),
         'shwfs_batch_destroy': (_I, [_P]),

@@ -63,7 +63,7 @@ def make_entry(name, plan, pupil, n_frames, S0=1.0, with_slopes=True, energy_th=
 class MultiSensorBatch:
     """shwfs_batch over a list of SensorEntry (see include/shwfs.h)."""
 
-    def __init__(self, entries, grid_blocks=0, order=0, device=0):
+    def __init__(self, entries, grid_blocks=0, order=0, device=0, fuse_slopes=True):
         self.entries = list(entries)
         self.device = device
         n = len(self.entries)
@@ -83,13 +83,29 @@ class MultiSensorBatch:
         _native.check(_native.lib().shwfs_batch_info(h, ctypes.byref(ni), ctypes.byref(gr)), 'shwfs_batch_info')
         self.n_items, self.grid = ni.value, gr.value
         self.graph = None
+        # sensors whose ShSlopec can run in the SH launch's epilogue (NGS MFMA np 8, Parseval, 8 or 16
+        # px per subaperture) get it there (the same slopes as SHPlan.forward_slopes); the others
+        # keep a CoG launch of their own after the SH launches
+        self.fused = [False] * n
+        if fuse_slopes:
+            for k, e in enumerate(self.entries):
+                if e.slopec is None or not e.plan.bind_slopec(e.slopec):
+                    continue
+                rc = _native.lib().shwfs_batch_set_slopes(h, k, 0.0, 0, _native.ptr(e.slopes), e.slopes.shape[1],
+                                                          _native.ptr(e.flux), _native.ptr(e.totals))
+                if rc == _native.SHWFS_ERR_UNSUPPORTED:
+                    continue
+                _native.check(rc, 'shwfs_batch_set_slopes')
+                self.fused[k] = True
 
     def forward(self, stream=None, slopes=True):
+        """The SH launches (with the fused sensors' CoG and their slopes), then, with ``slopes``,
+        the CoG launches of the other sensors."""
         st = _native.stream_handle(stream)
         _native.check(_native.lib().shwfs_batch_forward(self._h, st), 'shwfs_batch_forward')
         if slopes:
-            for e in self.entries:
-                if e.slopec is not None:
+            for e, fused in zip(self.entries, self.fused):
+                if e.slopec is not None and not fused:
                     e.slopec.forward(e.ccd, e.slopes, e.flux, e.totals, n_frames=e.n_frames, stream=stream)
 
     def latencies_us(self, stream=None):

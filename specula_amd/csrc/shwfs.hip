@@ -2162,9 +2162,20 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
             const int wv = threadIdx.x >> 6;
             float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
             const int wave_id = w.tile * NgsMfmaShape<8>::WPB + wv;
-            if (!g.parseval)       // image normalised by its own sum: partial sums, rescaled afterwards
+            if (!g.parseval) {     // image normalised by its own sum: partial sums, rescaled afterwards
                 ngs_mfma_frames<8, false, 0, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-            else if (g.npx == 16 && g.store_vec)
+            } else if (g.cog) {    // ShSlopec moments in the epilogue (shwfs_batch_set_slopes: npx 8 or 16)
+                const int mode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;
+                if (g.npx == 16) {
+                    if (mode == 3) ngs_mfma_frames<8, true, 16, 3, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+                    else if (mode == 1) ngs_mfma_frames<8, true, 16, 1, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+                    else ngs_mfma_frames<8, true, 16, 2, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+                } else {
+                    if (mode == 3) ngs_mfma_frames<8, true, 8, 3, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+                    else if (mode == 1) ngs_mfma_frames<8, true, 8, 1, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+                    else ngs_mfma_frames<8, true, 8, 2, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+                }
+            } else if (g.npx == 16 && g.store_vec)
                 ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
             else if (g.npx == 8 && g.store_vec)
                 ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
@@ -3563,9 +3574,23 @@ struct shwfs_batch {
         double S0;
         int n_frames;
     };
+    struct Slopes {                   // sensor whose ShSlopec runs in the SH launch's epilogue
+        const shslopec_plan* c;
+        double* tpart;                // [n_frames][n_parts][2] per-wave partial totals
+        int n_parts;
+        float* slopes;
+        int64_t slopes_stride;
+        double* flux;
+        double* totals;
+        int n_frames;
+    };
     std::vector<Resample> resample;
     std::vector<Rescale> rescale;
     std::vector<Direct> direct;
+    std::vector<Slopes> slopes;
+    std::vector<NgsArgs> h_args;      // host copy of d_args
+    std::vector<shwfs_plan*> plans;
+    std::vector<int> kinds, frames;
     std::vector<void*> bufs;          // device buffers of the above (resampled fields, partial sums)
 };
 
@@ -3743,7 +3768,45 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         (e = hipMemcpy(b->d_args, args.data(), n_sensors * sizeof(NgsArgs), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e)));
+    b->h_args = args;
+    b->plans.assign(plans, plans + n_sensors);
+    b->kinds = kinds;
+    b->frames.assign(n_frames, n_frames + n_sensors);
     *out = b;
+    return SHWFS_OK;
+}
+
+int shwfs_batch_set_slopes(shwfs_batch* b, int sensor, float thr, int thr_mode, float* slopes,
+                           int64_t slopes_stride, double* flux, double* totals) {
+    if (!b || !slopes || !flux || !totals) return set_error(SHWFS_ERR_ARG, "null argument");
+    if (sensor < 0 || sensor >= b->n_sensors) return set_error(SHWFS_ERR_ARG, "sensor index out of range");
+    if (thr_mode != 0 && thr_mode != 1) return set_error(SHWFS_ERR_ARG, "thr_mode must be 0 or 1");
+    shwfs_plan* p = b->plans[sensor];
+    const shslopec_plan* c = p->cog;
+    NgsArgs& a = b->h_args[sensor];
+    if (!c) return set_error(SHWFS_ERR_ARG, "no ShSlopec plan bound to the sensor's plan (shwfs_plan_bind_slopec)");
+    if (b->kinds[sensor] != 0 || !a.parseval || !a.store_vec || (a.npx != 8 && a.npx != 16))
+        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG in a batch needs an NGS MFMA np 8 sensor with Parseval "
+                                                "normalisation, 8 or 16 px per subaperture and 16-byte aligned CCD rows");
+    for (const auto& r : b->slopes)
+        if (r.c == c && r.tpart) return set_error(SHWFS_ERR_ARG, "this ShSlopec plan already serves a sensor of the batch");
+    const int nf = b->frames[sensor];
+    if (nf > c->max_frames) return set_error(SHWFS_ERR_ARG, "frame count above the ShSlopec plan's max_frames");
+    if (nf > 1 && slopes_stride < 2 * (int64_t)c->n_sub) return set_error(SHWFS_ERR_ARG, "slopes frame stride too small");
+    DeviceGuard dg(b->device);
+    void* tp = nullptr;
+    hipError_t e = hipMalloc(&tp, (size_t)nf * a.n_parts * 2 * sizeof(double));
+    if (e != hipSuccess) return set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e));
+    b->bufs.push_back(tp);
+    a.cog = thr_mode + 1;
+    a.thr = thr;
+    a.slot = p->d_slot;
+    a.mom = c->d_mom;
+    a.cogw = p->d_cogw;
+    a.tpart = static_cast<double*>(tp);
+    a.n_valid = c->n_sub;
+    SHWFS_HIP_CHECK(hipMemcpy(b->d_args + sensor, &a, sizeof(NgsArgs), hipMemcpyHostToDevice));
+    b->slopes.push_back({c, static_cast<double*>(tp), a.n_parts, slopes, slopes_stride, flux, totals, nf});
     return SHWFS_OK;
 }
 
@@ -3766,6 +3829,12 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
     if (b->n_ngs > 0) {
         hipLaunchKernelGGL(sh_batch_kernel<0>, dim3(std::min(b->grid, b->n_ngs)), dim3(256), kBatchNgsLds, st,
                            b->d_args, b->d_items + b->n_lgs, b->n_ngs, b->d_ts);
+        SHWFS_LAUNCH_CHECK();
+    }
+    for (const auto& r : b->slopes) {       // fused-CoG sensors: slopes, flux and totals from the moments
+        hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((r.c->n_sub + 1023) / 1024, r.n_frames), dim3(256), 0, st,
+                           r.c->d_mom, r.tpart, r.n_parts, r.c->n_sub, r.slopes, r.slopes_stride, r.flux, r.totals,
+                           (double*)nullptr);
         SHWFS_LAUNCH_CHECK();
     }
     for (const auto& c : b->rescale) {      // non-Parseval sensors: images normalised by their sums

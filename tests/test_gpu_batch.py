@@ -91,8 +91,9 @@ def test_autotuner(entries, tmp_path, monkeypatch):
 def test_full_multi_sensor_work_list_matches_per_sensor_and_oracle():
     """The whole uneven batch of the benchmark: 6 LGS 40x40 (N 128, launchers on a 20 m circle),
     NGS 10x10 and 20x20 (N 32) and 80x80 (N 64), and 2000 push-pull fields of the 80x80 sensor
-    (2009 frames) in one persistent SH launch plus the per-sensor CoG launches, replayed from a HIP
-    graph.  Images and slopes equal the per-sensor launches bit for bit; one frame of every sensor
+    (2009 frames) in one persistent SH launch (the NGS sensors' CoG in its epilogue) plus the LGS
+    sensors' CoG launches, replayed from a HIP graph.  Images and slopes equal the per-sensor
+    launches bit for bit; one frame of every sensor
     type (including the 10x10 / N 32 sensor and both signs of a push-pull pair) matches the FP64
     oracle chain (images 1e-4, slopes 1e-3 CCD px)."""
     from helpers import check_intensity, check_slopes, oracle_geometry
@@ -100,15 +101,25 @@ def test_full_multi_sensor_work_list_matches_per_sensor_and_oracle():
     full = workloads.multi_sensor_entries(n_lgs=6, n_pushpull=2000)
     assert [e.n_frames for e in full] == [1] * 9 + [2000]
     print('full work list: ' + ', '.join(f'{e.name} x{e.n_frames}' for e in full), flush=True)
+    b = MultiSensorBatch(full, grid_blocks=0, order=2)
+    # the NGS np 8 sensors run their CoG in the SH epilogue: the per-sensor reference is then the
+    # plan's own fused forward (static pupil registered, so its waves group the subapertures as the
+    # batch does), the others the two-pass chain
+    assert [e.name for e, f in zip(full, b.fused) if f] == ['scao_ngs_10x10', 'scao_ngs_20x20', 'elt_ngs_80x80',
+                                                            'elt80_pushpull']
     refs, srefs = [], []
-    for e in full:
+    for e, fused in zip(full, b.fused):
         out = torch.empty_like(e.ccd)
-        e.plan.forward(e.pupil, e.phase, out, e.S0, n_frames=e.n_frames)
         s = torch.empty_like(e.slopes)
-        e.slopec.forward(out, s, torch.empty_like(e.flux), torch.empty_like(e.totals), n_frames=e.n_frames)
+        if fused:
+            e.plan.set_static_pupil(e.pupil, e.S0)
+            e.plan.forward_slopes(e.pupil, e.phase, out, e.S0, s, torch.empty_like(e.flux), torch.empty_like(e.totals),
+                                  n_frames=e.n_frames)
+        else:
+            e.plan.forward(e.pupil, e.phase, out, e.S0, n_frames=e.n_frames)
+            e.slopec.forward(out, s, torch.empty_like(e.flux), torch.empty_like(e.totals), n_frames=e.n_frames)
         refs.append(out)
         srefs.append(s)
-    b = MultiSensorBatch(full, grid_blocks=0, order=2)
     b.capture()
     for rep in range(2):
         for e in full:
