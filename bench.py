@@ -42,7 +42,7 @@ FP32_PEAK_TFLOPS = 157.3
 # the 2.4 GHz peak engine clock is the issue roofline that bounds these kernels.
 MEASURED_ISSUE_PER_FRAME = {
     'sh_ngs_mfma_kernel<8, true, 16, 3, false>': (464_520, 38_400),     # (VALU wave instr, MFMA)
-    'sh_lgs_forward_kernel<16, 32, 8>': (974_000, 40_000),
+    'sh_lgs_forward_kernel<16, 32, 8, false>': (974_000, 40_000),
     'ccd_photon_lean_kernel': (6_960_000, 0),
 }
 N_SIMDS = 1024
@@ -190,18 +190,39 @@ def bench_lgs(torch, dev):
     _native.check(_native.lib().shwfs_plan_set_timing_events(plan._h, None, None), 'set_timing_events')
     torch.cuda.synchronize()
     kt = float(np.mean([hip_elapsed_ms(a, b) for a, b in ev])) * 1e-3
+    # SH + ShSlopec of the same frames: the CoG in the kernel's epilogue against its own pass
+    from specula_amd.lib.subaps import detect_subaps
+    from specula_amd.processing_objects.sh_slopec import SlopecPlan, compute_xy_weights
+    plan.forward(A, torch.zeros((1, g.ef_size, g.ef_size), dtype=torch.float32, device=dev), out[:1], 1.0)
+    idxs, _ = detect_subaps(out[0].cpu().numpy(), g.n_lenses, 0.5)
+    sc = SlopecPlan(idxs, (g.ccd_side, g.ccd_side), compute_xy_weights(g.subap_npx), max_frames=F, device=0)
+    n = idxs.shape[0]
+    sl = torch.empty((F, 2 * n), dtype=torch.float32, device=dev)
+    fl = torch.empty((F, n), dtype=torch.float64, device=dev)
+    tt = torch.empty((F, 2), dtype=torch.float64, device=dev)
+
+    def two_pass():
+        plan.forward(A, ph, out, 1.0, n_frames=F)
+        sc.forward(out, sl, fl, tt, n_frames=F)
+    dt_two = timed(two_pass, 10, torch.cuda.synchronize)
+    if not plan.bind_slopec(sc):
+        raise RuntimeError('lgs_40x40: the ShSlopec could not be fused')
+    dt_fused = timed(lambda: plan.forward_slopes(A, ph, out, 1.0, sl, fl, tt, n_frames=F), 10, torch.cuda.synchronize)
+    plan.bind_slopec(None)
     kbytes = g.n_lenses ** 2 * (2 * g.ovs_np_sub - 1) * g.ovs_np_sub * 8     # Hermitian half of the lag grid read
     frame_bytes = 4 * (2 * g.ef_size ** 2 + g.ccd_side ** 2) + kbytes          # pupil + phase + CCD + spectra
     gbps = frame_bytes * F / kt / 1e9
     return {'workload': 'lgs_40x40', 'frames_per_s': round(F / dt, 1), 'ms_per_frame': round(dt * 1e3 / F, 5),
-            'kernel': 'sh_lgs_forward_kernel<16, 32, 8>', 'kernel_ms_per_launch': round(kt * 1e3, 5),
+            'kernel': 'sh_lgs_forward_kernel<16, 32, 8, false>', 'kernel_ms_per_launch': round(kt * 1e3, 5),
             # algorithmic bytes count the kernel spectra on the Hermitian half of the lag grid
             # (|m| <= np - 1, m_x >= 0: all the correlation uses), not the full N x N spectra;
             # the HBM fraction is per kernel duration, frames_per_s per call (host launch included)
             'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'algorithmic_MB_per_frame': round(frame_bytes / 1e6, 2),
             'algorithmic_GBps': round(gbps, 1), 'hbm_frac': round(gbps / HBM_PEAK_GBS, 4),
             'measured_traffic_MB_per_frame': round(MEASURED_LGS_TRAFFIC_PER_FRAME / 1e6, 3),
-            'issue_roofline': issue_roofline('sh_lgs_forward_kernel<16, 32, 8>', kt / F),
+            'issue_roofline': issue_roofline('sh_lgs_forward_kernel<16, 32, 8, false>', kt / F),
+            'sh_slopes_frames_per_s': {'fused_cog': round(F / dt_fused, 1), 'two_pass': round(F / dt_two, 1),
+                                       'valid_subaps': int(n)},
             'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
 
 

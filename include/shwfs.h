@@ -167,9 +167,10 @@ int shslopec_forward(shslopec_plan* plan, int n_frames,
  * (frame scale x norm_const x the subaperture's sum(A^2)) rather than the sum of its FP32 pixels,
  * and without threshold the moments keep the FP32 image's rounding-level negative pixels, which the
  * reference's clamp at zero (sh_slopec.py:382-386) would remove from its exact, non-negative image.
- * shwfs_plan_bind_slopec checks that the pair can be fused: NGS MFMA plan with np_sub 8,
- * Parseval normalisation and 8 or 16 px per subaperture; ShSlopec plan made from block origins
- * lying on the lenslet grid of this CCD, with the default separable weights.  It returns
+ * shwfs_plan_bind_slopec checks that the pair can be fused: Parseval normalisation and an NGS
+ * MFMA plan with np_sub 8 and 8 or 16 px per subaperture, or an LGS plan on a specialised LGS
+ * kernel (np_sub 6, 8, 12 or 16, at most 16 px); ShSlopec plan made from block origins lying on
+ * the lenslet grid of this CCD, with the default separable weights.  It returns
  * SHWFS_ERR_UNSUPPORTED otherwise (then call the two functions).  The bound ShSlopec plan must
  * outlive its use; bind NULL to unbind.  Replaces the SH -> ShSlopec hand-over of one loop step
  * (processing_objects/sh.py:348-428 followed by sh_slopec.py:308-438). */
@@ -257,9 +258,10 @@ int shwfs_im_from_jacobian_noalloc(int n_lenses, int np_sub, int ef_size, const 
 
 /* ---- multi-sensor batch -----------------------------------------------------------------
  * No reference counterpart (the reference triggers one SH object at a time, each optionally a
- * captured CUDA graph, base_processing_obj.py:114-155).  One persistent launch per body type
- * (LGS np 16 / N 128 tiles, then NGS MFMA np 8 tiles of up to 4 streamed frames) runs every
- * (sensor, frame, subaperture tile) of several sensors of uneven shapes; sensors with field
+ * captured CUDA graph, base_processing_obj.py:114-155).  Persistent launches per body type (LGS
+ * np 16 / N 128 tiles, then NGS MFMA np 8 tiles of up to 4 streamed frames; one launch per body
+ * variant present, those with fewer work items first) run every (sensor, frame, subaperture tile)
+ * of several sensors of uneven shapes; sensors with field
  * resampling have their phases resampled before it and sensors normalised by their own image sum
  * are rescaled after it.  Any other plan (other np, the any-size path, round FoV masks) runs its
  * own launch in the same sequence (n_frames <= its max_frames).  All buffers are fixed at creation
@@ -274,10 +276,11 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
 int shwfs_batch_forward(shwfs_batch* batch, void* stream);
 /* Runs the ShSlopec of batch sensor `sensor` in the SH launch's epilogue, as shwfs_forward_slopes
  * does for one plan (ShSlopec.calc_slopes_nofor, sh_slopec.py:308-438): the sensor's plan must have
- * its ShSlopec plan bound (shwfs_plan_bind_slopec) and be an NGS MFMA np 8 plan with Parseval
- * normalisation and 8 or 16 px per subaperture; every later shwfs_batch_forward then also writes
- * its slopes (frames `slopes_stride` apart), flux and totals.  thr_mode 0 subtracts and clamps the
- * threshold, 1 is the pedestal mode. */
+ * its ShSlopec plan bound (shwfs_plan_bind_slopec), Parseval normalisation, and run in one of the
+ * batch's persistent bodies (NGS MFMA np 8 with 8 or 16 px per subaperture and 16-byte aligned CCD
+ * rows, or LGS np 16 / M 32); every later shwfs_batch_forward then also writes its slopes (frames
+ * `slopes_stride` apart), flux and totals.  thr_mode 0 subtracts and clamps the threshold, 1 is
+ * the pedestal mode.  A set-up call (synchronous): call it before capturing the batch in a graph. */
 int shwfs_batch_set_slopes(shwfs_batch* batch, int sensor, float thr, int thr_mode, float* slopes,
                            int64_t slopes_stride, double* flux, double* totals);
 /* Per-sensor completion time [us] of the last shwfs_batch_forward, from its start (synchronises

@@ -83,9 +83,9 @@ class MultiSensorBatch:
         _native.check(_native.lib().shwfs_batch_info(h, ctypes.byref(ni), ctypes.byref(gr)), 'shwfs_batch_info')
         self.n_items, self.grid = ni.value, gr.value
         self.graph = None
-        # sensors whose ShSlopec can run in the SH launch's epilogue (NGS MFMA np 8, Parseval, 8 or 16
-        # px per subaperture) get it there (the same slopes as SHPlan.forward_slopes); the others
-        # keep a CoG launch of their own after the SH launches
+        # sensors whose ShSlopec can run in the SH launch's epilogue (Parseval normalisation; NGS MFMA
+        # np 8 with 8 or 16 px per subaperture, or the LGS np 16 body) get it there (the same slopes
+        # as SHPlan.forward_slopes); the others keep a CoG launch of their own after the SH launches
         self.fused = [False] * n
         if fuse_slopes:
             for k, e in enumerate(self.entries):

@@ -27,6 +27,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/shwfs.h"
@@ -979,13 +980,23 @@ struct LgsShape {
     static constexpr int SPN = 16 / LH;           // subapertures per 16-column block of the MFMA phases
     static constexpr int WAVES = TS / (2 * SPN);  // two column blocks per wave
     static constexpr int BT = 64 * WAVES;         // threads per block
+    // fused CoG: LDS after the tile buffer for the per-wave frame-total partials (FP64 pairs) and
+    // the column square sums of the pupil samples (one per phase-1 task)
+    static constexpr int COG_BYTES = 2 * WAVES * 8 + TS * NP * 4;
 };
 
 // The lag-domain image D = C K is Hermitian on the (m_y, m_x) grid (C is an autocorrelation, K
 // the spectrum of a real kernel), and so are the binning operators (transforms of real ones):
 // Re sum_{m_y, m_x} P[Y][m_y] D[m_y][m_x] Q[X][m_x] only needs the columns m_x >= 0, the
 // m_x > 0 ones counted twice.  Phases 2-4 therefore carry LH = NP lag columns instead of 2NP-1.
-template <int NP, int M, int TS>
+//
+// COG: the ShSlopec centre of gravity (sh_slopec.py:365-404) of the tile's subapertures from the
+// binned pixels of phase 4 while they are in registers: per subaperture the thresholded total
+// and the x / y moments (FP32 over 32 pixels, FP64 across lanes) and the Parseval flux
+// (scale x norm_const x sum(A^2) of the subaperture), into g.mom at the subaperture's slot, plus
+// the tile's share of the frame totals into g.tpart; cog_finalize_parts_kernel turns them into
+// slopes.  `buf` then extends Sh::COG_BYTES past the tile buffer.
+template <int NP, int M, int TS, bool COG = false>
 __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, float2* buf, double* red) {
     constexpr int L = 2 * NP - 1;
     using Sh = LgsShape<NP, M, TS>;
@@ -995,6 +1006,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     const int n = g.n_lenses;
     const int sub0 = tile * TS;
     const float2* Qc = reinterpret_cast<const float2*>(g.colw);
+    double* cog_part = reinterpret_cast<double*>(buf + Sh::LDS_F2);     // [WAVES][2]
+    float* cog_sq = reinterpret_cast<float*>(cog_part + 2 * WAVES);    // [TS * NP]
 
     // tile position s -> lenslet index (the plan's lit-first order for a registered static pupil)
     auto lenslet = [&](int s) { const int pos = sub0 + s; return g.order ? g.order[pos] : pos; };
@@ -1020,12 +1033,33 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             lit |= av[y] != 0.f;
         }
     }
+    if constexpr (COG) {
+        if (t < TS * NP) {
+            float q = 0.f;
+#pragma unroll
+            for (int y = 0; y < NP; ++y) q = fmaf(av[y], av[y], q);
+            cog_sq[t] = q;                            // visible after the barrier below
+        }
+    }
     double scale = 1.0;                               // Parseval flux scale of the frame
     if (g.parseval) {
         const double* am = g.amp + (int64_t)f * g.amp_fstep;
         scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
     }
     if (!__syncthreads_or(lit)) {
+        if constexpr (COG) {
+            // every pixel of the tile is 0 x the scale: so are its moments and flux
+            const double zd = 0.0 * scale;
+            if (t < TS && sub0 + t < g.n_sub) {
+                const int slot = g.slot[lenslet(t)];
+                if (slot >= 0) {
+                    double* m = g.mom + ((int64_t)f * g.n_valid + slot) * 4;
+                    *reinterpret_cast<double2*>(m) = make_double2(zd, zd);
+                    *reinterpret_cast<double2*>(m + 2) = make_double2(zd, zd);
+                }
+            }
+            if (t == 0) *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + tile) * 2) = make_double2(zd, zd);
+        }
         // 0 times the scale: a frame without any light gives the reference's 0/0 (NaN) as the
         // full transform would
         const float z = (float)(0.0 * scale);
@@ -1211,6 +1245,22 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             qr[ks] = w * qc.x;
             nqi[ks] = -w * qc.y;
         }
+        const int Y = n16, X0 = 4 * k4;
+        // fused CoG: CoG weights of the lane's pixels (x of its 4 columns, y of its row; 0 outside
+        // the subaperture) and its per-subaperture FP32 partials: thresholded total, x and y moments
+        float cxw[4] = {0.f, 0.f, 0.f, 0.f}, cyw = 0.f;
+        float cpart[SPW][3];
+        int cmode = 0;
+        if constexpr (COG) {
+            cmode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;   // 3: no threshold (no clamp, as the NGS epilogue)
+            if (Y < g.npx) {
+                cyw = g.cogw[g.npx + Y];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) cxw[r] = X0 + r < g.npx ? g.cogw[X0 + r] : 0.f;
+            }
+#pragma unroll
+            for (int u = 0; u < SPW; ++u) cpart[u][0] = cpart[u][1] = cpart[u][2] = 0.f;
+        }
 #pragma unroll
         for (int u = 0; u < SPW; ++u) {
             const int s = wave * SPW + u;
@@ -1225,7 +1275,6 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(qr[ks], v.x, acc, 0, 0, 0);
                 acc = __builtin_amdgcn_mfma_f32_16x16x4f32(nqi[ks], v.y, acc, 0, 0, 0);
             }
-            const int Y = n16, X0 = 4 * k4;
             if (Y < g.npx && X0 < g.npx) {
                 const int i = sub / n, j = sub - (sub / n) * n;
                 float* dst = g.ccd + f * g.ccd_stride + (int64_t)(i * g.npx + Y) * W + (int64_t)j * g.npx + X0;
@@ -1238,6 +1287,79 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                     for (int r = 0; r < 4; ++r)
                         if (X0 + r < g.npx) dst[r] = o[r];
                 }
+                if constexpr (COG) {
+                    float w[4];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const float v = o[r];
+                        const float tv = cmode == 1 ? clamp0(v - g.thr) : cmode == 2 ? (v < g.thr ? 0.f : v) : v;
+                        w[r] = X0 + r < g.npx ? tv : 0.f;
+                    }
+                    const float tot = (w[0] + w[1]) + (w[2] + w[3]);
+                    cpart[u][0] = tot;
+                    cpart[u][1] = fmaf(w[3], cxw[3], fmaf(w[2], cxw[2], fmaf(w[1], cxw[1], w[0] * cxw[0])));
+                    cpart[u][2] = tot * cyw;
+                }
+            }
+        }
+        if constexpr (COG) {
+            // reduce each subaperture's partials over its 64 lanes (16 rows Y x 4 column groups):
+            // the column groups in FP32 (lane bits 4-5, splitting the subapertures between the
+            // halves and quarters of the wave so that one exchange serves two of them), then the
+            // 16 rows in FP64 (lane bits 0-3, the first step in FP32: 32 pixels per FP32 partial).
+            // Afterwards lane 16 r holds subaperture u = r (SPW 4) or r / 2 (SPW 2).
+            static_assert(SPW == 2 || SPW == 4, "2 or 4 subapertures per wave");
+            float c[3];
+            const bool hi32 = lane >= 32, hi16 = (lane & 16) != 0;
+            if constexpr (SPW == 2) {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    const float keep = hi32 ? cpart[1][v] : cpart[0][v], send = hi32 ? cpart[0][v] : cpart[1][v];
+                    c[v] = keep + __shfl_xor(send, 32, 64);
+                    c[v] += __shfl_xor(c[v], 16, 64);
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 3; ++v) {
+                    float a = hi32 ? cpart[2][v] : cpart[0][v], b = hi32 ? cpart[3][v] : cpart[1][v];
+                    a += __shfl_xor(hi32 ? cpart[0][v] : cpart[2][v], 32, 64);
+                    b += __shfl_xor(hi32 ? cpart[1][v] : cpart[3][v], 32, 64);
+                    c[v] = (hi16 ? b : a) + __shfl_xor(hi16 ? a : b, 16, 64);
+                }
+            }
+            const int u_own = SPW == 4 ? (lane >> 4) : (lane >> 5);
+            const int s_own = wave * SPW + u_own;
+            // sum(A^2) of the subaperture: one phase-1 column per lane n16 < NP
+            const float sq = n16 < NP ? cog_sq[s_own * NP + n16] : 0.f;
+#pragma unroll
+            for (int v = 0; v < 3; ++v) c[v] += __shfl_xor(c[v], 1, 64);
+            double m_t = c[0], m_x = c[1], m_y = c[2], m_q = (double)sq + (double)__shfl_xor(sq, 1, 64);
+#pragma unroll
+            for (int o = 2; o < 16; o <<= 1) {
+                m_t += __shfl_xor(m_t, o, 64);
+                m_x += __shfl_xor(m_x, o, 64);
+                m_y += __shfl_xor(m_y, o, 64);
+                m_q += __shfl_xor(m_q, o, 64);
+            }
+            const int slot = (sub0 + s_own < g.n_sub) ? g.slot[lenslet(s_own)] : -1;
+            const bool writer = n16 == 0 && (SPW == 4 || !hi16) && slot >= 0;
+            const double m_f = m_q * (scale * g.norm_const);   // Parseval flux of the subaperture
+            if (writer) {
+                double* m = g.mom + ((int64_t)f * g.n_valid + slot) * 4;
+                *reinterpret_cast<double2*>(m) = make_double2(m_f, m_t);
+                *reinterpret_cast<double2*>(m + 2) = make_double2(m_x, m_y);
+            }
+            // the tile's share of the frame totals (flux, thresholded total), fixed order
+            double p_f = writer ? m_f : 0.0, p_t = writer ? m_t : 0.0;
+            p_f += __shfl_xor(p_f, 16, 64); p_t += __shfl_xor(p_t, 16, 64);
+            p_f += __shfl_xor(p_f, 32, 64); p_t += __shfl_xor(p_t, 32, 64);
+            if (lane == 0) { cog_part[2 * wave] = p_f; cog_part[2 * wave + 1] = p_t; }
+            __syncthreads();
+            if (t == 0) {
+                double sf = 0.0, st = 0.0;
+#pragma unroll
+                for (int w = 0; w < WAVES; ++w) { sf += cog_part[2 * w]; st += cog_part[2 * w + 1]; }
+                *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + tile) * 2) = make_double2(sf, st);
             }
         }
     }
@@ -1247,12 +1369,12 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     }
 }
 
-template <int NP, int M, int TS>
+template <int NP, int M, int TS, bool COG>
 __global__ void __launch_bounds__((LgsShape<NP, M, TS>::BT)) __attribute__((amdgpu_waves_per_eu(4, 8)))
 sh_lgs_forward_kernel(NgsArgs g) {
     extern __shared__ float4 lds_raw[];
     __shared__ double red[4];
-    lgs_tile<NP, M, TS>(g, blockIdx.x, blockIdx.y, reinterpret_cast<float2*>(lds_raw), red);
+    lgs_tile<NP, M, TS, COG>(g, blockIdx.x, blockIdx.y, reinterpret_cast<float2*>(lds_raw), red);
 }
 
 // Lit flag of each subaperture (a nonzero pupil sample): one wave per subaperture, a ballot over
@@ -2133,7 +2255,7 @@ struct BatchItem {
 };
 
 constexpr int kBatchNgsLds = NgsMfmaShape<8>::LDS_F4 * 16;
-constexpr int kBatchLgsLds = LgsShape<16, 32, 8>::LDS_F2 * 8;
+constexpr int kBatchLgsLds = LgsShape<16, 32, 8>::LDS_F2 * 8 + LgsShape<16, 32, 8>::COG_BYTES;
 constexpr int kBatchNgsFpb = 4;          // frames streamed per NGS item (next-frame prefetch)
 
 __global__ void batch_stamp_kernel(unsigned long long* __restrict__ ts, int n_sensors) {
@@ -2146,9 +2268,41 @@ __global__ void batch_sensor_stamp_kernel(unsigned long long* __restrict__ ts, i
     if (threadIdx.x == 0) ts[1 + sensor] = (unsigned long long)wall_clock64();
 }
 
-// One persistent launch per body type (KIND 0: NGS MFMA np 8 tiles, KIND 1: LGS np 16 tiles), so
-// that each runs with its own register and LDS footprint; blocks stride over the item list.
-template <int KIND>
+// Body variant of a batch sensor: which instantiation of its body (KIND 0: NGS MFMA np 8 tiles,
+// KIND 1: LGS np 16 tiles) its items run.  NGS: 0 image normalised by its own sum, 1-3 fused CoG
+// with 16 px per subaperture (no threshold, subtracted threshold, pedestal), 4-6 the same with 8 px,
+// 7 / 8 plain with 16 / 8 px and 16-byte aligned CCD rows, 9 any other; LGS: 0 plain, 1 fused CoG.
+constexpr int kBatchVariants[2] = {10, 2};
+__host__ __device__ inline int batch_variant(const NgsArgs& g, int kind) {
+    if (kind == 1) return g.cog ? 1 : 0;
+    if (!g.parseval) return 0;
+    if (g.cog) {
+        const int mode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;
+        return (g.npx == 16 ? 1 : 4) + (mode == 3 ? 0 : mode);
+    }
+    if (g.npx == 16 && g.store_vec) return 7;
+    if (g.npx == 8 && g.store_vec) return 8;
+    return 9;
+}
+
+template <int VAR>
+__device__ __forceinline__ void batch_ngs_body(const NgsArgs& g, int wave_id, int f0, int f1, float* sw) {
+    if constexpr (VAR == 0) ngs_mfma_frames<8, false, 0, 0, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 1) ngs_mfma_frames<8, true, 16, 3, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 2) ngs_mfma_frames<8, true, 16, 1, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 3) ngs_mfma_frames<8, true, 16, 2, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 4) ngs_mfma_frames<8, true, 8, 3, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 5) ngs_mfma_frames<8, true, 8, 1, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 6) ngs_mfma_frames<8, true, 8, 2, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 7) ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, f0, f1, sw);
+    else if constexpr (VAR == 8) ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, f0, f1, sw);
+    else ngs_mfma_frames<8, true, 0, 0, false>(g, wave_id, f0, f1, sw);
+}
+
+// One persistent launch per body type and variant present, so that each runs with its own
+// register and LDS footprint (one kernel holding several variants spills registers); blocks
+// stride over the variant's items (grouped by shwfs_batch's layout).
+template <int KIND, int VAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8)))
 sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict__ items, int n_items,
                 unsigned long long* __restrict__ ts) {
@@ -2161,32 +2315,22 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
         if constexpr (KIND == 0) {
             const int wv = threadIdx.x >> 6;
             float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
-            const int wave_id = w.tile * NgsMfmaShape<8>::WPB + wv;
-            if (!g.parseval) {     // image normalised by its own sum: partial sums, rescaled afterwards
-                ngs_mfma_frames<8, false, 0, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-            } else if (g.cog) {    // ShSlopec moments in the epilogue (shwfs_batch_set_slopes: npx 8 or 16)
-                const int mode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;
-                if (g.npx == 16) {
-                    if (mode == 3) ngs_mfma_frames<8, true, 16, 3, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-                    else if (mode == 1) ngs_mfma_frames<8, true, 16, 1, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-                    else ngs_mfma_frames<8, true, 16, 2, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-                } else {
-                    if (mode == 3) ngs_mfma_frames<8, true, 8, 3, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-                    else if (mode == 1) ngs_mfma_frames<8, true, 8, 1, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-                    else ngs_mfma_frames<8, true, 8, 2, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-                }
-            } else if (g.npx == 16 && g.store_vec)
-                ngs_mfma_frames<8, true, 16, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-            else if (g.npx == 8 && g.store_vec)
-                ngs_mfma_frames<8, true, 8, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
-            else
-                ngs_mfma_frames<8, true, 0, 0, false>(g, wave_id, w.frame, w.frame + w.nf, sw);
+            batch_ngs_body<VAR>(g, w.tile * NgsMfmaShape<8>::WPB + wv, w.frame, w.frame + w.nf, sw);
         } else {
-            lgs_tile<16, 32, 8>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
+            lgs_tile<16, 32, 8, VAR == 1>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
         }
         __syncthreads();
         if (threadIdx.x == 0) atomicMax(&ts[1 + w.sensor], (unsigned long long)wall_clock64());
     }
+}
+
+using BatchKernel = void (*)(const NgsArgs*, const BatchItem*, int, unsigned long long*);
+
+template <int KIND, int... V>
+BatchKernel batch_kernel_of(int var, std::integer_sequence<int, V...>) {
+    BatchKernel k = nullptr;
+    ((k = (var == V) ? sh_batch_kernel<KIND, V> : k), ...);
+    return k;
 }
 
 }  // namespace
@@ -2289,7 +2433,7 @@ struct shslopec_plan {
     bool rows_path = false;        // cog_rows_kernel (separable weights, square blocks)
     bool vec4 = false;
     std::vector<int32_t> h_origin; // block origins (host copy; empty for index-list plans)
-    std::vector<double> h_sep;     // xv | yv of the separable weights (rows_path only)
+    std::vector<double> h_sep;     // xv | yv of separable weights of square blocks (empty otherwise)
     int32_t* d_sorted = nullptr;   // row-major sorted block origins
     int32_t* d_order = nullptr;    // subaperture index of each sorted slot
     double* d_sep = nullptr;       // xv | yv
@@ -2389,11 +2533,16 @@ using NgsLauncher = int (*)(shwfs_plan*, const NgsArgs&, int, hipStream_t);
 
 template <int NP, int M, int TS>
 int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
-    constexpr int lds = LgsShape<NP, M, TS>::LDS_F2 * (int)sizeof(float2);
-    static_assert(lds <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
+    using Sh = LgsShape<NP, M, TS>;
+    constexpr int lds = Sh::LDS_F2 * (int)sizeof(float2);
+    static_assert(lds + Sh::COG_BYTES <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
     dim3 grid(p->n_tiles, n_frames);
-    hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS>), grid, dim3((LgsShape<NP, M, TS>::BT)), lds, st, t_ev_start,
-                          t_ev_stop, 0, a);
+    if (a.cog)      // ShSlopec moments in the epilogue (shwfs_forward_slopes)
+        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, true>), grid, dim3(Sh::BT), lds + Sh::COG_BYTES, st,
+                              t_ev_start, t_ev_stop, 0, a);
+    else
+        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, false>), grid, dim3(Sh::BT), lds, st, t_ev_start,
+                              t_ev_stop, 0, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
 }
@@ -2925,11 +3074,15 @@ int shwfs_plan_bind_slopec(shwfs_plan* p, shslopec_plan* c) {
     if (!c) return SHWFS_OK;
     const shwfs_geom& g = p->g;
     const int n = g.n_lenses, npx = g.subap_npx, W = n * npx;
-    if (g.lgs || !p->mfma || g.np_sub != 8 || !g.parseval || (npx != 8 && npx != 16))
-        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs an NGS MFMA plan with np_sub 8, Parseval "
-                                                "normalisation and 8 or 16 px per subaperture");
+    int lgs_ts = 0;
+    const bool lgs_ok = g.lgs && !p->general && pick_lgs(g.np_sub, g.spec_size, &lgs_ts) && npx <= 16;
+    const bool ngs_ok = !g.lgs && p->mfma && g.np_sub == 8 && (npx == 8 || npx == 16);
+    if (!g.parseval || !(lgs_ok || ngs_ok))
+        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs Parseval normalisation and an NGS MFMA plan with "
+                                                "np_sub 8 and 8 or 16 px per subaperture, or an LGS plan on a "
+                                                "specialised LGS kernel");
     if (c->device != p->device) return set_error(SHWFS_ERR_ARG, "plans on different devices");
-    if (!c->rows_path || c->np_sub != npx || c->frame_h != W || c->frame_w != W)
+    if (c->h_sep.empty() || c->np_sub != npx || c->frame_h != W || c->frame_w != W)
         return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG needs square subaperture blocks of the SH pixel "
                                                 "grid with separable weights (the ShSlopec default)");
     std::vector<int32_t> slot((size_t)p->n_sub, -1);
@@ -3030,14 +3183,21 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
         if ((e = hipMalloc((void**)&p->d_origin, n_sub * sizeof(int32_t))) != hipSuccess ||
             (e = hipMemcpy(p->d_origin, block_origin, n_sub * sizeof(int32_t), hipMemcpyHostToDevice)) != hipSuccess)
             return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
-        // separable default weights -> row kernel with row-major processing order
-        bool sep = (np_sub == 4 || np_sub == 8 || np_sub == 16);
+        // separable default weights (x weight by column, y weight by row, no mask): kept for the
+        // fused CoG of shwfs_plan_bind_slopec; with 4, 8 or 16 px blocks also the row kernel with
+        // row-major processing order
+        bool sep = true;
         for (int64_t r = 0; sep && r < np_sub; ++r)
             for (int64_t c = 0; c < np_sub; ++c) {
                 const int64_t q = r * np_sub + c;
                 if (mw[q] != 1.0 || xw[q] != xw[c] || yw[q] != yw[r * np_sub]) { sep = false; break; }
             }
         if (sep) {
+            p->h_sep.resize(2 * (size_t)np_sub);
+            for (int c = 0; c < np_sub; ++c) p->h_sep[c] = xw[c];
+            for (int r = 0; r < np_sub; ++r) p->h_sep[np_sub + r] = yw[(int64_t)r * np_sub];
+        }
+        if (sep && (np_sub == 4 || np_sub == 8 || np_sub == 16)) {
             std::vector<int32_t> order(n_sub), sorted(n_sub);
             for (int s = 0; s < n_sub; ++s) order[s] = s;
             std::stable_sort(order.begin(), order.end(),
@@ -3059,7 +3219,6 @@ int shslopec_plan_create(int n_sub, int np_sub, int frame_h, int frame_w,
                 return cleanup(set_error(SHWFS_ERR_HIP, std::string("slopec plan: ") + hipGetErrorString(e)));
             p->rows_path = true;
             p->vec4 = aligned;
-            p->h_sep = sepw;
         }
     } else {
         if ((e = hipMalloc((void**)&p->d_idx, n_sub * npp * sizeof(int32_t))) != hipSuccess ||
@@ -3592,7 +3751,41 @@ struct shwfs_batch {
     std::vector<shwfs_plan*> plans;
     std::vector<int> kinds, frames;
     std::vector<void*> bufs;          // device buffers of the above (resampled fields, partial sums)
+    std::vector<BatchItem> h_items;   // the work list in the requested order (LGS items, then NGS)
+    int grid_req = 0;                 // blocks per launch requested (0: one per item)
+    struct Launch {                   // one persistent launch: items [offset, offset + count) of d_items
+        int kind, variant, offset, count;
+    };
+    std::vector<Launch> launches;
 };
+
+namespace {
+// Groups the work list by body variant (each variant is a kernel of its own, over a contiguous
+// range of d_items; the requested order holds within a variant) and orders the launches: LGS
+// before NGS, within a kind the variants with fewer items first, so that the sensors of a
+// real-time step complete before a large calibration batch.  Uploads d_items (synchronous).
+int batch_layout(shwfs_batch* b) {
+    b->launches.clear();
+    std::vector<BatchItem> sorted;
+    sorted.reserve(b->h_items.size());
+    for (int kind = 1; kind >= 0; --kind) {
+        std::vector<std::vector<BatchItem>> per(kBatchVariants[kind]);
+        for (const auto& it : b->h_items)
+            if (b->kinds[it.sensor] == kind) per[batch_variant(b->h_args[it.sensor], kind)].push_back(it);
+        std::vector<int> vs;
+        for (int v = 0; v < kBatchVariants[kind]; ++v)
+            if (!per[v].empty()) vs.push_back(v);
+        std::stable_sort(vs.begin(), vs.end(), [&](int x, int y) { return per[x].size() < per[y].size(); });
+        for (const int v : vs) {
+            b->launches.push_back({kind, v, (int)sorted.size(), (int)per[v].size()});
+            sorted.insert(sorted.end(), per[v].begin(), per[v].end());
+        }
+    }
+    const hipError_t e = hipMemcpy(b->d_items, sorted.data(), sorted.size() * sizeof(BatchItem), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return set_error(SHWFS_ERR_HIP, std::string("batch items: ") + hipGetErrorString(e));
+    return SHWFS_OK;
+}
+}  // namespace
 
 extern "C" {
 
@@ -3765,13 +3958,15 @@ int shwfs_batch_create(int n_sensors, shwfs_plan* const* plans, const int32_t* n
         }
     }
     if ((e = hipGetLastError()) != hipSuccess || (e = hipDeviceSynchronize()) != hipSuccess ||
-        (e = hipMemcpy(b->d_args, args.data(), n_sensors * sizeof(NgsArgs), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice)) != hipSuccess)
+        (e = hipMemcpy(b->d_args, args.data(), n_sensors * sizeof(NgsArgs), hipMemcpyHostToDevice)) != hipSuccess)
         return cleanup(set_error(SHWFS_ERR_HIP, std::string("batch setup: ") + hipGetErrorString(e)));
     b->h_args = args;
     b->plans.assign(plans, plans + n_sensors);
     b->kinds = kinds;
     b->frames.assign(n_frames, n_frames + n_sensors);
+    b->h_items = items;
+    b->grid_req = grid_blocks > 0 ? grid_blocks : 0;
+    if (const int rc = batch_layout(b)) return cleanup(rc);
     *out = b;
     return SHWFS_OK;
 }
@@ -3785,9 +3980,12 @@ int shwfs_batch_set_slopes(shwfs_batch* b, int sensor, float thr, int thr_mode, 
     const shslopec_plan* c = p->cog;
     NgsArgs& a = b->h_args[sensor];
     if (!c) return set_error(SHWFS_ERR_ARG, "no ShSlopec plan bound to the sensor's plan (shwfs_plan_bind_slopec)");
-    if (b->kinds[sensor] != 0 || !a.parseval || !a.store_vec || (a.npx != 8 && a.npx != 16))
-        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG in a batch needs an NGS MFMA np 8 sensor with Parseval "
-                                                "normalisation, 8 or 16 px per subaperture and 16-byte aligned CCD rows");
+    const bool ngs_ok = b->kinds[sensor] == 0 && a.store_vec && (a.npx == 8 || a.npx == 16);
+    const bool lgs_ok = b->kinds[sensor] == 1 && a.npx <= 16;
+    if (!a.parseval || !(ngs_ok || lgs_ok))
+        return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG in a batch needs Parseval normalisation and an NGS MFMA "
+                                                "np 8 sensor with 8 or 16 px per subaperture and 16-byte aligned CCD "
+                                                "rows, or an LGS sensor of the batch's LGS body");
     for (const auto& r : b->slopes)
         if (r.c == c && r.tpart) return set_error(SHWFS_ERR_ARG, "this ShSlopec plan already serves a sensor of the batch");
     const int nf = b->frames[sensor];
@@ -3807,7 +4005,7 @@ int shwfs_batch_set_slopes(shwfs_batch* b, int sensor, float thr, int thr_mode, 
     a.n_valid = c->n_sub;
     SHWFS_HIP_CHECK(hipMemcpy(b->d_args + sensor, &a, sizeof(NgsArgs), hipMemcpyHostToDevice));
     b->slopes.push_back({c, static_cast<double*>(tp), a.n_parts, slopes, slopes_stride, flux, totals, nf});
-    return SHWFS_OK;
+    return batch_layout(b);        // the sensor's items now run the fused-CoG variant of its body
 }
 
 int shwfs_batch_forward(shwfs_batch* b, void* stream) {
@@ -3821,14 +4019,12 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
                                        r.n_frames, st);
         if (rc) return rc;
     }
-    if (b->n_lgs > 0) {
-        hipLaunchKernelGGL(sh_batch_kernel<1>, dim3(std::min(b->grid, b->n_lgs)), dim3(256), kBatchLgsLds, st,
-                           b->d_args, b->d_items, b->n_lgs, b->d_ts);
-        SHWFS_LAUNCH_CHECK();
-    }
-    if (b->n_ngs > 0) {
-        hipLaunchKernelGGL(sh_batch_kernel<0>, dim3(std::min(b->grid, b->n_ngs)), dim3(256), kBatchNgsLds, st,
-                           b->d_args, b->d_items + b->n_lgs, b->n_ngs, b->d_ts);
+    for (const auto& L : b->launches) {     // the persistent SH launches (batch_layout)
+        const BatchKernel k = L.kind == 1 ? batch_kernel_of<1>(L.variant, std::make_integer_sequence<int, kBatchVariants[1]>{})
+                                          : batch_kernel_of<0>(L.variant, std::make_integer_sequence<int, kBatchVariants[0]>{});
+        const int grid = b->grid_req > 0 ? std::min(b->grid_req, L.count) : L.count;
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), L.kind == 1 ? kBatchLgsLds : kBatchNgsLds, st, b->d_args,
+                           b->d_items + L.offset, L.count, b->d_ts);
         SHWFS_LAUNCH_CHECK();
     }
     for (const auto& r : b->slopes) {       // fused-CoG sensors: slopes, flux and totals from the moments

@@ -91,8 +91,8 @@ def test_autotuner(entries, tmp_path, monkeypatch):
 def test_full_multi_sensor_work_list_matches_per_sensor_and_oracle():
     """The whole uneven batch of the benchmark: 6 LGS 40x40 (N 128, launchers on a 20 m circle),
     NGS 10x10 and 20x20 (N 32) and 80x80 (N 64), and 2000 push-pull fields of the 80x80 sensor
-    (2009 frames) in one persistent SH launch (the NGS sensors' CoG in its epilogue) plus the LGS
-    sensors' CoG launches, replayed from a HIP graph.  Images and slopes equal the per-sensor
+    (2009 frames) in persistent SH launches with every sensor's CoG in their epilogue, replayed
+    from a HIP graph.  Images and slopes equal the per-sensor
     launches bit for bit; one frame of every sensor
     type (including the 10x10 / N 32 sensor and both signs of a push-pull pair) matches the FP64
     oracle chain (images 1e-4, slopes 1e-3 CCD px)."""
@@ -102,11 +102,11 @@ def test_full_multi_sensor_work_list_matches_per_sensor_and_oracle():
     assert [e.n_frames for e in full] == [1] * 9 + [2000]
     print('full work list: ' + ', '.join(f'{e.name} x{e.n_frames}' for e in full), flush=True)
     b = MultiSensorBatch(full, grid_blocks=0, order=2)
-    # the NGS np 8 sensors run their CoG in the SH epilogue: the per-sensor reference is then the
-    # plan's own fused forward (static pupil registered, so its waves group the subapertures as the
-    # batch does), the others the two-pass chain
-    assert [e.name for e, f in zip(full, b.fused) if f] == ['scao_ngs_10x10', 'scao_ngs_20x20', 'elt_ngs_80x80',
-                                                            'elt80_pushpull']
+    # every sensor runs its CoG in the SH epilogue: the per-sensor reference is then the plan's own
+    # fused forward (static pupil registered, so its waves and tiles group the subapertures as the
+    # batch does)
+    assert [e.name for e, f in zip(full, b.fused) if f] == [f'lgs40_{k}' for k in range(6)] + [
+        'scao_ngs_10x10', 'scao_ngs_20x20', 'elt_ngs_80x80', 'elt80_pushpull']
     refs, srefs = [], []
     for e, fused in zip(full, b.fused):
         out = torch.empty_like(e.ccd)
