@@ -439,6 +439,25 @@ struct NgsStoreShape {
 // order like their int32 patterns and every negative float (sign bit set) maps to +0.
 __device__ __forceinline__ float clamp0(float v) { return __int_as_float(max(__float_as_int(v), 0)); }
 
+// Exchanges within each row of 16 lanes (DPP, no LDS): the value of lane i's partner i ^ 1, i ^ 2,
+// 7 - i (within its half row) or 15 - i.  Summing v and its partner's v along these four steps
+// leaves the row's total in every lane of the row.
+constexpr int kDppXor1 = 0xB1;          // quad_perm [1, 0, 3, 2]
+constexpr int kDppXor2 = 0x4E;          // quad_perm [2, 3, 0, 1]
+constexpr int kDppHalfMirror = 0x141;   // row_half_mirror
+constexpr int kDppMirror = 0x140;       // row_mirror
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 constexpr int kYwWord = 256;   // first padding word of subaperture 0 in a wave's LDS region
 
 // Hand-over between lanes of one wave through LDS: a wave's LDS operations execute in order, so
@@ -980,9 +999,9 @@ struct LgsShape {
     static constexpr int SPN = 16 / LH;           // subapertures per 16-column block of the MFMA phases
     static constexpr int WAVES = TS / (2 * SPN);  // two column blocks per wave
     static constexpr int BT = 64 * WAVES;         // threads per block
-    // fused CoG: LDS after the tile buffer for the per-wave frame-total partials (FP64 pairs) and
-    // the column square sums of the pupil samples (one per phase-1 task)
-    static constexpr int COG_BYTES = 2 * WAVES * 8 + TS * NP * 4;
+    // fused CoG: LDS after the tile buffer for the column square sums of the pupil samples (one
+    // per phase-1 task)
+    static constexpr int COG_BYTES = TS * NP * 4;
 };
 
 // The lag-domain image D = C K is Hermitian on the (m_y, m_x) grid (C is an autocorrelation, K
@@ -990,13 +1009,15 @@ struct LgsShape {
 // Re sum_{m_y, m_x} P[Y][m_y] D[m_y][m_x] Q[X][m_x] only needs the columns m_x >= 0, the
 // m_x > 0 ones counted twice.  Phases 2-4 therefore carry LH = NP lag columns instead of 2NP-1.
 //
-// COG: the ShSlopec centre of gravity (sh_slopec.py:365-404) of the tile's subapertures from the
+// COG (1 subtracted threshold, 2 pedestal, 3 no threshold): the ShSlopec centre of gravity
+// (sh_slopec.py:365-404) of the tile's subapertures from the
 // binned pixels of phase 4 while they are in registers: per subaperture the thresholded total
 // and the x / y moments (FP32 over 32 pixels, FP64 across lanes) and the Parseval flux
 // (scale x norm_const x sum(A^2) of the subaperture), into g.mom at the subaperture's slot, plus
-// the tile's share of the frame totals into g.tpart; cog_finalize_parts_kernel turns them into
-// slopes.  `buf` then extends Sh::COG_BYTES past the tile buffer.
-template <int NP, int M, int TS, bool COG = false>
+// its share of the frame totals into g.tpart (one pair per tile position: g.n_parts = tiles x TS);
+// cog_finalize_parts_kernel turns them into slopes.  `buf` then extends Sh::COG_BYTES past the
+// tile buffer.
+template <int NP, int M, int TS, int COG = 0>
 __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, float2* buf, double* red) {
     constexpr int L = 2 * NP - 1;
     using Sh = LgsShape<NP, M, TS>;
@@ -1006,8 +1027,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     const int n = g.n_lenses;
     const int sub0 = tile * TS;
     const float2* Qc = reinterpret_cast<const float2*>(g.colw);
-    double* cog_part = reinterpret_cast<double*>(buf + Sh::LDS_F2);     // [WAVES][2]
-    float* cog_sq = reinterpret_cast<float*>(cog_part + 2 * WAVES);    // [TS * NP]
+    float* cog_sq = reinterpret_cast<float*>(buf + Sh::LDS_F2);        // [TS * NP]
 
     // tile position s -> lenslet index (the plan's lit-first order for a registered static pupil)
     auto lenslet = [&](int s) { const int pos = sub0 + s; return g.order ? g.order[pos] : pos; };
@@ -1033,7 +1053,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             lit |= av[y] != 0.f;
         }
     }
-    if constexpr (COG) {
+    if constexpr (COG != 0) {
         if (t < TS * NP) {
             float q = 0.f;
 #pragma unroll
@@ -1047,18 +1067,19 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
         scale = g.S0 * g.pixel_area * am[0] / (g.norm_const * am[1]);
     }
     if (!__syncthreads_or(lit)) {
-        if constexpr (COG) {
+        if constexpr (COG != 0) {
             // every pixel of the tile is 0 x the scale: so are its moments and flux
             const double zd = 0.0 * scale;
-            if (t < TS && sub0 + t < g.n_sub) {
-                const int slot = g.slot[lenslet(t)];
+            if (t < TS) {
+                const int slot = sub0 + t < g.n_sub ? g.slot[lenslet(t)] : -1;
                 if (slot >= 0) {
                     double* m = g.mom + ((int64_t)f * g.n_valid + slot) * 4;
                     *reinterpret_cast<double2*>(m) = make_double2(zd, zd);
                     *reinterpret_cast<double2*>(m + 2) = make_double2(zd, zd);
                 }
+                const double z = slot >= 0 ? zd : 0.0;
+                *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + tile * TS + t) * 2) = make_double2(z, z);
             }
-            if (t == 0) *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + tile) * 2) = make_double2(zd, zd);
         }
         // 0 times the scale: a frame without any light gives the reference's 0/0 (NaN) as the
         // full transform would
@@ -1246,13 +1267,17 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             nqi[ks] = -w * qc.y;
         }
         const int Y = n16, X0 = 4 * k4;
-        // fused CoG: CoG weights of the lane's pixels (x of its 4 columns, y of its row; 0 outside
-        // the subaperture) and its per-subaperture FP32 partials: thresholded total, x and y moments
+        // fused CoG: weights of the lane's pixels (x of its 4 columns, y of its row), its FP32
+        // partials per subaperture (thresholded total, x and y moments), and, requested before the
+        // binning products so that the loads overlap them, the slot and the sum(A^2) columns of
+        // the subaperture whose sums this lane ends with (u_own, after the reduction below)
         float cxw[4] = {0.f, 0.f, 0.f, 0.f}, cyw = 0.f;
         float cpart[SPW][3];
-        int cmode = 0;
-        if constexpr (COG) {
-            cmode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;   // 3: no threshold (no clamp, as the NGS epilogue)
+        const int u_own = SPW == 4 ? (lane >> 4) : (lane >> 5), s_own = wave * SPW + u_own;
+        int slot = -1;
+        float sq = 0.f;
+        if constexpr (COG != 0) {
+            static_assert(SPW == 2 || SPW == 4, "2 or 4 subapertures per wave");
             if (Y < g.npx) {
                 cyw = g.cogw[g.npx + Y];
 #pragma unroll
@@ -1260,6 +1285,8 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
             }
 #pragma unroll
             for (int u = 0; u < SPW; ++u) cpart[u][0] = cpart[u][1] = cpart[u][2] = 0.f;
+            if (sub0 + s_own < g.n_sub) slot = g.slot[lenslet(s_own)];
+            if (n16 < NP) sq = cog_sq[s_own * NP + n16];
         }
 #pragma unroll
         for (int u = 0; u < SPW; ++u) {
@@ -1287,13 +1314,14 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                     for (int r = 0; r < 4; ++r)
                         if (X0 + r < g.npx) dst[r] = o[r];
                 }
-                if constexpr (COG) {
+                if constexpr (COG != 0) {
                     float w[4];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
-                        const float v = o[r];
-                        const float tv = cmode == 1 ? clamp0(v - g.thr) : cmode == 2 ? (v < g.thr ? 0.f : v) : v;
-                        w[r] = X0 + r < g.npx ? tv : 0.f;
+                        float v = o[r];
+                        if constexpr (COG == 1) v = clamp0(v - g.thr);          // subtract the threshold, clamp
+                        else if constexpr (COG == 2) v = v < g.thr ? 0.f : v;  // pedestal
+                        w[r] = X0 + r < g.npx ? v : 0.f;                        // columns beyond the subaperture
                     }
                     const float tot = (w[0] + w[1]) + (w[2] + w[3]);
                     cpart[u][0] = tot;
@@ -1302,13 +1330,12 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                 }
             }
         }
-        if constexpr (COG) {
+        if constexpr (COG != 0) {
             // reduce each subaperture's partials over its 64 lanes (16 rows Y x 4 column groups):
             // the column groups in FP32 (lane bits 4-5, splitting the subapertures between the
             // halves and quarters of the wave so that one exchange serves two of them), then the
-            // 16 rows in FP64 (lane bits 0-3, the first step in FP32: 32 pixels per FP32 partial).
-            // Afterwards lane 16 r holds subaperture u = r (SPW 4) or r / 2 (SPW 2).
-            static_assert(SPW == 2 || SPW == 4, "2 or 4 subapertures per wave");
+            // 16 rows within each row of lanes (DPP), the first step in FP32 (32 pixels per FP32
+            // partial), the others in FP64.  Afterwards every lane of row r holds subaperture u_own.
             float c[3];
             const bool hi32 = lane >= 32, hi16 = (lane & 16) != 0;
             if constexpr (SPW == 2) {
@@ -1327,39 +1354,26 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
                     c[v] = (hi16 ? b : a) + __shfl_xor(hi16 ? a : b, 16, 64);
                 }
             }
-            const int u_own = SPW == 4 ? (lane >> 4) : (lane >> 5);
-            const int s_own = wave * SPW + u_own;
-            // sum(A^2) of the subaperture: one phase-1 column per lane n16 < NP
-            const float sq = n16 < NP ? cog_sq[s_own * NP + n16] : 0.f;
 #pragma unroll
-            for (int v = 0; v < 3; ++v) c[v] += __shfl_xor(c[v], 1, 64);
-            double m_t = c[0], m_x = c[1], m_y = c[2], m_q = (double)sq + (double)__shfl_xor(sq, 1, 64);
+            for (int v = 0; v < 3; ++v) c[v] += dpp_f<kDppXor1>(c[v]);
+            sq += dpp_f<kDppXor1>(sq);
+            double m[4] = {c[0], c[1], c[2], sq};       // total, x and y moments, sum(A^2)
 #pragma unroll
-            for (int o = 2; o < 16; o <<= 1) {
-                m_t += __shfl_xor(m_t, o, 64);
-                m_x += __shfl_xor(m_x, o, 64);
-                m_y += __shfl_xor(m_y, o, 64);
-                m_q += __shfl_xor(m_q, o, 64);
+            for (int v = 0; v < 4; ++v) {
+                m[v] += dpp_d<kDppXor2>(m[v]);
+                m[v] += dpp_d<kDppHalfMirror>(m[v]);
+                m[v] += dpp_d<kDppMirror>(m[v]);
             }
-            const int slot = (sub0 + s_own < g.n_sub) ? g.slot[lenslet(s_own)] : -1;
-            const bool writer = n16 == 0 && (SPW == 4 || !hi16) && slot >= 0;
-            const double m_f = m_q * (scale * g.norm_const);   // Parseval flux of the subaperture
-            if (writer) {
-                double* m = g.mom + ((int64_t)f * g.n_valid + slot) * 4;
-                *reinterpret_cast<double2*>(m) = make_double2(m_f, m_t);
-                *reinterpret_cast<double2*>(m + 2) = make_double2(m_x, m_y);
-            }
-            // the tile's share of the frame totals (flux, thresholded total), fixed order
-            double p_f = writer ? m_f : 0.0, p_t = writer ? m_t : 0.0;
-            p_f += __shfl_xor(p_f, 16, 64); p_t += __shfl_xor(p_t, 16, 64);
-            p_f += __shfl_xor(p_f, 32, 64); p_t += __shfl_xor(p_t, 32, 64);
-            if (lane == 0) { cog_part[2 * wave] = p_f; cog_part[2 * wave + 1] = p_t; }
-            __syncthreads();
-            if (t == 0) {
-                double sf = 0.0, st = 0.0;
-#pragma unroll
-                for (int w = 0; w < WAVES; ++w) { sf += cog_part[2 * w]; st += cog_part[2 * w + 1]; }
-                *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + tile) * 2) = make_double2(sf, st);
+            if (n16 == 0 && (SPW == 4 || !hi16)) {       // one lane per subaperture
+                const double m_f = m[3] * (scale * g.norm_const);   // Parseval flux of the subaperture
+                if (slot >= 0) {
+                    double* mo = g.mom + ((int64_t)f * g.n_valid + slot) * 4;
+                    *reinterpret_cast<double2*>(mo) = make_double2(m_f, m[0]);
+                    *reinterpret_cast<double2*>(mo + 2) = make_double2(m[1], m[2]);
+                }
+                // its share of the frame totals (flux, thresholded total)
+                *reinterpret_cast<double2*>(g.tpart + ((int64_t)f * g.n_parts + tile * TS + s_own) * 2) =
+                    slot >= 0 ? make_double2(m_f, m[0]) : make_double2(0.0, 0.0);
             }
         }
     }
@@ -1369,7 +1383,7 @@ __device__ __forceinline__ void lgs_tile(const NgsArgs& g, int tile, int f, floa
     }
 }
 
-template <int NP, int M, int TS, bool COG>
+template <int NP, int M, int TS, int COG>
 __global__ void __launch_bounds__((LgsShape<NP, M, TS>::BT)) __attribute__((amdgpu_waves_per_eu(4, 8)))
 sh_lgs_forward_kernel(NgsArgs g) {
     extern __shared__ float4 lds_raw[];
@@ -2271,15 +2285,14 @@ __global__ void batch_sensor_stamp_kernel(unsigned long long* __restrict__ ts, i
 // Body variant of a batch sensor: which instantiation of its body (KIND 0: NGS MFMA np 8 tiles,
 // KIND 1: LGS np 16 tiles) its items run.  NGS: 0 image normalised by its own sum, 1-3 fused CoG
 // with 16 px per subaperture (no threshold, subtracted threshold, pedestal), 4-6 the same with 8 px,
-// 7 / 8 plain with 16 / 8 px and 16-byte aligned CCD rows, 9 any other; LGS: 0 plain, 1 fused CoG.
-constexpr int kBatchVariants[2] = {10, 2};
+// 7 / 8 plain with 16 / 8 px and 16-byte aligned CCD rows, 9 any other; LGS: 0 plain, 1-3 fused
+// CoG (no threshold, subtracted threshold, pedestal).
+constexpr int kBatchVariants[2] = {10, 4};
 __host__ __device__ inline int batch_variant(const NgsArgs& g, int kind) {
-    if (kind == 1) return g.cog ? 1 : 0;
+    const int mode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;
+    if (kind == 1) return g.cog ? (mode == 3 ? 1 : mode + 1) : 0;
     if (!g.parseval) return 0;
-    if (g.cog) {
-        const int mode = (g.cog == 1 && g.thr == 0.f) ? 3 : g.cog;
-        return (g.npx == 16 ? 1 : 4) + (mode == 3 ? 0 : mode);
-    }
+    if (g.cog) return (g.npx == 16 ? 1 : 4) + (mode == 3 ? 0 : mode);
     if (g.npx == 16 && g.store_vec) return 7;
     if (g.npx == 8 && g.store_vec) return 8;
     return 9;
@@ -2317,7 +2330,8 @@ sh_batch_kernel(const NgsArgs* __restrict__ sensors, const BatchItem* __restrict
             float* sw = reinterpret_cast<float*>(lds_raw + wv * NgsMfmaShape<8>::WAVE_F4);
             batch_ngs_body<VAR>(g, w.tile * NgsMfmaShape<8>::WPB + wv, w.frame, w.frame + w.nf, sw);
         } else {
-            lgs_tile<16, 32, 8, VAR == 1>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
+            constexpr int COG = VAR == 0 ? 0 : VAR == 1 ? 3 : VAR - 1;
+            lgs_tile<16, 32, 8, COG>(g, w.tile, w.frame, reinterpret_cast<float2*>(lds_raw), red);
         }
         __syncthreads();
         if (threadIdx.x == 0) atomicMax(&ts[1 + w.sensor], (unsigned long long)wall_clock64());
@@ -2374,6 +2388,7 @@ struct shwfs_plan {
     int ts = 0;
     int n_tiles = 0;
     int n_parts = 0;               // non-Parseval partial sums per frame
+    int n_cog_parts = 0;           // fused-CoG frame-total partials per frame (LGS: one per tile position)
     int amp_blocks = 0;
     int rows_alloc = 0;            // rows of the uploaded binning operators (zero padded)
     bool mfma = false;             // binning on the matrix cores (sh_ngs_mfma_kernel)
@@ -2537,11 +2552,18 @@ int launch_lgs(shwfs_plan* p, const NgsArgs& a, int n_frames, hipStream_t st) {
     constexpr int lds = Sh::LDS_F2 * (int)sizeof(float2);
     static_assert(lds + Sh::COG_BYTES <= 64 * 1024, "LGS tile exceeds 64 KiB of LDS");
     dim3 grid(p->n_tiles, n_frames);
-    if (a.cog)      // ShSlopec moments in the epilogue (shwfs_forward_slopes)
-        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, true>), grid, dim3(Sh::BT), lds + Sh::COG_BYTES, st,
+    const int mode = (a.cog == 1 && a.thr == 0.f) ? 3 : a.cog;   // ShSlopec moments in the epilogue
+    if (mode == 3)
+        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, 3>), grid, dim3(Sh::BT), lds + Sh::COG_BYTES, st,
+                              t_ev_start, t_ev_stop, 0, a);
+    else if (mode == 1)
+        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, 1>), grid, dim3(Sh::BT), lds + Sh::COG_BYTES, st,
+                              t_ev_start, t_ev_stop, 0, a);
+    else if (mode == 2)
+        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, 2>), grid, dim3(Sh::BT), lds + Sh::COG_BYTES, st,
                               t_ev_start, t_ev_stop, 0, a);
     else
-        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, false>), grid, dim3(Sh::BT), lds, st, t_ev_start,
+        hipExtLaunchKernelGGL((sh_lgs_forward_kernel<NP, M, TS, 0>), grid, dim3(Sh::BT), lds, st, t_ev_start,
                               t_ev_stop, 0, a);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
@@ -2946,7 +2968,7 @@ int sh_forward_impl(shwfs_plan* p, int n_frames, const float* A, int64_t a_strid
     a.ccd = ccd; a.ccd_stride = ccd_stride;
     a.roww = (p->mfma && !g.lgs) ? p->d_rowlag : p->d_roww; a.colw = p->d_colw;
     a.amp = amp; a.amp_fstep = (a_stride == 0) ? 0 : 2;
-    a.bpart = p->d_bpart; a.n_parts = p->n_parts;
+    a.bpart = p->d_bpart; a.n_parts = cog ? p->n_cog_parts : p->n_parts;
     a.n_lenses = g.n_lenses; a.ef_size = g.ef_size; a.npx = g.subap_npx; a.npx_pad = p->npx_pad;
     a.n_sub = p->n_sub; a.n_tiles = p->n_tiles; a.parseval = g.parseval;
     a.k = g.k_rad_per_nm; a.tilt = g.tilt_rad; a.a0 = g.tilt_a0;
@@ -3099,8 +3121,9 @@ int shwfs_plan_bind_slopec(shwfs_plan* p, shslopec_plan* c) {
     for (int k = 0; k < 2 * npx; ++k) w[k] = (float)c->h_sep[k];
     if (!p->d_slot) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_slot, (size_t)p->n_sub * sizeof(int32_t)));
     if (!p->d_cogw) SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_cogw, 2 * (size_t)npx * sizeof(float)));
+    p->n_cog_parts = g.lgs ? p->n_tiles * lgs_ts : p->n_parts;
     if (!p->d_tpart)
-        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_tpart, (size_t)g.max_frames * p->n_parts * 2 * sizeof(double)));
+        SHWFS_HIP_CHECK(hipMalloc((void**)&p->d_tpart, (size_t)g.max_frames * p->n_cog_parts * 2 * sizeof(double)));
     SHWFS_HIP_CHECK(hipMemcpy(p->d_slot, slot.data(), slot.size() * sizeof(int32_t), hipMemcpyHostToDevice));
     SHWFS_HIP_CHECK(hipMemcpy(p->d_cogw, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
     p->cog = c;
@@ -3134,7 +3157,7 @@ int shwfs_forward_slopes(shwfs_plan* p, int n_frames, const float* A, int64_t a_
     if (rc) return rc;
     DeviceGuard dg(p->device);
     hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((c->n_sub + 1023) / 1024, n_frames), dim3(256), 0, st,
-                       c->d_mom, p->d_tpart, p->n_parts, c->n_sub, slopes, slopes_stride, flux, totals,
+                       c->d_mom, p->d_tpart, p->n_cog_parts, c->n_sub, slopes, slopes_stride, flux, totals,
                        (double*)nullptr);
     SHWFS_LAUNCH_CHECK();
     return SHWFS_OK;
@@ -3992,6 +4015,7 @@ int shwfs_batch_set_slopes(shwfs_batch* b, int sensor, float thr, int thr_mode, 
     if (nf > c->max_frames) return set_error(SHWFS_ERR_ARG, "frame count above the ShSlopec plan's max_frames");
     if (nf > 1 && slopes_stride < 2 * (int64_t)c->n_sub) return set_error(SHWFS_ERR_ARG, "slopes frame stride too small");
     DeviceGuard dg(b->device);
+    if (b->kinds[sensor] == 1) a.n_parts = a.n_tiles * 8;   // LGS: one partial per tile position
     void* tp = nullptr;
     hipError_t e = hipMalloc(&tp, (size_t)nf * a.n_parts * 2 * sizeof(double));
     if (e != hipSuccess) return set_error(SHWFS_ERR_HIP, std::string("batch alloc: ") + hipGetErrorString(e));
