@@ -42,7 +42,7 @@ FP32_PEAK_TFLOPS = 157.3
 # the 2.4 GHz peak engine clock is the issue roofline that bounds these kernels.
 MEASURED_ISSUE_PER_FRAME = {
     'sh_ngs_mfma_kernel<8, true, 16, 3, false>': (464_520, 38_400),     # (VALU wave instr, MFMA)
-    'sh_lgs_forward_kernel<16, 32, 8, false>': (974_000, 40_000),
+    'sh_lgs_forward_kernel<16, 32, 8, 0>': (974_000, 40_000),
     'ccd_photon_lean_kernel': (6_960_000, 0),
 }
 N_SIMDS = 1024
@@ -213,14 +213,14 @@ def bench_lgs(torch, dev):
     frame_bytes = 4 * (2 * g.ef_size ** 2 + g.ccd_side ** 2) + kbytes          # pupil + phase + CCD + spectra
     gbps = frame_bytes * F / kt / 1e9
     return {'workload': 'lgs_40x40', 'frames_per_s': round(F / dt, 1), 'ms_per_frame': round(dt * 1e3 / F, 5),
-            'kernel': 'sh_lgs_forward_kernel<16, 32, 8, false>', 'kernel_ms_per_launch': round(kt * 1e3, 5),
+            'kernel': 'sh_lgs_forward_kernel<16, 32, 8, 0>', 'kernel_ms_per_launch': round(kt * 1e3, 5),
             # algorithmic bytes count the kernel spectra on the Hermitian half of the lag grid
             # (|m| <= np - 1, m_x >= 0: all the correlation uses), not the full N x N spectra;
             # the HBM fraction is per kernel duration, frames_per_s per call (host launch included)
             'kernel_spectra_MB_per_frame': round(kbytes / 1e6, 2), 'algorithmic_MB_per_frame': round(frame_bytes / 1e6, 2),
             'algorithmic_GBps': round(gbps, 1), 'hbm_frac': round(gbps / HBM_PEAK_GBS, 4),
             'measured_traffic_MB_per_frame': round(MEASURED_LGS_TRAFFIC_PER_FRAME / 1e6, 3),
-            'issue_roofline': issue_roofline('sh_lgs_forward_kernel<16, 32, 8, false>', kt / F),
+            'issue_roofline': issue_roofline('sh_lgs_forward_kernel<16, 32, 8, 0>', kt / F),
             'sh_slopes_frames_per_s': {'fused_cog': round(F / dt_fused, 1), 'two_pass': round(F / dt_two, 1),
                                        'valid_subaps': int(n)},
             'fft_size': g.fft_size, 'np_sub': g.ovs_np_sub}
