@@ -1740,12 +1740,14 @@ __global__ void cog_finalize_kernel(const double* __restrict__ mom, int n_sub,
 // Fused-path variant: the frame totals come as per-wave partial sums from the SH kernel, so every
 // block reduces them (same fixed order in every block) and handles its own slice of subapertures
 // (grid-stride over them).  grid = (a few blocks, n_frames).
-__global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const double* __restrict__ tpart,
-                                          int n_parts, int n_sub, float* __restrict__ slopes,
-                                          int64_t slopes_stride, double* __restrict__ flux,
-                                          double* __restrict__ totals, double* __restrict__ factor_out) {
-    __shared__ double red[32];
-    const int f = blockIdx.y;
+// Slopes, flux and totals of frame f from the fused-CoG moments (sh_slopec.py:365-438): the
+// frame totals from the n_parts partial pairs, then per subaperture factor = 1 / total, set to 0
+// below 1e-3 of the mean total (sh_slopec.py:393-400).  One block row per frame.
+__device__ __forceinline__ void cog_finalize_frame(const double* __restrict__ mom, const double* __restrict__ tpart,
+                                                   int n_parts, int n_sub, float* __restrict__ slopes,
+                                                   int64_t slopes_stride, double* __restrict__ flux,
+                                                   double* __restrict__ totals, double* __restrict__ factor_out,
+                                                   int f, double* red) {
     const double* tp = tpart + (int64_t)f * n_parts * 2;
     double sf = 0.0, st = 0.0;
     for (int b = threadIdx.x; b < n_parts; b += blockDim.x) {
@@ -1772,6 +1774,39 @@ __global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const 
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         totals[2 * f] = sf;
         totals[2 * f + 1] = sf / n_sub;
+    }
+}
+
+__global__ void cog_finalize_parts_kernel(const double* __restrict__ mom, const double* __restrict__ tpart,
+                                          int n_parts, int n_sub, float* __restrict__ slopes,
+                                          int64_t slopes_stride, double* __restrict__ flux,
+                                          double* __restrict__ totals, double* __restrict__ factor_out) {
+    __shared__ double red[32];
+    cog_finalize_frame(mom, tpart, n_parts, n_sub, slopes, slopes_stride, flux, totals, factor_out, blockIdx.y, red);
+}
+
+// The same for every fused-CoG sensor of a multi-sensor batch in one launch: block row y serves
+// frame map[y].y of record map[y].x.
+struct CogFinalizeRec {
+    const double* mom;
+    const double* tpart;
+    int n_parts, n_sub;
+    float* slopes;
+    int64_t slopes_stride;
+    double* flux;
+    double* totals;
+};
+
+__global__ void cog_finalize_batch_kernel(const CogFinalizeRec* __restrict__ recs, const int2* __restrict__ map,
+                                          int n_rows) {
+    __shared__ double red[32];
+    for (int y = blockIdx.y; y < n_rows; y += gridDim.y) {
+        const int2 rf = map[y];
+        const CogFinalizeRec& r = recs[rf.x];
+        if ((int)blockIdx.x * (int)blockDim.x < r.n_sub || blockIdx.x == 0)
+            cog_finalize_frame(r.mom, r.tpart, r.n_parts, r.n_sub, r.slopes, r.slopes_stride, r.flux, r.totals,
+                               nullptr, rf.y, red);
+        __syncthreads();                             // red is reused by the next row
     }
 }
 
@@ -3780,6 +3815,9 @@ struct shwfs_batch {
         int kind, variant, offset, count;
     };
     std::vector<Launch> launches;
+    CogFinalizeRec* d_fin = nullptr;  // fused-CoG sensors' finalize records, one launch for all
+    int2* d_fin_map = nullptr;        // [fin_rows]: (record, frame)
+    int fin_rows = 0, fin_bx = 0;
 };
 
 namespace {
@@ -3817,6 +3855,8 @@ int shwfs_batch_destroy(shwfs_batch* b) {
     DeviceGuard dg(b->device);
     (void)hipFree(b->d_args);
     (void)hipFree(b->d_items);
+    (void)hipFree(b->d_fin);
+    (void)hipFree(b->d_fin_map);
     (void)hipFree(b->d_amp);
     (void)hipFree(b->d_scale);
     (void)hipFree(b->d_amp_part);
@@ -4029,6 +4069,23 @@ int shwfs_batch_set_slopes(shwfs_batch* b, int sensor, float thr, int thr_mode, 
     a.n_valid = c->n_sub;
     SHWFS_HIP_CHECK(hipMemcpy(b->d_args + sensor, &a, sizeof(NgsArgs), hipMemcpyHostToDevice));
     b->slopes.push_back({c, static_cast<double*>(tp), a.n_parts, slopes, slopes_stride, flux, totals, nf});
+    // the finalize records of every fused sensor, for one launch after the SH launches
+    std::vector<CogFinalizeRec> recs;
+    std::vector<int2> map;
+    int max_sub = 0;
+    for (const auto& r : b->slopes) {
+        for (int f = 0; f < r.n_frames; ++f) map.push_back(make_int2((int)recs.size(), f));
+        recs.push_back({r.c->d_mom, r.tpart, r.n_parts, r.c->n_sub, r.slopes, r.slopes_stride, r.flux, r.totals});
+        max_sub = std::max(max_sub, r.c->n_sub);
+    }
+    (void)hipFree(b->d_fin); b->d_fin = nullptr;
+    (void)hipFree(b->d_fin_map); b->d_fin_map = nullptr;
+    SHWFS_HIP_CHECK(hipMalloc((void**)&b->d_fin, recs.size() * sizeof(CogFinalizeRec)));
+    SHWFS_HIP_CHECK(hipMalloc((void**)&b->d_fin_map, map.size() * sizeof(int2)));
+    SHWFS_HIP_CHECK(hipMemcpy(b->d_fin, recs.data(), recs.size() * sizeof(CogFinalizeRec), hipMemcpyHostToDevice));
+    SHWFS_HIP_CHECK(hipMemcpy(b->d_fin_map, map.data(), map.size() * sizeof(int2), hipMemcpyHostToDevice));
+    b->fin_rows = (int)map.size();
+    b->fin_bx = (max_sub + 1023) / 1024;
     return batch_layout(b);        // the sensor's items now run the fused-CoG variant of its body
 }
 
@@ -4051,10 +4108,9 @@ int shwfs_batch_forward(shwfs_batch* b, void* stream) {
                            b->d_items + L.offset, L.count, b->d_ts);
         SHWFS_LAUNCH_CHECK();
     }
-    for (const auto& r : b->slopes) {       // fused-CoG sensors: slopes, flux and totals from the moments
-        hipLaunchKernelGGL(cog_finalize_parts_kernel, dim3((r.c->n_sub + 1023) / 1024, r.n_frames), dim3(256), 0, st,
-                           r.c->d_mom, r.tpart, r.n_parts, r.c->n_sub, r.slopes, r.slopes_stride, r.flux, r.totals,
-                           (double*)nullptr);
+    if (b->fin_rows > 0) {                  // fused-CoG sensors: slopes, flux and totals from the moments
+        hipLaunchKernelGGL(cog_finalize_batch_kernel, dim3(b->fin_bx, std::min(b->fin_rows, 65535)), dim3(256), 0, st,
+                           b->d_fin, b->d_fin_map, b->fin_rows);
         SHWFS_LAUNCH_CHECK();
     }
     for (const auto& c : b->rescale) {      // non-Parseval sensors: images normalised by their sums
