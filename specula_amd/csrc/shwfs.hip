@@ -4049,6 +4049,7 @@ int shwfs_batch_set_slopes(shwfs_batch* b, int sensor, float thr, int thr_mode, 
         return set_error(SHWFS_ERR_UNSUPPORTED, "fused CoG in a batch needs Parseval normalisation and an NGS MFMA "
                                                 "np 8 sensor with 8 or 16 px per subaperture and 16-byte aligned CCD "
                                                 "rows, or an LGS sensor of the batch's LGS body");
+    if (a.cog) return set_error(SHWFS_ERR_ARG, "the sensor's ShSlopec already runs in the batch");
     for (const auto& r : b->slopes)
         if (r.c == c && r.tpart) return set_error(SHWFS_ERR_ARG, "this ShSlopec plan already serves a sensor of the batch");
     const int nf = b->frames[sensor];

@@ -221,3 +221,23 @@ def test_batch_with_resampled_and_cut_sensors_matches_per_sensor_and_oracle():
         torch.cuda.synchronize()
         for e, f0 in zip(ents, first):
             assert torch.equal(e.ccd, f0), e.name
+
+
+def test_batch_set_slopes_argument_checks():
+    """shwfs_batch_set_slopes refuses a sensor whose ShSlopec already runs in the batch, a bad
+    sensor index and thr_mode, and a sensor without a bound ShSlopec plan; the batch still runs."""
+    from specula_amd import _native
+    ents = workloads.multi_sensor_entries(n_lgs=1, n_pushpull=2)
+    b = MultiSensorBatch(ents, order=2)
+    assert all(b.fused)
+    lib, e = _native.lib(), ents[0]
+    args = (0.0, 0, _native.ptr(e.slopes), e.slopes.shape[1], _native.ptr(e.flux), _native.ptr(e.totals))
+    assert lib.shwfs_batch_set_slopes(b._h, 0, *args) == _native.SHWFS_ERR_ARG        # already fused
+    assert lib.shwfs_batch_set_slopes(b._h, len(ents), *args) == _native.SHWFS_ERR_ARG
+    assert lib.shwfs_batch_set_slopes(b._h, 0, 0.0, 2, *args[2:]) == _native.SHWFS_ERR_ARG
+    b2 = MultiSensorBatch(ents, order=2, fuse_slopes=False)
+    e.plan.bind_slopec(None)
+    assert lib.shwfs_batch_set_slopes(b2._h, 0, *args) == _native.SHWFS_ERR_ARG       # no ShSlopec bound
+    b.forward()
+    b2.forward()
+    torch.cuda.synchronize()
