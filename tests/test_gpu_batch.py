@@ -241,3 +241,37 @@ def test_batch_set_slopes_argument_checks():
     b.forward()
     b2.forward()
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize('pedestal', [False, True])
+def test_batch_fused_threshold_modes(pedestal):
+    """The threshold variants of the batch bodies (subtracted threshold / pedestal, LGS and NGS
+    with 8 and 16 px) through the C ABI: slopes, flux and totals equal each plan's own fused
+    forward with the same threshold."""
+    from specula_amd import _native
+    ents = workloads.multi_sensor_entries(n_lgs=2, n_pushpull=4)
+    b = MultiSensorBatch(ents, order=2, fuse_slopes=False)
+    lib = _native.lib()
+    thrs = []
+    for k, e in enumerate(ents):
+        assert e.plan.bind_slopec(e.slopec)
+        img = torch.empty_like(e.ccd)
+        e.plan.forward(e.pupil, e.phase, img, e.S0, n_frames=e.n_frames)
+        thr = 0.3 * float(img.mean())
+        thrs.append(thr)
+        rc = lib.shwfs_batch_set_slopes(b._h, k, thr, 1 if pedestal else 0, _native.ptr(e.slopes), e.slopes.shape[1],
+                                        _native.ptr(e.flux), _native.ptr(e.totals))
+        _native.check(rc, f'shwfs_batch_set_slopes {e.name}')
+    for e in ents:
+        e.slopes.zero_()
+    b.forward(slopes=False)
+    torch.cuda.synchronize()
+    for e, thr in zip(ents, thrs):
+        out, s = torch.empty_like(e.ccd), torch.empty_like(e.slopes)
+        fl, tt = torch.empty_like(e.flux), torch.empty_like(e.totals)
+        e.plan.set_static_pupil(e.pupil, e.S0)
+        e.plan.forward_slopes(e.pupil, e.phase, out, e.S0, s, fl, tt, thr=thr, pedestal=pedestal, n_frames=e.n_frames)
+        torch.cuda.synchronize()
+        print(f'{e.name}: thr {thr:.3e} pedestal={pedestal}', flush=True)
+        assert torch.equal(e.ccd, out) and torch.equal(e.slopes, s), e.name
+        assert torch.equal(e.flux, fl) and torch.equal(e.totals, tt), e.name
