@@ -793,10 +793,13 @@ __device__ __forceinline__ void ngs_mfma_frame(const NgsArgs& g, const NgsWave& 
             // the total comes from the same pixels as the x and y sums (sh_slopec.py:392), so that
             // errors common to the subaperture's pixels cancel in the slopes
             double m_x = l_x, m_y = l_y, m_t = (double)((cw0 + cw1) + (cw2 + cw3));
-#pragma unroll
-            for (int o = 1; o < X4; o <<= 1) {
-                m_t += __shfl_xor(m_t, o, kWave);
-                m_x += __shfl_xor(m_x, o, kWave); m_y += __shfl_xor(m_y, o, kWave);
+            // the column groups of one row: lanes x4 ^ 1 and x4 ^ 2 of the same quad (DPP)
+            if constexpr (X4 > 1) {
+                m_t += dpp_d<kDppXor1>(m_t); m_x += dpp_d<kDppXor1>(m_x); m_y += dpp_d<kDppXor1>(m_y);
+            }
+            if constexpr (X4 > 2) {
+                static_assert(X4 == 4, "at most 4 column groups");
+                m_t += dpp_d<kDppXor2>(m_t); m_x += dpp_d<kDppXor2>(m_x); m_y += dpp_d<kDppXor2>(m_y);
             }
 #pragma unroll
             for (int o = LPR; o < 64; o <<= 1) {
